@@ -1,0 +1,152 @@
+"""Benchmark of the clumped-sphere DEM timestep on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 200,000
+clumps from 8 random templates of 2-8 sub-spheres (lognormal radii, median
+1 mm, sigma 0.15, clipped to [0.6, 1.6] mm), about 1.12M spheres, deposited in
+a cylinder wall of radius 60 mm; E = 1e8 Pa, nu = 0.3, rho = 2650 kg/m^3,
+e = 0.5, mu = 0.5, dt = 5e-7 s, uniform grid. A "step" is one full timestep
+of the whole bed (kinematics, rebuild when flagged, contacts, walls, clump
+reduction, integration) with all state resident in HBM.
+
+value: sphere-steps/s of the FP32-mixed form; the plain FP64 form runs the
+same steps beside it. roofline: the contact kernel (k_contacts), the dominant
+kernel, with algorithmic bytes per launch
+    N * 72 B + C_ss * 80 B + C_w * 72 B
+(per-array accounting of the force inputs/outputs; C_ss / C_w = touching
+sphere-sphere / wall contacts counted on the device each step), timed with HIP
+events on the library stream. cpu_baseline: the FP64 oracle (the reference's
+slot-padded dataflow restated in C++ with OpenMP) on a bounded sample of the
+same bed.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nbody-clump_amd"))
+
+HBM_PEAK = 8.0e12
+
+
+def run_form(bed, precision, steps, warmup):
+    import nbc
+
+    opt = nbc.make_options(precision=precision, wall_type=nbc.NBC_WALL_CYLINDER)
+    params = bed.finalized_params(opt)
+    ctx = nbc.Context(params, opt)
+    ctx.upload(bed, bed.wall)
+    ctx.step(warmup)
+    s0 = ctx.stats()
+    ctx.profile(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)  # returns after a stream synchronize
+    t1 = time.perf_counter()
+    s1 = ctx.stats()
+    k3_ms, k3_n = ctx.profile_get(1)
+    out = dict(seconds=t1 - t0, k3_ms=k3_ms, k3_n=k3_n, rebuilds=s1.rebuilds - s0.rebuilds,
+               touching_ss=0.5 * (s0.touching_ss + s1.touching_ss), touching_wall=0.5 * (s0.touching_wall + s1.touching_wall),
+               front_pairs=s1.front_pairs)
+    for cls in (0, 2, 3, 4):
+        out[f"cls{cls}_ms"], out[f"cls{cls}_n"] = ctx.profile_get(cls)
+    ctx.close()
+    return out
+
+
+def cpu_baseline(bed, sample_steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+
+    orc.set_threads(os.cpu_count() or 1)
+    p = orc.finalize_params(bed)
+    o = orc.Oracle(p)
+    o.upload(bed, bed.wall)
+    o.step(1)  # first step builds the lists
+    t0 = time.perf_counter()
+    o.step(sample_steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return dict(value=bed.n * sample_steps / dt, unit="sphere-steps/s", cores=orc.threads(), kind="port",
+                sample=f"FP64 oracle (slot-padded reference dataflow, OpenMP), same 1.12M-sphere bed, "
+                       f"{sample_steps} steps after the first (list-building) step, BranchLmtLen "
+                       f"{bed.params.BranchLmtLen}, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nclumps", type=int, default=200000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import nbc
+
+    if nbc.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    bed = nbc.generate_bed(1, a.nclumps, seed=1)
+    bed.params.BranchLmtLen = 64
+    print(f"[bench] bed: {bed.n} spheres, {bed.nc} clumps", file=sys.stderr, flush=True)
+    mixed = run_form(bed, nbc.NBC_MIXED, a.steps, a.warmup)
+    print(f"[bench] mixed: {mixed}", file=sys.stderr, flush=True)
+    fp64 = None
+    if not a.no_fp64:
+        fp64 = run_form(bed, nbc.NBC_FP64, a.steps, a.warmup)
+        print(f"[bench] fp64: {fp64}", file=sys.stderr, flush=True)
+    N = bed.n
+    value = N * a.steps / mixed["seconds"]
+    k3_avg = mixed["k3_ms"] / max(mixed["k3_n"], 1) * 1e-3
+    k3_bytes = N * 72 + mixed["touching_ss"] * 80 + mixed["touching_wall"] * 72
+    achieved = k3_bytes / k3_avg
+    # whole-step algorithmic bytes (B_step, mixed): N*236 + Nc*336 + C_ss*80 + C_w*72
+    step_bytes = N * 236 + bed.nc * 336 + mixed["touching_ss"] * 80 + mixed["touching_wall"] * 72
+    res = {
+        "metric": "sphere-steps/s @1M clumped spheres, FP32-mixed vs FP64; % HBM peak",
+        "value": value,
+        "unit": "sphere-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": mixed["seconds"] / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces / f64 integration",
+        "data": "synthetic (deterministic generator, seed 1)",
+        "config": {
+            "workload": "C1 bed: 200,000 clumps from 8 templates of 2-8 spheres, lognormal radii, "
+                        "cylinder R 60 mm, dt 5e-7 s, uniform grid",
+            "n_spheres": N, "n_clumps": bed.nc, "touching_ss_per_sphere": mixed["touching_ss"] / N,
+            "list_pairs_per_sphere": mixed["front_pairs"] / N, "rebuilds_in_timed_steps": mixed["rebuilds"],
+        },
+        "roofline": {
+            "kernel": "k_contacts",
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "avg_launch_us": k3_avg * 1e6, "algorithmic_bytes_per_launch": k3_bytes,
+        },
+        "step_hbm_frac": step_bytes * a.steps / mixed["seconds"] / HBM_PEAK,
+    }
+    if fp64:
+        v64 = N * a.steps / fp64["seconds"]
+        res["fp64_value"] = v64
+        res["speedup_vs_fp64"] = value / v64
+    if not a.no_cpu:
+        try:
+            res["cpu_baseline"] = cpu_baseline(bed, a.cpu_steps)
+        except Exception as e:  # report, do not hide
+            res["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

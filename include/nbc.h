@@ -1,0 +1,326 @@
+/*
+ * nbc.h — C ABI of the MI355X clumped-sphere DEM timestep (libnbc.so).
+ *
+ * Plain C: fixed-width integers, doubles, pointers and sizes. No HIP or torch
+ * types cross this boundary; the library owns its device memory and its HIP
+ * stream. Every entry point returns an nbc error code (0 = success) and never
+ * calls exit(); nbc_last_error() gives the message. The reference aborts the
+ * process in the same three situations (neighbour-list overflow, invalid or
+ * non-finite wall input, bad input files); a caller that wants the reference's
+ * behaviour prints the message and exits with EXIT_FAILURE, which is what the
+ * bundled nbody_clump driver does.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   nbc_run_simulation_clump   DEMClumpMain::RunSimulationClump   src/dem_clump.h:95,
+ *                                                                 src/dem_simulation_clump.cu:3-261
+ *   nbc_load_world_par         DEMSubSphereIo::LoadParameters     src/dem_subsphere_io.cu:40-149
+ *   nbc_finalize_params        DEMSubSphereIo::SetMeshSize        src/dem_subsphere_io.cu:195-241
+ *   nbc_load_phase             DEMSubSphereIo::LoadPhase          src/dem_subsphere_io.cu:151-193
+ *   nbc_load_clumps            DEMClumpSphereIo::LoadPhase        src/dem_clumpsphere_io.cu:24-62
+ *   nbc_load_cube_wall         DEMWallCubeIo::LoadInfo            src/dem_cube_wall_io.cu:21-53
+ *   nbc_load_cylinder_wall     DEMWallCylinderIo::LoadInfo        src/dem_cylind_wall_io.cu:20-56
+ *   nbc_load_penetrator        DEMPeneCylinderIo::LoadInfo        src/dem_penetrator_io.cu:21-116
+ *   nbc_create / nbc_upload    DEMClumpMain::AllocateGpuMemory + CopyDataFromHostToDevice,
+ *                              DEMClumpSphere::AllocateGpuMemory, DEMGrid::GridConfig,
+ *                              DEMForce::ForceConfig, DEMWallCube::AllocateMemory,
+ *                              DEMPeneCylinder::AllocateMemory    src/dem_clump.cu:49-82,
+ *                              src/dem_clumpsphere.cu:28-61, src/dem_grid.cu:212-271,
+ *                              src/dem_force.cu:32-66, src/dem_cube_wall.cu:213-319,
+ *                              src/dem_penetrator.cu:89-163
+ *   nbc_step                   one pass of the loop body          src/dem_simulation_clump.cu:110-226
+ *   nbc_rebuild                DEMGrid::NeighborSearch + SortArray + CreateContList
+ *                                                                 src/dem_grid.cu:649-750
+ *   nbc_get_*                  CopyDataToIo / CopyInfoToIo        src/dem_clump.cu:151-191,
+ *                                                                 src/dem_clumpsphere.cu:63-70,
+ *                                                                 src/dem_penetrator.cu:207-218
+ *   nbc_write_*                dem_binary::* writers              src/dem_binary_io.h:100-180,
+ *                                                                 src/dem_subsphere_io.cu:262-311
+ */
+#ifndef NBC_H
+#define NBC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+enum {
+    NBC_OK = 0,
+    NBC_E_BADINPUT = 1,          /* invalid argument / malformed input file          */
+    NBC_E_NEIGHBOR_OVERFLOW = 2, /* a sphere has more than BranchLmtLen/2 neighbours
+                                    on one side (src/dem_grid.cu:709-719)            */
+    NBC_E_NONFINITE = 3,         /* invalid or non-finite sphere/wall state
+                                    (src/dem_force_cube_wall_clump.cu:191-203,295-297) */
+    NBC_E_HIP = 4,               /* a HIP runtime call failed                        */
+    NBC_E_NODEVICE = 5,          /* no usable GPU                                    */
+    NBC_E_IO = 6,                /* file could not be opened or written              */
+    NBC_E_PARAM = 7              /* world.par: unknown key or missing keys
+                                    (src/dem_subsphere_io.cu:130,136-141)            */
+};
+
+/* ---- enumerations ------------------------------------------------------ */
+enum { NBC_WALL_NONE = 0, NBC_WALL_CUBE = 1, NBC_WALL_CYLINDER = 2 };
+enum { NBC_FP64 = 0, NBC_MIXED = 1 };
+/* PenetratorShape (src/dem_parameters.h:100) plus the triangulated cone. */
+enum { NBC_PENE_CYLINDER = 0, NBC_PENE_SPHERE = 1, NBC_PENE_CONE = 2 };
+/* PenetratorMotionMode (src/dem_parameters.h:101). */
+enum { NBC_PENE_CONSTANT_VELOCITY = 0, NBC_PENE_GRAVITY = 1 };
+
+/* ---- run parameters ----------------------------------------------------
+ * The 38 world.par keys (src/world.par:18-64, parsed by
+ * src/dem_subsphere_io.cu:83-128 into Parameters/GlobalFlg,
+ * src/dem_parameters.h:7-78) followed by the derived fields that
+ * LoadParameters/SetMeshSize fill in. */
+typedef struct nbc_params {
+    int32_t ParticleNum;     /* NumberBody   */
+    int32_t ClumpNum;
+    int32_t BranchLmtLen;
+    int32_t StartStep;
+    int32_t EndStep;
+    int32_t OutputInterval;
+    double StepSize;
+    double Ymod, Prat, Frcoe, NormRes, Frvel, Beta, MU_T, MU_R, Cohesion, Gravity;
+    double Thresh, GridLevels;
+    double Update;           /* initial rebuild flag (key "Update")            */
+    double WorldSize[3], WorldOrigin[3], BoxUpLmt[3], BoxLowLmt[3];
+    int32_t isIntruder, isSelfGravity, isWallForce, isQuaternion;
+    /* derived: Damping (src/dem_subsphere_io.cu:143-145), uniform mesh
+       (src/dem_subsphere_io.cu:197-230). WorldOrigin/WorldSize above are the
+       values after SetMeshSize once nbc_finalize_params has run. */
+    double Damping;
+    double CellSize[3];
+    double i_CellSize[3];
+    uint32_t GridDim[3];
+    int32_t CellNum;
+    int32_t finalized;
+} nbc_params;
+
+/* Options that are not world.par keys (read from the optional nbc.par file by
+ * the driver so that world.par stays 38-key compatible). Zero = default. */
+typedef struct nbc_options {
+    int32_t precision;       /* NBC_FP64 (default) or NBC_MIXED                */
+    int32_t wall_type;       /* 0 = NBC_WALL_CUBE default when isWallForce     */
+    int32_t block_size;      /* threads per block for per-sphere kernels (64..1024) */
+    int32_t radix_bits;      /* LSD radix digit width for the cell sort (1..8) */
+    int32_t pair_capacity;   /* front-pair slots reserved per sphere (average) */
+    int32_t reserved0;
+    double pene_frcoe;       /* penetrator friction; <= 0 means Frcoe          */
+    double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
+    double reserved[4];
+} nbc_options;
+
+/* ---- boundaries -------------------------------------------------------- */
+/* Cube (Rocks/CubeWall.bt, src/dem_cube_wall_io.cu:43-48): half side lengths,
+ * pose and motion. Cylinder (Rocks/CylindWall.bt,
+ * src/dem_cylind_wall_io.cu:41-49): radius, half height, axis, property. */
+typedef struct nbc_wall {
+    int32_t type;            /* NBC_WALL_* */
+    int32_t reserved;
+    double side[3];          /* cube: SideLength (half-extents)               */
+    double radius, height;   /* cylinder: Radius, Height (half height)        */
+    double dir[3];           /* cylinder: Dirct                               */
+    double prop[4];          /* cylinder: Property                            */
+    double pos[3], vel[3], quat[4], angv[3];
+} nbc_wall;
+
+/* Penetrator (Rocks/Penetrator.bt, src/dem_penetrator_io.cu:21-116,
+ * struct Cylinder src/dem_parameters.h:85-98). A cone is given as a closed
+ * triangle mesh in the penetrator frame (positions relative to pos). */
+typedef struct nbc_penetrator {
+    int32_t shape;           /* NBC_PENE_* */
+    int32_t motion;          /* NBC_PENE_CONSTANT_VELOCITY / NBC_PENE_GRAVITY */
+    int32_t wait_for_contact;
+    int32_t contact_state;
+    double radius, height;
+    double dir[3];
+    double prop[4];          /* Ix Iy Iz mass */
+    double pos[3], vel[3], quat[4], angv[3];
+    int32_t ntri;            /* cone only: number of triangles                */
+    int32_t reserved;
+    const double* tri;       /* cone only: 9*ntri doubles (v0 v1 v2), may be NULL otherwise */
+} nbc_penetrator;
+
+/* ---- host-side state (file order, double3 packed as 3 doubles) -------- */
+typedef struct nbc_spheres_in {      /* ph.00000000.bt columns */
+    int32_t n;
+    int32_t reserved;
+    const int32_t* clump_id;
+    const double* pos;       /* 3n */
+    const double* local;     /* 3n */
+    const double* mass;      /* n  */
+    const double* radius;    /* n  */
+} nbc_spheres_in;
+
+typedef struct nbc_clumps_in {       /* ch.00000000.bt columns */
+    int32_t n;
+    int32_t reserved;
+    const double* pos;       /* 3n */
+    const double* vel;       /* 3n */
+    const double* quat;      /* 4n (x y z w) */
+    const double* angv;      /* 3n */
+    const double* prop;      /* 4n (Ix Iy Iz mass) */
+} nbc_clumps_in;
+
+/* Per-step statistics of the last step (FP64 global sums). */
+typedef struct nbc_stats {
+    int64_t step;            /* completed steps since upload                  */
+    int64_t rebuilds;        /* neighbour-list rebuilds since upload          */
+    int64_t front_pairs;     /* list pairs (i<j) incl. intra-clump            */
+    int64_t back_entries;
+    int64_t touching_ss;     /* touching sphere-sphere contacts, different clumps */
+    int64_t touching_wall;   /* touching wall + penetrator contacts          */
+    double kinetic_energy;   /* sum over clumps of 1/2 m v^2 + 1/2 sum I w^2 */
+    double pene_force[3];    /* reaction force on the penetrator (-sum F)    */
+    double pene_torque[3];
+    double max_cum_disp;     /* reserved */
+} nbc_stats;
+
+typedef struct nbc_ctx nbc_ctx;
+
+/* ---- library / errors -------------------------------------------------- */
+const char* nbc_version(void);
+const char* nbc_last_error(nbc_ctx* ctx);    /* ctx may be NULL: thread-global message */
+int nbc_device_count(int32_t* out);
+
+/* ---- drop-in file formats (host only, no device use) ------------------ */
+/* world.par: 38 keys exactly, unknown key -> NBC_E_PARAM, fewer -> NBC_E_PARAM.
+ * iniconfile receives the IniConFile value. Damping is derived. */
+int nbc_load_world_par(const char* path, nbc_params* out, char* iniconfile, size_t iniconfile_len);
+/* SetMeshSize: cell = 3*max radius (or opt->cell_size), origin -= cell,
+ * size += 2*cell, GridDim = ceil(size/cell). opt may be NULL. */
+int nbc_finalize_params(nbc_params* p, const double* radius, int32_t n, const nbc_options* opt);
+/* Optional extension file nbc.par ("key = value"); missing file leaves defaults. */
+int nbc_load_options(const char* path, nbc_options* out);
+/* ph text: n rows "id sub clumpId x y z lx ly lz mass radius". Buffers caller-owned. */
+int nbc_load_phase(const char* path, int32_t n, int32_t* clump_id, double* pos, double* local,
+                   double* mass, double* radius);
+/* ch text: n rows "id x y z vx vy vz qx qy qz qw wx wy wz Ix Iy Iz mass". */
+int nbc_load_clumps(const char* path, int32_t n, double* pos, double* vel, double* quat,
+                    double* angv, double* prop);
+int nbc_load_cube_wall(const char* path, nbc_wall* out);
+int nbc_load_cylinder_wall(const char* path, nbc_wall* out);
+/* Penetrator.bt; a "cone R H" shape token is an extension. The cone's mesh
+ * buffer is library-owned and lives until nbc_free_mesh. */
+int nbc_load_penetrator(const char* path, nbc_penetrator* out);
+void nbc_free_mesh(nbc_penetrator* p);
+/* Triangulate a cone of base radius R and height H in the penetrator frame:
+ * base disc at z = 0, apex at z = -H. nseg rim segments give 2*nseg
+ * triangles (9 doubles each) written to tri_out when it is not NULL.
+ * Returns the triangle count or < 0 on error. */
+int nbc_make_cone_mesh(double radius, double height, int32_t nseg, double* tri_out);
+
+/* ---- synthetic beds --------------------------------------------------- */
+/* kind 0 = C0: nclumps (default 2 000) collinear 3-sphere clumps, r = 1 mm
+ *          x U[0.9, 1.1], 0.8 r spacing, shuffled lattice in a cube
+ *          40 x 40 x 80 mm (container[0..2] override the edge lengths);
+ * kind 1 = C1: nclumps (default 200 000) clumps from 8 random templates of
+ *          2-8 spheres, lognormal radii (median 1 mm, sigma 0.15, clipped to
+ *          [0.6, 1.6] mm), sequential deposition into a cylinder of radius
+ *          60 mm (container[0] overrides) whose height fits the bed.
+ * The generator allocates the output arrays; release them with nbc_free_bed. */
+typedef struct nbc_bed_spec {
+    int32_t kind;
+    int32_t nclumps;
+    uint64_t seed;
+    double container[4];
+} nbc_bed_spec;
+
+typedef struct nbc_bed_out {
+    int32_t n, nc;           /* sphere and clump counts                       */
+    int32_t* clump_id;       /* n   */
+    double* pos;             /* 3n  */
+    double* local;           /* 3n  */
+    double* mass;            /* n   */
+    double* radius;          /* n   */
+    double* cpos;            /* 3nc */
+    double* cvel;            /* 3nc */
+    double* cquat;           /* 4nc */
+    double* cangv;           /* 3nc */
+    double* cprop;           /* 4nc */
+    nbc_wall wall;           /* container                                      */
+    nbc_params params;       /* world.par values for the bed (not finalized)   */
+} nbc_bed_out;
+
+int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out);
+void nbc_free_bed(nbc_bed_out* out);
+/* Writes world.par, ph.00000000.bt, ch.00000000.bt, Rocks/{CubeWall,CylindWall,
+ * Penetrator}.bt and nbc.par for a generated bed (pene may be NULL). */
+int nbc_save_bed_text(const char* dir, const nbc_bed_out* bed, const nbc_penetrator* pene);
+
+/* ---- device context ---------------------------------------------------- */
+int nbc_create(nbc_ctx** out, const nbc_params* params, const nbc_options* opt, int32_t device);
+int nbc_destroy(nbc_ctx* ctx);
+/* Copy host state in (file order) and reset histories; wall may be NULL when
+ * isWallForce == 0; pene may be NULL when isIntruder == 0. */
+int nbc_upload(nbc_ctx* ctx, const nbc_spheres_in* s, const nbc_clumps_in* c,
+               const nbc_wall* wall, const nbc_penetrator* pene);
+/* Advance nsteps full timesteps (src/dem_simulation_clump.cu:110-226). */
+int nbc_step(nbc_ctx* ctx, int32_t nsteps);
+/* Force a neighbour-list rebuild on the current positions. */
+int nbc_rebuild(nbc_ctx* ctx);
+/* Record per-contact exports (penetrator per-sphere Fn/Ft/Tn/Tt, pair normal
+ * force and contact point) during the next step. */
+int nbc_set_record(nbc_ctx* ctx, int32_t on);
+int nbc_synchronize(nbc_ctx* ctx);
+int nbc_get_stats(nbc_ctx* ctx, nbc_stats* out);
+
+/* ---- getters (sorted order; any pointer may be NULL) ------------------ */
+int nbc_get_spheres(nbc_ctx* ctx, int32_t* file_id, int32_t* clump_id, double* pos, double* cum,
+                    double* local, double* mass, double* radius);
+int nbc_get_grid(nbc_ctx* ctx, int32_t* cell, int32_t* perm, int32_t* cell_beg, int32_t* cell_end);
+int nbc_get_pair_counts(nbc_ctx* ctx, int64_t* nfront, int64_t* nback);
+/* Front list (j > i, reference search order), flags bit0 = intra-clump;
+ * back list of each sphere (j < i, reverse search order) with the owning
+ * front slot (-1 if the owner's search did not find the pair). */
+int nbc_get_pairs(nbc_ctx* ctx, int32_t* front_off, int32_t* front_j, uint8_t* front_flags,
+                  int32_t* back_off, int32_t* back_i, int32_t* back_slot);
+/* Tangential/twisting/rolling history written by the last step per front
+ * slot (9 doubles; zero where the pair was not touching). */
+int nbc_get_pair_history(nbc_ctx* ctx, double* hist9);
+/* Per-sphere force / torque of the last step recorded with nbc_set_record:
+ * kind 0 = total (contact + wall + penetrator, src/dem_clump.cu:97-98),
+ * 1 = sphere-sphere, 2 = wall, 3 = penetrator. */
+int nbc_get_sphere_forces(nbc_ctx* ctx, int32_t kind, double* force, double* torque);
+int nbc_get_clumps(nbc_ctx* ctx, double* pos, double* vel, double* quat, double* angv,
+                   double* force, double* torque);
+/* Penetrator pose and the reaction history cache (OutputInterval entries). */
+int nbc_get_penetrator(nbc_ctx* ctx, double* pos, double* vel, double* quat, double* angv,
+                       double* cache_f, double* cache_t, int32_t* contact_state);
+/* Recorded exports of the last step with recording on. */
+int nbc_get_pene_exports(nbc_ctx* ctx, double* fn, double* ft, double* tn, double* tt);
+int nbc_get_pair_exports(nbc_ctx* ctx, double* fcn, double* cpt);
+
+/* ---- timing of library kernels on the library stream ----------------- */
+/* Kernel classes: 0 kinematics(K1+K2), 1 contacts, 2 clump reduction,
+ * 3 rebuild, 4 penetrator sum. */
+int nbc_profile_enable(nbc_ctx* ctx, int32_t on);
+int nbc_profile_get(nbc_ctx* ctx, int32_t kernel_class, double* total_ms, int64_t* launches);
+
+/* ---- binary writers (src/dem_binary_io.h layout, native little-endian) */
+int nbc_write_particles_bin(const char* dir, int32_t step, int32_t n, const int32_t* sorted_id,
+                            const int32_t* file_id, const int32_t* clump_id, const double* pos,
+                            const double* local, const double* mass, const double* radius);
+int nbc_write_clumps_bin(const char* dir, int32_t step, int32_t nc, const double* pos,
+                         const double* vel, const double* quat, const double* angv,
+                         const double* prop, const double* pene_pos, const double* pene_vel,
+                         const double* pene_quat, const double* pene_angv);
+int nbc_write_force_history_bin(const char* dir, int32_t step, int32_t interval,
+                                const double* force, const double* torque);
+int nbc_write_yi_bin(const char* dir, int32_t step, int32_t n, const int32_t* clump_id,
+                     const double* fn, const double* ft, const double* tn, const double* tt);
+int nbc_write_contacts_bin(const char* dir, int32_t step, int32_t n, int32_t branch,
+                           const int32_t* nei, const double* fcn, const double* cpt);
+
+/* ---- the drop-in entry point ------------------------------------------ */
+/* Runs the whole simulation like DEMClumpMain::RunSimulationClump from the
+ * files in workdir (world.par, IniConFile, ch.00000000.bt, Rocks/, optional
+ * nbc.par), writing OutPart/OutClump/OutForce/OutCnt there. */
+int nbc_run_simulation_clump(const char* workdir, int32_t device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBC_H */

@@ -1,0 +1,114 @@
+// Device-side types and math for the MI355X clumped-sphere DEM kernels.
+//
+// Arithmetic helpers evaluate in the same order as the reference's
+// src/type.h operators so that the FP64 instantiation, compiled with FP
+// contraction off, reproduces the reference formulas bit for bit. The FP32
+// ("mixed") instantiation uses the same code with T = float.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace nbc {
+
+struct alignas(32) D4 { double x, y, z, w; };
+struct alignas(16) F4 { float x, y, z, w; };
+
+template <class T> struct Vec4T;
+template <> struct Vec4T<double> { typedef D4 type; };
+template <> struct Vec4T<float> { typedef F4 type; };
+
+template <class T>
+struct V3 {
+    T x, y, z;
+};
+
+template <class T> __host__ __device__ __forceinline__ V3<T> mk(T x, T y, T z) { V3<T> v; v.x = x; v.y = y; v.z = z; return v; }
+template <class T> __host__ __device__ __forceinline__ V3<T> operator+(const V3<T>& a, const V3<T>& b) { return mk<T>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> __host__ __device__ __forceinline__ V3<T> operator-(const V3<T>& a, const V3<T>& b) { return mk<T>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class T> __host__ __device__ __forceinline__ V3<T> operator*(T s, const V3<T>& u) { return mk<T>(s * u.x, s * u.y, s * u.z); }
+template <class T> __host__ __device__ __forceinline__ T Dot(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __host__ __device__ __forceinline__ V3<T> Cross(const V3<T>& a, const V3<T>& b) {
+    T x = a.y * b.z - a.z * b.y;
+    T y = a.z * b.x - a.x * b.z;
+    T z = a.x * b.y - a.y * b.x;
+    return mk<T>(x, y, z);
+}
+// |v| with the reference's cut-off: zero at or below |v|^2 = 1e-32.
+template <class T> __host__ __device__ __forceinline__ T Length(const V3<T>& v) {
+    T t = v.x * v.x + v.y * v.y + v.z * v.z;
+    return t > T(1e-32) ? sqrt(t) : T(0);
+}
+template <class T> __host__ __device__ __forceinline__ V3<T> cvt3(const V3<double>& v) { return mk<T>((T)v.x, (T)v.y, (T)v.z); }
+
+// Body -> world rotation of quaternion (x, y, z, w).
+template <class T> __host__ __device__ __forceinline__ void QuatIvDCM(T qx, T qy, T qz, T qw, T m[3][3]) {
+    m[0][0] = qx * qx - qy * qy - qz * qz + qw * qw;
+    m[1][0] = T(2.0) * (qx * qy + qz * qw);
+    m[2][0] = T(2.0) * (qx * qz - qy * qw);
+    m[0][1] = T(2.0) * (qx * qy - qz * qw);
+    m[1][1] = -qx * qx + qy * qy - qz * qz + qw * qw;
+    m[2][1] = T(2.0) * (qy * qz + qx * qw);
+    m[0][2] = T(2.0) * (qx * qz + qy * qw);
+    m[1][2] = T(2.0) * (qy * qz - qx * qw);
+    m[2][2] = -qx * qx - qy * qy + qz * qz + qw * qw;
+}
+// World -> body rotation.
+template <class T> __host__ __device__ __forceinline__ void QuatDCM(T qx, T qy, T qz, T qw, T m[3][3]) {
+    m[0][0] = qx * qx - qy * qy - qz * qz + qw * qw;
+    m[1][0] = T(2.0) * (qx * qy - qz * qw);
+    m[2][0] = T(2.0) * (qx * qz + qy * qw);
+    m[0][1] = T(2.0) * (qx * qy + qz * qw);
+    m[1][1] = -qx * qx + qy * qy - qz * qz + qw * qw;
+    m[2][1] = T(2.0) * (qy * qz - qx * qw);
+    m[0][2] = T(2.0) * (qx * qz - qy * qw);
+    m[1][2] = T(2.0) * (qy * qz + qx * qw);
+    m[2][2] = -qx * qx - qy * qy + qz * qz + qw * qw;
+}
+template <class T> __host__ __device__ __forceinline__ V3<T> Map(const T m[3][3], const V3<T>& u) {
+    T a = m[0][0] * u.x + m[0][1] * u.y + m[0][2] * u.z;
+    T b = m[1][0] * u.x + m[1][1] * u.y + m[1][2] * u.z;
+    T c = m[2][0] * u.x + m[2][1] * u.y + m[2][2] * u.z;
+    return mk<T>(a, b, c);
+}
+
+// ---- pair-word layout of the front list ------------------------------------
+// bits 0..28  partner sphere index j (sorted order)
+// bit  29     intra-clump pair (listed, never a contact)
+// bit  30/31  history in buffer 0/1 is valid (written by the step that wrote it)
+constexpr uint32_t JMASK = 0x1FFFFFFFu;
+constexpr uint32_t INTRA = 1u << 29;
+__host__ __device__ __forceinline__ uint32_t VALID(int p) { return 1u << (30 + p); }
+
+constexpr int HSTRIDE = 9;   // tangential(3) twisting(3) rolling(3) per history record
+constexpr int MAX_FACES = 7; // cube 6 (or cylinder 3) + penetrator 1
+
+// ---- run parameters passed by value ----------------------------------------
+struct DevParams {
+    int N, Nc, B, CellNum;
+    uint32_t G[3];
+    double origin[3], cell[3], icell[3], boxlo[3], boxup[3];
+    double dt, Ymod, Prat, Frcoe, Beta, Cohesion, Gravity, Damping, Thresh;
+    double pene_frcoe;
+    int wall_type;        // NBC_WALL_*
+    int nfaces;           // wall faces in use (6 cube, 3 cylinder, 0 none)
+    int pene_on;          // isIntruder
+    int pene_face;        // history slot of the penetrator (= nfaces)
+    // wall
+    double wside[3], wrad, wheight, wdir[3], wpos[3], wvel[3], wangv[3];
+    // penetrator (static part)
+    int pshape, pmotion, pwait, ntri;
+    double prad, pheight, pdir[3], pmass;
+    const double* ptri;   // 9*ntri, penetrator frame
+};
+
+// Penetrator dynamic state, device resident.
+struct PeneState {
+    double pos[3], vel[3], angv[3], quat[4];
+    double totF[3], totT[3];
+    int contact_state;
+    int pad;
+};
+
+}  // namespace nbc

@@ -1,0 +1,828 @@
+// Device context and step orchestration behind the nbc C ABI.
+//
+// The loop body of src/dem_simulation_clump.cu:110-226 becomes, per step:
+//   K1 k_clump_before, K2 k_sphere_pose (sets the rebuild flag),
+//   one 8-byte status read-back, the rebuild sequence only when flagged or on
+//   the first step, K3 k_contacts, k_pene_finalize (penetrator runs only),
+//   K4 k_clump_after.
+// The status read-back also reports K3's invalid-input index from the
+// previous step, so the reference's fatal wall checks
+// (src/dem_force_cube_wall_clump.cu:322-331) surface as NBC_E_NONFINITE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nbc_engine.h"
+#include "nbc_kernels.h"
+
+namespace nbc {
+
+#define HIPC(x)                                                                 \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) return set_hip_error(e_, #x, __LINE__);           \
+    } while (0)
+#define KC(x)                                                                   \
+    do {                                                                        \
+        int e_ = (x);                                                           \
+        if (e_ != 0) return set_hip_error((hipError_t)e_, #x, __LINE__);        \
+    } while (0)
+
+int EngineBase::set_hip_error(hipError_t e, const char* what, int line) {
+    char b[512];
+    std::snprintf(b, sizeof b, "HIP error %d (%s) in %s [nbc_engine.cpp:%d]", (int)e, hipGetErrorString(e), what, line);
+    err = b;
+    return NBC_E_HIP;
+}
+int EngineBase::fail(int code, const std::string& m) {
+    err = m;
+    return code;
+}
+
+static void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+template <class T>
+static hipError_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+
+static int bits_for(long long maxkey) {
+    int b = 1;
+    while (b < 31 && (1ll << b) <= maxkey) ++b;
+    return b;
+}
+
+template <class R>
+class Engine : public EngineBase {
+  public:
+    typedef typename Vec4T<R>::type R4;
+
+    Engine() {}
+    ~Engine() override { release(); }
+
+    int init(const nbc_params& p, const nbc_options& o, int dev) override {
+        P = p;
+        O = o;
+        device = dev;
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        N = P.ParticleNum;
+        Nc = P.ClumpNum;
+        if (N <= 0 || Nc <= 0 || P.BranchLmtLen < 2 || P.CellNum <= 0)
+            return fail(NBC_E_BADINPUT, "nbc_create: ParticleNum, ClumpNum, BranchLmtLen and the mesh must be set "
+                                        "(call nbc_finalize_params first)");
+        if ((long long)N > (long long)JMASK)
+            return fail(NBC_E_BADINPUT, "nbc_create: too many spheres for the 29-bit pair index");
+        block = O.block_size > 0 ? O.block_size : 256;
+        if (block < 64 || block > 1024 || (block % 64) != 0)
+            return fail(NBC_E_BADINPUT, "nbc_create: block_size must be a multiple of 64 in [64, 1024]");
+        if (block > 256) block = 256;  // k_contacts is launched with __launch_bounds__(256)
+        rbits = O.radix_bits > 0 ? O.radix_bits : 8;
+        if (rbits > 8) rbits = 8;
+        wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
+        nfaces = wall_type == NBC_WALL_CUBE ? 6 : (wall_type == NBC_WALL_CYLINDER ? 3 : 0);
+        pair_cap_per = O.pair_capacity > 0 ? O.pair_capacity : 16;
+        // device parameters
+        std::memset(&dp, 0, sizeof(dp));
+        dp.N = N; dp.Nc = Nc; dp.B = P.BranchLmtLen; dp.CellNum = P.CellNum;
+        for (int a = 0; a < 3; ++a) {
+            dp.G[a] = P.GridDim[a];
+            dp.origin[a] = P.WorldOrigin[a];
+            dp.cell[a] = P.CellSize[a];
+            dp.icell[a] = P.i_CellSize[a];
+            dp.boxlo[a] = P.BoxLowLmt[a];
+            dp.boxup[a] = P.BoxUpLmt[a];
+        }
+        dp.dt = P.StepSize; dp.Ymod = P.Ymod; dp.Prat = P.Prat; dp.Frcoe = P.Frcoe; dp.Beta = P.Beta;
+        dp.Cohesion = P.Cohesion; dp.Gravity = P.Gravity; dp.Damping = P.Damping; dp.Thresh = P.Thresh;
+        dp.pene_frcoe = O.pene_frcoe > 0.0 ? O.pene_frcoe : P.Frcoe;
+        dp.wall_type = wall_type; dp.nfaces = nfaces;
+        dp.pene_on = P.isIntruder ? 1 : 0;
+        dp.pene_face = nfaces;
+        nwh = nfaces + (dp.pene_on ? 1 : 0);
+        oi = P.OutputInterval > 0 ? P.OutputInterval : 1;
+        // spheres (double-buffered for the rebuild gather)
+        for (int b = 0; b < 2; ++b) {
+            SphereBuf<R>& s = S[b];
+            HIPC(dalloc(&s.pos, N)); HIPC(dalloc(&s.cum, N)); HIPC(dalloc(&s.loc, N)); HIPC(dalloc(&s.kin, N));
+            HIPC(dalloc(&s.cid, N)); HIPC(dalloc(&s.fid, N)); HIPC(dalloc(&s.wflag, N));
+            HIPC(dalloc(&s.wh, (size_t)std::max(nwh, 1) * N * HSTRIDE));
+            if (b == 0) HIPC(dalloc(&s.sft, 2 * (size_t)N));
+            else s.sft = S[0].sft;
+        }
+        HIPC(dalloc(&C.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C.prop, Nc));
+        HIPC(dalloc(&C.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C.snap, 2 * (size_t)Nc));
+        // lists
+        for (int b = 0; b < 2; ++b) {
+            HIPC(dalloc(&Lf[b].foff, (size_t)N + 1));
+            Lf[b].cap = 0; Lf[b].fword = nullptr; Lf[b].hist[0] = Lf[b].hist[1] = nullptr;
+        }
+        HIPC(dalloc(&boff, (size_t)N + 1));
+        bcap = 0; bent = nullptr;
+        HIPC(grow_front(0, (long long)N * pair_cap_per));
+        HIPC(grow_front(1, (long long)N * pair_cap_per));
+        HIPC(grow_back((long long)N * pair_cap_per));
+        // sort / grid scratch
+        const int nk = std::max(N, Nc);
+        HIPC(dalloc(&key0, nk)); HIPC(dalloc(&key1, nk)); HIPC(dalloc(&val0, nk)); HIPC(dalloc(&val1, nk));
+        HIPC(dalloc(&hist, radix_tmp_ints(nk, 8))); HIPC(dalloc(&hscan, radix_tmp_ints(nk, 8)));
+        HIPC(dalloc(&stmp, scan_tmp_ints(std::max(radix_tmp_ints(nk, 8), N + 1))));
+        HIPC(dalloc(&stotal, 4));
+        HIPC(dalloc(&cellkey, N)); HIPC(dalloc(&perm, N));
+        HIPC(dalloc(&cbeg, P.CellNum)); HIPC(dalloc(&cend, P.CellNum));
+        HIPC(dalloc(&fcnt, (size_t)N + 1)); HIPC(dalloc(&bcnt, (size_t)N + 1));
+        HIPC(dalloc(&mbeg, Nc)); HIPC(dalloc(&mend, Nc)); HIPC(dalloc(&mem, N));
+        HIPC(dalloc(&dstatus, 4));
+        HIPC(dalloc(&counters, 2));
+        HIPC(dalloc(&pst, 1));
+        nbk = contacts_blocks(N, block);
+        HIPC(dalloc(&pene_partial, 6 * (size_t)nbk));
+        HIPC(dalloc(&cache_f, 3 * (size_t)oi)); HIPC(dalloc(&cache_t, 3 * (size_t)oi));
+        HIPC(dalloc(&ke_partial, (Nc + 255) / 256));
+        HIPC(hipHostMalloc((void**)&hstatus, 4 * sizeof(uint32_t), hipHostMallocDefault));
+        std::memset(&rec, 0, sizeof(rec));
+        return 0;
+    }
+
+    hipError_t grow_front(int b, long long cap) {
+        FrontList& L = Lf[b];
+        if (cap <= L.cap) return hipSuccess;
+        dfree(L.fword);
+        dfree(L.hist[0]);
+        dfree(L.hist[1]);
+        L.fword = nullptr; L.hist[0] = L.hist[1] = nullptr; L.cap = 0;
+        hipError_t e = dalloc(&L.fword, (size_t)cap);
+        if (e != hipSuccess) return e;
+        e = dalloc(&L.hist[0], (size_t)cap * HSTRIDE);
+        if (e != hipSuccess) return e;
+        e = dalloc(&L.hist[1], (size_t)cap * HSTRIDE);
+        if (e != hipSuccess) return e;
+        L.cap = cap;
+        return hipSuccess;
+    }
+    hipError_t grow_back(long long cap) {
+        if (cap <= bcap) return hipSuccess;
+        dfree(bent);
+        bent = nullptr; bcap = 0;
+        hipError_t e = dalloc(&bent, (size_t)cap);
+        if (e != hipSuccess) return e;
+        bcap = cap;
+        return hipSuccess;
+    }
+
+    void release() {
+        if (device < 0) return;
+        (void)hipSetDevice(device);
+        if (st) (void)hipStreamSynchronize(st);
+        auto F = [](void* p) { dfree(p); };
+        for (int b = 0; b < 2; ++b) {
+            SphereBuf<R>& s = S[b];
+            F(s.pos); F(s.cum); F(s.loc); F(s.kin); F(s.cid); F(s.fid); F(s.wflag); F(s.wh);
+            if (b == 0) F(s.sft);
+            F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
+        }
+        F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap);
+        F(boff); F(bent);
+        F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
+        F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
+        F(dstatus); F(counters); F(pst); F(pene_partial); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
+        F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
+        if (hstatus) (void)hipHostFree(hstatus);
+        for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
+        ev_pool.clear();
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+        device = -1;
+    }
+
+    // ------------------------------------------------------------------
+    int upload(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w, const nbc_penetrator* pe) override {
+        HIPC(hipSetDevice(device));
+        if (!s || !c || s->n != N || c->n != Nc)
+            return fail(NBC_E_BADINPUT, "nbc_upload: sphere/clump counts differ from the context parameters");
+        for (int i = 0; i < N; ++i)
+            if (s->clump_id[i] < 0 || s->clump_id[i] >= Nc) {
+                char b[160];
+                std::snprintf(b, sizeof b,
+                              "FATAL: Invalid input or nonfinite wall force/torque at sorted particle index %d. "
+                              "Simulation stopped.", i);
+                return fail(NBC_E_NONFINITE, b);
+            }
+        if (nfaces > 0) {
+            if (!w) return fail(NBC_E_BADINPUT, "nbc_upload: isWallForce is set but no wall was given");
+            bool okw = true;
+            for (int a = 0; a < 3; ++a)
+                okw = okw && std::isfinite(w->pos[a]) && std::isfinite(w->vel[a]) && std::isfinite(w->angv[a]);
+            if (wall_type == NBC_WALL_CUBE)
+                for (int a = 0; a < 3; ++a) okw = okw && std::isfinite(w->side[a]) && w->side[a] > 0.0;
+            else
+                okw = okw && std::isfinite(w->radius) && std::isfinite(w->height) && w->radius > 0.0 && w->height > 0.0;
+            if (!okw)
+                return fail(NBC_E_NONFINITE, "FATAL: Invalid input or nonfinite wall force/torque at sorted particle "
+                                             "index 0. Simulation stopped.");
+            for (int a = 0; a < 3; ++a) {
+                dp.wside[a] = w->side[a]; dp.wdir[a] = w->dir[a]; dp.wpos[a] = w->pos[a];
+                dp.wvel[a] = w->vel[a]; dp.wangv[a] = w->angv[a];
+            }
+            dp.wrad = w->radius; dp.wheight = w->height;
+        }
+        std::vector<D4> pos(N), cum(N);
+        std::vector<R4> loc(N), kin(N);
+        std::vector<int> fid(N);
+        for (int i = 0; i < N; ++i) {
+            pos[i].x = s->pos[3 * i]; pos[i].y = s->pos[3 * i + 1]; pos[i].z = s->pos[3 * i + 2]; pos[i].w = s->radius[i];
+            cum[i].x = cum[i].y = cum[i].z = cum[i].w = 0.0;
+            loc[i].x = (R)s->local[3 * i]; loc[i].y = (R)s->local[3 * i + 1]; loc[i].z = (R)s->local[3 * i + 2];
+            loc[i].w = (R)s->mass[i];
+            kin[i].x = kin[i].y = kin[i].z = R(0); kin[i].w = (R)s->mass[i];
+            fid[i] = i;
+        }
+        SphereBuf<R>& s0 = S[scur = 0];
+        HIPC(hipMemcpy(s0.pos, pos.data(), sizeof(D4) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s0.cum, cum.data(), sizeof(D4) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s0.loc, loc.data(), sizeof(R4) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s0.kin, kin.data(), sizeof(R4) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s0.cid, s->clump_id, sizeof(int) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(s0.fid, fid.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemset(s0.wflag, 0, sizeof(uint32_t) * N));
+        HIPC(hipMemset(s0.wh, 0, sizeof(R) * (size_t)std::max(nwh, 1) * N * HSTRIDE));
+        HIPC(hipMemset(S[1].wflag, 0, sizeof(uint32_t) * N));
+        HIPC(hipMemset(s0.sft, 0, sizeof(R4) * 2 * (size_t)N));
+        std::vector<D4> pq(2 * (size_t)Nc), vw(2 * (size_t)Nc), prop(Nc), ft(2 * (size_t)Nc);
+        for (int i = 0; i < Nc; ++i) {
+            pq[2 * i] = D4{c->pos[3 * i], c->pos[3 * i + 1], c->pos[3 * i + 2], 0.0};
+            pq[2 * i + 1] = D4{c->quat[4 * i], c->quat[4 * i + 1], c->quat[4 * i + 2], c->quat[4 * i + 3]};
+            vw[2 * i] = D4{c->vel[3 * i], c->vel[3 * i + 1], c->vel[3 * i + 2], 0.0};
+            vw[2 * i + 1] = D4{c->angv[3 * i], c->angv[3 * i + 1], c->angv[3 * i + 2], 0.0};
+            prop[i] = D4{c->prop[4 * i], c->prop[4 * i + 1], c->prop[4 * i + 2], c->prop[4 * i + 3]};
+            ft[2 * i] = D4{0, 0, 0, 0};
+            ft[2 * i + 1] = D4{0, 0, 0, 0};
+        }
+        HIPC(hipMemcpy(C.pq, pq.data(), sizeof(D4) * pq.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(C.vw, vw.data(), sizeof(D4) * vw.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(C.prop, prop.data(), sizeof(D4) * prop.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(C.ft, ft.data(), sizeof(D4) * ft.size(), hipMemcpyHostToDevice));
+        HIPC(hipMemset(C.snap, 0, sizeof(R4) * 2 * (size_t)Nc));
+        // empty lists (src/dem_grid.cu:228-247: Nsum = 0, Lpb_pre = -1)
+        lcur = 0;
+        HIPC(hipMemset(Lf[0].foff, 0, sizeof(int) * ((size_t)N + 1)));
+        HIPC(hipMemset(Lf[1].foff, 0, sizeof(int) * ((size_t)N + 1)));
+        HIPC(hipMemset(boff, 0, sizeof(int) * ((size_t)N + 1)));
+        nfront = nback = 0;
+        HIPC(hipMemset(perm, 0, sizeof(int) * N));
+        HIPC(hipMemset(cellkey, 0, sizeof(uint32_t) * N));
+        KC(launch_fill_i32(st, cbeg, P.CellNum, -1));
+        KC(launch_fill_i32(st, cend, P.CellNum, -1));
+        KC(launch_fill_i32(st, mbeg, Nc, -1));
+        KC(launch_fill_i32(st, mend, Nc, -1));
+        HIPC(hipMemset(dstatus, 0, 4 * sizeof(uint32_t)));
+        {
+            const int neg = 0x7fffffff;
+            HIPC(hipMemcpy(dstatus + 1, &neg, sizeof(int), hipMemcpyHostToDevice));
+        }
+        HIPC(hipMemset(counters, 0, 2 * sizeof(unsigned long long)));
+        // penetrator
+        PeneState ps;
+        std::memset(&ps, 0, sizeof(ps));
+        if (dp.pene_on) {
+            if (!pe) return fail(NBC_E_BADINPUT, "nbc_upload: isIntruder is set but no penetrator was given");
+            for (int a = 0; a < 3; ++a) {
+                ps.pos[a] = pe->pos[a]; ps.vel[a] = pe->vel[a]; ps.angv[a] = pe->angv[a]; dp.pdir[a] = pe->dir[a];
+            }
+            for (int a = 0; a < 4; ++a) ps.quat[a] = pe->quat[a];
+            ps.contact_state = pe->contact_state;
+            dp.pshape = pe->shape; dp.pmotion = pe->motion; dp.pwait = pe->wait_for_contact;
+            dp.prad = pe->radius; dp.pheight = pe->height; dp.pmass = pe->prop[3];
+            dp.ntri = 0; dp.ptri = nullptr;
+            if (pe->shape == NBC_PENE_CONE) {
+                if (!pe->tri || pe->ntri <= 0) return fail(NBC_E_BADINPUT, "nbc_upload: cone penetrator without a mesh");
+                dfree(dtri);
+                HIPC(dalloc(&dtri, 9 * (size_t)pe->ntri));
+                HIPC(hipMemcpy(dtri, pe->tri, sizeof(double) * 9 * pe->ntri, hipMemcpyHostToDevice));
+                dp.ntri = pe->ntri;
+                dp.ptri = dtri;
+            }
+        }
+        HIPC(hipMemcpy(pst, &ps, sizeof(ps), hipMemcpyHostToDevice));
+        HIPC(hipMemset(cache_f, 0, sizeof(double) * 3 * oi));
+        HIPC(hipMemset(cache_t, 0, sizeof(double) * 3 * oi));
+        init_update = P.Update != 0.0;
+        first = true;
+        nstep = 0;
+        nrebuild = 0;
+        count = 0;
+        uploaded = true;
+        HIPC(hipDeviceSynchronize());
+        return 0;
+    }
+
+    // ------------------------------------------------------------------
+    // Neighbour-list rebuild for step parity p.
+    int rebuild(int p) {
+        SphereBuf<R>& so = S[scur];
+        SphereBuf<R>& sn = S[scur ^ 1];
+        const double t0 = prof ? 1.0 : 0.0;
+        (void)t0;
+        hipEvent_t e0 = prof_begin();
+        // 1. hash + stable sort by cell (src/dem_grid.cu:155-305)
+        KC(launch_hash(st, dp, so.pos, key0, val0));
+        int second = 0;
+        KC(launch_radix_sort(st, key0, val0, key1, val1, N, bits_for(P.CellNum - 1), rbits, hist, hscan, stmp,
+                             stotal, &second));
+        const uint32_t* skey = second ? key1 : key0;
+        const int* sval = second ? val1 : val0;
+        HIPC(hipMemcpyAsync(cellkey, skey, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+        HIPC(hipMemcpyAsync(perm, sval, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
+        // 2. cell bounds (src/dem_grid.cu:105-153)
+        KC(launch_seg_bounds(st, cellkey, N, P.CellNum, cbeg, cend));
+        // 3. gather every per-sphere array (src/dem_simulation_clump.cu:126-172)
+        KC(launch_gather<R>(st, N, nwh, perm, so, sn));
+        scur ^= 1;
+        SphereBuf<R>& s = S[scur];
+        // 4. neighbour counts, overflow check (src/dem_grid.cu:466-552, 709-719)
+        FrontList& Lo = Lf[lcur];
+        FrontList& Ln = Lf[lcur ^ 1];
+        KC(launch_fill_i32(st, fcnt + N, 1, 0));
+        KC(launch_fill_i32(st, bcnt + N, 1, 0));
+        KC(launch_fill_u32(st, dstatus + 2, 1, 0u));
+        KC(launch_neighbors(st, dp, false, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, nullptr, nullptr, nullptr,
+                            nullptr, dstatus + 2, block));
+        KC(launch_exclusive_scan(st, fcnt, Ln.foff, N + 1, stmp, stotal));
+        KC(launch_exclusive_scan(st, bcnt, boff, N + 1, stmp, stotal + 1));
+        int tot[2];
+        uint32_t ovf = 0;
+        HIPC(hipMemcpyAsync(tot, stotal, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&ovf, dstatus + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        if (ovf) {
+            char b[320];
+            std::snprintf(b, sizeof b,
+                          "FATAL: Neighbor list overflow (BranchLmtLen=%d). Simulation stopped before using the "
+                          "incomplete list. Review neighbor capacity and GPU memory before rerunning.",
+                          P.BranchLmtLen);
+            return fail(NBC_E_NEIGHBOR_OVERFLOW, b);
+        }
+        if (tot[0] < 0 || tot[1] < 0) return fail(NBC_E_NEIGHBOR_OVERFLOW, "neighbour list exceeds 2^31 entries");
+        if ((long long)tot[0] > Ln.cap) {
+            const long long nc = (long long)(tot[0] * 1.25) + 1024;
+            HIPC(grow_front(lcur ^ 1, nc));
+            if (rec.pair_fcn) { dfree(rec.pair_fcn); dfree(rec.pair_cpt); rec.pair_fcn = rec.pair_cpt = nullptr; rec_cap = 0; }
+        }
+        if ((long long)tot[1] > bcap) HIPC(grow_back((long long)(tot[1] * 1.25) + 1024));
+        // 5. fill rows, resolve back slots, remap histories (src/dem_grid.cu:561-647)
+        KC(launch_neighbors(st, dp, true, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, Ln.foff, boff, Ln.fword, bent,
+                            dstatus + 2, block));
+        KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
+        KC(launch_hist_remap<R>(st, N, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword, Lo.hist[p], block));
+        lcur ^= 1;
+        nfront = tot[0];
+        nback = tot[1];
+        // 6. clump member lists in ascending sorted order (replaces the per-step
+        //    thrust::sort_by_key of src/dem_clumpsphere.cu:123-150)
+        KC(launch_keys_from_int(st, s.cid, key0, val0, N));
+        KC(launch_radix_sort(st, key0, val0, key1, val1, N, bits_for(Nc - 1), rbits, hist, hscan, stmp, stotal,
+                             &second));
+        const uint32_t* ck = second ? key1 : key0;
+        const int* cv = second ? val1 : val0;
+        HIPC(hipMemcpyAsync(mem, cv, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
+        KC(launch_seg_bounds(st, ck, N, Nc, mbeg, mend));
+        nrebuild++;
+        prof_end(3, e0);
+        return 0;
+    }
+
+    int ensure_record() {
+        if (!rec.tot) {
+            HIPC(dalloc(&rec.tot, 2 * (size_t)N));
+            HIPC(dalloc(&rec.parts, 6 * (size_t)N));
+            HIPC(dalloc(&rec.pene, 4 * (size_t)N));
+        }
+        if (!rec.pair_fcn || rec_cap < Lf[lcur].cap) {
+            if (rec.pair_fcn) { dfree(rec.pair_fcn); dfree(rec.pair_cpt); }
+            rec_cap = std::max(Lf[0].cap, Lf[1].cap);
+            HIPC(dalloc(&rec.pair_fcn, (size_t)rec_cap));
+            HIPC(dalloc(&rec.pair_cpt, (size_t)rec_cap));
+        }
+        return 0;
+    }
+
+    int check_status(bool after_step) {
+        (void)after_step;
+        const int bad = (int)hstatus[1];
+        if (bad != 0x7fffffff) {
+            char b[200];
+            std::snprintf(b, sizeof b,
+                          "FATAL: Invalid input or nonfinite wall force/torque at sorted particle index %d. "
+                          "Simulation stopped.", bad);
+            return fail(NBC_E_NONFINITE, b);
+        }
+        return 0;
+    }
+
+    int step(int nsteps) override {
+        if (!uploaded) return fail(NBC_E_BADINPUT, "nbc_step: no state uploaded");
+        HIPC(hipSetDevice(device));
+        for (int k = 0; k < nsteps; ++k) {
+            const int p = (int)(nstep & 1);
+            hipEvent_t e0 = prof_begin();
+            KC(launch_clump_before<R>(st, dp, C, pst, counters, block));
+            KC(launch_sphere_pose<R>(st, dp, S[scur], C, dstatus, block));
+            prof_end(0, e0);
+            HIPC(hipMemcpyAsync(hstatus, dstatus, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            int e = check_status(false);
+            if (e) return e;
+            const bool rb = first || init_update || hstatus[0] != 0;
+            if (rb) {
+                e = rebuild(p);
+                if (e) return e;
+                init_update = false;
+                KC(launch_fill_u32(st, dstatus, 1, 0u));
+            }
+            first = false;
+            const bool record = record_next;
+            if (record) {
+                e = ensure_record();
+                if (e) return e;
+                KC(launch_fill_f64(st, (double*)rec.pene, 16 * (long long)N, 0.0));
+                KC(launch_fill_f64(st, (double*)rec.pair_fcn, 4 * (long long)rec_cap, 0.0));
+                KC(launch_fill_f64(st, (double*)rec.pair_cpt, 4 * (long long)rec_cap, 0.0));
+            }
+            ListBuf<R> L;
+            L.foff = Lf[lcur].foff; L.fword = Lf[lcur].fword; L.boff = boff; L.bent = bent;
+            L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
+            e0 = prof_begin();
+            KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, pene_partial, counters, (int*)(dstatus + 1), rec,
+                                  record, block, 1));
+            prof_end(1, e0);
+            if (dp.pene_on) {
+                e0 = prof_begin();
+                KC(launch_pene_finalize(st, dp, pst, pene_partial, nbk, count, cache_f, cache_t));
+                prof_end(4, e0);
+            }
+            e0 = prof_begin();
+            KC(launch_clump_after<R>(st, dp, S[scur], C, mbeg, mend, mem, block));
+            prof_end(2, e0);
+            record_last = record;
+            if (record) record_next = false;
+            count++;
+            if (count == oi) count = 0;
+            nstep++;
+        }
+        HIPC(hipMemcpyAsync(hstatus, dstatus, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        prof_collect();
+        return check_status(true);
+    }
+
+    int force_rebuild() override {
+        if (!uploaded) return fail(NBC_E_BADINPUT, "nbc_rebuild: no state uploaded");
+        HIPC(hipSetDevice(device));
+        int e = rebuild((int)(nstep & 1));
+        if (e) return e;
+        first = false;
+        init_update = false;
+        KC(launch_fill_u32(st, dstatus, 1, 0u));
+        HIPC(hipStreamSynchronize(st));
+        prof_collect();
+        return 0;
+    }
+
+    // ------------------------------------------------------------------
+    // profiling with events on the library stream
+    hipEvent_t prof_begin() {
+        if (!prof) return nullptr;
+        hipEvent_t e = get_event();
+        (void)hipEventRecord(e, st);
+        return e;
+    }
+    void prof_end(int cls, hipEvent_t b) {
+        if (!prof || !b) return;
+        hipEvent_t e = get_event();
+        (void)hipEventRecord(e, st);
+        pending.push_back({cls, b, e});
+    }
+    hipEvent_t get_event() {
+        if (ev_free.empty()) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            ev_pool.push_back(e);
+            return e;
+        }
+        hipEvent_t e = ev_free.back();
+        ev_free.pop_back();
+        return e;
+    }
+    void prof_collect() {
+        for (auto& q : pending) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, q.b, q.e);
+            prof_ms[q.cls] += ms;
+            prof_n[q.cls] += 1;
+            ev_free.push_back(q.b);
+            ev_free.push_back(q.e);
+        }
+        pending.clear();
+    }
+    int profile_enable(int on) override {
+        prof = on != 0;
+        for (int k = 0; k < 8; ++k) { prof_ms[k] = 0.0; prof_n[k] = 0; }
+        return 0;
+    }
+    int profile_get(int cls, double* ms, int64_t* n) override {
+        if (cls < 0 || cls >= 8) return fail(NBC_E_BADINPUT, "nbc_profile_get: bad kernel class");
+        if (ms) *ms = prof_ms[cls];
+        if (n) *n = prof_n[cls];
+        return 0;
+    }
+
+    // ------------------------------------------------------------------
+    // getters
+    int set_record(int on) override {
+        record_next = on != 0;
+        return 0;
+    }
+    int sync() override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        return 0;
+    }
+
+    int get_spheres(int32_t* fid, int32_t* cid, double* pos, double* cum, double* local, double* mass,
+                    double* radius) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        SphereBuf<R>& s = S[scur];
+        if (fid) HIPC(hipMemcpy(fid, s.fid, sizeof(int) * N, hipMemcpyDeviceToHost));
+        if (cid) HIPC(hipMemcpy(cid, s.cid, sizeof(int) * N, hipMemcpyDeviceToHost));
+        if (pos || radius) {
+            std::vector<D4> v(N);
+            HIPC(hipMemcpy(v.data(), s.pos, sizeof(D4) * N, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; ++i) {
+                if (pos) { pos[3 * i] = v[i].x; pos[3 * i + 1] = v[i].y; pos[3 * i + 2] = v[i].z; }
+                if (radius) radius[i] = v[i].w;
+            }
+        }
+        if (cum) {
+            std::vector<D4> v(N);
+            HIPC(hipMemcpy(v.data(), s.cum, sizeof(D4) * N, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; ++i) { cum[3 * i] = v[i].x; cum[3 * i + 1] = v[i].y; cum[3 * i + 2] = v[i].z; }
+        }
+        if (local || mass) {
+            std::vector<R4> v(N);
+            HIPC(hipMemcpy(v.data(), s.loc, sizeof(R4) * N, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; ++i) {
+                if (local) { local[3 * i] = v[i].x; local[3 * i + 1] = v[i].y; local[3 * i + 2] = v[i].z; }
+                if (mass) mass[i] = v[i].w;
+            }
+        }
+        return 0;
+    }
+
+    int get_grid(int32_t* cell, int32_t* pm, int32_t* cb, int32_t* ce) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        if (cell) HIPC(hipMemcpy(cell, cellkey, sizeof(int) * N, hipMemcpyDeviceToHost));
+        if (pm) HIPC(hipMemcpy(pm, perm, sizeof(int) * N, hipMemcpyDeviceToHost));
+        if (cb) HIPC(hipMemcpy(cb, cbeg, sizeof(int) * P.CellNum, hipMemcpyDeviceToHost));
+        if (ce) HIPC(hipMemcpy(ce, cend, sizeof(int) * P.CellNum, hipMemcpyDeviceToHost));
+        return 0;
+    }
+
+    int get_pair_counts(int64_t* nf, int64_t* nb) override {
+        if (nf) *nf = nfront;
+        if (nb) *nb = nback;
+        return 0;
+    }
+
+    int get_pairs(int32_t* fo, int32_t* fj, uint8_t* ff, int32_t* bo, int32_t* bi, int32_t* bs) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        const FrontList& L = Lf[lcur];
+        if (fo) HIPC(hipMemcpy(fo, L.foff, sizeof(int) * ((size_t)N + 1), hipMemcpyDeviceToHost));
+        if (fj || ff) {
+            std::vector<uint32_t> w(std::max<long long>(nfront, 1));
+            if (nfront) HIPC(hipMemcpy(w.data(), L.fword, sizeof(uint32_t) * nfront, hipMemcpyDeviceToHost));
+            for (long long k = 0; k < nfront; ++k) {
+                if (fj) fj[k] = (int32_t)(w[k] & JMASK);
+                if (ff) ff[k] = (w[k] & INTRA) ? 1 : 0;
+            }
+        }
+        if (bo) HIPC(hipMemcpy(bo, boff, sizeof(int) * ((size_t)N + 1), hipMemcpyDeviceToHost));
+        if (bi || bs) {
+            std::vector<int2> v(std::max<long long>(nback, 1));
+            if (nback) HIPC(hipMemcpy(v.data(), bent, sizeof(int2) * nback, hipMemcpyDeviceToHost));
+            for (long long k = 0; k < nback; ++k) {
+                if (bi) bi[k] = v[k].x;
+                if (bs) bs[k] = v[k].y;
+            }
+        }
+        return 0;
+    }
+
+    int get_pair_history(double* h9) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        if (!nfront) return 0;
+        const FrontList& L = Lf[lcur];
+        const int q = (int)((nstep + 1) & 1);  // buffer written by the last step
+        std::vector<uint32_t> w(nfront);
+        std::vector<R> h((size_t)nfront * HSTRIDE);
+        HIPC(hipMemcpy(w.data(), L.fword, sizeof(uint32_t) * nfront, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(h.data(), L.hist[q], sizeof(R) * h.size(), hipMemcpyDeviceToHost));
+        for (long long k = 0; k < nfront; ++k)
+            for (int t = 0; t < HSTRIDE; ++t)
+                h9[k * HSTRIDE + t] = (w[k] & VALID(q)) ? (double)h[k * HSTRIDE + t] : 0.0;
+        return 0;
+    }
+
+    int get_sphere_forces(int kind, double* f, double* t) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        if (!record_last || !rec.tot)
+            return fail(NBC_E_BADINPUT, "nbc_get_sphere_forces: enable nbc_set_record before the step");
+        std::vector<D4> v;
+        if (kind == 0) {
+            v.resize(2 * (size_t)N);
+            HIPC(hipMemcpy(v.data(), rec.tot, sizeof(D4) * v.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; ++i) {
+                if (f) { f[3 * i] = v[2 * i].x; f[3 * i + 1] = v[2 * i].y; f[3 * i + 2] = v[2 * i].z; }
+                if (t) { t[3 * i] = v[2 * i + 1].x; t[3 * i + 1] = v[2 * i + 1].y; t[3 * i + 2] = v[2 * i + 1].z; }
+            }
+        } else if (kind >= 1 && kind <= 3) {
+            v.resize(6 * (size_t)N);
+            HIPC(hipMemcpy(v.data(), rec.parts, sizeof(D4) * v.size(), hipMemcpyDeviceToHost));
+            const int o = 2 * (kind - 1);
+            for (int i = 0; i < N; ++i) {
+                const D4 a = v[6 * i + o], b = v[6 * i + o + 1];
+                if (f) { f[3 * i] = a.x; f[3 * i + 1] = a.y; f[3 * i + 2] = a.z; }
+                if (t) { t[3 * i] = b.x; t[3 * i + 1] = b.y; t[3 * i + 2] = b.z; }
+            }
+        } else {
+            return fail(NBC_E_BADINPUT, "nbc_get_sphere_forces: bad kind");
+        }
+        return 0;
+    }
+
+    int get_clumps(double* pos, double* vel, double* quat, double* angv, double* F, double* T) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        std::vector<D4> a(2 * (size_t)Nc);
+        if (pos || quat) {
+            HIPC(hipMemcpy(a.data(), C.pq, sizeof(D4) * a.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < Nc; ++i) {
+                if (pos) { pos[3 * i] = a[2 * i].x; pos[3 * i + 1] = a[2 * i].y; pos[3 * i + 2] = a[2 * i].z; }
+                if (quat) {
+                    quat[4 * i] = a[2 * i + 1].x; quat[4 * i + 1] = a[2 * i + 1].y;
+                    quat[4 * i + 2] = a[2 * i + 1].z; quat[4 * i + 3] = a[2 * i + 1].w;
+                }
+            }
+        }
+        if (vel || angv) {
+            HIPC(hipMemcpy(a.data(), C.vw, sizeof(D4) * a.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < Nc; ++i) {
+                if (vel) { vel[3 * i] = a[2 * i].x; vel[3 * i + 1] = a[2 * i].y; vel[3 * i + 2] = a[2 * i].z; }
+                if (angv) { angv[3 * i] = a[2 * i + 1].x; angv[3 * i + 1] = a[2 * i + 1].y; angv[3 * i + 2] = a[2 * i + 1].z; }
+            }
+        }
+        if (F || T) {
+            HIPC(hipMemcpy(a.data(), C.ft, sizeof(D4) * a.size(), hipMemcpyDeviceToHost));
+            for (int i = 0; i < Nc; ++i) {
+                if (F) { F[3 * i] = a[2 * i].x; F[3 * i + 1] = a[2 * i].y; F[3 * i + 2] = a[2 * i].z; }
+                if (T) { T[3 * i] = a[2 * i + 1].x; T[3 * i + 1] = a[2 * i + 1].y; T[3 * i + 2] = a[2 * i + 1].z; }
+            }
+        }
+        return 0;
+    }
+
+    int get_penetrator(double* pos, double* vel, double* quat, double* angv, double* cf, double* ct,
+                       int32_t* state) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        PeneState ps;
+        HIPC(hipMemcpy(&ps, pst, sizeof(ps), hipMemcpyDeviceToHost));
+        if (pos) for (int a = 0; a < 3; ++a) pos[a] = ps.pos[a];
+        if (vel) for (int a = 0; a < 3; ++a) vel[a] = ps.vel[a];
+        if (quat) for (int a = 0; a < 4; ++a) quat[a] = ps.quat[a];
+        if (angv) for (int a = 0; a < 3; ++a) angv[a] = ps.angv[a];
+        if (cf) HIPC(hipMemcpy(cf, cache_f, sizeof(double) * 3 * oi, hipMemcpyDeviceToHost));
+        if (ct) HIPC(hipMemcpy(ct, cache_t, sizeof(double) * 3 * oi, hipMemcpyDeviceToHost));
+        if (state) *state = ps.contact_state;
+        return 0;
+    }
+
+    int get_pene_exports(double* fn, double* ft, double* tn, double* tt) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        if (!record_last || !rec.pene) return fail(NBC_E_BADINPUT, "nbc_get_pene_exports: no recorded step");
+        std::vector<D4> v(4 * (size_t)N);
+        HIPC(hipMemcpy(v.data(), rec.pene, sizeof(D4) * v.size(), hipMemcpyDeviceToHost));
+        double* o[4] = {fn, ft, tn, tt};
+        for (int k = 0; k < 4; ++k)
+            if (o[k])
+                for (int i = 0; i < N; ++i) {
+                    o[k][3 * i] = v[4 * i + k].x; o[k][3 * i + 1] = v[4 * i + k].y; o[k][3 * i + 2] = v[4 * i + k].z;
+                }
+        return 0;
+    }
+
+    int get_pair_exports(double* fcn, double* cpt) override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        if (!record_last || !rec.pair_fcn) return fail(NBC_E_BADINPUT, "nbc_get_pair_exports: no recorded step");
+        if (!nfront) return 0;
+        std::vector<D4> v(nfront);
+        if (fcn) {
+            HIPC(hipMemcpy(v.data(), rec.pair_fcn, sizeof(D4) * nfront, hipMemcpyDeviceToHost));
+            for (long long k = 0; k < nfront; ++k) { fcn[3 * k] = v[k].x; fcn[3 * k + 1] = v[k].y; fcn[3 * k + 2] = v[k].z; }
+        }
+        if (cpt) {
+            HIPC(hipMemcpy(v.data(), rec.pair_cpt, sizeof(D4) * nfront, hipMemcpyDeviceToHost));
+            for (long long k = 0; k < nfront; ++k) { cpt[3 * k] = v[k].x; cpt[3 * k + 1] = v[k].y; cpt[3 * k + 2] = v[k].z; }
+        }
+        return 0;
+    }
+
+    int get_stats(nbc_stats* s) override {
+        HIPC(hipSetDevice(device));
+        std::memset(s, 0, sizeof(*s));
+        KC(launch_kinetic(st, Nc, C.vw, C.prop, ke_partial));
+        std::vector<double> kp((Nc + 255) / 256);
+        unsigned long long ctr[2];
+        PeneState ps;
+        HIPC(hipMemcpyAsync(kp.data(), ke_partial, sizeof(double) * kp.size(), hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(ctr, counters, sizeof(ctr), hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(&ps, pst, sizeof(ps), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        double ke = 0.0;
+        for (double v : kp) ke += v;
+        s->step = nstep;
+        s->rebuilds = nrebuild;
+        s->front_pairs = nfront;
+        s->back_entries = nback;
+        s->touching_ss = (int64_t)ctr[0];
+        s->touching_wall = (int64_t)ctr[1];
+        s->kinetic_energy = ke;
+        for (int a = 0; a < 3; ++a) { s->pene_force[a] = -ps.totF[a]; s->pene_torque[a] = ps.totT[a]; }
+        return 0;
+    }
+
+  private:
+    struct FrontList {
+        int* foff = nullptr;
+        uint32_t* fword = nullptr;
+        R* hist[2] = {nullptr, nullptr};
+        long long cap = 0;
+    };
+    struct Pending {
+        int cls;
+        hipEvent_t b, e;
+    };
+    nbc_params P;
+    nbc_options O;
+    DevParams dp;
+    hipStream_t st = nullptr;
+    int N = 0, Nc = 0, block = 256, rbits = 8, wall_type = 0, nfaces = 0, nwh = 0, oi = 1, nbk = 1;
+    int pair_cap_per = 16;
+    SphereBuf<R> S[2] = {};
+    int scur = 0;
+    ClumpBuf<R> C = {};
+    FrontList Lf[2];
+    int lcur = 0;
+    int* boff = nullptr;
+    int2* bent = nullptr;
+    long long bcap = 0, nfront = 0, nback = 0, rec_cap = 0;
+    uint32_t *key0 = nullptr, *key1 = nullptr, *cellkey = nullptr;
+    int *val0 = nullptr, *val1 = nullptr, *hist = nullptr, *hscan = nullptr, *stmp = nullptr, *stotal = nullptr;
+    int *perm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
+    int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;
+    uint32_t* dstatus = nullptr;  // [0] rebuild flag, [1] smallest bad sphere, [2] overflow
+    uint32_t* hstatus = nullptr;
+    unsigned long long* counters = nullptr;
+    PeneState* pst = nullptr;
+    double *pene_partial = nullptr, *cache_f = nullptr, *cache_t = nullptr, *ke_partial = nullptr, *dtri = nullptr;
+    RecordBuf rec;
+    bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;
+    long long nstep = 0, nrebuild = 0;
+    int count = 0;
+    double prof_ms[8] = {0};
+    int64_t prof_n[8] = {0};
+    std::vector<hipEvent_t> ev_pool, ev_free;
+    std::vector<Pending> pending;
+};
+
+EngineBase* make_engine(int precision) {
+    if (precision == NBC_MIXED) return new Engine<float>();
+    return new Engine<double>();
+}
+
+}  // namespace nbc

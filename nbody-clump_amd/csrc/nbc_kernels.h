@@ -1,0 +1,95 @@
+// Device buffer views and launcher declarations shared by the kernels and the
+// engine. All launchers return 0 or a hipError_t value.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nbc.h"
+#include "nbc_device.h"
+
+namespace nbc {
+
+// Per-sphere arrays, sorted order. pos.w holds the FP64 radius so that one
+// 32-byte load gives a neighbour's centre and size; loc/kin .w hold the mass.
+template <class R>
+struct SphereBuf {
+    D4* pos;                          // x y z radius (FP64)
+    D4* cum;                          // displacement since the last rebuild (FP64)
+    typename Vec4T<R>::type* loc;     // local offset (body frame), mass
+    typename Vec4T<R>::type* kin;     // R(q) * local (world frame), mass
+    int* cid;                         // clump index
+    int* fid;                         // file index (part_fix)
+    uint32_t* wflag;                  // bit f: wall/penetrator history f valid
+    R* wh;                            // [MAX_FACES][N][HSTRIDE] wall/penetrator histories
+    typename Vec4T<R>::type* sft;     // [2N] force, torque about the clump centre
+};
+
+// Per-clump arrays (FP64 state; snap is the contact kernel's copy in R).
+template <class R>
+struct ClumpBuf {
+    D4* pq;                           // [2Nc] position, quaternion (x y z w)
+    D4* vw;                           // [2Nc] velocity, angular velocity
+    D4* prop;                         // Ix Iy Iz mass
+    D4* ft;                           // [2Nc] force, torque (FP64)
+    typename Vec4T<R>::type* snap;    // [2Nc] velocity (w = state finite), angular velocity
+};
+
+// Verlet list: CSR front rows (owner i, partner j > i) with ping-pong history,
+// CSR back rows (i, owner j < i, owner slot).
+template <class R>
+struct ListBuf {
+    int* foff;
+    uint32_t* fword;
+    int* boff;
+    int2* bent;
+    R* hist[2];
+};
+
+struct RecordBuf {
+    D4* tot;       // [2N] total force, total torque (before the clump term)
+    D4* parts;     // [6N] contact F,T; wall F,T; penetrator F,T
+    D4* pene;      // [4N] penetrator Fn, Ft, Tn, Tt
+    D4* pair_fcn;  // [front slots] -Fn of the pair (CacheFcn)
+    D4* pair_cpt;  // [front slots] contact point (CacheCtp)
+};
+
+int launch_fill_i32(hipStream_t st, int* a, long long n, int v);
+int launch_fill_u32(hipStream_t st, uint32_t* a, long long n, uint32_t v);
+int launch_fill_f64(hipStream_t st, double* a, long long n, double v);
+int scan_tmp_ints(int n);
+int launch_exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* tmp, int* total);
+int radix_tmp_ints(int n, int rbits);
+int launch_radix_sort(hipStream_t st, uint32_t* k0, int* v0, uint32_t* k1, int* v1, int n, int bits, int rbits,
+                      int* hist, int* hscan, int* stmp, int* stotal, int* in_second);
+int launch_keys_from_int(hipStream_t st, const int* a, uint32_t* k, int* v, int n);
+int launch_hash(hipStream_t st, const DevParams& P, const D4* pos, uint32_t* key, int* val);
+int launch_seg_bounds(hipStream_t st, const uint32_t* key, int n, int nkeys, int* beg, int* end);
+int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* pos, const uint32_t* cell,
+                     const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, const int* foff,
+                     const int* boff, uint32_t* fword, int2* bent, unsigned* overflow, int block);
+int launch_back_resolve(hipStream_t st, int N, const int* foff, const uint32_t* fword, const int* boff, int2* bent,
+                        int block);
+int launch_kinetic(hipStream_t st, int Nc, const D4* vw, const D4* prop, double* partial);
+int launch_pene_finalize(hipStream_t st, const DevParams& P, PeneState* pst, const double* partial, int nb, int count,
+                         double* cf, double* ct);
+int contacts_blocks(int N, int block);
+
+template <class R>
+int launch_clump_before(hipStream_t st, const DevParams& P, ClumpBuf<R> C, PeneState* pst, unsigned long long* ctr,
+                        int block);
+template <class R>
+int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, unsigned* flag, int block);
+template <class R>
+int launch_gather(hipStream_t st, int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst);
+template <class R>
+int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
+                      const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
+template <class R>
+int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                    PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
+                    int block, int remap);
+template <class R>
+int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
+                       const int* mend, const int* mem, int block);
+
+}  // namespace nbc

@@ -1,0 +1,1355 @@
+// HIP kernels of the clumped-sphere DEM timestep for gfx950 (MI355X).
+//
+// One steady-state step is four launches on the library stream:
+//   k_clump_before   per clump   FP64 half-kick, drift, quaternion update
+//                                (src/dem_integrate_clumpsphere.cu:3-90)
+//   k_sphere_pose    per sphere  FP64 sphere pose, displacement, rebuild flag
+//                                (src/dem_clump.cu:113-138)
+//   k_contacts       per sphere  sphere-sphere, wall and penetrator contacts,
+//                                per-sphere force and clump torque
+//                                (src/dem_force_clump.cu:3-191,
+//                                src/dem_force_cube_wall_clump.cu:16-298,
+//                                src/dem_force_cylind_wall.cu:152-237,
+//                                src/dem_contact_penetrator.cu:8-84,
+//                                src/dem_force_penetrator_clump.cu:4-182,
+//                                src/dem_clump.cu:89-101,
+//                                src/dem_clumpsphere.cu:86-107)
+//   k_clump_after    per clump   deterministic member sum + second half-kick
+//                                (src/dem_clumpsphere.cu:110-151,
+//                                src/dem_integrate_clumpsphere.cu:110-141)
+// plus k_pene_finalize (one block) when a penetrator is present.
+//
+// A rebuild (hash, stable LSD radix sort, cell bounds, gather, two-pass
+// neighbour search into CSR lists, back-slot resolution, history remap,
+// clump member lists) replaces src/dem_grid.cu:105-750.
+//
+// Every sphere-sphere pair is stored once (owner = lower sorted index) and
+// evaluated by both spheres' threads from the same inputs, so each thread
+// only writes its own sphere: no float atomics, results independent of
+// scheduling. The owner writes the new history into the other buffer of a
+// ping-pong pair; bits 30/31 of the pair word say whether each buffer holds a
+// valid history, which replaces the reference's per-step memsets of every slot.
+#include <hip/hip_runtime.h>
+
+#include "nbc_device.h"
+#include "nbc_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace nbc {
+
+// ---------------------------------------------------------------------------
+// small helpers
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// Logical block index that gives each XCD a contiguous run of blocks
+// (workgroups are dispatched round-robin over the 8 XCDs, each with its own L2).
+__device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
+    if (!remap) return b;
+    const int nx = 8, q = nb / nx, r = nb % nx, x = b % nx, k = b / nx;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+template <class R> struct Ld4;
+template <> struct Ld4<double> {
+    static __device__ __forceinline__ V3<double> xyz(const D4& v) { return mk<double>(v.x, v.y, v.z); }
+};
+template <> struct Ld4<float> {
+    static __device__ __forceinline__ V3<float> xyz(const F4& v) { return mk<float>(v.x, v.y, v.z); }
+};
+template <class R> __device__ __forceinline__ typename Vec4T<R>::type mk4(R x, R y, R z, R w) {
+    typename Vec4T<R>::type v;
+    v.x = x; v.y = y; v.z = z; v.w = w;
+    return v;
+}
+__device__ __forceinline__ V3<double> xyz(const D4& v) { return mk<double>(v.x, v.y, v.z); }
+__device__ __forceinline__ bool finite3(const V3<double>& v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+// ---------------------------------------------------------------------------
+// K1: clump half-kick, drift and attitude (FP64). Thread 0 of block 0 also
+// advances the penetrator (src/dem_integrate_penetrator.cu:3-41) and clears
+// the step counters.
+template <class R>
+__global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsigned long long* counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counters[0] = 0ull;
+        counters[1] = 0ull;
+        if (P.pene_on) {
+            const double Dt = P.dt;
+            if (P.pmotion == NBC_PENE_GRAVITY && (!P.pwait || pst->contact_state != 0)) {
+                V3<double> F = mk<double>(pst->totF[0], pst->totF[1], pst->totF[2]);
+                V3<double> pf = (-1.0 / P.pmass) * F;
+                pf.z = pf.z + P.Gravity;
+                V3<double> pos = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
+                V3<double> vel = mk<double>(pst->vel[0], pst->vel[1], pst->vel[2]);
+                pos = pos + Dt * vel + 0.5 * Dt * Dt * pf;
+                vel = vel + 0.5 * Dt * pf;
+                pst->pos[0] = pos.x; pst->pos[1] = pos.y; pst->pos[2] = pos.z;
+                pst->vel[0] = vel.x; pst->vel[1] = vel.y; pst->vel[2] = vel.z;
+            } else {
+                pst->pos[0] = pst->pos[0] + Dt * pst->vel[0];
+                pst->pos[1] = pst->pos[1] + Dt * pst->vel[1];
+                pst->pos[2] = pst->pos[2] + Dt * pst->vel[2];
+            }
+        }
+    }
+    if (i >= P.Nc) return;
+    const double Dt = P.dt;
+    const D4 pr = C.prop[i];
+    D4 Fr = C.ft[2 * i], Tr = C.ft[2 * i + 1];
+    D4 pc = C.pq[2 * i], qc = C.pq[2 * i + 1];
+    D4 vr = C.vw[2 * i], wr = C.vw[2 * i + 1];
+    V3<double> pos = xyz(pc), vel = xyz(vr), w = xyz(wr);
+    V3<double> F = xyz(Fr), T = xyz(Tr);
+    F = (1.0 / pr.w) * F;
+    F.z = F.z + P.Gravity;
+    T.x = (1.0 / pr.x) * T.x;
+    T.y = (1.0 / pr.y) * T.y;
+    T.z = (1.0 / pr.z) * T.z;
+    V3<double> dist = Dt * vel + 0.5 * Dt * Dt * F;
+    pos = pos + dist;
+    vel = vel + 0.5 * Dt * F;
+    w = w + 0.5 * Dt * T;
+    // attitude, src/dem_integrate_clumpsphere.cu:45-90 (T is the same scaled torque)
+    double M[3][3];
+    QuatDCM<double>(qc.x, qc.y, qc.z, qc.w, M);
+    V3<double> Wb = Map<double>(M, w), Wd = Map<double>(M, T);
+    const double qx = qc.x, qy = qc.y, qz = qc.z, qw = qc.w;
+    double dx = 0.50 * (Wb.x * qw - Wb.y * qz + Wb.z * qy);
+    double dy = 0.50 * (Wb.x * qz + Wb.y * qw - Wb.z * qx);
+    double dz = 0.50 * (-Wb.x * qy + Wb.y * qx + Wb.z * qw);
+    double dw = 0.50 * (-Wb.x * qx - Wb.y * qy - Wb.z * qz);
+    double ddx = (Wd.x * qw - Wd.y * qz + Wd.z * qy) * 0.50 + (Wb.x * dw - Wb.y * dz + Wb.z * dy) * 0.50;
+    double ddy = (Wd.x * qz + Wd.y * qw - Wd.z * qx) * 0.50 + (Wb.x * dz + Wb.y * dw - Wb.z * dx) * 0.50;
+    double ddz = (-Wd.x * qy + Wd.y * qx + Wd.z * qw) * 0.50 + (-Wb.x * dy + Wb.y * dx + Wb.z * dw) * 0.50;
+    double ddw = (-Wd.x * qx - Wd.y * qy - Wd.z * qz) * 0.50 + (-Wb.x * dx - Wb.y * dy - Wb.z * dz) * 0.50;
+    const double h2 = 0.5 * Dt * Dt;
+    double nx = qx + Dt * dx + h2 * ddx;
+    double ny = qy + Dt * dy + h2 * ddy;
+    double nz = qz + Dt * dz + h2 * ddz;
+    double nw = qw + Dt * dw + h2 * ddw;
+    double norm = sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
+    const double s = 1.0 / norm;
+    nx = s * nx; ny = s * ny; nz = s * nz; nw = s * nw;
+    D4 po; po.x = pos.x; po.y = pos.y; po.z = pos.z; po.w = 0.0;
+    D4 qo; qo.x = nx; qo.y = ny; qo.z = nz; qo.w = nw;
+    D4 vo; vo.x = vel.x; vo.y = vel.y; vo.z = vel.z; vo.w = 0.0;
+    D4 wo; wo.x = w.x; wo.y = w.y; wo.z = w.z; wo.w = 0.0;
+    C.pq[2 * i] = po;
+    C.pq[2 * i + 1] = qo;
+    C.vw[2 * i] = vo;
+    C.vw[2 * i + 1] = wo;
+    // reduced-precision kinematics snapshot for the contact kernel; .w of the
+    // first record flags a finite clump state (checked as in
+    // src/dem_force_cube_wall_clump.cu:196-197)
+    const bool ok = finite3(vel) && finite3(w) && isfinite(nx) && isfinite(ny) && isfinite(nz) && isfinite(nw);
+    C.snap[2 * i] = mk4<R>((R)vel.x, (R)vel.y, (R)vel.z, ok ? R(1) : R(0));
+    C.snap[2 * i + 1] = mk4<R>((R)w.x, (R)w.y, (R)w.z, R(0));
+}
+
+// ---------------------------------------------------------------------------
+// K2: sphere pose from the clump pose, cumulative displacement, rebuild flag.
+template <class R>
+__global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, unsigned* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool over = false;
+    if (i < P.N) {
+        const int c = S.cid[i];
+        const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
+        const typename Vec4T<R>::type lo = S.loc[i];
+        double M[3][3];
+        QuatIvDCM<double>(q.x, q.y, q.z, q.w, M);
+        V3<double> arm = Map<double>(M, mk<double>((double)lo.x, (double)lo.y, (double)lo.z));
+        V3<double> up = xyz(pc) + arm;
+        const D4 po = S.pos[i];
+        const D4 cu = S.cum[i];
+        V3<double> cum = xyz(cu) + up - xyz(po);
+        over = Length<double>(cum) > P.Thresh;
+        D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
+        D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = 0.0;
+        S.pos[i] = pn;
+        S.cum[i] = cn;
+        S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
+    }
+    const uint64_t b = __ballot(over);
+    if (b != 0ull && lane_id() == (int)__ffsll((unsigned long long)b) - 1) atomicOr(flag, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Grid hash, src/dem_grid.cu:155-202. The clamp of the reference runs in
+// unsigned arithmetic (GridDim is unsigned), so a negative index maps to GridDim-1.
+__device__ __forceinline__ int clamp_u(uint32_t G, int v) {
+    const uint32_t u = (uint32_t)v, hi = G - 1u;
+    return (int)(u < hi ? u : hi);
+}
+
+__global__ void k_hash(DevParams P, const D4* pos, uint32_t* key, int* val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const D4 p = pos[i];
+    val[i] = i;
+    int idx;
+    if (p.z >= P.boxup[2] || p.y >= P.boxup[1] || p.x >= P.boxup[0] || p.z < P.boxlo[2] || p.y < P.boxlo[1] ||
+        p.x < P.boxlo[0]) {
+        idx = P.CellNum - 1;
+    } else {
+        int cx = (int)floor((p.x - P.origin[0]) / P.cell[0]);
+        int cy = (int)floor((p.y - P.origin[1]) / P.cell[1]);
+        int cz = (int)floor((p.z - P.origin[2]) / P.cell[2]);
+        cx = clamp_u(P.G[0], cx);
+        cy = clamp_u(P.G[1], cy);
+        cz = clamp_u(P.G[2], cz);
+        idx = (int)((uint32_t)cz * P.G[1] * P.G[0] + (uint32_t)cy * P.G[0] + (uint32_t)cx);
+        if (idx < 0) idx = 0;
+        if (idx >= P.CellNum) idx = P.CellNum - 1;
+    }
+    key[i] = (uint32_t)idx;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (uint32 key, int value), 256-thread tiles, one key
+// per thread. Within a wave the lanes holding the same digit are found with
+// one ballot per digit bit; ranks combine wave counts in thread order, so
+// equal digits keep their input order (stable, like cub::DeviceRadixSort).
+__device__ __forceinline__ uint64_t digit_match(int dig, bool valid, int rbits) {
+    uint64_t m = __ballot(valid);
+    for (int b = 0; b < rbits; ++b) {
+        const bool bit = (dig >> b) & 1;
+        const uint64_t bb = __ballot(valid && bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t* keys, int n, int shift, int rbits, int* hist, int nb) {
+    __shared__ int cnt[4][256];
+    const int t = threadIdx.x, wv = t >> 6;
+    for (int k = t; k < 1024; k += 256) (&cnt[0][0])[k] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * 256 + t;
+    const bool valid = i < n;
+    const int mask = (1 << rbits) - 1;
+    const int dig = valid ? (int)((keys[i] >> shift) & (uint32_t)mask) : 0;
+    const uint64_t m = digit_match(dig, valid, rbits);
+    if (valid && (m & lanemask_lt()) == 0ull) cnt[wv][dig] = __popcll(m);
+    __syncthreads();
+    for (int d = t; d <= mask; d += 256) hist[d * nb + blockIdx.x] = cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t* kin, const int* vin, uint32_t* kout, int* vout,
+                                                    int n, int shift, int rbits, const int* hscan, int nb) {
+    __shared__ int cnt[4][256];
+    const int t = threadIdx.x, wv = t >> 6;
+    for (int k = t; k < 1024; k += 256) (&cnt[0][0])[k] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * 256 + t;
+    const bool valid = i < n;
+    const int mask = (1 << rbits) - 1;
+    uint32_t key = 0;
+    int val = 0;
+    if (valid) { key = kin[i]; val = vin[i]; }
+    const int dig = (int)((key >> shift) & (uint32_t)mask);
+    const uint64_t m = digit_match(dig, valid, rbits);
+    if (valid && (m & lanemask_lt()) == 0ull) cnt[wv][dig] = __popcll(m);
+    __syncthreads();
+    for (int d = t; d <= mask; d += 256) {
+        const int a = cnt[0][d], b = cnt[1][d], c = cnt[2][d];
+        cnt[0][d] = 0; cnt[1][d] = a; cnt[2][d] = a + b; cnt[3][d] = a + b + c;
+    }
+    __syncthreads();
+    if (valid) {
+        const int pos = hscan[dig * nb + blockIdx.x] + cnt[wv][dig] + __popcll(m & lanemask_lt());
+        kout[pos] = key;
+        vout[pos] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of int32 (tiles of 2048 = 256 threads x 8).
+__device__ __forceinline__ int block_excl_scan256(int v, int* lds /*>= 4*/, int* total) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wv] = x;
+    __syncthreads();
+    int add = 0, tot = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int s = lds[k];
+        if (k < wv) add += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return x - v + add;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* in, int n, int* bsum) {
+    __shared__ int lds[4];
+    const int base = blockIdx.x * 2048 + threadIdx.x * 8;
+    int s = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int i = base + k;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    block_excl_scan256(s, lds, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(int* bsum, int nb, int* total_out) {
+    __shared__ int lds[4];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 256) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        int tot;
+        const int ex = block_excl_scan256(v, lds, &tot);
+        if (i < nb) bsum[i] = ex + carry;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const int* in, int* out, int n, const int* bsum) {
+    __shared__ int lds[4];
+    const int base = blockIdx.x * 2048 + threadIdx.x * 8;
+    int v[8];
+    int s = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_excl_scan256(s, lds, &tot) + bsum[blockIdx.x];
+    for (int k = 0; k < 8; ++k) {
+        const int i = base + k;
+        if (i < n) out[i] = ex;
+        ex += v[k];
+    }
+}
+
+__global__ void k_keys_from_int(const int* a, uint32_t* k, int* v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    k[i] = (uint32_t)a[i];
+    v[i] = i;
+}
+
+__global__ void k_fill_i32(int* a, long long n, int v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* a, long long n, uint32_t v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_f64(double* a, long long n, double v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Segment bounds of a sorted key array (src/dem_grid.cu:105-153): beg/end of
+// each key value, -1 where absent (caller fills -1 first).
+__global__ void k_seg_bounds(const uint32_t* key, int n, int nkeys, int* beg, int* end) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = (int)key[i];
+    if (c < 0 || c >= nkeys) return;
+    if (i == 0 || (int)key[i - 1] != c) beg[c] = i;
+    if (i == n - 1 || (int)key[i + 1] != c) end[c] = i + 1;
+}
+
+// Gather of every per-sphere array into the new sorted order
+// (src/dem_simulation_clump.cu:126-164); the displacement restarts at zero
+// (src/dem_simulation_clump.cu:172).
+template <class R>
+__global__ void k_gather(int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int o = perm[i];
+    dst.pos[i] = src.pos[o];
+    D4 z; z.x = 0.0; z.y = 0.0; z.z = 0.0; z.w = 0.0;
+    dst.cum[i] = z;
+    dst.loc[i] = src.loc[o];
+    dst.kin[i] = src.kin[o];
+    dst.cid[i] = src.cid[o];
+    dst.fid[i] = src.fid[o];
+    const uint32_t wf = src.wflag[o];
+    dst.wflag[i] = wf;
+    for (int f = 0; f < nface; ++f) {
+        if (!((wf >> f) & 1u)) continue;
+        const R* a = src.wh + ((size_t)f * N + o) * HSTRIDE;
+        R* b = dst.wh + ((size_t)f * N + i) * HSTRIDE;
+        for (int k = 0; k < HSTRIDE; ++k) b[k] = a[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour search, src/dem_grid.cu:466-552: 27 cells (z outer, x inner),
+// ascending j inside a cell, cut-off 1.5 (ri + rj). The search cell uses the
+// reciprocal cell size (not the hash's division), unclamped. FILL = false
+// counts, FILL = true writes the CSR rows: front j > i in discovery order,
+// back j < i in reverse discovery order (the gather order of
+// src/dem_force_clump.cu:183-188).
+template <bool FILL>
+__global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, const int* cbeg, const int* cend,
+                            const int* cid, int* fcnt, int* bcnt, const int* foff, const int* boff, uint32_t* fword,
+                            int2* bent, unsigned* overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const int half = P.B / 2;
+    if ((int)cell[i] == P.CellNum - 1) {
+        if (!FILL) { fcnt[i] = 0; bcnt[i] = 0; }
+        return;
+    }
+    const D4 pi = pos[i];
+    const double ri = pi.w;
+    const int cix = (int)floor((pi.x - P.origin[0]) * P.icell[0]);
+    const int ciy = (int)floor((pi.y - P.origin[1]) * P.icell[1]);
+    const int ciz = (int)floor((pi.z - P.origin[2]) * P.icell[2]);
+    int ng = 0, ni = 0;
+    int fb = 0, bb = 0, nbi = 0, ci = 0;
+    if (FILL) { fb = foff[i]; bb = boff[i]; nbi = bcnt[i]; ci = cid[i]; }
+    for (int cz = -1; cz <= 1; cz++)
+        for (int cy = -1; cy <= 1; cy++)
+            for (int cx = -1; cx <= 1; cx++) {
+                const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
+                if (jx < 0 || (uint32_t)jx >= P.G[0] || jy < 0 || (uint32_t)jy >= P.G[1] || jz < 0 ||
+                    (uint32_t)jz >= P.G[2])
+                    continue;
+                const int cidx = (int)(((uint32_t)jz * P.G[1] + (uint32_t)jy) * P.G[0] + (uint32_t)jx);
+                const int start = cbeg[cidx];
+                if (start == -1) continue;
+                const int end = cend[cidx];
+                for (int j = start; j < end; j++) {
+                    if (j == i) continue;
+                    const D4 pj = pos[j];
+                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const double ds = dx * dx + dy * dy + dz * dz;
+                    const double lim = 1.50 * (ri + pj.w);
+                    if (ds < lim * lim) {
+                        if (i < j) {
+                            if (FILL && ng < half) fword[fb + ng] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+                            ng++;
+                        } else {
+                            if (FILL && ni < half) bent[bb + (nbi - 1 - ni)] = make_int2(j, -1);
+                            ni++;
+                        }
+                    }
+                }
+            }
+    if (!FILL) {
+        fcnt[i] = ng;
+        bcnt[i] = ni;
+        if (ng > half || ni > half) atomicOr(overflow, 1u);
+    }
+}
+
+// Owner slot of every back entry: src/dem_grid.cu:591-599 (Aref of the back
+// rows), -1 when the owner's own search did not list the pair.
+__global__ void k_back_resolve(int N, const int* foff, const uint32_t* fword, const int* boff, int2* bent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int e = boff[i]; e < boff[i + 1]; ++e) {
+        const int o = bent[e].x;
+        int slot = -1;
+        for (int s = foff[o]; s < foff[o + 1]; ++s)
+            if ((int)(fword[s] & JMASK) == i) slot = s;
+        bent[e].y = slot;
+    }
+}
+
+// History inheritance, src/dem_grid.cu:603-647: each new front pair looks up
+// its pair in the old list of the lower old index and copies the history of
+// buffer p, negated when the orientation flipped. Unmatched pairs start from
+// zero.
+template <class R>
+__global__ void k_hist_remap(int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
+                             const int* ofoff, const uint32_t* ofword, const R* ohist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int io = perm[i];
+    for (int s = nfoff[i]; s < nfoff[i + 1]; ++s) {
+        uint32_t w = nfword[s];
+        const int j = (int)(w & JMASK);
+        const int jo = perm[j];
+        const int a = io < jo ? io : jo, b = io < jo ? jo : io;
+        const bool neg = jo < io;
+        int q = -1;
+        for (int t = ofoff[a]; t < ofoff[a + 1]; ++t)
+            if ((int)(ofword[t] & JMASK) == b) q = t;
+        w &= ~(VALID(0) | VALID(1));
+        if (q >= 0 && (ofword[q] & VALID(p)) && !(w & INTRA)) {
+            const R* h = ohist + (size_t)q * HSTRIDE;
+            R* d = nhist + (size_t)s * HSTRIDE;
+            for (int k = 0; k < HSTRIDE; ++k) d[k] = neg ? R(-1.0) * h[k] : h[k];
+            w |= VALID(p);
+        }
+        nfword[s] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hertz-Mindlin sphere-sphere contact of the pair (I, J), I < J, as in
+// src/dem_force_clump.cu:54-161. Geometry (Dist = x_J - x_I, DistS, DistL,
+// Dn > 0) comes in FP64; R is the arithmetic type of the force law.
+template <class R>
+struct PairIn {
+    V3<R> Dist;
+    R DistS, DistL, Dn, rI, rJ, mI, mJ;
+    V3<R> armI, armJ;  // R(q) * local of each sphere
+    V3<R> vI, wI, vJ, wJ;
+    V3<R> H, Mt, Mr;   // tangential, twisting, rolling history
+};
+template <class R>
+struct PairOut {
+    V3<R> cfc, ctc, ctc2, ht, htw, hr, fcn;
+    R Ap;
+    V3<R> DistU;
+};
+
+template <class R>
+__device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& in, PairOut<R>& o) {
+    const R Ymod = (R)P.Ymod, Prat = (R)P.Prat, Damping = (R)P.Damping, Beta = (R)P.Beta, Frcoe = (R)P.Frcoe;
+    const R dt = (R)P.dt, Coh = (R)P.Cohesion;
+    const R DistR = R(1.0) / in.DistL;
+    const V3<R> DistU = DistR * in.Dist;
+    const R Ap = R(0.50) * (in.rI * in.rI - in.rJ * in.rJ + in.DistS) * DistR;
+    const R An = in.DistL - Ap;
+    const R Rij = in.rI * in.rJ / (in.rI + in.rJ);
+    const R Mij = in.mI * in.mJ / (in.mI + in.mJ);
+    const R Kn = R(2.0) * Ymod * sqrt(Rij * in.Dn) / (R(3.0) * (R(1.0) - Prat * Prat));
+    const R Cn = Damping * sqrt(Kn * Mij);
+    const R Ks = R(2.0) * Ymod / (R(1.0) + Prat) / (R(2.0) - Prat) * sqrt(Rij * in.Dn);
+    const R Cs = Damping * sqrt(Ks * Mij);
+    const R Kr = R(0.25) * Kn * (Beta * Rij) * (Beta * Rij);
+    const R Cr = R(0.25) * Cn * (Beta * Rij) * (Beta * Rij);
+    const R Kt = R(0.50) * Ks * (Beta * Rij) * (Beta * Rij);
+    const R Ct = R(0.50) * Cs * (Beta * Rij) * (Beta * Rij);
+    const V3<R> ARMi = in.armI + Ap * DistU;
+    const V3<R> ARMj = in.armJ - An * DistU;
+    const V3<R> Vrel = in.vJ - in.vI;
+    const V3<R> Vrot = Cross<R>(in.wJ, ARMj) - Cross<R>(in.wI, ARMi);
+    const V3<R> Vtot = Vrel + Vrot;
+    const R ERR = Dot<R>(DistU, Vtot);
+    const V3<R> Vnor = ERR * DistU;
+    const V3<R> Vtan = Vtot - Vnor;
+    const V3<R> nf = Kn * in.Dn * DistU + Cn * Vnor;
+    const R nfL = Length<R>(nf);
+    const V3<R> Ds = dt * Vtan;
+    V3<R> tf = -Ks * Ds + Cs * Vtan + in.H;
+    const R tfL = Length<R>(tf);
+    V3<R> th;
+    if (tfL > nfL * Frcoe) {
+        tf = Frcoe * nfL * (R(1.0) / tfL) * tf;
+        th = tf;
+    } else {
+        th = tf - Cs * Vtan;
+    }
+    const V3<R> cf = Coh * (Beta * Rij) * (Beta * Rij) * DistU;
+    V3<R> cfc = R(-1.0) * nf - tf + cf;
+    V3<R> ctc = -Ap * Cross<R>(DistU, tf);
+    V3<R> ctc2 = -An * Cross<R>(DistU, tf);
+    const V3<R> Dth = dt * (in.wI - in.wJ);
+    const R DthL = Dot<R>(Dth, DistU);
+    const V3<R> DthT = DthL * DistU;
+    const V3<R> DthR = Dth - DthT;
+    V3<R> rm = Kr * DthR + in.Mr;
+    const R rmL = Length<R>(rm);
+    V3<R> rh;
+    if (rmL > R(0.525) * Beta * Rij * nfL) {
+        rm = R(0.525) * Beta * Rij * nfL * (R(1.0) / rmL) * rm;
+        rh = rm;
+    } else {
+        rh = rm;
+        rm = rm - Cr * (R(1.0) / dt) * DthR;
+    }
+    V3<R> tm = Kt * DthT + in.Mt;
+    const R tmL = Length<R>(tm);
+    V3<R> twh;
+    if (tmL > R(0.65) * Frcoe * Beta * Rij * nfL) {
+        tm = R(0.65) * Frcoe * Beta * Rij * nfL * (R(1.0) / tmL) * tm;
+        twh = tm;
+    } else {
+        twh = tm;
+        tm = tm - Ct * (R(1.0) / dt) * DthT;
+    }
+    ctc = ctc - rm - tm;
+    ctc2 = ctc2 + rm + tm;
+    o.cfc = cfc;
+    o.ctc = ctc;
+    o.ctc2 = ctc2;
+    o.ht = th;
+    o.htw = twh;
+    o.hr = rh;
+    o.fcn = R(-1.0) * nf;
+    o.Ap = Ap;
+    o.DistU = DistU;
+}
+
+// Wall / penetrator contact of one sphere (src/dem_force_cube_wall_clump.cu:49-165
+// for walls with CsFactor = 2, src/dem_force_penetrator_clump.cu:35-177 for the
+// penetrator with CsFactor = 1). Writes the force/torque terms and the new history.
+template <class R>
+struct WallOut {
+    V3<R> nf, tf, rm, tm, ht, htw, hr, arm;
+};
+template <class R>
+__device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFactor, const V3<R>& DistU, R Ap, R Dn,
+                                           R Rij, R Mij, const V3<R>& arm0, const V3<R>& RVo, const V3<R>& cv,
+                                           const V3<R>& cw, const V3<R>& wv, const V3<R>& ww, const V3<R>& H,
+                                           const V3<R>& Mt, const V3<R>& Mr, WallOut<R>& o) {
+    const R Ymod = (R)P.Ymod, Prat = (R)P.Prat, Damping = (R)P.Damping, Beta = (R)P.Beta, dt = (R)P.dt;
+    const R Kn = R(2.0) * Ymod * sqrt(Rij * Dn) / (R(3.0) * (R(1.0) - Prat * Prat));
+    const R Cn = Damping * sqrt(Kn * Mij);
+    const R Ks = R(2.0) * Ymod / (R(1.0) + Prat) / (R(2.0) - Prat) * sqrt(Rij * Dn);
+    const R Cs = CsFactor == R(1.0) ? Damping * sqrt(Ks * Mij) : CsFactor * Damping * sqrt(Ks * Mij);
+    const R Kr = R(0.25) * Kn * (Beta * Rij) * (Beta * Rij);
+    const R Cr = R(0.25) * Cn * (Beta * Rij) * (Beta * Rij);
+    const R Kt = R(0.50) * Ks * (Beta * Rij) * (Beta * Rij);
+    const R Ct = R(0.50) * Cs * (Beta * Rij) * (Beta * Rij);
+    const V3<R> Arm = R(-1.0) * arm0 + Ap * DistU;
+    const V3<R> Vwall = Cross<R>(ww, RVo);
+    const V3<R> Vrel = cv - wv;
+    const V3<R> Vrot = Cross<R>(Arm, cw) - Vwall;
+    const V3<R> Vtot = Vrel + Vrot;
+    const R ERR = Dot<R>(DistU, Vtot);
+    const V3<R> Vnor = ERR * DistU;
+    const V3<R> Vtan = Vtot - Vnor;
+    const V3<R> nf = Kn * Dn * DistU + Cn * Vnor;
+    const R nfL = Length<R>(nf);
+    const V3<R> Ds = dt * Vtan;
+    V3<R> tf = -Ks * Ds + Cs * Vtan + H;
+    const R tfL = Length<R>(tf);
+    V3<R> th;
+    if (tfL > nfL * Frcoe) {
+        tf = Frcoe * nfL * (R(1.0) / tfL) * tf;
+        th = tf;
+    } else {
+        th = tf - Cs * Vtan;
+    }
+    const V3<R> Dth = dt * (cw - ww);
+    const R DthL = Dot<R>(Dth, DistU);
+    const V3<R> DthT = DthL * DistU;
+    const V3<R> DthR = Dth - DthT;
+    V3<R> rm = Kr * DthR + Mr;
+    const R rmL = Length<R>(rm);
+    V3<R> rh;
+    if (rmL > R(0.525) * Beta * Rij * nfL) {
+        rm = R(0.525) * Beta * Rij * nfL * (R(1.0) / rmL) * rm;
+        rh = rm;
+    } else {
+        rh = rm;
+        rm = rm - Cr * (R(1.0) / dt) * DthR;
+    }
+    V3<R> tm = Kt * DthT + Mt;
+    const R tmL = Length<R>(tm);
+    V3<R> twh;
+    if (tmL > R(0.65) * Frcoe * Beta * Rij * nfL) {
+        tm = R(0.65) * Frcoe * Beta * Rij * nfL * (R(1.0) / tmL) * tm;
+        twh = tm;
+    } else {
+        twh = tm;
+        tm = tm - Ct * (R(1.0) / dt) * DthT;
+    }
+    o.nf = nf; o.tf = tf; o.rm = rm; o.tm = tm; o.ht = th; o.htw = twh; o.hr = rh; o.arm = Arm;
+}
+
+template <class R>
+__device__ __forceinline__ void load_hist(const R* h, V3<R>& ht, V3<R>& htw, V3<R>& hr) {
+    ht = mk<R>(h[0], h[1], h[2]);
+    htw = mk<R>(h[3], h[4], h[5]);
+    hr = mk<R>(h[6], h[7], h[8]);
+}
+template <class R>
+__device__ __forceinline__ void store_hist(R* h, const V3<R>& ht, const V3<R>& htw, const V3<R>& hr) {
+    h[0] = ht.x; h[1] = ht.y; h[2] = ht.z;
+    h[3] = htw.x; h[4] = htw.y; h[5] = htw.z;
+    h[6] = hr.x; h[7] = hr.y; h[8] = hr.z;
+}
+
+// Closest point on triangle abc to p (Ericson, Real-Time Collision Detection 5.1.5).
+__device__ __forceinline__ V3<double> closest_on_tri(const V3<double>& p, const V3<double>& a, const V3<double>& b,
+                                                     const V3<double>& c) {
+    const V3<double> ab = b - a, ac = c - a, ap = p - a;
+    const double d1 = Dot<double>(ab, ap), d2 = Dot<double>(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return a;
+    const V3<double> bp = p - b;
+    const double d3 = Dot<double>(ab, bp), d4 = Dot<double>(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) { const double v = d1 / (d1 - d3); return a + v * ab; }
+    const V3<double> cp = p - c;
+    const double d5 = Dot<double>(ab, cp), d6 = Dot<double>(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) { const double w = d2 / (d2 - d6); return a + w * ac; }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return b + w * (c - b);
+    }
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
+    return a + v * ab + w * ac;
+}
+
+// ---------------------------------------------------------------------------
+// K3: all contacts of sphere i, then its force and clump-torque contribution.
+template <class R, bool RECORD>
+__global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                                                  PeneState* pst, double* pene_partial,
+                                                  unsigned long long* counters, int* bad, RecordBuf rec, int remap) {
+    typedef typename Vec4T<R>::type R4;
+    const int lb = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int i = lb * blockDim.x + threadIdx.x;
+    const bool active = i < P.N;
+    int n_ss = 0, n_w = 0;
+    V3<double> pf_sum = mk<double>(0.0, 0.0, 0.0), pt_sum = mk<double>(0.0, 0.0, 0.0);
+    if (active) {
+        const D4 pi4 = S.pos[i];
+        const V3<double> xi = xyz(pi4);
+        const double ri = pi4.w;
+        const R4 ki = S.kin[i];
+        const int ci_raw = S.cid[i];
+        const bool cid_ok = ci_raw >= 0 && ci_raw < P.Nc;
+        const int ci = cid_ok ? ci_raw : 0;
+        const R4 sv = C.snap[2 * ci], sw = C.snap[2 * ci + 1];
+        const V3<R> armi = Ld4<R>::xyz(ki), vi = Ld4<R>::xyz(sv), wi = Ld4<R>::xyz(sw);
+        const R mi = ki.w;
+        V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss, Fw = Fss, Tw = Fss, Fp = Fss, Tp = Fss;
+        R* hcur = L.hist[p];
+        R* hnew = L.hist[p ^ 1];
+        // front pairs: this sphere owns the pair and writes its history
+        const int f0 = L.foff[i], f1 = L.foff[i + 1];
+        for (int s = f0; s < f1; ++s) {
+            const uint32_t w = L.fword[s];
+            if (w & INTRA) continue;
+            const int j = (int)(w & JMASK);
+            const D4 pj4 = S.pos[j];
+            const V3<double> Dd = xyz(pj4) - xi;
+            const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+            const double DistL = sqrt(DistS);
+            const double Dn = ri + pj4.w - DistL;
+            if (!(Dn > 0.0)) {
+                if (w & VALID(p ^ 1)) L.fword[s] = w & ~VALID(p ^ 1);
+                continue;
+            }
+            const R4 kj = S.kin[j];
+            const int cj = S.cid[j];
+            const R4 svj = C.snap[2 * cj], swj = C.snap[2 * cj + 1];
+            PairIn<R> in;
+            in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
+            in.rI = (R)ri; in.rJ = (R)pj4.w; in.mI = mi; in.mJ = kj.w;
+            in.armI = armi; in.armJ = Ld4<R>::xyz(kj);
+            in.vI = vi; in.wI = wi; in.vJ = Ld4<R>::xyz(svj); in.wJ = Ld4<R>::xyz(swj);
+            if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
+            else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
+            PairOut<R> o;
+            pair_force<R>(P, in, o);
+            store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
+            if (!(w & VALID(p ^ 1))) L.fword[s] = w | VALID(p ^ 1);
+            Fss = Fss + o.cfc;
+            Tss = Tss + o.ctc;
+            n_ss++;
+            if (RECORD) {
+                D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
+                const V3<double> cpt = xi + (double)o.Ap * mk<double>((double)o.DistU.x, (double)o.DistU.y, (double)o.DistU.z);
+                D4 b; b.x = cpt.x; b.y = cpt.y; b.z = cpt.z; b.w = 0.0;
+                rec.pair_fcn[s] = a;
+                rec.pair_cpt[s] = b;
+            }
+        }
+        // back pairs: the owner (lower index) holds the history; recompute the
+        // same pair from the same inputs and take the reaction
+        const int b0 = L.boff[i], b1 = L.boff[i + 1];
+        for (int e = b0; e < b1; ++e) {
+            const int2 be = L.bent[e];
+            if (be.y < 0) continue;
+            const uint32_t w = L.fword[be.y];
+            if (w & INTRA) continue;
+            const int o_ = be.x;
+            const D4 po4 = S.pos[o_];
+            const V3<double> Dd = xi - xyz(po4);
+            const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+            const double DistL = sqrt(DistS);
+            const double Dn = po4.w + ri - DistL;
+            if (!(Dn > 0.0)) continue;
+            const R4 ko = S.kin[o_];
+            const int co = S.cid[o_];
+            const R4 svo = C.snap[2 * co], swo = C.snap[2 * co + 1];
+            PairIn<R> in;
+            in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
+            in.rI = (R)po4.w; in.rJ = (R)ri; in.mI = ko.w; in.mJ = mi;
+            in.armI = Ld4<R>::xyz(ko); in.armJ = armi;
+            in.vI = Ld4<R>::xyz(svo); in.wI = Ld4<R>::xyz(swo); in.vJ = vi; in.wJ = wi;
+            if (w & VALID(p)) load_hist<R>(hcur + (size_t)be.y * HSTRIDE, in.H, in.Mt, in.Mr);
+            else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
+            PairOut<R> o;
+            pair_force<R>(P, in, o);
+            Fss = Fss - o.cfc;
+            Tss = Tss + o.ctc2;
+        }
+        // walls and penetrator
+        uint32_t wf = S.wflag[i];
+        const uint32_t wf0 = wf;
+        const bool clump_ok = sv.w != R(0);
+        if (P.nfaces > 0 || P.pene_on) {
+            const R4 li = S.loc[i];
+            const bool ok = cid_ok && clump_ok && finite3(xi) && isfinite((double)li.x) &&
+                            isfinite((double)li.y) && isfinite((double)li.z) && isfinite(ri) && ri > 0.0 &&
+                            isfinite((double)mi) && mi > R(0);
+            if (P.nfaces > 0 && !ok) atomicMin(bad, i);
+            const R Rr = (R)ri;
+            if (P.nfaces > 0 && ok) {
+                const V3<double> wpos = mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
+                const V3<R> wv = mk<R>((R)P.wvel[0], (R)P.wvel[1], (R)P.wvel[2]);
+                const V3<R> ww = mk<R>((R)P.wangv[0], (R)P.wangv[1], (R)P.wangv[2]);
+                const V3<double> RV = xi - wpos;
+                if (P.wall_type == NBC_WALL_CUBE) {
+                    // faces 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
+                    for (int f = 0; f < 6; ++f) {
+                        const int a = f >> 1;
+                        const bool up = (f & 1) == 0;
+                        const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
+                        const double Sd = P.wside[a];
+                        const bool cand = up ? (E > 0 && E + ri - Sd > 0) : (E < 0 && E - ri + Sd < 0);
+                        bool touch = false;
+                        if (cand) {
+                            const V3<double> ea = mk<double>(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
+                            const V3<double> RVb = RV - E * ea, RVt = E * ea;
+                            const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
+                            const V3<double> RVn = RV - RVo;
+                            const double sgn = up ? -1.0 : 1.0;
+                            const V3<double> nin = mk<double>(a == 0 ? sgn : 0.0, a == 1 ? sgn : 0.0, a == 2 ? sgn : 0.0);
+                            const double Ap = Dot<double>(RVn, nin);
+                            const double Dn = ri - Ap;
+                            if (Dn > 0.0) {
+                                touch = true;
+                                V3<R> H, Mt, Mr;
+                                R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
+                                if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                                else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                                WallOut<R> o;
+                                wall_force<R>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
+                                              cvt3<R>(RVo), vi, wi, wv, ww, H, Mt, Mr, o);
+                                Fw = Fw + o.nf + o.tf;
+                                Tw = Tw - (R)Ap * Cross<R>(cvt3<R>(nin), o.tf);
+                                Tw = Tw - o.rm - o.tm;
+                                store_hist<R>(hp, o.ht, o.htw, o.hr);
+                                n_w++;
+                            }
+                        }
+                        if (touch) wf |= 1u << f; else wf &= ~(1u << f);
+                    }
+                } else {
+                    // cylinder surfaces 0 bottom cap, 1 top cap, 2 side
+                    const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+                    const double ERR = Dot<double>(RV, Wv);
+                    const V3<double> RVb = RV - ERR * Wv, RVt = ERR * Wv;
+                    const double norm = Length<double>(RVb);
+                    const double d_cap = fabs(ERR) - P.wheight + ri;
+                    const double d_side = norm - P.wrad + ri;
+                    for (int f = 0; f < 3; ++f) {
+                        bool cand;
+                        V3<double> RVo;
+                        if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
+                        else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
+                        else { cand = d_side >= 1e-14; RVo = (P.wrad / norm) * RVb + RVt; }
+                        bool touch = false;
+                        if (cand) {
+                            const V3<double> RVn = RV - RVo;
+                            const double Dn = ri - Length<double>(RVn);
+                            if (Dn > 0.0) {
+                                touch = true;
+                                const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                                const double DistL = sqrt(DistS);
+                                const R DistLr = (R)DistL;
+                                const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
+                                V3<R> H, Mt, Mr;
+                                R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
+                                if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                                else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                                WallOut<R> o;
+                                wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
+                                              vi, wi, wv, ww, H, Mt, Mr, o);
+                                Fw = Fw + o.nf + o.tf;
+                                Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
+                                Tw = Tw - o.rm - o.tm;
+                                store_hist<R>(hp, o.ht, o.htw, o.hr);
+                                n_w++;
+                            }
+                        }
+                        if (touch) wf |= 1u << f; else wf &= ~(1u << f);
+                    }
+                }
+                if (!isfinite((double)Fw.x) || !isfinite((double)Fw.y) || !isfinite((double)Fw.z) ||
+                    !isfinite((double)Tw.x) || !isfinite((double)Tw.y) || !isfinite((double)Tw.z))
+                    atomicMin(bad, i);
+            }
+            if (P.pene_on) {
+                const int pf = P.pene_face;
+                const V3<double> pp = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
+                const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
+                const V3<double> RV = xi - pp;
+                bool touch = false;
+                V3<double> RVo = mk<double>(0.0, 0.0, 0.0), Dist = RVo, DistU64 = RVo;
+                double DistL = 0.0, Dn = 0.0;
+                bool have_u = false;
+                if (P.pshape == NBC_PENE_SPHERE) {
+                    // src/dem_sphere_intruder.cuh:27-50
+                    const double cd = Length<double>(RV);
+                    V3<double> nrm;
+                    if (cd > 1.0e-15) {
+                        nrm = (1.0 / cd) * RV;
+                    } else {
+                        const double t = sqrt(pdir.x * pdir.x + pdir.y * pdir.y + pdir.z * pdir.z);
+                        nrm = t > 1e-20 ? mk<double>(pdir.x / t, pdir.y / t, pdir.z / t) : mk<double>(0.0, 0.0, 0.0);
+                        if (Length<double>(nrm) < 1.0e-15) nrm = mk<double>(0.0, 0.0, 1.0);
+                    }
+                    const double gap = cd - P.prad;
+                    const double ov = ri - gap;
+                    if (ov > 0.0) {
+                        touch = true;
+                        RVo = P.prad * nrm;
+                        Dist = gap * nrm;
+                        DistL = gap;
+                        Dn = ov;
+                        DistU64 = nrm;
+                        have_u = true;
+                        if (P.pwait) atomicCAS(&pst->contact_state, 0, 1);
+                    }
+                } else if (P.pshape == NBC_PENE_CONE) {
+                    double best = 1e300;
+                    V3<double> bq = mk<double>(0.0, 0.0, 0.0);
+                    for (int t = 0; t < P.ntri; ++t) {
+                        const double* v = P.ptri + 9 * t;
+                        const V3<double> q = closest_on_tri(RV, mk<double>(v[0], v[1], v[2]), mk<double>(v[3], v[4], v[5]),
+                                                            mk<double>(v[6], v[7], v[8]));
+                        const V3<double> dq = RV - q;
+                        const double ds = Dot<double>(dq, dq);
+                        if (ds < best) { best = ds; bq = q; }
+                    }
+                    const V3<double> RVn = RV - bq;
+                    const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                    const double dl = sqrt(DistS);
+                    if (P.ntri > 0 && ri - dl > 0.0) {
+                        touch = true; RVo = bq; Dist = RVn; DistL = dl; Dn = ri - dl;
+                    }
+                } else {
+                    // src/dem_contact_penetrator.cu:42-83
+                    const double LatDx = P.cell[0];
+                    const double ERR = Dot<double>(RV, pdir);
+                    if (fabs(ERR) < (P.pheight + 0.80 * LatDx)) {
+                        const V3<double> RVb = RV - ERR * pdir;
+                        const double norm = Length<double>(RVb);
+                        if (norm < (P.prad + 0.80 * LatDx)) {
+                            const V3<double> RVt = ERR * pdir;
+                            V3<double> o3;
+                            if (fabs(ERR) < P.pheight) o3 = (1.0 / norm) * P.prad * RVb + RVt;
+                            else if (norm < P.prad) o3 = (1.0 / fabs(ERR)) * P.pheight * RVt + RVb;
+                            else o3 = (1.0 / norm) * P.prad * RVb + (1.0 / fabs(ERR)) * P.pheight * RVt;
+                            const V3<double> RVn = RV - o3;
+                            const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                            const double dl = sqrt(DistS);
+                            if (ri - dl > 0.0) { touch = true; RVo = o3; Dist = RVn; DistL = dl; Dn = ri - dl; }
+                        }
+                    }
+                }
+                if (touch) {
+                    const R DistLr = (R)DistL;
+                    const V3<R> DistU = have_u ? cvt3<R>(DistU64) : (R(1.0) / DistLr) * cvt3<R>(Dist);
+                    R Rij = Rr;
+                    if (P.pshape == NBC_PENE_SPHERE) Rij = Rr * (R)P.prad / (Rr + (R)P.prad);
+                    R Mij = mi;
+                    if (P.pmotion == NBC_PENE_GRAVITY) Mij = mi * (R)P.pmass / (mi + (R)P.pmass);
+                    V3<R> H, Mt, Mr;
+                    R* hp = S.wh + ((size_t)pf * P.N + i) * HSTRIDE;
+                    if ((wf >> pf) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                    const V3<R> pv = mk<R>((R)pst->vel[0], (R)pst->vel[1], (R)pst->vel[2]);
+                    const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
+                    const V3<R> RVor = cvt3<R>(RVo);
+                    WallOut<R> o;
+                    wall_force<R>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
+                                  pw, H, Mt, Mr, o);
+                    const V3<R> cf = mk<R>(R(0), R(0), R(0));
+                    Fp = o.nf + o.tf;
+                    Tp = -DistLr * Cross<R>(DistU, o.tf);
+                    V3<R> pTc = R(-1.0) * Cross<R>(RVor, (o.nf + o.tf + cf));
+                    Tp = Tp - o.rm - o.tm;
+                    pTc = pTc + o.rm + o.tm;
+                    store_hist<R>(hp, o.ht, o.htw, o.hr);
+                    pf_sum = mk<double>((double)Fp.x, (double)Fp.y, (double)Fp.z);
+                    pt_sum = mk<double>((double)pTc.x, (double)pTc.y, (double)pTc.z);
+                    n_w++;
+                    if (RECORD) {
+                        const V3<R> tn = Cross<R>(o.nf, o.arm);
+                        const V3<R> tt = Cross<R>(o.tf, o.arm) - o.rm - o.tm;
+                        D4 a; a.x = o.nf.x; a.y = o.nf.y; a.z = o.nf.z; a.w = 0.0;
+                        D4 b; b.x = o.tf.x; b.y = o.tf.y; b.z = o.tf.z; b.w = 0.0;
+                        D4 c; c.x = tn.x; c.y = tn.y; c.z = tn.z; c.w = 0.0;
+                        D4 d; d.x = tt.x; d.y = tt.y; d.z = tt.z; d.w = 0.0;
+                        rec.pene[4 * (size_t)i] = a; rec.pene[4 * (size_t)i + 1] = b;
+                        rec.pene[4 * (size_t)i + 2] = c; rec.pene[4 * (size_t)i + 3] = d;
+                    }
+                }
+                if (touch) wf |= 1u << pf; else wf &= ~(1u << pf);
+            }
+        }
+        if (wf != wf0) S.wflag[i] = wf;
+        // totF/totT (src/dem_clump.cu:97-98) and the clump torque term
+        // T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
+        const V3<R> tF = Fss + Fw + Fp;
+        const V3<R> tT = Tss + Tw + Tp;
+        V3<R> r;
+        if (sizeof(R) == sizeof(double)) {
+            const D4 pc = C.pq[2 * ci];
+            r = cvt3<R>(mk<double>(xi.x - pc.x, xi.y - pc.y, xi.z - pc.z));
+        } else {
+            r = armi;  // x_s - x_c = R(q) * local up to FP64 rounding
+        }
+        const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
+                               tT.z + (r.x * tF.y - r.y * tF.x));
+        S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
+        S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+        if (RECORD) {
+            D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
+            D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
+            rec.tot[2 * (size_t)i] = a;
+            rec.tot[2 * (size_t)i + 1] = b;
+            D4 q[6];
+            const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
+            for (int k = 0; k < 6; ++k) {
+                q[k].x = parts[k].x; q[k].y = parts[k].y; q[k].z = parts[k].z; q[k].w = 0.0;
+                rec.parts[6 * (size_t)i + k] = q[k];
+            }
+        }
+    }
+    // block reductions: touching counts and FP64 penetrator partial sums
+    __shared__ double red[6][4];
+    __shared__ unsigned int cnt[2][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double v[6] = {pf_sum.x, pf_sum.y, pf_sum.z, pt_sum.x, pt_sum.y, pt_sum.z};
+    unsigned a0 = (unsigned)n_ss, a1 = (unsigned)n_w;
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int k = 0; k < 6; ++k) v[k] += __shfl_down(v[k], o, 64);
+        a0 += __shfl_down(a0, o, 64);
+        a1 += __shfl_down(a1, o, 64);
+    }
+    if (lane == 0) {
+        for (int k = 0; k < 6; ++k) red[k][wv] = v[k];
+        cnt[0][wv] = a0;
+        cnt[1][wv] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        unsigned long long s0 = 0, s1 = 0;
+        for (int k = 0; k < nw; ++k) { s0 += cnt[0][k]; s1 += cnt[1][k]; }
+        if (s0) atomicAdd(&counters[0], s0);
+        if (s1) atomicAdd(&counters[1], s1);
+        if (P.pene_on) {
+            for (int k = 0; k < 6; ++k) {
+                double s = 0.0;
+                for (int q = 0; q < nw; ++q) s += red[k][q];
+                pene_partial[6 * (size_t)lb + k] = s;
+            }
+        }
+    }
+}
+
+// Penetrator reaction: FP64 sum of the per-block partials in block order, then
+// src/dem_integrate_penetrator.cu:79-115.
+__global__ __launch_bounds__(256) void k_pene_finalize(DevParams P, PeneState* pst, const double* partial, int nb,
+                                                       int count, double* cache_f, double* cache_t) {
+    __shared__ double red[6][256];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int k = 0; k < 6; ++k) v[k] += partial[6 * (size_t)b + k];
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; ++k) { pst->totF[k] = red[k][0]; pst->totT[k] = red[3 + k][0]; }
+        if (P.pwait && pst->contact_state == 1) pst->contact_state = 2;
+        const V3<double> reaction = -1.0 * mk<double>(pst->totF[0], pst->totF[1], pst->totF[2]);
+        if (P.pmotion == NBC_PENE_GRAVITY && (!P.pwait || pst->contact_state != 0)) {
+            V3<double> pf = (1.0 / P.pmass) * reaction;
+            pf.z = pf.z + P.Gravity;
+            const double h = 0.5 * P.dt;
+            pst->vel[0] = pst->vel[0] + h * pf.x;
+            pst->vel[1] = pst->vel[1] + h * pf.y;
+            pst->vel[2] = pst->vel[2] + h * pf.z;
+        }
+        cache_f[3 * count] = reaction.x; cache_f[3 * count + 1] = reaction.y; cache_f[3 * count + 2] = reaction.z;
+        cache_t[3 * count] = pst->totT[0]; cache_t[3 * count + 1] = pst->totT[1]; cache_t[3 * count + 2] = pst->totT[2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4: clump force/torque as the sum of its spheres in ascending sorted order
+// (FP64 accumulation), then the second half-kick.
+template <class R>
+__global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg, const int* mend,
+                              const int* mem) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.Nc) return;
+    const int b = mbeg[c], e = mend[c];
+    D4 Fr, Tr;
+    if (b >= 0 && e > b) {
+        typedef typename Vec4T<R>::type R4;
+        int s = mem[b];
+        R4 f = S.sft[2 * (size_t)s], t = S.sft[2 * (size_t)s + 1];
+        V3<double> F = mk<double>(f.x, f.y, f.z), T = mk<double>(t.x, t.y, t.z);
+        for (int k = b + 1; k < e; ++k) {
+            s = mem[k];
+            f = S.sft[2 * (size_t)s];
+            t = S.sft[2 * (size_t)s + 1];
+            F = F + mk<double>(f.x, f.y, f.z);
+            T = T + mk<double>(t.x, t.y, t.z);
+        }
+        Fr.x = F.x; Fr.y = F.y; Fr.z = F.z; Fr.w = 0.0;
+        Tr.x = T.x; Tr.y = T.y; Tr.z = T.z; Tr.w = 0.0;
+        C.ft[2 * c] = Fr;
+        C.ft[2 * c + 1] = Tr;
+    } else {
+        Fr = C.ft[2 * c];
+        Tr = C.ft[2 * c + 1];
+    }
+    const D4 pr = C.prop[c];
+    V3<double> F = xyz(Fr), T = xyz(Tr);
+    F = (1.0 / pr.w) * F;
+    F.z = F.z + P.Gravity;
+    T.x = (1.0 / pr.x) * T.x;
+    T.y = (1.0 / pr.y) * T.y;
+    T.z = (1.0 / pr.z) * T.z;
+    const D4 vr = C.vw[2 * c], wr = C.vw[2 * c + 1];
+    const V3<double> v = xyz(vr) + 0.5 * P.dt * F;
+    const V3<double> w = xyz(wr) + 0.5 * P.dt * T;
+    D4 vo; vo.x = v.x; vo.y = v.y; vo.z = v.z; vo.w = 0.0;
+    D4 wo; wo.x = w.x; wo.y = w.y; wo.z = w.z; wo.w = 0.0;
+    C.vw[2 * c] = vo;
+    C.vw[2 * c + 1] = wo;
+}
+
+// Kinetic energy partial sums (FP64), one value per block.
+__global__ __launch_bounds__(256) void k_kinetic(int Nc, const D4* vw, const D4* prop, double* partial) {
+    __shared__ double red[256];
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    double e = 0.0;
+    if (c < Nc) {
+        const D4 v = vw[2 * c], w = vw[2 * c + 1], p = prop[c];
+        e = 0.5 * p.w * (v.x * v.x + v.y * v.y + v.z * v.z) + 0.5 * (p.x * w.x * w.x + p.y * w.y * w.y + p.z * w.z * w.z);
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ===========================================================================
+// host-side launchers
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define LAUNCH_CHECK()                                    \
+    do {                                                  \
+        hipError_t e_ = hipGetLastError();                \
+        if (e_ != hipSuccess) return (int)e_;             \
+    } while (0)
+
+int launch_fill_i32(hipStream_t st, int* a, long long n, int v) {
+    if (n <= 0) return 0;
+    k_fill_i32<<<nblk(n, 256), 256, 0, st>>>(a, n, v);
+    LAUNCH_CHECK();
+    return 0;
+}
+int launch_fill_u32(hipStream_t st, uint32_t* a, long long n, uint32_t v) {
+    if (n <= 0) return 0;
+    k_fill_u32<<<nblk(n, 256), 256, 0, st>>>(a, n, v);
+    LAUNCH_CHECK();
+    return 0;
+}
+int launch_fill_f64(hipStream_t st, double* a, long long n, double v) {
+    if (n <= 0) return 0;
+    k_fill_f64<<<nblk(n, 256), 256, 0, st>>>(a, n, v);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_keys_from_int(hipStream_t st, const int* a, uint32_t* k, int* v, int n) {
+    if (n <= 0) return 0;
+    k_keys_from_int<<<nblk(n, 256), 256, 0, st>>>(a, k, v, n);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int scan_tmp_ints(int n) { return (int)nblk(n, 2048) + 1; }
+
+int launch_exclusive_scan(hipStream_t st, const int* in, int* out, int n, int* tmp, int* total) {
+    const int nb = (int)nblk(n, 2048);
+    if (n <= 0) return launch_fill_i32(st, total, 1, 0);
+    k_scan_reduce<<<nb, 256, 0, st>>>(in, n, tmp);
+    LAUNCH_CHECK();
+    k_scan_top<<<1, 256, 0, st>>>(tmp, nb, total);
+    LAUNCH_CHECK();
+    k_scan_down<<<nb, 256, 0, st>>>(in, out, n, tmp);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int radix_tmp_ints(int n, int rbits) { return (1 << rbits) * (int)nblk(n, 256); }
+
+// Sorts (keys, vals) by the low `bits` bits of the keys. Result in (k0, v0)
+// when the pass count is even, else in (k1, v1); *in_second reports which.
+int launch_radix_sort(hipStream_t st, uint32_t* k0, int* v0, uint32_t* k1, int* v1, int n, int bits, int rbits,
+                      int* hist, int* hscan, int* stmp, int* stotal, int* in_second) {
+    const int nb = (int)nblk(n, 256);
+    if (rbits < 1) rbits = 1;
+    if (rbits > 8) rbits = 8;
+    int passes = (bits + rbits - 1) / rbits;
+    if (passes < 1) passes = 1;
+    uint32_t* ki = k0; int* vi = v0; uint32_t* ko = k1; int* vo = v1;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int shift = ps * rbits;
+        const int rb = (shift + rbits <= 32) ? rbits : 32 - shift;
+        const int nh = (1 << rb) * nb;
+        k_rs_hist<<<nb, 256, 0, st>>>(ki, n, shift, rb, hist, nb);
+        LAUNCH_CHECK();
+        int e = launch_exclusive_scan(st, hist, hscan, nh, stmp, stotal);
+        if (e) return e;
+        k_rs_scatter<<<nb, 256, 0, st>>>(ki, vi, ko, vo, n, shift, rb, hscan, nb);
+        LAUNCH_CHECK();
+        uint32_t* tk = ki; ki = ko; ko = tk;
+        int* tv = vi; vi = vo; vo = tv;
+    }
+    *in_second = (passes & 1) ? 1 : 0;
+    return 0;
+}
+
+int launch_hash(hipStream_t st, const DevParams& P, const D4* pos, uint32_t* key, int* val) {
+    k_hash<<<nblk(P.N, 256), 256, 0, st>>>(P, pos, key, val);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_seg_bounds(hipStream_t st, const uint32_t* key, int n, int nkeys, int* beg, int* end) {
+    int e = launch_fill_i32(st, beg, nkeys, -1);
+    if (e) return e;
+    e = launch_fill_i32(st, end, nkeys, -1);
+    if (e) return e;
+    k_seg_bounds<<<nblk(n, 256), 256, 0, st>>>(key, n, nkeys, beg, end);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* pos, const uint32_t* cell,
+                     const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, const int* foff,
+                     const int* boff, uint32_t* fword, int2* bent, unsigned* overflow, int block) {
+    if (fill)
+        k_neighbors<true><<<nblk(P.N, block), block, 0, st>>>(P, pos, cell, cbeg, cend, cid, fcnt, bcnt, foff, boff,
+                                                              fword, bent, overflow);
+    else
+        k_neighbors<false><<<nblk(P.N, block), block, 0, st>>>(P, pos, cell, cbeg, cend, cid, fcnt, bcnt, foff, boff,
+                                                               fword, bent, overflow);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_back_resolve(hipStream_t st, int N, const int* foff, const uint32_t* fword, const int* boff, int2* bent,
+                        int block) {
+    k_back_resolve<<<nblk(N, block), block, 0, st>>>(N, foff, fword, boff, bent);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_kinetic(hipStream_t st, int Nc, const D4* vw, const D4* prop, double* partial) {
+    k_kinetic<<<nblk(Nc, 256), 256, 0, st>>>(Nc, vw, prop, partial);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_pene_finalize(hipStream_t st, const DevParams& P, PeneState* pst, const double* partial, int nb, int count,
+                         double* cf, double* ct) {
+    k_pene_finalize<<<1, 256, 0, st>>>(P, pst, partial, nb, count, cf, ct);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+template <class R>
+int launch_clump_before(hipStream_t st, const DevParams& P, ClumpBuf<R> C, PeneState* pst, unsigned long long* ctr,
+                        int block) {
+    const int n = P.Nc > 1 ? P.Nc : 1;
+    k_clump_before<R><<<nblk(n, block), block, 0, st>>>(P, C, pst, ctr);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, unsigned* flag, int block) {
+    k_sphere_pose<R><<<nblk(P.N, block), block, 0, st>>>(P, S, C, flag);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_gather(hipStream_t st, int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst) {
+    k_gather<R><<<nblk(N, 256), 256, 0, st>>>(N, nface, perm, src, dst);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
+                      const int* ofoff, const uint32_t* ofword, const R* ohist, int block) {
+    k_hist_remap<R><<<nblk(N, block), block, 0, st>>>(N, p, perm, nfoff, nfword, nhist, ofoff, ofword, ohist);
+    LAUNCH_CHECK();
+    return 0;
+}
+int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
+template <class R>
+int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                    PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
+                    int block, int remap) {
+    const unsigned nb = nblk(P.N, block);
+    if (record)
+        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap);
+    else
+        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
+                       const int* mend, const int* mem, int block) {
+    k_clump_after<R><<<nblk(P.Nc, block), block, 0, st>>>(P, S, C, mbeg, mend, mem);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+#define INST(R)                                                                                                      \
+    template int launch_clump_before<R>(hipStream_t, const DevParams&, ClumpBuf<R>, PeneState*, unsigned long long*, \
+                                        int);                                                                        \
+    template int launch_sphere_pose<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, unsigned*, int);    \
+    template int launch_gather<R>(hipStream_t, int, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
+    template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
+                                      const uint32_t*, const R*, int);                                               \
+    template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
+                                    PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, int, int);      \
+    template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, const int*,         \
+                                       const int*, const int*, int);
+INST(double)
+INST(float)
+
+}  // namespace nbc

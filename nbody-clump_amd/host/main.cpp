@@ -1,0 +1,21 @@
+// nbody_clump: drop-in executable for the reference's main()
+// (src/main.cpp:4-17). Runs the simulation from the current directory (or the
+// directory given as the first argument) and exits with EXIT_FAILURE and the
+// reference's message when the library reports a fatal condition.
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+
+#include "../../include/nbc.h"
+
+int main(int argc, char** argv) {
+    const char* dir = argc > 1 ? argv[1] : ".";
+    const int device = argc > 2 ? std::atoi(argv[2]) : 0;
+    const int e = nbc_run_simulation_clump(dir, device);
+    if (e) {
+        std::fprintf(stderr, "%s\n", nbc_last_error(nullptr));
+        return e == 7 ? 1 : EXIT_FAILURE;
+    }
+    std::cout << "Final!!" << std::endl;
+    return 0;
+}

@@ -1,0 +1,425 @@
+"""ctypes binding of libnbc.so (include/nbc.h) for tests, bench and scripts.
+
+The library is the product: this module only marshals numpy arrays across
+the C ABI. It raises if libnbc.so is missing — there is no fallback path.
+Structures mirror include/nbc.h field for field.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnbc.so")
+
+NBC_OK, NBC_E_BADINPUT, NBC_E_NEIGHBOR_OVERFLOW, NBC_E_NONFINITE, NBC_E_HIP, NBC_E_NODEVICE, NBC_E_IO, NBC_E_PARAM = range(8)
+NBC_WALL_NONE, NBC_WALL_CUBE, NBC_WALL_CYLINDER = 0, 1, 2
+NBC_FP64, NBC_MIXED = 0, 1
+NBC_PENE_CYLINDER, NBC_PENE_SPHERE, NBC_PENE_CONE = 0, 1, 2
+NBC_PENE_CONSTANT_VELOCITY, NBC_PENE_GRAVITY = 0, 1
+
+D3 = C.c_double * 3
+D4 = C.c_double * 4
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("ParticleNum", C.c_int32), ("ClumpNum", C.c_int32), ("BranchLmtLen", C.c_int32),
+        ("StartStep", C.c_int32), ("EndStep", C.c_int32), ("OutputInterval", C.c_int32),
+        ("StepSize", C.c_double),
+        ("Ymod", C.c_double), ("Prat", C.c_double), ("Frcoe", C.c_double), ("NormRes", C.c_double),
+        ("Frvel", C.c_double), ("Beta", C.c_double), ("MU_T", C.c_double), ("MU_R", C.c_double),
+        ("Cohesion", C.c_double), ("Gravity", C.c_double),
+        ("Thresh", C.c_double), ("GridLevels", C.c_double), ("Update", C.c_double),
+        ("WorldSize", D3), ("WorldOrigin", D3), ("BoxUpLmt", D3), ("BoxLowLmt", D3),
+        ("isIntruder", C.c_int32), ("isSelfGravity", C.c_int32), ("isWallForce", C.c_int32),
+        ("isQuaternion", C.c_int32),
+        ("Damping", C.c_double), ("CellSize", D3), ("i_CellSize", D3), ("GridDim", C.c_uint32 * 3),
+        ("CellNum", C.c_int32), ("finalized", C.c_int32),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("wall_type", C.c_int32), ("block_size", C.c_int32),
+        ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("reserved0", C.c_int32),
+        ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("reserved", C.c_double * 4),
+    ]
+
+
+class Wall(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("reserved", C.c_int32), ("side", D3), ("radius", C.c_double),
+        ("height", C.c_double), ("dir", D3), ("prop", D4), ("pos", D3), ("vel", D3), ("quat", D4),
+        ("angv", D3),
+    ]
+
+
+class Penetrator(C.Structure):
+    _fields_ = [
+        ("shape", C.c_int32), ("motion", C.c_int32), ("wait_for_contact", C.c_int32),
+        ("contact_state", C.c_int32), ("radius", C.c_double), ("height", C.c_double), ("dir", D3),
+        ("prop", D4), ("pos", D3), ("vel", D3), ("quat", D4), ("angv", D3), ("ntri", C.c_int32),
+        ("reserved", C.c_int32), ("tri", C.POINTER(C.c_double)),
+    ]
+
+
+class SpheresIn(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("reserved", C.c_int32), ("clump_id", C.POINTER(C.c_int32)),
+        ("pos", C.POINTER(C.c_double)), ("local", C.POINTER(C.c_double)),
+        ("mass", C.POINTER(C.c_double)), ("radius", C.POINTER(C.c_double)),
+    ]
+
+
+class ClumpsIn(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("reserved", C.c_int32), ("pos", C.POINTER(C.c_double)),
+        ("vel", C.POINTER(C.c_double)), ("quat", C.POINTER(C.c_double)),
+        ("angv", C.POINTER(C.c_double)), ("prop", C.POINTER(C.c_double)),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("rebuilds", C.c_int64), ("front_pairs", C.c_int64),
+        ("back_entries", C.c_int64), ("touching_ss", C.c_int64), ("touching_wall", C.c_int64),
+        ("kinetic_energy", C.c_double), ("pene_force", D3), ("pene_torque", D3),
+        ("max_cum_disp", C.c_double),
+    ]
+
+
+class BedSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nclumps", C.c_int32), ("seed", C.c_uint64), ("container", D4)]
+
+
+class BedOut(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("nc", C.c_int32), ("clump_id", C.POINTER(C.c_int32)),
+        ("pos", C.POINTER(C.c_double)), ("local", C.POINTER(C.c_double)),
+        ("mass", C.POINTER(C.c_double)), ("radius", C.POINTER(C.c_double)),
+        ("cpos", C.POINTER(C.c_double)), ("cvel", C.POINTER(C.c_double)),
+        ("cquat", C.POINTER(C.c_double)), ("cangv", C.POINTER(C.c_double)),
+        ("cprop", C.POINTER(C.c_double)), ("wall", Wall), ("params", Params),
+    ]
+
+
+# Every exported symbol of include/nbc.h, with its argtypes.
+P = C.c_void_p
+I32 = C.c_int32
+_SIGS = {
+    "nbc_version": ([], C.c_char_p),
+    "nbc_last_error": ([P], C.c_char_p),
+    "nbc_device_count": ([C.POINTER(I32)], C.c_int),
+    "nbc_load_world_par": ([C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_size_t], C.c_int),
+    "nbc_finalize_params": ([C.POINTER(Params), P, I32, C.POINTER(Options)], C.c_int),
+    "nbc_load_options": ([C.c_char_p, C.POINTER(Options)], C.c_int),
+    "nbc_load_phase": ([C.c_char_p, I32, P, P, P, P, P], C.c_int),
+    "nbc_load_clumps": ([C.c_char_p, I32, P, P, P, P, P], C.c_int),
+    "nbc_load_cube_wall": ([C.c_char_p, C.POINTER(Wall)], C.c_int),
+    "nbc_load_cylinder_wall": ([C.c_char_p, C.POINTER(Wall)], C.c_int),
+    "nbc_load_penetrator": ([C.c_char_p, C.POINTER(Penetrator)], C.c_int),
+    "nbc_free_mesh": ([C.POINTER(Penetrator)], None),
+    "nbc_make_cone_mesh": ([C.c_double, C.c_double, I32, P], C.c_int),
+    "nbc_generate_bed": ([C.POINTER(BedSpec), C.POINTER(BedOut)], C.c_int),
+    "nbc_free_bed": ([C.POINTER(BedOut)], None),
+    "nbc_save_bed_text": ([C.c_char_p, C.POINTER(BedOut), C.POINTER(Penetrator)], C.c_int),
+    "nbc_create": ([C.POINTER(P), C.POINTER(Params), C.POINTER(Options), I32], C.c_int),
+    "nbc_destroy": ([P], C.c_int),
+    "nbc_upload": ([P, C.POINTER(SpheresIn), C.POINTER(ClumpsIn), C.POINTER(Wall), C.POINTER(Penetrator)], C.c_int),
+    "nbc_step": ([P, I32], C.c_int),
+    "nbc_rebuild": ([P], C.c_int),
+    "nbc_set_record": ([P, I32], C.c_int),
+    "nbc_synchronize": ([P], C.c_int),
+    "nbc_get_stats": ([P, C.POINTER(Stats)], C.c_int),
+    "nbc_get_spheres": ([P, P, P, P, P, P, P, P], C.c_int),
+    "nbc_get_grid": ([P, P, P, P, P], C.c_int),
+    "nbc_get_pair_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "nbc_get_pairs": ([P, P, P, P, P, P, P], C.c_int),
+    "nbc_get_pair_history": ([P, P], C.c_int),
+    "nbc_get_sphere_forces": ([P, I32, P, P], C.c_int),
+    "nbc_get_clumps": ([P, P, P, P, P, P, P], C.c_int),
+    "nbc_get_penetrator": ([P, P, P, P, P, P, P, P], C.c_int),
+    "nbc_get_pene_exports": ([P, P, P, P, P], C.c_int),
+    "nbc_get_pair_exports": ([P, P, P], C.c_int),
+    "nbc_profile_enable": ([P, I32], C.c_int),
+    "nbc_profile_get": ([P, I32, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
+    "nbc_write_particles_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P], C.c_int),
+    "nbc_write_clumps_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P, P, P], C.c_int),
+    "nbc_write_force_history_bin": ([C.c_char_p, I32, I32, P, P], C.c_int),
+    "nbc_write_yi_bin": ([C.c_char_p, I32, I32, P, P, P, P, P], C.c_int),
+    "nbc_write_contacts_bin": ([C.c_char_p, I32, I32, I32, P, P, P], C.c_int),
+    "nbc_run_simulation_clump": ([C.c_char_p, I32], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libnbc.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libnbc.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+class NbcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nbc error {code}: {msg}")
+        self.code = code
+
+
+def check(code, ctx=None):
+    if code != 0:
+        m = lib().nbc_last_error(ctx)
+        raise NbcError(code, m.decode() if m else "")
+    return code
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count():
+    n = I32(0)
+    rc = lib().nbc_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+# ---------------------------------------------------------------------------
+class Bed:
+    """Host state of one bed in file order (numpy copies)."""
+
+    def __init__(self, params, clump_id, pos, local, mass, radius, cpos, cvel, cquat, cangv, cprop, wall):
+        self.params = params
+        self.clump_id = np.ascontiguousarray(clump_id, dtype=np.int32)
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.local = np.ascontiguousarray(local, dtype=np.float64).reshape(-1, 3)
+        self.mass = np.ascontiguousarray(mass, dtype=np.float64)
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.cpos = np.ascontiguousarray(cpos, dtype=np.float64).reshape(-1, 3)
+        self.cvel = np.ascontiguousarray(cvel, dtype=np.float64).reshape(-1, 3)
+        self.cquat = np.ascontiguousarray(cquat, dtype=np.float64).reshape(-1, 4)
+        self.cangv = np.ascontiguousarray(cangv, dtype=np.float64).reshape(-1, 3)
+        self.cprop = np.ascontiguousarray(cprop, dtype=np.float64).reshape(-1, 4)
+        self.wall = wall
+
+    @property
+    def n(self):
+        return len(self.radius)
+
+    @property
+    def nc(self):
+        return len(self.cprop)
+
+    def spheres_in(self):
+        return SpheresIn(self.n, 0, self.clump_id.ctypes.data_as(C.POINTER(C.c_int32)),
+                         self.pos.ctypes.data_as(C.POINTER(C.c_double)),
+                         self.local.ctypes.data_as(C.POINTER(C.c_double)),
+                         self.mass.ctypes.data_as(C.POINTER(C.c_double)),
+                         self.radius.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def clumps_in(self):
+        return ClumpsIn(self.nc, 0, self.cpos.ctypes.data_as(C.POINTER(C.c_double)),
+                        self.cvel.ctypes.data_as(C.POINTER(C.c_double)),
+                        self.cquat.ctypes.data_as(C.POINTER(C.c_double)),
+                        self.cangv.ctypes.data_as(C.POINTER(C.c_double)),
+                        self.cprop.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def finalized_params(self, opt=None):
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self.params), C.sizeof(Params))
+        check(lib().nbc_finalize_params(C.byref(p), ptr(self.radius), self.n, C.byref(opt) if opt else None))
+        return p
+
+
+def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
+    spec = BedSpec(kind, nclumps, seed, D4(*container))
+    out = BedOut()
+    check(lib().nbc_generate_bed(C.byref(spec), C.byref(out)))
+    try:
+        n, nc = out.n, out.nc
+
+        def arr(p, m):
+            return np.ctypeslib.as_array(p, shape=(m,)).copy()
+
+        params = Params()
+        C.memmove(C.byref(params), C.byref(out.params), C.sizeof(Params))
+        wall = Wall()
+        C.memmove(C.byref(wall), C.byref(out.wall), C.sizeof(Wall))
+        bed = Bed(params, arr(out.clump_id, n), arr(out.pos, 3 * n), arr(out.local, 3 * n), arr(out.mass, n),
+                  arr(out.radius, n), arr(out.cpos, 3 * nc), arr(out.cvel, 3 * nc), arr(out.cquat, 4 * nc),
+                  arr(out.cangv, 3 * nc), arr(out.cprop, 4 * nc), wall)
+    finally:
+        lib().nbc_free_bed(C.byref(out))
+    return bed
+
+
+def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
+                 cell_size=0.0):
+    o = Options()
+    o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
+    o.pair_capacity, o.pene_frcoe, o.cell_size = pair_capacity, pene_frcoe, cell_size
+    return o
+
+
+def sphere_penetrator(radius, pos, vel, mass=1.0, motion=NBC_PENE_CONSTANT_VELOCITY):
+    p = Penetrator()
+    p.shape, p.motion, p.radius, p.height = NBC_PENE_SPHERE, motion, radius, radius
+    p.dir[:] = (0.0, 0.0, 1.0)
+    p.prop[:] = (1.0, 1.0, 1.0, mass)
+    p.pos[:] = pos
+    p.vel[:] = vel
+    p.quat[:] = (0.0, 0.0, 0.0, 1.0)
+    return p
+
+
+def cone_penetrator(radius, height, pos, vel, nseg=64, mass=1.0):
+    p = Penetrator()
+    p.shape, p.motion, p.radius, p.height = NBC_PENE_CONE, NBC_PENE_CONSTANT_VELOCITY, radius, height
+    p.dir[:] = (0.0, 0.0, 1.0)
+    p.prop[:] = (1.0, 1.0, 1.0, mass)
+    p.pos[:] = pos
+    p.vel[:] = vel
+    p.quat[:] = (0.0, 0.0, 0.0, 1.0)
+    nt = lib().nbc_make_cone_mesh(radius, height, nseg, None)
+    tri = np.zeros(9 * nt)
+    lib().nbc_make_cone_mesh(radius, height, nseg, ptr(tri))
+    p._tri = tri  # keep alive
+    p.ntri = nt
+    p.tri = tri.ctypes.data_as(C.POINTER(C.c_double))
+    return p
+
+
+class Context:
+    """Device context: one bed resident on one GPU."""
+
+    def __init__(self, params, opt=None, device=0):
+        self.params = params
+        self.h = C.c_void_p()
+        check(lib().nbc_create(C.byref(self.h), C.byref(params), C.byref(opt) if opt else None, device))
+        self.N = params.ParticleNum
+        self.Nc = params.ClumpNum
+
+    def close(self):
+        if self.h:
+            lib().nbc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return check(rc, self.h)
+
+    def upload(self, bed, wall=None, pene=None):
+        s, c = bed.spheres_in(), bed.clumps_in()
+        self._bed = bed
+        self._pene = pene
+        self._c(lib().nbc_upload(self.h, C.byref(s), C.byref(c), C.byref(wall) if wall else None,
+                                 C.byref(pene) if pene else None))
+
+    def step(self, n=1):
+        self._c(lib().nbc_step(self.h, n))
+
+    def rebuild(self):
+        self._c(lib().nbc_rebuild(self.h))
+
+    def set_record(self, on=True):
+        self._c(lib().nbc_set_record(self.h, int(on)))
+
+    def sync(self):
+        self._c(lib().nbc_synchronize(self.h))
+
+    def stats(self):
+        s = Stats()
+        self._c(lib().nbc_get_stats(self.h, C.byref(s)))
+        return s
+
+    def spheres(self):
+        N = self.N
+        d = dict(fid=np.zeros(N, np.int32), cid=np.zeros(N, np.int32), pos=np.zeros((N, 3)),
+                 cum=np.zeros((N, 3)), local=np.zeros((N, 3)), mass=np.zeros(N), radius=np.zeros(N))
+        self._c(lib().nbc_get_spheres(self.h, ptr(d["fid"]), ptr(d["cid"]), ptr(d["pos"]), ptr(d["cum"]),
+                                      ptr(d["local"]), ptr(d["mass"]), ptr(d["radius"])))
+        return d
+
+    def grid(self):
+        N, CN = self.N, self.params.CellNum
+        d = dict(cell=np.zeros(N, np.int32), perm=np.zeros(N, np.int32), beg=np.zeros(CN, np.int32),
+                 end=np.zeros(CN, np.int32))
+        self._c(lib().nbc_get_grid(self.h, ptr(d["cell"]), ptr(d["perm"]), ptr(d["beg"]), ptr(d["end"])))
+        return d
+
+    def pair_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._c(lib().nbc_get_pair_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def pairs(self):
+        nf, nb = self.pair_counts()
+        N = self.N
+        d = dict(foff=np.zeros(N + 1, np.int32), fj=np.zeros(max(nf, 1), np.int32), fflag=np.zeros(max(nf, 1), np.uint8),
+                 boff=np.zeros(N + 1, np.int32), bi=np.zeros(max(nb, 1), np.int32), bslot=np.zeros(max(nb, 1), np.int32))
+        self._c(lib().nbc_get_pairs(self.h, ptr(d["foff"]), ptr(d["fj"]), ptr(d["fflag"]), ptr(d["boff"]),
+                                    ptr(d["bi"]), ptr(d["bslot"])))
+        d["fj"], d["fflag"], d["bi"], d["bslot"] = d["fj"][:nf], d["fflag"][:nf], d["bi"][:nb], d["bslot"][:nb]
+        return d
+
+    def pair_history(self):
+        nf, _ = self.pair_counts()
+        h = np.zeros((max(nf, 1), 9))
+        self._c(lib().nbc_get_pair_history(self.h, ptr(h)))
+        return h[:nf]
+
+    def sphere_forces(self, kind=0):
+        f, t = np.zeros((self.N, 3)), np.zeros((self.N, 3))
+        self._c(lib().nbc_get_sphere_forces(self.h, kind, ptr(f), ptr(t)))
+        return f, t
+
+    def clumps(self):
+        Nc = self.Nc
+        d = dict(pos=np.zeros((Nc, 3)), vel=np.zeros((Nc, 3)), quat=np.zeros((Nc, 4)), angv=np.zeros((Nc, 3)),
+                 F=np.zeros((Nc, 3)), T=np.zeros((Nc, 3)))
+        self._c(lib().nbc_get_clumps(self.h, ptr(d["pos"]), ptr(d["vel"]), ptr(d["quat"]), ptr(d["angv"]),
+                                     ptr(d["F"]), ptr(d["T"])))
+        return d
+
+    def penetrator(self):
+        oi = max(self.params.OutputInterval, 1)
+        d = dict(pos=np.zeros(3), vel=np.zeros(3), quat=np.zeros(4), angv=np.zeros(3), cache_f=np.zeros((oi, 3)),
+                 cache_t=np.zeros((oi, 3)))
+        st = I32()
+        self._c(lib().nbc_get_penetrator(self.h, ptr(d["pos"]), ptr(d["vel"]), ptr(d["quat"]), ptr(d["angv"]),
+                                         ptr(d["cache_f"]), ptr(d["cache_t"]), C.byref(st)))
+        d["state"] = st.value
+        return d
+
+    def pene_exports(self):
+        a = [np.zeros((self.N, 3)) for _ in range(4)]
+        self._c(lib().nbc_get_pene_exports(self.h, *[ptr(x) for x in a]))
+        return a
+
+    def pair_exports(self):
+        nf, _ = self.pair_counts()
+        a, b = np.zeros((max(nf, 1), 3)), np.zeros((max(nf, 1), 3))
+        self._c(lib().nbc_get_pair_exports(self.h, ptr(a), ptr(b)))
+        return a[:nf], b[:nf]
+
+    def profile(self, on=True):
+        self._c(lib().nbc_profile_enable(self.h, int(on)))
+
+    def profile_get(self, cls):
+        ms, n = C.c_double(), C.c_int64()
+        self._c(lib().nbc_profile_get(self.h, cls, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,200 @@
+"""ctypes wrapper of the FP64 host oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker. Struct layouts are the
+ones of include/nbc.h (taken from the nbc binding module).
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "nbody-clump_amd"))
+import nbc  # noqa: E402  (struct definitions only)
+
+P = C.c_void_p
+I32 = C.c_int32
+_SIGS = {
+    "orc_finalize_params": ([C.POINTER(nbc.Params), P, I32, C.c_double], C.c_int),
+    "orc_create": ([C.POINTER(nbc.Params), C.POINTER(nbc.Options)], P),
+    "orc_destroy": ([P], None),
+    "orc_last_error": ([P], C.c_char_p),
+    "orc_set_threads": ([I32], None),
+    "orc_get_threads": ([], C.c_int),
+    "orc_upload": ([P, C.POINTER(nbc.SpheresIn), C.POINTER(nbc.ClumpsIn), C.POINTER(nbc.Wall),
+                    C.POINTER(nbc.Penetrator)], C.c_int),
+    "orc_step": ([P, I32], C.c_int),
+    "orc_rebuild": ([P], C.c_int),
+    "orc_get_spheres": ([P, P, P, P, P, P, P, P], C.c_int),
+    "orc_get_grid": ([P, P, P, P, P], C.c_int),
+    "orc_get_pair_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "orc_get_pairs": ([P, P, P, P, P, P, P], C.c_int),
+    "orc_get_pair_history": ([P, P], C.c_int),
+    "orc_get_sphere_forces": ([P, I32, P, P], C.c_int),
+    "orc_get_clumps": ([P, P, P, P, P, P, P], C.c_int),
+    "orc_get_penetrator": ([P, P, P, P, P, P, P, P], C.c_int),
+    "orc_get_pene_exports": ([P, P, P, P, P], C.c_int),
+    "orc_get_pair_exports": ([P, P, P], C.c_int),
+    "orc_get_stats": ([P, C.POINTER(nbc.Stats)], C.c_int),
+    "orc_bruteforce_pairs": ([P, P, P, C.c_longlong], C.c_longlong),
+}
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built ({LIB_PATH}); run make -C oracle")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n):
+    lib().orc_set_threads(n)
+
+
+def threads():
+    return lib().orc_get_threads()
+
+
+def finalize_params(bed, cell_size=0.0):
+    p = nbc.Params()
+    C.memmove(C.byref(p), C.byref(bed.params), C.sizeof(nbc.Params))
+    rc = lib().orc_finalize_params(C.byref(p), ptr(bed.radius), bed.n, cell_size)
+    assert rc == 0
+    return p
+
+
+class Oracle:
+    """Same getter surface as nbc.Context, computed by the FP64 restatement."""
+
+    def __init__(self, params, opt=None):
+        self.params = params
+        self.h = lib().orc_create(C.byref(params), C.byref(opt) if opt else None)
+        self.N = params.ParticleNum
+        self.Nc = params.ClumpNum
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        if rc != 0:
+            raise nbc.NbcError(rc, lib().orc_last_error(self.h).decode())
+        return rc
+
+    def upload(self, bed, wall=None, pene=None):
+        s, c = bed.spheres_in(), bed.clumps_in()
+        self._bed, self._pene = bed, pene
+        self._c(lib().orc_upload(self.h, C.byref(s), C.byref(c), C.byref(wall) if wall else None,
+                                 C.byref(pene) if pene else None))
+
+    def step(self, n=1):
+        self._c(lib().orc_step(self.h, n))
+
+    def rebuild(self):
+        self._c(lib().orc_rebuild(self.h))
+
+    def set_record(self, on=True):
+        pass
+
+    def stats(self):
+        s = nbc.Stats()
+        self._c(lib().orc_get_stats(self.h, C.byref(s)))
+        return s
+
+    def spheres(self):
+        N = self.N
+        d = dict(fid=np.zeros(N, np.int32), cid=np.zeros(N, np.int32), pos=np.zeros((N, 3)),
+                 cum=np.zeros((N, 3)), local=np.zeros((N, 3)), mass=np.zeros(N), radius=np.zeros(N))
+        self._c(lib().orc_get_spheres(self.h, ptr(d["fid"]), ptr(d["cid"]), ptr(d["pos"]), ptr(d["cum"]),
+                                      ptr(d["local"]), ptr(d["mass"]), ptr(d["radius"])))
+        return d
+
+    def grid(self):
+        N, CN = self.N, self.params.CellNum
+        d = dict(cell=np.zeros(N, np.int32), perm=np.zeros(N, np.int32), beg=np.zeros(CN, np.int32),
+                 end=np.zeros(CN, np.int32))
+        self._c(lib().orc_get_grid(self.h, ptr(d["cell"]), ptr(d["perm"]), ptr(d["beg"]), ptr(d["end"])))
+        return d
+
+    def pair_counts(self):
+        a, b = C.c_int64(), C.c_int64()
+        self._c(lib().orc_get_pair_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def pairs(self):
+        nf, nb = self.pair_counts()
+        N = self.N
+        d = dict(foff=np.zeros(N + 1, np.int32), fj=np.zeros(max(nf, 1), np.int32), fflag=np.zeros(max(nf, 1), np.uint8),
+                 boff=np.zeros(N + 1, np.int32), bi=np.zeros(max(nb, 1), np.int32), bslot=np.zeros(max(nb, 1), np.int32))
+        self._c(lib().orc_get_pairs(self.h, ptr(d["foff"]), ptr(d["fj"]), ptr(d["fflag"]), ptr(d["boff"]),
+                                    ptr(d["bi"]), ptr(d["bslot"])))
+        d["fj"], d["fflag"], d["bi"], d["bslot"] = d["fj"][:nf], d["fflag"][:nf], d["bi"][:nb], d["bslot"][:nb]
+        return d
+
+    def pair_history(self):
+        nf, _ = self.pair_counts()
+        h = np.zeros((max(nf, 1), 9))
+        self._c(lib().orc_get_pair_history(self.h, ptr(h)))
+        return h[:nf]
+
+    def sphere_forces(self, kind=0):
+        f, t = np.zeros((self.N, 3)), np.zeros((self.N, 3))
+        self._c(lib().orc_get_sphere_forces(self.h, kind, ptr(f), ptr(t)))
+        return f, t
+
+    def clumps(self):
+        Nc = self.Nc
+        d = dict(pos=np.zeros((Nc, 3)), vel=np.zeros((Nc, 3)), quat=np.zeros((Nc, 4)), angv=np.zeros((Nc, 3)),
+                 F=np.zeros((Nc, 3)), T=np.zeros((Nc, 3)))
+        self._c(lib().orc_get_clumps(self.h, ptr(d["pos"]), ptr(d["vel"]), ptr(d["quat"]), ptr(d["angv"]),
+                                     ptr(d["F"]), ptr(d["T"])))
+        return d
+
+    def penetrator(self):
+        oi = max(self.params.OutputInterval, 1)
+        d = dict(pos=np.zeros(3), vel=np.zeros(3), cache_f=np.zeros((oi, 3)), cache_t=np.zeros((oi, 3)),
+                 totf=np.zeros(3), tott=np.zeros(3))
+        st = I32()
+        self._c(lib().orc_get_penetrator(self.h, ptr(d["pos"]), ptr(d["vel"]), ptr(d["cache_f"]), ptr(d["cache_t"]),
+                                         C.byref(st), ptr(d["totf"]), ptr(d["tott"])))
+        d["state"] = st.value
+        return d
+
+    def pene_exports(self):
+        a = [np.zeros((self.N, 3)) for _ in range(4)]
+        self._c(lib().orc_get_pene_exports(self.h, *[ptr(x) for x in a]))
+        return a
+
+    def pair_exports(self):
+        nf, _ = self.pair_counts()
+        a, b = np.zeros((max(nf, 1), 3)), np.zeros((max(nf, 1), 3))
+        self._c(lib().orc_get_pair_exports(self.h, ptr(a), ptr(b)))
+        return a[:nf], b[:nf]
+
+    def bruteforce_pairs(self):
+        n = lib().orc_bruteforce_pairs(self.h, None, None, 0)
+        a, b = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        lib().orc_bruteforce_pairs(self.h, ptr(a), ptr(b), n)
+        return a[:n], b[:n]

@@ -1,0 +1,229 @@
+"""Drop-in boundary on the CPU: file formats of the reference, failure
+semantics of its loaders, the binary output layout, and the C ABI itself
+(every symbol of include/nbc.h is exported by libnbc.so). No GPU calls."""
+import ctypes as C
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+import nbc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "reference_inputs")
+ROOT = os.path.dirname(HERE)
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "nbc.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbc_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = C.CDLL(nbc.LIB_PATH)
+    names = header_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding knows all of them
+    assert set(names) <= set(nbc._SIGS), set(names) - set(nbc._SIGS)
+
+
+def load_params(path):
+    p = nbc.Params()
+    ini = C.create_string_buffer(256)
+    rc = nbc.lib().nbc_load_world_par(path.encode(), C.byref(p), ini, 256)
+    return rc, p, ini.value.decode()
+
+
+def test_world_par_fixture():
+    rc, p, ini = load_params(os.path.join(FIX, "world.par"))
+    assert rc == 0
+    assert (p.ParticleNum, p.ClumpNum, p.BranchLmtLen) == (576, 144, 256)
+    assert ini == "ph.00000000.bt"
+    assert (p.StepSize, p.EndStep, p.StartStep, p.OutputInterval) == (1e-5, 200000, 0, 10000)
+    assert (p.Ymod, p.Prat, p.Frcoe, p.NormRes, p.Beta, p.Gravity, p.Thresh) == (2e7, 0.2, 0.5, 0.2, 0.5, -9.81, 1e-3)
+    assert list(p.WorldSize) == [0.4, 0.4, 1.0] and list(p.BoxUpLmt) == [0.4, 0.4, 1.0]
+    assert (p.isIntruder, p.isWallForce, p.isSelfGravity, p.isQuaternion) == (1, 1, 0, 1)
+    ln = np.log(0.2)
+    assert p.Damping == pytest.approx(2 * np.sqrt(5 / 6) * ln / np.sqrt(ln * ln + 3.14159265359 ** 2), rel=1e-15)
+    assert p.Damping < 0
+
+
+def test_world_par_failures(tmp_path):
+    src = open(os.path.join(FIX, "world.par")).read()
+    bad = tmp_path / "missing.par"
+    bad.write_text(src.replace("Cohesion    = 0.0", ""))
+    assert load_params(str(bad))[0] == nbc.NBC_E_PARAM
+    assert "missing" in nbc.lib().nbc_last_error(None).decode()
+    bad.write_text(src + "\nUnknownKey = 3\n")
+    assert load_params(str(bad))[0] == nbc.NBC_E_PARAM
+    assert "Unrecoganized item UnknownKey" in nbc.lib().nbc_last_error(None).decode()
+    assert load_params(str(tmp_path / "nope.par"))[0] == nbc.NBC_E_IO
+
+
+def test_mesh_size_matches_reference_rule():
+    rc, p, _ = load_params(os.path.join(FIX, "world.par"))
+    r = np.full(576, 0.005)
+    assert nbc.lib().nbc_finalize_params(C.byref(p), nbc.ptr(r), 576, None) == 0
+    assert list(p.CellSize) == [0.015] * 3
+    assert list(p.WorldOrigin) == [-0.015] * 3
+    assert p.WorldSize[2] == pytest.approx(1.03)
+    assert list(p.GridDim) == [int(np.ceil(0.43 / 0.015))] * 2 + [int(np.ceil(1.03 / 0.015))]
+    assert p.CellNum == p.GridDim[0] * p.GridDim[1] * p.GridDim[2]
+
+
+def test_phase_and_clump_fixtures():
+    n, nc = 576, 144
+    cid = np.zeros(n, np.int32)
+    pos, loc, m, r = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    rc = nbc.lib().nbc_load_phase(os.path.join(FIX, "ph.00000000.bt").encode(), n, nbc.ptr(cid), nbc.ptr(pos),
+                                  nbc.ptr(loc), nbc.ptr(m), nbc.ptr(r))
+    assert rc == 0
+    raw = np.loadtxt(os.path.join(FIX, "ph.00000000.bt"))
+    assert np.array_equal(cid, raw[:, 2].astype(int))
+    assert np.array_equal(pos, raw[:, 3:6]) and np.array_equal(r, raw[:, 10])
+    cp, cv, cq, cw, pr = (np.zeros((nc, 3)), np.zeros((nc, 3)), np.zeros((nc, 4)), np.zeros((nc, 3)), np.zeros((nc, 4)))
+    rc = nbc.lib().nbc_load_clumps(os.path.join(FIX, "ch.00000000.bt").encode(), nc, *[nbc.ptr(a) for a in (cp, cv, cq, cw, pr)])
+    assert rc == 0
+    rawc = np.loadtxt(os.path.join(FIX, "ch.00000000.bt"))
+    assert np.array_equal(cq, rawc[:, 7:11]) and np.array_equal(pr, rawc[:, 14:18])
+    # every sphere's file position equals its clump pose applied to its local offset
+    from util import quat_to_matrix
+    for i in range(0, n, 37):
+        c = cid[i]
+        x = cp[c] + quat_to_matrix(cq[c]) @ loc[i]
+        assert np.allclose(x, pos[i], atol=1e-8)
+
+
+def test_rocks_fixtures(tmp_path):
+    w = nbc.Wall()
+    assert nbc.lib().nbc_load_cube_wall(os.path.join(FIX, "Rocks", "CubeWall.bt").encode(), C.byref(w)) == 0
+    assert list(w.side) == [0.2, 0.2, 0.5] and list(w.pos) == [0.2, 0.2, 0.5]
+    c = nbc.Wall()
+    assert nbc.lib().nbc_load_cylinder_wall(os.path.join(FIX, "Rocks", "CylindWall.bt").encode(), C.byref(c)) == 0
+    assert (c.radius, c.height, list(c.dir)) == (0.05, 0.5, [0.0, 0.0, 1.0])
+    p = nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(os.path.join(FIX, "Rocks", "Penetrator.bt").encode(), C.byref(p)) == 0
+    assert (p.shape, p.motion, p.radius, p.height) == (nbc.NBC_PENE_SPHERE, nbc.NBC_PENE_CONSTANT_VELOCITY, 0.02, 0.02)
+    assert list(p.vel) == [0.0, 0.0, -0.05]
+    g = nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(os.path.join(FIX, "Rocks", "Penetrator_gravity.bt").encode(), C.byref(g)) == 0
+    assert g.motion == nbc.NBC_PENE_GRAVITY and g.wait_for_contact == 0
+    f = tmp_path / "p.bt"
+    f.write_text("cylinder 0.01 0.05 0 0 1\n1 1 1 2\n0 0 0\n0 0 0\n0 0 0 1\n0 0 0\nmotion gravity_after_contact\n")
+    q = nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(str(f).encode(), C.byref(q)) == 0
+    assert (q.shape, q.motion, q.wait_for_contact, q.prop[3]) == (nbc.NBC_PENE_CYLINDER, nbc.NBC_PENE_GRAVITY, 1, 2.0)
+    f.write_text("torus 0.01\n")
+    assert nbc.lib().nbc_load_penetrator(str(f).encode(), C.byref(q)) == nbc.NBC_E_BADINPUT
+    f.write_text("sphere -1\n1 1 1 1\n0 0 0\n0 0 0\n0 0 0 1\n0 0 0\n")
+    assert nbc.lib().nbc_load_penetrator(str(f).encode(), C.byref(q)) == nbc.NBC_E_BADINPUT
+    f.write_text("sphere 0.01\n1 1 1 1\n0 0 0\n0 0 0\n0 0 0 1\n0 0 0\nmotion teleport\n")
+    assert nbc.lib().nbc_load_penetrator(str(f).encode(), C.byref(q)) == nbc.NBC_E_BADINPUT
+    f.write_text("cone 0.01 0.0173 32\n1 1 1 1\n0 0 0.1\n0 0 -0.1\n0 0 0 1\n0 0 0\n")
+    k = nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(str(f).encode(), C.byref(k)) == 0
+    assert k.shape == nbc.NBC_PENE_CONE and k.ntri == 64
+    tri = np.ctypeslib.as_array(k.tri, shape=(64, 3, 3))
+    assert np.isclose(tri[:, :, 2].min(), -0.0173) and np.isclose(np.hypot(tri[:, :, 0], tri[:, :, 1]).max(), 0.01)
+    nbc.lib().nbc_free_mesh(C.byref(k))
+
+
+def test_binary_writers_layout(tmp_path):
+    n = 5
+    sid = np.arange(n, dtype=np.int32)
+    fid = np.array([3, 1, 4, 0, 2], np.int32)
+    cid = np.array([0, 0, 1, 1, 1], np.int32)
+    pos = np.random.default_rng(0).random((n, 3))
+    loc = np.random.default_rng(1).random((n, 3))
+    m, r = np.arange(n) + 0.5, np.arange(n) + 0.25
+    d = str(tmp_path / "OutPart")
+    assert nbc.lib().nbc_write_particles_bin(d.encode(), 100, n, *[nbc.ptr(a) for a in (sid, fid, cid, pos, loc, m, r)]) == 0
+    raw = open(os.path.join(d, "ph.00000100.bin"), "rb").read()
+    assert len(raw) == 4 + 3 * 4 * n + 2 * 24 * n + 2 * 8 * n
+    assert np.frombuffer(raw[:4], np.int32)[0] == n
+    ints = np.frombuffer(raw[4:4 + 12 * n], np.int32).reshape(3, n)
+    assert np.array_equal(ints[1], fid) and np.array_equal(ints[2], cid)
+    dbl = np.frombuffer(raw[4 + 12 * n:], np.float64)
+    assert np.array_equal(dbl[:3 * n].reshape(n, 3), pos) and np.array_equal(dbl[-n:], r)
+    # clumps with the penetrator first (id -1, zero property)
+    nc = 2
+    cp, cv, cw = np.ones((nc, 3)), 2 * np.ones((nc, 3)), 3 * np.ones((nc, 3))
+    cq, pr = 4 * np.ones((nc, 4)), 5 * np.ones((nc, 4))
+    pp, pv, pq, pw = np.full(3, 6.0), np.full(3, 7.0), np.full(4, 8.0), np.full(3, 9.0)
+    d2 = str(tmp_path / "OutClump")
+    assert nbc.lib().nbc_write_clumps_bin(d2.encode(), 7, nc, *[nbc.ptr(a) for a in (cp, cv, cq, cw, pr, pp, pv, pq, pw)]) == 0
+    raw = open(os.path.join(d2, "ch.00000007.bin"), "rb").read()
+    hdr = np.frombuffer(raw[:16], np.int32)
+    assert list(hdr) == [3, -1, 0, 1]
+    body = np.frombuffer(raw[16:], np.float64)
+    assert len(body) == 3 * 3 + 3 * 3 + 3 * 4 + 3 * 3 + 3 * 4
+    assert list(body[:3]) == [6.0] * 3 and list(body[3:9]) == [1.0] * 6
+    assert list(body[-12:-8]) == [0.0] * 4
+    # force history: every 100th entry of the interval
+    oi = 250
+    f = np.arange(3 * oi, dtype=float).reshape(oi, 3)
+    d3 = str(tmp_path / "OutForce")
+    assert nbc.lib().nbc_write_force_history_bin(d3.encode(), 250, oi, nbc.ptr(f), nbc.ptr(-f)) == 0
+    raw = open(os.path.join(d3, "fc.00000250.bin"), "rb").read()
+    assert np.frombuffer(raw[:4], np.int32)[0] == 3
+    assert list(np.frombuffer(raw[4:16], np.int32)) == [0, 100, 200]
+    vals = np.frombuffer(raw[16:], np.float64).reshape(2, 3, 3)
+    assert np.array_equal(vals[0], f[[0, 100, 200]]) and np.array_equal(vals[1], -f[[0, 100, 200]])
+    # yi: only spheres with |Fn|^2 > 1e-28
+    fn = np.zeros((n, 3))
+    fn[1] = [0, 0, 1e-15]
+    fn[3] = [1.0, 0, 0]
+    assert nbc.lib().nbc_write_yi_bin(d3.encode(), 250, n, nbc.ptr(cid), nbc.ptr(fn), nbc.ptr(fn), nbc.ptr(fn), nbc.ptr(fn)) == 0
+    raw = open(os.path.join(d3, "yi.00000250.bin"), "rb").read()
+    assert np.frombuffer(raw[:4], np.int32)[0] == 1 and np.frombuffer(raw[4:8], np.int32)[0] == cid[3]
+    # contacts
+    B = 4
+    nei = np.arange(n * B, dtype=np.int32)
+    fc = np.ones((n * B, 3))
+    d4 = str(tmp_path / "OutCnt")
+    assert nbc.lib().nbc_write_contacts_bin(d4.encode(), 20, n, B, nbc.ptr(nei), nbc.ptr(fc), nbc.ptr(2 * fc)) == 0
+    for pre, size in (("Nei", 8 + 4 * n * B), ("Fcn", 8 + 24 * n * B), ("Cpt", 8 + 24 * n * B)):
+        raw = open(os.path.join(d4, f"{pre}.00000020.bin"), "rb").read()
+        assert len(raw) == size and list(np.frombuffer(raw[:8], np.int32)) == [n, B]
+
+
+def test_generator_is_deterministic_and_fp32_exact():
+    a = nbc.generate_bed(1, 200, seed=42)
+    b = nbc.generate_bed(1, 200, seed=42)
+    c = nbc.generate_bed(1, 200, seed=43)
+    assert np.array_equal(a.pos, b.pos) and not np.array_equal(a.pos, c.pos)
+    for arr in (a.local, a.mass, a.radius):
+        assert np.array_equal(arr.astype(np.float32).astype(np.float64), arr)
+    assert a.radius.min() >= 0.6e-3 and a.radius.max() <= 1.6e-3
+    # sub-sphere mass is an equal share of the clump mass
+    for k in range(0, a.nc, 17):
+        m = a.mass[a.clump_id == k]
+        assert np.allclose(m.sum(), a.cprop[k, 3], rtol=1e-6)
+    # deposited bed: no overlaps beyond rounding between different clumps
+    from scipy.spatial import cKDTree
+    t = cKDTree(a.pos)
+    for i, j in t.query_pairs(2 * a.radius.max()):
+        if a.clump_id[i] != a.clump_id[j]:
+            assert np.linalg.norm(a.pos[i] - a.pos[j]) >= a.radius[i] + a.radius[j] - 1e-9
+    # inside the cylinder
+    assert np.all(np.hypot(a.pos[:, 0], a.pos[:, 1]) + a.radius <= a.wall.radius + 1e-9)
+
+
+def test_save_bed_text_roundtrip(tmp_path):
+    a = nbc.generate_bed(0, 40, seed=3)
+    out = nbc.BedOut()
+    spec = nbc.BedSpec(0, 40, 3, nbc.D4(0, 0, 0, 0))
+    assert nbc.lib().nbc_generate_bed(C.byref(spec), C.byref(out)) == 0
+    assert nbc.lib().nbc_save_bed_text(str(tmp_path).encode(), C.byref(out), None) == 0
+    nbc.lib().nbc_free_bed(C.byref(out))
+    rc, p, ini = load_params(str(tmp_path / "world.par"))
+    assert rc == 0 and p.ParticleNum == a.n and p.ClumpNum == a.nc
+    cid = np.zeros(a.n, np.int32)
+    pos, loc, m, r = np.zeros((a.n, 3)), np.zeros((a.n, 3)), np.zeros(a.n), np.zeros(a.n)
+    assert nbc.lib().nbc_load_phase(str(tmp_path / ini).encode(), a.n, *[nbc.ptr(x) for x in (cid, pos, loc, m, r)]) == 0
+    assert np.array_equal(pos, a.pos) and np.array_equal(r, a.radius) and np.array_equal(cid, a.clump_id)
