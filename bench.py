@@ -59,9 +59,11 @@ def cpu_baseline(bed, sample_steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
+    import nbc
+
     orc.set_threads(os.cpu_count() or 1)
     p = orc.finalize_params(bed)
-    o = orc.Oracle(p)
+    o = orc.Oracle(p, nbc.make_options(wall_type=nbc.NBC_WALL_CYLINDER))
     o.upload(bed, bed.wall)
     o.step(1)  # first step builds the lists
     t0 = time.perf_counter()
@@ -94,7 +96,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     bed = nbc.generate_bed(1, a.nclumps, seed=1)
-    bed.params.BranchLmtLen = 64
+    bed.params.BranchLmtLen = 128
     print(f"[bench] bed: {bed.n} spheres, {bed.nc} clumps", file=sys.stderr, flush=True)
     mixed = run_form(bed, nbc.NBC_MIXED, a.steps, a.warmup)
     print(f"[bench] mixed: {mixed}", file=sys.stderr, flush=True)

@@ -106,7 +106,7 @@ typedef struct nbc_options {
     int32_t block_size;      /* threads per block for per-sphere kernels (64..1024) */
     int32_t radix_bits;      /* LSD radix digit width for the cell sort (1..8) */
     int32_t pair_capacity;   /* front-pair slots reserved per sphere (average) */
-    int32_t reserved0;
+    int32_t batch_steps;     /* steps enqueued per host round trip (default 64) */
     double pene_frcoe;       /* penetrator friction; <= 0 means Frcoe          */
     double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
     double reserved[4];

@@ -100,6 +100,7 @@ struct DevParams {
     // penetrator (static part)
     int pshape, pmotion, pwait, ntri;
     double prad, pheight, pdir[3], pmass;
+    double pbound;        // radius of a sphere about the penetrator origin enclosing the mesh
     const double* ptri;   // 9*ntri, penetrator frame
 };
 

@@ -91,6 +91,7 @@ class Engine : public EngineBase {
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
         nfaces = wall_type == NBC_WALL_CUBE ? 6 : (wall_type == NBC_WALL_CYLINDER ? 3 : 0);
         pair_cap_per = O.pair_capacity > 0 ? O.pair_capacity : 16;
+        batch = O.batch_steps > 0 ? O.batch_steps : 64;
         // device parameters
         std::memset(&dp, 0, sizeof(dp));
         dp.N = N; dp.Nc = Nc; dp.B = P.BranchLmtLen; dp.CellNum = P.CellNum;
@@ -284,10 +285,9 @@ class Engine : public EngineBase {
         KC(launch_fill_i32(st, cend, P.CellNum, -1));
         KC(launch_fill_i32(st, mbeg, Nc, -1));
         KC(launch_fill_i32(st, mend, Nc, -1));
-        HIPC(hipMemset(dstatus, 0, 4 * sizeof(uint32_t)));
         {
-            const int neg = 0x7fffffff;
-            HIPC(hipMemcpy(dstatus + 1, &neg, sizeof(int), hipMemcpyHostToDevice));
+            const int init[4] = {HALT_NONE, HALT_NONE, 0, 0};
+            HIPC(hipMemcpy(dstatus, init, sizeof(init), hipMemcpyHostToDevice));
         }
         HIPC(hipMemset(counters, 0, 2 * sizeof(unsigned long long)));
         // penetrator
@@ -310,6 +310,12 @@ class Engine : public EngineBase {
                 HIPC(hipMemcpy(dtri, pe->tri, sizeof(double) * 9 * pe->ntri, hipMemcpyHostToDevice));
                 dp.ntri = pe->ntri;
                 dp.ptri = dtri;
+                double b2 = 0.0;
+                for (int k = 0; k < 3 * pe->ntri; ++k) {
+                    const double* v = pe->tri + 3 * k;
+                    b2 = std::max(b2, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                }
+                dp.pbound = std::sqrt(b2) * (1.0 + 1e-12);
             }
         }
         HIPC(hipMemcpy(pst, &ps, sizeof(ps), hipMemcpyHostToDevice));
@@ -396,7 +402,7 @@ class Engine : public EngineBase {
         HIPC(hipMemcpyAsync(mem, cv, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
         KC(launch_seg_bounds(st, ck, N, Nc, mbeg, mend));
         nrebuild++;
-        prof_end(3, e0);
+        prof_end(3, e0, -1);
         return 0;
     }
 
@@ -415,10 +421,11 @@ class Engine : public EngineBase {
         return 0;
     }
 
-    int check_status(bool after_step) {
-        (void)after_step;
+    static constexpr int HALT_NONE = 0x7fffffff;
+
+    int check_status() {
         const int bad = (int)hstatus[1];
-        if (bad != 0x7fffffff) {
+        if (bad != HALT_NONE) {
             char b[200];
             std::snprintf(b, sizeof b,
                           "FATAL: Invalid input or nonfinite wall force/torque at sorted particle index %d. "
@@ -428,60 +435,106 @@ class Engine : public EngineBase {
         return 0;
     }
 
+    int read_status() {
+        HIPC(hipMemcpyAsync(hstatus, dstatus, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        return 0;
+    }
+
+    // K1 + K2 of step n (src/dem_simulation_clump.cu:113-119)
+    int enqueue_head(long long n) {
+        hipEvent_t e0 = prof_begin();
+        KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (const int*)dstatus, (int)n));
+        KC(launch_sphere_pose<R>(st, dp, S[scur], C, (int*)dstatus, (int)n, block));
+        prof_end(0, e0, n);
+        return 0;
+    }
+
+    // K3, penetrator sum, K4 of step n (src/dem_simulation_clump.cu:178-225)
+    int enqueue_tail(long long n, bool record) {
+        const int p = (int)(n & 1);
+        if (record) {
+            int e = ensure_record();
+            if (e) return e;
+            KC(launch_fill_f64(st, (double*)rec.pene, 16 * (long long)N, 0.0));
+            KC(launch_fill_f64(st, (double*)rec.pair_fcn, 4 * (long long)rec_cap, 0.0));
+            KC(launch_fill_f64(st, (double*)rec.pair_cpt, 4 * (long long)rec_cap, 0.0));
+        }
+        ListBuf<R> L;
+        L.foff = Lf[lcur].foff; L.fword = Lf[lcur].fword; L.boff = boff; L.bent = bent;
+        L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
+        const int* halt = (const int*)dstatus;
+        hipEvent_t e0 = prof_begin();
+        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, pene_partial, counters, (int*)(dstatus + 1), rec, record,
+                              block, 1, halt, (int)n));
+        prof_end(1, e0, n);
+        if (dp.pene_on) {
+            e0 = prof_begin();
+            KC(launch_pene_finalize(st, dp, pst, pene_partial, nbk, (int)(n % oi), cache_f, cache_t, halt, (int)n));
+            prof_end(4, e0, n);
+        }
+        e0 = prof_begin();
+        KC(launch_clump_after<R>(st, dp, S[scur], C, mbeg, mend, mem, block, halt, (int)n));
+        prof_end(2, e0, n);
+        return 0;
+    }
+
+    int clear_halt() {
+        KC(launch_fill_i32(st, (int*)dstatus, 1, HALT_NONE));
+        return 0;
+    }
+
+    // Steps are enqueued in batches without host round trips. When a step
+    // needs a rebuild, the device halt step stops the rest of the batch at
+    // that step's contact kernel; the host then rebuilds and replays from
+    // there, so rebuilds happen at exactly the reference's steps.
     int step(int nsteps) override {
         if (!uploaded) return fail(NBC_E_BADINPUT, "nbc_step: no state uploaded");
         HIPC(hipSetDevice(device));
-        for (int k = 0; k < nsteps; ++k) {
-            const int p = (int)(nstep & 1);
-            hipEvent_t e0 = prof_begin();
-            KC(launch_clump_before<R>(st, dp, C, pst, counters, block));
-            KC(launch_sphere_pose<R>(st, dp, S[scur], C, dstatus, block));
-            prof_end(0, e0);
-            HIPC(hipMemcpyAsync(hstatus, dstatus, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            HIPC(hipStreamSynchronize(st));
-            int e = check_status(false);
-            if (e) return e;
-            const bool rb = first || init_update || hstatus[0] != 0;
-            if (rb) {
-                e = rebuild(p);
-                if (e) return e;
-                init_update = false;
-                KC(launch_fill_u32(st, dstatus, 1, 0u));
+        long long n = nstep;
+        const long long end = nstep + nsteps;
+        const bool rec = record_next;
+        int e;
+        while (n < end) {
+            if (first || init_update) {
+                // the first step always rebuilds (src/dem_simulation_clump.cu:124)
+                if ((e = enqueue_head(n))) return e;
+                if ((e = rebuild((int)(n & 1)))) return e;
+                if ((e = clear_halt())) return e;
+                first = init_update = false;
+                if ((e = enqueue_tail(n, rec && n == end - 1))) return e;
+                ++n;
+                continue;
             }
-            first = false;
-            const bool record = record_next;
-            if (record) {
-                e = ensure_record();
-                if (e) return e;
-                KC(launch_fill_f64(st, (double*)rec.pene, 16 * (long long)N, 0.0));
-                KC(launch_fill_f64(st, (double*)rec.pair_fcn, 4 * (long long)rec_cap, 0.0));
-                KC(launch_fill_f64(st, (double*)rec.pair_cpt, 4 * (long long)rec_cap, 0.0));
+            const long long m = std::min<long long>(batch, end - n);
+            for (long long k = 0; k < m; ++k) {
+                if ((e = enqueue_head(n + k))) return e;
+                if ((e = enqueue_tail(n + k, rec && n + k == end - 1))) return e;
             }
-            ListBuf<R> L;
-            L.foff = Lf[lcur].foff; L.fword = Lf[lcur].fword; L.boff = boff; L.bent = bent;
-            L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
-            e0 = prof_begin();
-            KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, pene_partial, counters, (int*)(dstatus + 1), rec,
-                                  record, block, 1));
-            prof_end(1, e0);
-            if (dp.pene_on) {
-                e0 = prof_begin();
-                KC(launch_pene_finalize(st, dp, pst, pene_partial, nbk, count, cache_f, cache_t));
-                prof_end(4, e0);
+            if ((e = read_status())) return e;
+            if ((e = check_status())) return e;
+            const int h = (int)hstatus[0];
+            if (h == HALT_NONE) {
+                n += m;
+                continue;
             }
-            e0 = prof_begin();
-            KC(launch_clump_after<R>(st, dp, S[scur], C, mbeg, mend, mem, block));
-            prof_end(2, e0);
-            record_last = record;
-            if (record) record_next = false;
-            count++;
-            if (count == oi) count = 0;
-            nstep++;
+            // steps n .. h-1 completed; step h stopped after k_sphere_pose
+            prof_drop(h);
+            if ((e = rebuild(h & 1))) return e;
+            if ((e = clear_halt())) return e;
+            if ((e = enqueue_tail(h, rec && h == end - 1))) return e;
+            n = (long long)h + 1;
         }
-        HIPC(hipMemcpyAsync(hstatus, dstatus, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPC(hipStreamSynchronize(st));
+        nstep = end;
+        if ((e = read_status())) return e;
         prof_collect();
-        return check_status(true);
+        if (rec && nsteps > 0) {
+            record_last = true;
+            record_next = false;
+        } else if (nsteps > 0) {
+            record_last = false;
+        }
+        return check_status();
     }
 
     int force_rebuild() override {
@@ -491,7 +544,7 @@ class Engine : public EngineBase {
         if (e) return e;
         first = false;
         init_update = false;
-        KC(launch_fill_u32(st, dstatus, 1, 0u));
+        if ((e = clear_halt())) return e;
         HIPC(hipStreamSynchronize(st));
         prof_collect();
         return 0;
@@ -505,11 +558,21 @@ class Engine : public EngineBase {
         (void)hipEventRecord(e, st);
         return e;
     }
-    void prof_end(int cls, hipEvent_t b) {
+    void prof_end(int cls, hipEvent_t b, long long step = -1) {
         if (!prof || !b) return;
         hipEvent_t e = get_event();
         (void)hipEventRecord(e, st);
-        pending.push_back({cls, b, e});
+        pending.push_back({cls, step, b, e});
+    }
+    // forget the timings of launches that stood down after a halt at step h
+    void prof_drop(long long h) {
+        std::vector<Pending> keep;
+        for (auto& q : pending) {
+            const bool skipped = q.step >= 0 && ((q.cls == 0 && q.step > h) || (q.cls != 0 && q.cls != 3 && q.step >= h));
+            if (skipped) { ev_free.push_back(q.b); ev_free.push_back(q.e); }
+            else keep.push_back(q);
+        }
+        pending.swap(keep);
     }
     hipEvent_t get_event() {
         if (ev_free.empty()) {
@@ -785,13 +848,14 @@ class Engine : public EngineBase {
     };
     struct Pending {
         int cls;
+        long long step;
         hipEvent_t b, e;
     };
     nbc_params P;
     nbc_options O;
     DevParams dp;
     hipStream_t st = nullptr;
-    int N = 0, Nc = 0, block = 256, rbits = 8, wall_type = 0, nfaces = 0, nwh = 0, oi = 1, nbk = 1;
+    int N = 0, Nc = 0, block = 256, rbits = 8, wall_type = 0, nfaces = 0, nwh = 0, oi = 1, nbk = 1, batch = 64;
     int pair_cap_per = 16;
     SphereBuf<R> S[2] = {};
     int scur = 0;
@@ -805,7 +869,7 @@ class Engine : public EngineBase {
     int *val0 = nullptr, *val1 = nullptr, *hist = nullptr, *hscan = nullptr, *stmp = nullptr, *stotal = nullptr;
     int *perm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
     int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;
-    uint32_t* dstatus = nullptr;  // [0] rebuild flag, [1] smallest bad sphere, [2] overflow
+    uint32_t* dstatus = nullptr;  // [0] halt step, [1] smallest bad sphere, [2] overflow
     uint32_t* hstatus = nullptr;
     unsigned long long* counters = nullptr;
     PeneState* pst = nullptr;

@@ -71,14 +71,18 @@ int launch_back_resolve(hipStream_t st, int N, const int* foff, const uint32_t* 
                         int block);
 int launch_kinetic(hipStream_t st, int Nc, const D4* vw, const D4* prop, double* partial);
 int launch_pene_finalize(hipStream_t st, const DevParams& P, PeneState* pst, const double* partial, int nb, int count,
-                         double* cf, double* ct);
+                         double* cf, double* ct, const int* halt, int step);
 int contacts_blocks(int N, int block);
 
+// Step kernels take their absolute step index and the device "halt step":
+// k_sphere_pose lowers it to the step that needs a rebuild; every later
+// kernel of that step, and every kernel of later steps, returns at entry.
 template <class R>
 int launch_clump_before(hipStream_t st, const DevParams& P, ClumpBuf<R> C, PeneState* pst, unsigned long long* ctr,
-                        int block);
+                        int block, const int* halt, int step);
 template <class R>
-int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, unsigned* flag, int block);
+int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int step,
+                       int block);
 template <class R>
 int launch_gather(hipStream_t st, int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst);
 template <class R>
@@ -87,9 +91,9 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
                     PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                    int block, int remap);
+                    int block, int remap, const int* halt, int step);
 template <class R>
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
-                       const int* mend, const int* mem, int block);
+                       const int* mend, const int* mem, int block, const int* halt, int step);
 
 }  // namespace nbc

@@ -71,7 +71,9 @@ __device__ __forceinline__ bool finite3(const V3<double>& v) { return isfinite(v
 // advances the penetrator (src/dem_integrate_penetrator.cu:3-41) and clears
 // the step counters.
 template <class R>
-__global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsigned long long* counters) {
+__global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsigned long long* counters,
+                               const int* halt, int n) {
+    if (*halt < n) return;  // a previous step asked for a rebuild: replayed later
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         counters[0] = 0ull;
@@ -152,7 +154,8 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
 // ---------------------------------------------------------------------------
 // K2: sphere pose from the clump pose, cumulative displacement, rebuild flag.
 template <class R>
-__global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, unsigned* flag) {
+__global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int n) {
+    if (*halt < n) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false;
     if (i < P.N) {
@@ -173,8 +176,10 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, unsign
         S.cum[i] = cn;
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
     }
+    // the rebuild request (src/dem_clump.cu:132-134) records the step index:
+    // later kernels of this step and all kernels of later steps stand down
     const uint64_t b = __ballot(over);
-    if (b != 0ull && lane_id() == (int)__ffsll((unsigned long long)b) - 1) atomicOr(flag, 1u);
+    if (b != 0ull && lane_id() == (int)__ffsll((unsigned long long)b) - 1) atomicMin(halt, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -706,8 +711,10 @@ __device__ __forceinline__ V3<double> closest_on_tri(const V3<double>& p, const 
 template <class R, bool RECORD>
 __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
                                                   PeneState* pst, double* pene_partial,
-                                                  unsigned long long* counters, int* bad, RecordBuf rec, int remap) {
+                                                  unsigned long long* counters, int* bad, RecordBuf rec, int remap,
+                                                  const int* halt, int n) {
     typedef typename Vec4T<R>::type R4;
+    if (*halt <= n) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int i = lb * blockDim.x + threadIdx.x;
     const bool active = i < P.N;
@@ -927,6 +934,9 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                         if (P.pwait) atomicCAS(&pst->contact_state, 0, 1);
                     }
                 } else if (P.pshape == NBC_PENE_CONE) {
+                    // spheres outside the mesh's bounding sphere cannot touch it
+                    const double reach = P.pbound + ri;
+                    if (Dot<double>(RV, RV) < reach * reach) {
                     double best = 1e300;
                     V3<double> bq = mk<double>(0.0, 0.0, 0.0);
                     for (int t = 0; t < P.ntri; ++t) {
@@ -942,6 +952,7 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                     const double dl = sqrt(DistS);
                     if (P.ntri > 0 && ri - dl > 0.0) {
                         touch = true; RVo = bq; Dist = RVn; DistL = dl; Dn = ri - dl;
+                    }
                     }
                 } else {
                     // src/dem_contact_penetrator.cu:42-83
@@ -1069,7 +1080,9 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
 // Penetrator reaction: FP64 sum of the per-block partials in block order, then
 // src/dem_integrate_penetrator.cu:79-115.
 __global__ __launch_bounds__(256) void k_pene_finalize(DevParams P, PeneState* pst, const double* partial, int nb,
-                                                       int count, double* cache_f, double* cache_t) {
+                                                       int count, double* cache_f, double* cache_t, const int* halt,
+                                                       int n) {
+    if (*halt <= n) return;
     __shared__ double red[6][256];
     double v[6] = {0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nb; b += 256)
@@ -1103,7 +1116,8 @@ __global__ __launch_bounds__(256) void k_pene_finalize(DevParams P, PeneState* p
 // (FP64 accumulation), then the second half-kick.
 template <class R>
 __global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg, const int* mend,
-                              const int* mem) {
+                              const int* mem, const int* halt, int n) {
+    if (*halt <= n) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.Nc) return;
     const int b = mbeg[c], e = mend[c];
@@ -1284,23 +1298,24 @@ int launch_kinetic(hipStream_t st, int Nc, const D4* vw, const D4* prop, double*
 }
 
 int launch_pene_finalize(hipStream_t st, const DevParams& P, PeneState* pst, const double* partial, int nb, int count,
-                         double* cf, double* ct) {
-    k_pene_finalize<<<1, 256, 0, st>>>(P, pst, partial, nb, count, cf, ct);
+                         double* cf, double* ct, const int* halt, int n) {
+    k_pene_finalize<<<1, 256, 0, st>>>(P, pst, partial, nb, count, cf, ct, halt, n);
     LAUNCH_CHECK();
     return 0;
 }
 
 template <class R>
 int launch_clump_before(hipStream_t st, const DevParams& P, ClumpBuf<R> C, PeneState* pst, unsigned long long* ctr,
-                        int block) {
+                        int block, const int* halt, int step) {
     const int n = P.Nc > 1 ? P.Nc : 1;
-    k_clump_before<R><<<nblk(n, block), block, 0, st>>>(P, C, pst, ctr);
+    k_clump_before<R><<<nblk(n, block), block, 0, st>>>(P, C, pst, ctr, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
 template <class R>
-int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, unsigned* flag, int block) {
-    k_sphere_pose<R><<<nblk(P.N, block), block, 0, st>>>(P, S, C, flag);
+int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int step,
+                       int block) {
+    k_sphere_pose<R><<<nblk(P.N, block), block, 0, st>>>(P, S, C, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1321,34 +1336,35 @@ int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
                     PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                    int block, int remap) {
+                    int block, int remap, const int* halt, int step) {
     const unsigned nb = nblk(P.N, block);
     if (record)
-        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap);
+        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap, halt, step);
     else
-        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap);
+        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
 template <class R>
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
-                       const int* mend, const int* mem, int block) {
-    k_clump_after<R><<<nblk(P.Nc, block), block, 0, st>>>(P, S, C, mbeg, mend, mem);
+                       const int* mend, const int* mem, int block, const int* halt, int step) {
+    k_clump_after<R><<<nblk(P.Nc, block), block, 0, st>>>(P, S, C, mbeg, mend, mem, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
 
 #define INST(R)                                                                                                      \
     template int launch_clump_before<R>(hipStream_t, const DevParams&, ClumpBuf<R>, PeneState*, unsigned long long*, \
-                                        int);                                                                        \
-    template int launch_sphere_pose<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, unsigned*, int);    \
+                                        int, const int*, int);                                                       \
+    template int launch_sphere_pose<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, int*, int, int);    \
     template int launch_gather<R>(hipStream_t, int, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
     template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
-                                    PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, int, int);      \
+                                    PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, int, int,       \
+                                    const int*, int);                                                                \
     template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, const int*,         \
-                                       const int*, const int*, int);
+                                       const int*, const int*, int, const int*, int);
 INST(double)
 INST(float)
 

@@ -165,6 +165,9 @@ void rand_quat(std::mt19937_64& g, double q[4]) {
 
 struct Placed { double z, x, y, r; };
 
+// drop points tried per clump in the deposition (the lowest landing wins)
+constexpr int kDropTries = 8;
+
 }  // namespace
 
 extern "C" void nbc_free_bed(nbc_bed_out* o) {
@@ -324,24 +327,33 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
                 rxy = std::max(rxy, std::sqrt(off[s].x * off[s].x + off[s].y * off[s].y) + t.r[s]);
             }
             const double rr = Rc - rxy - 1e-6;
-            double x, y;
-            do { x = (2 * U(g) - 1) * rr; y = (2 * U(g) - 1) * rr; } while (x * x + y * y > rr * rr);
-            double best = -1e300;
-            for (size_t s = 0; s < t.off.size(); ++s) {
-                const double rs = t.r[s], xs = x + off[s].x, ys = y + off[s].y;
-                best = std::max(best, rs - off[s].z);
-                int cx, cy;
-                colidx(xs, ys, cx, cy);
-                for (int a = std::max(0, cx - 1); a <= std::min(nco - 1, cx + 1); ++a)
-                    for (int b = std::max(0, cy - 1); b <= std::min(nco - 1, cy + 1); ++b) {
-                        const std::vector<Placed>& c = col[(size_t)a * nco + b];
-                        for (const Placed& p : c) {  // descending z
-                            if (p.z + rs + rmax - off[s].z <= best) break;
-                            const double dx = xs - p.x, dy = ys - p.y, d2 = dx * dx + dy * dy, rsum = rs + p.r;
-                            if (d2 >= rsum * rsum) continue;
-                            best = std::max(best, p.z + std::sqrt(rsum * rsum - d2) - off[s].z);
+            auto landing = [&](double x, double y) {
+                double best = -1e300;
+                for (size_t s = 0; s < t.off.size(); ++s) {
+                    const double rs = t.r[s], xs = x + off[s].x, ys = y + off[s].y;
+                    best = std::max(best, rs - off[s].z);
+                    int cx, cy;
+                    colidx(xs, ys, cx, cy);
+                    for (int a = std::max(0, cx - 1); a <= std::min(nco - 1, cx + 1); ++a)
+                        for (int b = std::max(0, cy - 1); b <= std::min(nco - 1, cy + 1); ++b) {
+                            const std::vector<Placed>& c = col[(size_t)a * nco + b];
+                            for (const Placed& p : c) {  // descending z
+                                if (p.z + rs + rmax - off[s].z <= best) break;
+                                const double dx = xs - p.x, dy = ys - p.y, d2 = dx * dx + dy * dy, rsum = rs + p.r;
+                                if (d2 >= rsum * rsum) continue;
+                                best = std::max(best, p.z + std::sqrt(rsum * rsum - d2) - off[s].z);
+                            }
                         }
-                    }
+                }
+                return best;
+            };
+            // the lowest of a few random drop points: fills pockets, denser bed
+            double x = 0, y = 0, best = 1e300;
+            for (int tries = 0; tries < kDropTries; ++tries) {
+                double cx_, cy_;
+                do { cx_ = (2 * U(g) - 1) * rr; cy_ = (2 * U(g) - 1) * rr; } while (cx_ * cx_ + cy_ * cy_ > rr * rr);
+                const double h = landing(cx_, cy_);
+                if (h < best) { best = h; x = cx_; y = cy_; }
             }
             cpos[k] = {x, y, best};
             for (size_t s = 0; s < t.off.size(); ++s) {

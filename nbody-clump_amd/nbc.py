@@ -42,7 +42,7 @@ class Params(C.Structure):
 class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("wall_type", C.c_int32), ("block_size", C.c_int32),
-        ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("reserved0", C.c_int32),
+        ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("reserved", C.c_double * 4),
     ]
 
@@ -263,10 +263,10 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0):
+                 cell_size=0.0, batch_steps=0):
     o = Options()
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
-    o.pair_capacity, o.pene_frcoe, o.cell_size = pair_capacity, pene_frcoe, cell_size
+    o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
     return o
 
 

@@ -114,6 +114,7 @@ struct orc_ctx {
     // penetrator (src/dem_penetrator.h)
     nbc_penetrator Pn;
     std::vector<double> pene_tri;
+    double pene_bound = 0.0;
     std::vector<d3> PFc, PTc, PpeneTc, PFcn, PFct, PTcn, PTct, PRVo, PRVn;
     std::vector<int> PLcFlg;
     Hist3 PH, PHpre;
@@ -239,8 +240,14 @@ extern "C" int orc_upload(orc_ctx* c, const nbc_spheres_in* s, const nbc_clumps_
     // penetrator
     if (pe) {
         c->Pn = *pe;
-        if (pe->shape == NBC_PENE_CONE && pe->tri && pe->ntri > 0)
+        if (pe->shape == NBC_PENE_CONE && pe->tri && pe->ntri > 0) {
             c->pene_tri.assign(pe->tri, pe->tri + 9 * (size_t)pe->ntri);
+            double b2 = 0.0;
+            for (size_t k = 0; k < c->pene_tri.size(); k += 3)
+                b2 = std::max(b2, c->pene_tri[k] * c->pene_tri[k] + c->pene_tri[k + 1] * c->pene_tri[k + 1] +
+                                      c->pene_tri[k + 2] * c->pene_tri[k + 2]);
+            c->pene_bound = std::sqrt(b2) * (1.0 + 1e-12);
+        }
         c->Pn.tri = nullptr;
         c->pene_pos = mk3(pe->pos[0], pe->pos[1], pe->pos[2]);
         c->pene_vel = mk3(pe->vel[0], pe->vel[1], pe->vel[2]);
@@ -917,6 +924,8 @@ static void pene_precontact(orc_ctx* c) {
         d3 RV = c->Pos[i] - pp;
         if (p.shape == NBC_PENE_CONE) {
             // closest point on the mesh (penetrator frame = world frame offset by pos)
+            const double reach = c->pene_bound + c->Radius[i];
+            if (Dot(RV, RV) >= reach * reach) continue;
             const int nt = p.ntri;
             double best = std::numeric_limits<double>::infinity();
             d3 bq{0, 0, 0};

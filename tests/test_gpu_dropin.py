@@ -1,0 +1,103 @@
+"""The drop-in entry (nbc_run_simulation_clump, the RunSimulationClump
+replacement) on the reference's own shipped inputs, on the GPU: output files,
+their layout, and the state they hold against the FP64 oracle run on the same
+inputs."""
+import ctypes as C
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import nbc
+import orc
+
+pytestmark = pytest.mark.gpu
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_inputs")
+
+
+def prepare(tmp_path, end_step, interval, extra=None):
+    d = tmp_path / "run"
+    shutil.copytree(FIX, d)
+    txt = open(d / "world.par").read()
+    txt = txt.replace("EndStep\t\t= 200000", f"EndStep\t\t= {end_step}")
+    txt = txt.replace("OutputInterval\t= 10000", f"OutputInterval\t= {interval}")
+    assert f"= {end_step}" in txt and f"= {interval}" in txt
+    (d / "world.par").write_text(txt)
+    if extra:
+        (d / "nbc.par").write_text(extra)
+    return d
+
+
+def read_ph(path):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:4], np.int32)[0])
+    ints = np.frombuffer(raw[4:4 + 12 * n], np.int32).reshape(3, n)
+    dbl = np.frombuffer(raw[4 + 12 * n:], np.float64)
+    pos = dbl[:3 * n].reshape(n, 3)
+    return n, ints[1], ints[2], pos
+
+
+def oracle_for_fixture(steps):
+    p = nbc.Params()
+    ini = C.create_string_buffer(256)
+    assert nbc.lib().nbc_load_world_par(os.path.join(FIX, "world.par").encode(), C.byref(p), ini, 256) == 0
+    n, nc = p.ParticleNum, p.ClumpNum
+    cid = np.zeros(n, np.int32)
+    pos, loc, m, r = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    assert nbc.lib().nbc_load_phase(os.path.join(FIX, "ph.00000000.bt").encode(), n,
+                                    *[nbc.ptr(a) for a in (cid, pos, loc, m, r)]) == 0
+    cp, cv, cq, cw, pr = np.zeros((nc, 3)), np.zeros((nc, 3)), np.zeros((nc, 4)), np.zeros((nc, 3)), np.zeros((nc, 4))
+    assert nbc.lib().nbc_load_clumps(os.path.join(FIX, "ch.00000000.bt").encode(), nc,
+                                     *[nbc.ptr(a) for a in (cp, cv, cq, cw, pr)]) == 0
+    wall = nbc.Wall()
+    assert nbc.lib().nbc_load_cube_wall(os.path.join(FIX, "Rocks", "CubeWall.bt").encode(), C.byref(wall)) == 0
+    pene = nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(os.path.join(FIX, "Rocks", "Penetrator.bt").encode(), C.byref(pene)) == 0
+    bed = nbc.Bed(p, cid, pos, loc, m, r, cp, cv, cq, cw, pr, wall)
+    o = orc.Oracle(orc.finalize_params(bed))
+    o.upload(bed, wall, pene)
+    o.step(steps)
+    return o, bed
+
+
+def test_run_simulation_clump_on_reference_inputs(tmp_path):
+    print("[case] drop-in run on the shipped inputs, 300 steps", flush=True)
+    d = prepare(tmp_path, 300, 150)
+    rc = nbc.lib().nbc_run_simulation_clump(str(d).encode(), 0)
+    assert rc == 0, nbc.lib().nbc_last_error(None)
+    for f in ("OutPart/ph.00000150.bin", "OutPart/ph.00000300.bin", "OutClump/ch.00000300.bin",
+              "OutForce/fc.00000300.bin", "OutForce/yi.00000300.bin", "OutCnt/Nei.00000300.bin",
+              "OutCnt/Fcn.00000300.bin", "OutCnt/Cpt.00000300.bin", "OutClump/PenetratorShape.bt"):
+        assert (d / f).exists(), f
+    assert open(d / "OutClump/PenetratorShape.bt").read().split()[0] == "sphere"
+    n, fid, cid, pos = read_ph(d / "OutPart/ph.00000300.bin")
+    assert n == 576
+    o, bed = oracle_for_fixture(300)
+    so = o.spheres()
+    assert np.array_equal(fid, so["fid"]) and np.array_equal(cid, so["cid"])
+    assert np.abs(pos - so["pos"]).max() <= 1e-12
+    # Nei rows: front neighbours first, back ones at the end of each row
+    raw = open(d / "OutCnt/Nei.00000300.bin", "rb").read()
+    N, B = np.frombuffer(raw[:8], np.int32)
+    nei = np.frombuffer(raw[8:], np.int32).reshape(N, B)
+    pr = o.pairs()
+    for i in range(0, N, 29):
+        ng = pr["foff"][i + 1] - pr["foff"][i]
+        assert np.array_equal(nei[i, :ng], pr["fj"][pr["foff"][i]:pr["foff"][i + 1]])
+        nb = pr["boff"][i + 1] - pr["boff"][i]
+        if nb:
+            assert np.array_equal(nei[i, B - nb:], pr["bi"][pr["boff"][i]:pr["boff"][i + 1]])
+        assert np.all(nei[i, ng:B - nb] == -1)
+
+
+def test_mixed_precision_run_and_overflow_error(tmp_path):
+    print("[case] drop-in mixed run + tiny BranchLmtLen overflow", flush=True)
+    d = prepare(tmp_path, 100, 100, extra="Precision = mixed\n")
+    assert nbc.lib().nbc_run_simulation_clump(str(d).encode(), 0) == 0
+    d2 = prepare(tmp_path / "b", 10, 10)
+    txt = open(d2 / "world.par").read().replace("BranchLmtLen         = 256", "BranchLmtLen         = 2")
+    (d2 / "world.par").write_text(txt)
+    rc = nbc.lib().nbc_run_simulation_clump(str(d2).encode(), 0)
+    assert rc == nbc.NBC_E_NEIGHBOR_OVERFLOW
+    assert "Neighbor list overflow (BranchLmtLen=2)" in nbc.lib().nbc_last_error(None).decode()

@@ -99,6 +99,7 @@ def test_multi_step_with_rebuilds(prec):
     banner(f"200 steps with rebuilds, precision {prec}")
     bed = nbc.generate_bed(1, 300, seed=5)
     bed.params.Thresh = 2e-6  # force frequent rebuilds
+    bed.params.StepSize = 1e-5  # about 20 um of free fall in 200 steps
     opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
     ctx, ref = pair_setup(bed, opt)
     ctx.step(200)
@@ -121,8 +122,10 @@ def test_cube_wall_and_sphere_penetrator():
     banner("cube wall + sphere penetrator, fp64 and mixed")
     bed = nbc.generate_bed(0, 96, seed=3, container=(0.016, 0.016, 0.024, 0))
     # push a sphere penetrator into the top layer
-    top = bed.pos[:, 2].max()
-    pene = nbc.sphere_penetrator(0.004, (0.008, 0.008, top + 0.004), (0.0, 0.0, -0.5))
+    k = int(np.argmax(bed.pos[:, 2]))
+    x, y, z = bed.pos[k]
+    Rp = 0.004
+    pene = nbc.sphere_penetrator(Rp, (x, y, z + bed.radius[k] + Rp - 2e-5), (0.0, 0.0, -0.5))
     bed.params.isIntruder = 1
     bed.params.OutputInterval = 50
     for prec, tol in ((nbc.NBC_FP64, FP64_TOL), (nbc.NBC_MIXED, MIXED_TOL)):

@@ -201,9 +201,8 @@ def test_cell_hash_known_points():
 
 def test_newton_third_law_in_a_bed():
     bed = nbc.generate_bed(1, 300, seed=8)
-    bed.params.Gravity = -9.81
-    bed.params.isWallForce = 0
-    o = run(bed, steps=200, wall=None)
+    bed.params.StepSize = 1e-5  # let the deposited bed settle onto its floor
+    o = run(bed, steps=300, wall=bed.wall, opt=nbc.make_options(wall_type=nbc.NBC_WALL_CYLINDER))
     f, _ = o.sphere_forces(1)
     assert np.abs(f).sum() > 0
     assert np.abs(f.sum(axis=0)).max() <= 1e-12 * np.abs(f).sum()
