@@ -55,6 +55,39 @@ def run_form(bed, precision, steps, warmup):
     return out
 
 
+def settle(bed, steps, dt):
+    """Let the deposited bed collapse under gravity (FP32-mixed form, larger
+    step) so the benchmark runs on a contact-rich packing."""
+    import nbc
+
+    opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER)
+    saved = bed.params.StepSize
+    bed.params.StepSize = dt
+    ctx = nbc.Context(bed.finalized_params(opt), opt)
+    ctx.upload(bed, bed.wall)
+    ctx.step(steps)
+    out = nbc.download_bed(ctx, bed)
+    ctx.close()
+    bed.params.StepSize = saved
+    out.params.StepSize = saved
+    return out
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: reading its FETCH_SIZE and
+    WRITE_SIZE counters gives the factor that converts those counters into HBM
+    bytes for the contact kernel."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 5}))
+
+
 def cpu_baseline(bed, sample_steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
@@ -85,19 +118,39 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--settle-steps", type=int, default=6000)
+    ap.add_argument("--settle-dt", type=float, default=5e-6)
+    ap.add_argument("--bed-cache", default=None,
+                    help="load the settled bed from this .npz if it exists, else build it and save it there")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); this build has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe_mib > 0:
+        copy_probe(a.copy_probe_mib)
+        return
     import nbc
 
     if nbc.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    bed = nbc.generate_bed(1, a.nclumps, seed=1)
-    bed.params.BranchLmtLen = 128
-    print(f"[bench] bed: {bed.n} spheres, {bed.nc} clumps", file=sys.stderr, flush=True)
+    if a.bed_cache and os.path.exists(a.bed_cache):
+        bed = nbc.Bed.load(a.bed_cache)
+        print(f"[bench] bed loaded from {a.bed_cache}: {bed.n} spheres, {bed.nc} clumps", file=sys.stderr, flush=True)
+    else:
+        bed = nbc.generate_bed(1, a.nclumps, seed=1)
+        bed.params.BranchLmtLen = 128
+        print(f"[bench] bed: {bed.n} spheres, {bed.nc} clumps", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        if a.settle_steps > 0:
+            bed = settle(bed, a.settle_steps, a.settle_dt)
+        print(f"[bench] settled {a.settle_steps} steps of {a.settle_dt} s in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if a.bed_cache:
+            bed.save(a.bed_cache)
     mixed = run_form(bed, nbc.NBC_MIXED, a.steps, a.warmup)
     print(f"[bench] mixed: {mixed}", file=sys.stderr, flush=True)
     fp64 = None
@@ -126,7 +179,8 @@ def main():
         "data": "synthetic (deterministic generator, seed 1)",
         "config": {
             "workload": "C1 bed: 200,000 clumps from 8 templates of 2-8 spheres, lognormal radii, "
-                        "cylinder R 60 mm, dt 5e-7 s, uniform grid",
+                        "cylinder R 60 mm, dt 5e-7 s, uniform grid; deposited, then settled for "
+                        f"{a.settle_steps} steps of {a.settle_dt} s before timing",
             "n_spheres": N, "n_clumps": bed.nc, "touching_ss_per_sphere": mixed["touching_ss"] / N,
             "list_pairs_per_sphere": mixed["front_pairs"] / N, "rebuilds_in_timed_steps": mixed["rebuilds"],
         },

@@ -81,6 +81,9 @@ constexpr uint32_t JMASK = 0x1FFFFFFFu;
 constexpr uint32_t INTRA = 1u << 29;
 __host__ __device__ __forceinline__ uint32_t VALID(int p) { return 1u << (30 + p); }
 
+// back entries (owner, owner slot): bit 30 of the owner marks an intra-clump pair
+constexpr int BINTRA = 1 << 30;
+
 constexpr int HSTRIDE = 9;   // tangential(3) twisting(3) rolling(3) per history record
 constexpr int MAX_FACES = 7; // cube 6 (or cylinder 3) + penetrator 1
 
@@ -91,6 +94,7 @@ struct DevParams {
     double origin[3], cell[3], icell[3], boxlo[3], boxup[3];
     double dt, Ymod, Prat, Frcoe, Beta, Cohesion, Gravity, Damping, Thresh;
     double pene_frcoe;
+    double cKn, cKs, idt; // 2E/(3(1-nu^2)), 2E/(1+nu)/(2-nu), 1/dt (FP32 force path)
     int wall_type;        // NBC_WALL_*
     int nfaces;           // wall faces in use (6 cube, 3 cylinder, 0 none)
     int pene_on;          // isIntruder

@@ -106,6 +106,9 @@ class Engine : public EngineBase {
         dp.dt = P.StepSize; dp.Ymod = P.Ymod; dp.Prat = P.Prat; dp.Frcoe = P.Frcoe; dp.Beta = P.Beta;
         dp.Cohesion = P.Cohesion; dp.Gravity = P.Gravity; dp.Damping = P.Damping; dp.Thresh = P.Thresh;
         dp.pene_frcoe = O.pene_frcoe > 0.0 ? O.pene_frcoe : P.Frcoe;
+        dp.cKn = 2.0 * P.Ymod / (3.0 * (1.0 - P.Prat * P.Prat));
+        dp.cKs = 2.0 * P.Ymod / (1.0 + P.Prat) / (2.0 - P.Prat);
+        dp.idt = 1.0 / P.StepSize;
         dp.wall_type = wall_type; dp.nfaces = nfaces;
         dp.pene_on = P.isIntruder ? 1 : 0;
         dp.pene_face = nfaces;
@@ -685,7 +688,7 @@ class Engine : public EngineBase {
             std::vector<int2> v(std::max<long long>(nback, 1));
             if (nback) HIPC(hipMemcpy(v.data(), bent, sizeof(int2) * nback, hipMemcpyDeviceToHost));
             for (long long k = 0; k < nback; ++k) {
-                if (bi) bi[k] = v[k].x;
+                if (bi) bi[k] = v[k].x & ~BINTRA;
                 if (bs) bs[k] = v[k].y;
             }
         }

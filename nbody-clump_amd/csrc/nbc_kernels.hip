@@ -468,6 +468,7 @@ __global__ void k_back_resolve(int N, const int* foff, const uint32_t* fword, co
         for (int s = foff[o]; s < foff[o + 1]; ++s)
             if ((int)(fword[s] & JMASK) == i) slot = s;
         bent[e].y = slot;
+        if (slot >= 0 && (fword[slot] & INTRA)) bent[e].x = o | BINTRA;
     }
 }
 
@@ -520,9 +521,43 @@ struct PairOut {
     V3<R> DistU;
 };
 
+// Hertz-Mindlin stiffness and damping. FP64: the reference's expressions in
+// its evaluation order (src/dem_force_clump.cu:68-77). FP32: the same
+// quantities with the material constants hoisted into DevParams.
+template <class R>
+struct Coef {
+    R Kn, Cn, Ks, Cs, Kr, Cr, Kt, Ct, idt;
+};
+template <class R>
+__device__ __forceinline__ void hm_coef(const DevParams& P, R Rij, R Mij, R Dn, R CsFactor, Coef<R>& c) {
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const R Ymod = P.Ymod, Prat = P.Prat, Damping = P.Damping, Beta = P.Beta;
+        c.Kn = R(2.0) * Ymod * sqrt(Rij * Dn) / (R(3.0) * (R(1.0) - Prat * Prat));
+        c.Cn = Damping * sqrt(c.Kn * Mij);
+        c.Ks = R(2.0) * Ymod / (R(1.0) + Prat) / (R(2.0) - Prat) * sqrt(Rij * Dn);
+        c.Cs = CsFactor == R(1.0) ? Damping * sqrt(c.Ks * Mij) : CsFactor * Damping * sqrt(c.Ks * Mij);
+        c.Kr = R(0.25) * c.Kn * (Beta * Rij) * (Beta * Rij);
+        c.Cr = R(0.25) * c.Cn * (Beta * Rij) * (Beta * Rij);
+        c.Kt = R(0.50) * c.Ks * (Beta * Rij) * (Beta * Rij);
+        c.Ct = R(0.50) * c.Cs * (Beta * Rij) * (Beta * Rij);
+        c.idt = R(1.0) / (R)P.dt;
+    } else {
+        const R sq = sqrt(Rij * Dn), D = (R)P.Damping, bR = (R)P.Beta * Rij, b2 = bR * bR;
+        c.Kn = (R)P.cKn * sq;
+        c.Cn = D * sqrt(c.Kn * Mij);
+        c.Ks = (R)P.cKs * sq;
+        c.Cs = CsFactor * D * sqrt(c.Ks * Mij);
+        c.Kr = R(0.25) * c.Kn * b2;
+        c.Cr = R(0.25) * c.Cn * b2;
+        c.Kt = R(0.50) * c.Ks * b2;
+        c.Ct = R(0.50) * c.Cs * b2;
+        c.idt = (R)P.idt;
+    }
+}
+
 template <class R>
 __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& in, PairOut<R>& o) {
-    const R Ymod = (R)P.Ymod, Prat = (R)P.Prat, Damping = (R)P.Damping, Beta = (R)P.Beta, Frcoe = (R)P.Frcoe;
+    const R Beta = (R)P.Beta, Frcoe = (R)P.Frcoe;
     const R dt = (R)P.dt, Coh = (R)P.Cohesion;
     const R DistR = R(1.0) / in.DistL;
     const V3<R> DistU = DistR * in.Dist;
@@ -530,14 +565,9 @@ __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& 
     const R An = in.DistL - Ap;
     const R Rij = in.rI * in.rJ / (in.rI + in.rJ);
     const R Mij = in.mI * in.mJ / (in.mI + in.mJ);
-    const R Kn = R(2.0) * Ymod * sqrt(Rij * in.Dn) / (R(3.0) * (R(1.0) - Prat * Prat));
-    const R Cn = Damping * sqrt(Kn * Mij);
-    const R Ks = R(2.0) * Ymod / (R(1.0) + Prat) / (R(2.0) - Prat) * sqrt(Rij * in.Dn);
-    const R Cs = Damping * sqrt(Ks * Mij);
-    const R Kr = R(0.25) * Kn * (Beta * Rij) * (Beta * Rij);
-    const R Cr = R(0.25) * Cn * (Beta * Rij) * (Beta * Rij);
-    const R Kt = R(0.50) * Ks * (Beta * Rij) * (Beta * Rij);
-    const R Ct = R(0.50) * Cs * (Beta * Rij) * (Beta * Rij);
+    Coef<R> k;
+    hm_coef<R>(P, Rij, Mij, in.Dn, R(1.0), k);
+    const R Kn = k.Kn, Cn = k.Cn, Ks = k.Ks, Cs = k.Cs, Kr = k.Kr, Cr = k.Cr, Kt = k.Kt, Ct = k.Ct;
     const V3<R> ARMi = in.armI + Ap * DistU;
     const V3<R> ARMj = in.armJ - An * DistU;
     const V3<R> Vrel = in.vJ - in.vI;
@@ -574,7 +604,7 @@ __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& 
         rh = rm;
     } else {
         rh = rm;
-        rm = rm - Cr * (R(1.0) / dt) * DthR;
+        rm = rm - Cr * k.idt * DthR;
     }
     V3<R> tm = Kt * DthT + in.Mt;
     const R tmL = Length<R>(tm);
@@ -584,7 +614,7 @@ __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& 
         twh = tm;
     } else {
         twh = tm;
-        tm = tm - Ct * (R(1.0) / dt) * DthT;
+        tm = tm - Ct * k.idt * DthT;
     }
     ctc = ctc - rm - tm;
     ctc2 = ctc2 + rm + tm;
@@ -611,15 +641,10 @@ __device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFact
                                            R Rij, R Mij, const V3<R>& arm0, const V3<R>& RVo, const V3<R>& cv,
                                            const V3<R>& cw, const V3<R>& wv, const V3<R>& ww, const V3<R>& H,
                                            const V3<R>& Mt, const V3<R>& Mr, WallOut<R>& o) {
-    const R Ymod = (R)P.Ymod, Prat = (R)P.Prat, Damping = (R)P.Damping, Beta = (R)P.Beta, dt = (R)P.dt;
-    const R Kn = R(2.0) * Ymod * sqrt(Rij * Dn) / (R(3.0) * (R(1.0) - Prat * Prat));
-    const R Cn = Damping * sqrt(Kn * Mij);
-    const R Ks = R(2.0) * Ymod / (R(1.0) + Prat) / (R(2.0) - Prat) * sqrt(Rij * Dn);
-    const R Cs = CsFactor == R(1.0) ? Damping * sqrt(Ks * Mij) : CsFactor * Damping * sqrt(Ks * Mij);
-    const R Kr = R(0.25) * Kn * (Beta * Rij) * (Beta * Rij);
-    const R Cr = R(0.25) * Cn * (Beta * Rij) * (Beta * Rij);
-    const R Kt = R(0.50) * Ks * (Beta * Rij) * (Beta * Rij);
-    const R Ct = R(0.50) * Cs * (Beta * Rij) * (Beta * Rij);
+    const R Beta = (R)P.Beta, dt = (R)P.dt;
+    Coef<R> k;
+    hm_coef<R>(P, Rij, Mij, Dn, CsFactor, k);
+    const R Kn = k.Kn, Cn = k.Cn, Ks = k.Ks, Cs = k.Cs, Kr = k.Kr, Cr = k.Cr, Kt = k.Kt, Ct = k.Ct;
     const V3<R> Arm = R(-1.0) * arm0 + Ap * DistU;
     const V3<R> Vwall = Cross<R>(ww, RVo);
     const V3<R> Vrel = cv - wv;
@@ -652,7 +677,7 @@ __device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFact
         rh = rm;
     } else {
         rh = rm;
-        rm = rm - Cr * (R(1.0) / dt) * DthR;
+        rm = rm - Cr * k.idt * DthR;
     }
     V3<R> tm = Kt * DthT + Mt;
     const R tmL = Length<R>(tm);
@@ -662,7 +687,7 @@ __device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFact
         twh = tm;
     } else {
         twh = tm;
-        tm = tm - Ct * (R(1.0) / dt) * DthT;
+        tm = tm - Ct * k.idt * DthT;
     }
     o.nf = nf; o.tf = tf; o.rm = rm; o.tm = tm; o.ht = th; o.htw = twh; o.hr = rh; o.arm = Arm;
 }
@@ -715,15 +740,164 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                                                   const int* halt, int n) {
     typedef typename Vec4T<R>::type R4;
     if (*halt <= n) return;
+    // Per-wave queue of touching pairs: lanes scan their list rows, push the
+    // pairs that touch, and the queue is then evaluated one pair per lane so
+    // the force law runs on full waves. Each lane adds its own results in
+    // queue order, which is its slot order (front rows, then back rows).
+    constexpr int Q = 128;
+    __shared__ int q_slot[4][Q];
+    __shared__ int q_oth[4][Q];
+    __shared__ int q_inf[4][Q];
+    __shared__ R q_res[4][Q][6];
     const int lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int i = lb * blockDim.x + threadIdx.x;
     const bool active = i < P.N;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wbase = i - lane;
     int n_ss = 0, n_w = 0;
     V3<double> pf_sum = mk<double>(0.0, 0.0, 0.0), pt_sum = mk<double>(0.0, 0.0, 0.0);
+    D4 pi4;
+    pi4.x = pi4.y = pi4.z = pi4.w = 0.0;
+    if (active) pi4 = S.pos[i];
+    const V3<double> xi = xyz(pi4);
+    const double ri = pi4.w;
+    V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss, Fw = Fss, Tw = Fss, Fp = Fss, Tp = Fss;
+    R* hcur = L.hist[p];
+    R* hnew = L.hist[p ^ 1];
+    int qn = 0;  // wave-uniform queue length
+
+    auto flush = [&]() {
+        for (int r0 = 0; r0 < qn; r0 += 64) {
+            const int e = r0 + lane;
+            if (e < qn) {
+                const int inf = q_inf[wv][e];
+                const int side = inf >> 6;
+                const int me = wbase + (inf & 63);
+                const int oth = q_oth[wv][e];
+                const int s = q_slot[wv][e];
+                const int I = side ? oth : me, J = side ? me : oth;
+                const D4 pI = S.pos[I], pJ = S.pos[J];
+                const R4 kI = S.kin[I], kJ = S.kin[J];
+                const int cI = S.cid[I], cJ = S.cid[J];
+                const R4 svI = C.snap[2 * cI], swI = C.snap[2 * cI + 1];
+                const R4 svJ = C.snap[2 * cJ], swJ = C.snap[2 * cJ + 1];
+                const V3<double> Dd = xyz(pJ) - xyz(pI);
+                const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+                const double DistL = sqrt(DistS);
+                const double Dn = pI.w + pJ.w - DistL;
+                const uint32_t w = L.fword[s];
+                PairIn<R> in;
+                in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
+                in.rI = (R)pI.w; in.rJ = (R)pJ.w; in.mI = kI.w; in.mJ = kJ.w;
+                in.armI = Ld4<R>::xyz(kI); in.armJ = Ld4<R>::xyz(kJ);
+                in.vI = Ld4<R>::xyz(svI); in.wI = Ld4<R>::xyz(swI); in.vJ = Ld4<R>::xyz(svJ); in.wJ = Ld4<R>::xyz(swJ);
+                if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
+                else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
+                PairOut<R> o;
+                pair_force<R>(P, in, o);
+                V3<R> f, t;
+                if (side == 0) {
+                    store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
+                    if (!(w & VALID(p ^ 1))) L.fword[s] = w | VALID(p ^ 1);
+                    f = o.cfc;
+                    t = o.ctc;
+                    if (RECORD) {
+                        D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
+                        const V3<double> cpt = xyz(pI) + (double)o.Ap * mk<double>((double)o.DistU.x, (double)o.DistU.y,
+                                                                                  (double)o.DistU.z);
+                        D4 b; b.x = cpt.x; b.y = cpt.y; b.z = cpt.z; b.w = 0.0;
+                        rec.pair_fcn[s] = a;
+                        rec.pair_cpt[s] = b;
+                    }
+                } else {
+                    f = R(-1.0) * o.cfc;  // x - y == x + (-y) exactly: same sums as subtracting
+                    t = o.ctc2;
+                }
+                R* rr = q_res[wv][e];
+                rr[0] = f.x; rr[1] = f.y; rr[2] = f.z; rr[3] = t.x; rr[4] = t.y; rr[5] = t.z;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int e = 0; e < qn; ++e) {
+            if ((q_inf[wv][e] & 63) == lane) {
+                const R* rr = q_res[wv][e];
+                Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
+                Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        qn = 0;
+    };
+
+    // front rows: pairs this sphere owns (it writes their history)
+    {
+        const int f0 = active ? L.foff[i] : 0, f1 = active ? L.foff[i + 1] : 0;
+        for (int k = 0;; ++k) {
+            const int s = f0 + k;
+            const bool has = s < f1;
+            if (__ballot(has) == 0ull) break;
+            bool touch = false;
+            int j = 0;
+            if (has) {
+                const uint32_t w = L.fword[s];
+                if (!(w & INTRA)) {
+                    j = (int)(w & JMASK);
+                    const D4 pj4 = S.pos[j];
+                    const V3<double> Dd = xyz(pj4) - xi;
+                    const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+                    const double rs = ri + pj4.w;
+                    // clearly apart: the exact test would fail too (margin >> rounding)
+                    if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
+                    if (!touch && (w & VALID(p ^ 1))) L.fword[s] = w & ~VALID(p ^ 1);
+                }
+            }
+            const uint64_t m = __ballot(touch);
+            if (touch) {
+                const int pos = qn + __popcll(m & lanemask_lt());
+                q_slot[wv][pos] = s;
+                q_oth[wv][pos] = j;
+                q_inf[wv][pos] = lane;
+                n_ss++;
+            }
+            qn += __popcll(m);
+            if (qn > Q - 64) flush();
+        }
+    }
+    // back rows: the owner holds the history; the same pair is evaluated again
+    // from the same inputs and this sphere takes the reaction
+    {
+        const int b0 = active ? L.boff[i] : 0, b1 = active ? L.boff[i + 1] : 0;
+        for (int k = 0;; ++k) {
+            const int e = b0 + k;
+            const bool has = e < b1;
+            if (__ballot(has) == 0ull) break;
+            bool touch = false;
+            int2 be = make_int2(0, -1);
+            if (has) {
+                be = L.bent[e];
+                if (be.y >= 0 && !(be.x & BINTRA)) {
+                    const D4 po4 = S.pos[be.x];
+                    const V3<double> Dd = xi - xyz(po4);
+                    const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+                    const double rs = po4.w + ri;
+                    if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
+                }
+            }
+            const uint64_t m = __ballot(touch);
+            if (touch) {
+                const int pos = qn + __popcll(m & lanemask_lt());
+                q_slot[wv][pos] = be.y;
+                q_oth[wv][pos] = be.x;
+                q_inf[wv][pos] = lane | 64;
+            }
+            qn += __popcll(m);
+            if (qn > Q - 64) flush();
+        }
+    }
+    if (qn > 0) flush();
     if (active) {
-        const D4 pi4 = S.pos[i];
-        const V3<double> xi = xyz(pi4);
-        const double ri = pi4.w;
         const R4 ki = S.kin[i];
         const int ci_raw = S.cid[i];
         const bool cid_ok = ci_raw >= 0 && ci_raw < P.Nc;
@@ -731,79 +905,6 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
         const R4 sv = C.snap[2 * ci], sw = C.snap[2 * ci + 1];
         const V3<R> armi = Ld4<R>::xyz(ki), vi = Ld4<R>::xyz(sv), wi = Ld4<R>::xyz(sw);
         const R mi = ki.w;
-        V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss, Fw = Fss, Tw = Fss, Fp = Fss, Tp = Fss;
-        R* hcur = L.hist[p];
-        R* hnew = L.hist[p ^ 1];
-        // front pairs: this sphere owns the pair and writes its history
-        const int f0 = L.foff[i], f1 = L.foff[i + 1];
-        for (int s = f0; s < f1; ++s) {
-            const uint32_t w = L.fword[s];
-            if (w & INTRA) continue;
-            const int j = (int)(w & JMASK);
-            const D4 pj4 = S.pos[j];
-            const V3<double> Dd = xyz(pj4) - xi;
-            const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-            const double DistL = sqrt(DistS);
-            const double Dn = ri + pj4.w - DistL;
-            if (!(Dn > 0.0)) {
-                if (w & VALID(p ^ 1)) L.fword[s] = w & ~VALID(p ^ 1);
-                continue;
-            }
-            const R4 kj = S.kin[j];
-            const int cj = S.cid[j];
-            const R4 svj = C.snap[2 * cj], swj = C.snap[2 * cj + 1];
-            PairIn<R> in;
-            in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
-            in.rI = (R)ri; in.rJ = (R)pj4.w; in.mI = mi; in.mJ = kj.w;
-            in.armI = armi; in.armJ = Ld4<R>::xyz(kj);
-            in.vI = vi; in.wI = wi; in.vJ = Ld4<R>::xyz(svj); in.wJ = Ld4<R>::xyz(swj);
-            if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
-            else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
-            PairOut<R> o;
-            pair_force<R>(P, in, o);
-            store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
-            if (!(w & VALID(p ^ 1))) L.fword[s] = w | VALID(p ^ 1);
-            Fss = Fss + o.cfc;
-            Tss = Tss + o.ctc;
-            n_ss++;
-            if (RECORD) {
-                D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
-                const V3<double> cpt = xi + (double)o.Ap * mk<double>((double)o.DistU.x, (double)o.DistU.y, (double)o.DistU.z);
-                D4 b; b.x = cpt.x; b.y = cpt.y; b.z = cpt.z; b.w = 0.0;
-                rec.pair_fcn[s] = a;
-                rec.pair_cpt[s] = b;
-            }
-        }
-        // back pairs: the owner (lower index) holds the history; recompute the
-        // same pair from the same inputs and take the reaction
-        const int b0 = L.boff[i], b1 = L.boff[i + 1];
-        for (int e = b0; e < b1; ++e) {
-            const int2 be = L.bent[e];
-            if (be.y < 0) continue;
-            const uint32_t w = L.fword[be.y];
-            if (w & INTRA) continue;
-            const int o_ = be.x;
-            const D4 po4 = S.pos[o_];
-            const V3<double> Dd = xi - xyz(po4);
-            const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-            const double DistL = sqrt(DistS);
-            const double Dn = po4.w + ri - DistL;
-            if (!(Dn > 0.0)) continue;
-            const R4 ko = S.kin[o_];
-            const int co = S.cid[o_];
-            const R4 svo = C.snap[2 * co], swo = C.snap[2 * co + 1];
-            PairIn<R> in;
-            in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
-            in.rI = (R)po4.w; in.rJ = (R)ri; in.mI = ko.w; in.mJ = mi;
-            in.armI = Ld4<R>::xyz(ko); in.armJ = armi;
-            in.vI = Ld4<R>::xyz(svo); in.wI = Ld4<R>::xyz(swo); in.vJ = vi; in.wJ = wi;
-            if (w & VALID(p)) load_hist<R>(hcur + (size_t)be.y * HSTRIDE, in.H, in.Mt, in.Mr);
-            else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
-            PairOut<R> o;
-            pair_force<R>(P, in, o);
-            Fss = Fss - o.cfc;
-            Tss = Tss + o.ctc2;
-        }
         // walls and penetrator
         uint32_t wf = S.wflag[i];
         const uint32_t wf0 = wf;
@@ -1047,7 +1148,6 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
     // block reductions: touching counts and FP64 penetrator partial sums
     __shared__ double red[6][4];
     __shared__ unsigned int cnt[2][4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double v[6] = {pf_sum.x, pf_sum.y, pf_sum.z, pt_sum.x, pt_sum.y, pt_sum.z};
     unsigned a0 = (unsigned)n_ss, a1 = (unsigned)n_w;
     for (int o = 32; o > 0; o >>= 1) {

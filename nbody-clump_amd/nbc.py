@@ -239,6 +239,22 @@ class Bed:
         check(lib().nbc_finalize_params(C.byref(p), ptr(self.radius), self.n, C.byref(opt) if opt else None))
         return p
 
+    _ARRAYS = ("clump_id", "pos", "local", "mass", "radius", "cpos", "cvel", "cquat", "cangv", "cprop")
+
+    def save(self, path):
+        """Write the bed (arrays, parameter block, wall) to an .npz file."""
+        np.savez(path, params=np.frombuffer(bytes(self.params), np.uint8),
+                 wall=np.frombuffer(bytes(self.wall), np.uint8), **{k: getattr(self, k) for k in self._ARRAYS})
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path) as z:
+            if z["params"].size != C.sizeof(Params) or z["wall"].size != C.sizeof(Wall):
+                raise ValueError(f"{path}: parameter block layout does not match this library")
+            p = Params.from_buffer_copy(z["params"].tobytes())
+            w = Wall.from_buffer_copy(z["wall"].tobytes())
+            return cls(p, *[z[k] for k in cls._ARRAYS], w)
+
 
 def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
     spec = BedSpec(kind, nclumps, seed, D4(*container))
@@ -423,3 +439,17 @@ class Context:
         ms, n = C.c_double(), C.c_int64()
         self._c(lib().nbc_profile_get(self.h, cls, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def download_bed(ctx, template):
+    """Current device state as a Bed in file order (sphere file ids restored).
+
+    `template` supplies the parameters and wall of the returned bed."""
+    sp = ctx.spheres()
+    cl = ctx.clumps()
+    order = np.argsort(sp["fid"], kind="stable")
+    p = Params()
+    C.memmove(C.byref(p), C.byref(template.params), C.sizeof(Params))
+    prop = template.cprop.copy()
+    return Bed(p, sp["cid"][order], sp["pos"][order], sp["local"][order], sp["mass"][order], sp["radius"][order],
+               cl["pos"], cl["vel"], cl["quat"], cl["angv"], prop, template.wall)
