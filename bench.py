@@ -49,7 +49,7 @@ def run_form(bed, precision, steps, warmup):
     out = dict(seconds=t1 - t0, k3_ms=k3_ms, k3_n=k3_n, rebuilds=s1.rebuilds - s0.rebuilds,
                touching_ss=0.5 * (s0.touching_ss + s1.touching_ss), touching_wall=0.5 * (s0.touching_wall + s1.touching_wall),
                front_pairs=s1.front_pairs)
-    for cls in (0, 2, 3, 4):
+    for cls in (0, 2, 3, 4, 5):
         out[f"cls{cls}_ms"], out[f"cls{cls}_n"] = ctx.profile_get(cls)
     ctx.close()
     return out
@@ -65,7 +65,12 @@ def settle(bed, steps, dt):
     bed.params.StepSize = dt
     ctx = nbc.Context(bed.finalized_params(opt), opt)
     ctx.upload(bed, bed.wall)
-    ctx.step(steps)
+    chunk = 5000
+    for done in range(0, steps, chunk):
+        ctx.step(min(chunk, steps - done))
+        s = ctx.stats()
+        print(f"[bench] settle step {s.step}: touching {s.touching_ss / bed.n:.3f}/sphere, wall {s.touching_wall}, "
+              f"KE {s.kinetic_energy:.4e} J, rebuilds {s.rebuilds}", file=sys.stderr, flush=True)
     out = nbc.download_bed(ctx, bed)
     ctx.close()
     bed.params.StepSize = saved
@@ -118,7 +123,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--settle-steps", type=int, default=6000)
+    ap.add_argument("--settle-steps", type=int, default=60000)
     ap.add_argument("--settle-dt", type=float, default=5e-6)
     ap.add_argument("--bed-cache", default=None,
                     help="load the settled bed from this .npz if it exists, else build it and save it there")

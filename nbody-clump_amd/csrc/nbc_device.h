@@ -95,6 +95,10 @@ struct DevParams {
     double dt, Ymod, Prat, Frcoe, Beta, Cohesion, Gravity, Damping, Thresh;
     double pene_frcoe;
     double cKn, cKs, idt; // 2E/(3(1-nu^2)), 2E/(1+nu)/(2-nu), 1/dt (FP32 force path)
+    double pc[3];         // box centre: origin of the FP32 position copy
+    double half;          // largest box half-extent
+    float f32_marg;       // absolute slack of the FP32 distance pre-filter
+    double near_add;      // 2 * Thresh: gap below which a pair enters the near list
     int wall_type;        // NBC_WALL_*
     int nfaces;           // wall faces in use (6 cube, 3 cylinder, 0 none)
     int pene_on;          // isIntruder

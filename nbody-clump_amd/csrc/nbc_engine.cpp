@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -109,6 +110,15 @@ class Engine : public EngineBase {
         dp.cKn = 2.0 * P.Ymod / (3.0 * (1.0 - P.Prat * P.Prat));
         dp.cKs = 2.0 * P.Ymod / (1.0 + P.Prat) / (2.0 - P.Prat);
         dp.idt = 1.0 / P.StepSize;
+        dp.half = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            dp.pc[a] = 0.5 * (P.BoxLowLmt[a] + P.BoxUpLmt[a]);
+            dp.half = std::max(dp.half, 0.5 * (P.BoxUpLmt[a] - P.BoxLowLmt[a]));
+        }
+        dp.f32_marg = (float)(4.0 * dp.half * std::ldexp(1.0, -23));
+        // pairs farther apart than 2 * Thresh at a rebuild cannot touch before the next one
+        dp.near_add = (std::isfinite(P.Thresh) && P.Thresh >= 0.0) ? 2.0 * P.Thresh
+                                                                 : std::numeric_limits<double>::infinity();
         dp.wall_type = wall_type; dp.nfaces = nfaces;
         dp.pene_on = P.isIntruder ? 1 : 0;
         dp.pene_face = nfaces;
@@ -120,8 +130,8 @@ class Engine : public EngineBase {
             HIPC(dalloc(&s.pos, N)); HIPC(dalloc(&s.cum, N)); HIPC(dalloc(&s.loc, N)); HIPC(dalloc(&s.kin, N));
             HIPC(dalloc(&s.cid, N)); HIPC(dalloc(&s.fid, N)); HIPC(dalloc(&s.wflag, N));
             HIPC(dalloc(&s.wh, (size_t)std::max(nwh, 1) * N * HSTRIDE));
-            if (b == 0) HIPC(dalloc(&s.sft, 2 * (size_t)N));
-            else s.sft = S[0].sft;
+            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.p32, N)); }
+            else { s.sft = S[0].sft; s.p32 = S[0].p32; }
         }
         HIPC(dalloc(&C.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C.prop, Nc));
         HIPC(dalloc(&C.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C.snap, 2 * (size_t)Nc));
@@ -132,6 +142,7 @@ class Engine : public EngineBase {
         }
         HIPC(dalloc(&boff, (size_t)N + 1));
         bcap = 0; bent = nullptr;
+        HIPC(dalloc(&nfoff, (size_t)N + 1)); HIPC(dalloc(&nboff, (size_t)N + 1));
         HIPC(grow_front(0, (long long)N * pair_cap_per));
         HIPC(grow_front(1, (long long)N * pair_cap_per));
         HIPC(grow_back((long long)N * pair_cap_per));
@@ -150,6 +161,8 @@ class Engine : public EngineBase {
         HIPC(dalloc(&pst, 1));
         nbk = contacts_blocks(N, block);
         HIPC(dalloc(&pene_partial, 6 * (size_t)nbk));
+        HIPC(dalloc(&cand.cnt, nbk)); HIPC(dalloc(&cand.idx, (size_t)nbk * block));
+        HIPC(dalloc(&cand.raw, 2 * (size_t)nbk * block));
         HIPC(dalloc(&cache_f, 3 * (size_t)oi)); HIPC(dalloc(&cache_t, 3 * (size_t)oi));
         HIPC(dalloc(&ke_partial, (Nc + 255) / 256));
         HIPC(hipHostMalloc((void**)&hstatus, 4 * sizeof(uint32_t), hipHostMallocDefault));
@@ -183,6 +196,27 @@ class Engine : public EngineBase {
         return hipSuccess;
     }
 
+    // near lists hold at most as many entries as the front / back lists
+    hipError_t grow_near(long long fcap, long long bc) {
+        if (fcap > ncap_f) {
+            dfree(nfl); dfree(slot2near);
+            nfl = nullptr; slot2near = nullptr; ncap_f = 0;
+            hipError_t e = dalloc(&nfl, (size_t)fcap);
+            if (e != hipSuccess) return e;
+            e = dalloc(&slot2near, (size_t)fcap);
+            if (e != hipSuccess) return e;
+            ncap_f = fcap;
+        }
+        if (bc > ncap_b) {
+            dfree(nbl);
+            nbl = nullptr; ncap_b = 0;
+            hipError_t e = dalloc(&nbl, (size_t)bc);
+            if (e != hipSuccess) return e;
+            ncap_b = bc;
+        }
+        return hipSuccess;
+    }
+
     void release() {
         if (device < 0) return;
         (void)hipSetDevice(device);
@@ -191,14 +225,14 @@ class Engine : public EngineBase {
         for (int b = 0; b < 2; ++b) {
             SphereBuf<R>& s = S[b];
             F(s.pos); F(s.cum); F(s.loc); F(s.kin); F(s.cid); F(s.fid); F(s.wflag); F(s.wh);
-            if (b == 0) F(s.sft);
+            if (b == 0) { F(s.sft); F(s.p32); }
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap);
-        F(boff); F(bent);
+        F(boff); F(bent); F(nfoff); F(nboff); F(nfl); F(nbl); F(slot2near);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
-        F(dstatus); F(counters); F(pst); F(pene_partial); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
+        F(dstatus); F(counters); F(pst); F(pene_partial); F(cand.cnt); F(cand.idx); F(cand.raw); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         if (hstatus) (void)hipHostFree(hstatus);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
@@ -261,6 +295,7 @@ class Engine : public EngineBase {
         HIPC(hipMemset(s0.wh, 0, sizeof(R) * (size_t)std::max(nwh, 1) * N * HSTRIDE));
         HIPC(hipMemset(S[1].wflag, 0, sizeof(uint32_t) * N));
         HIPC(hipMemset(s0.sft, 0, sizeof(R4) * 2 * (size_t)N));
+        HIPC(hipMemset(s0.p32, 0xff, sizeof(F4) * (size_t)N));  // NaN: pre-filter passes everything
         std::vector<D4> pq(2 * (size_t)Nc), vw(2 * (size_t)Nc), prop(Nc), ft(2 * (size_t)Nc);
         for (int i = 0; i < Nc; ++i) {
             pq[2 * i] = D4{c->pos[3 * i], c->pos[3 * i + 1], c->pos[3 * i + 2], 0.0};
@@ -281,6 +316,8 @@ class Engine : public EngineBase {
         HIPC(hipMemset(Lf[0].foff, 0, sizeof(int) * ((size_t)N + 1)));
         HIPC(hipMemset(Lf[1].foff, 0, sizeof(int) * ((size_t)N + 1)));
         HIPC(hipMemset(boff, 0, sizeof(int) * ((size_t)N + 1)));
+        HIPC(hipMemset(nfoff, 0, sizeof(int) * ((size_t)N + 1)));
+        HIPC(hipMemset(nboff, 0, sizeof(int) * ((size_t)N + 1)));
         nfront = nback = 0;
         HIPC(hipMemset(perm, 0, sizeof(int) * N));
         HIPC(hipMemset(cellkey, 0, sizeof(uint32_t) * N));
@@ -354,7 +391,7 @@ class Engine : public EngineBase {
         // 2. cell bounds (src/dem_grid.cu:105-153)
         KC(launch_seg_bounds(st, cellkey, N, P.CellNum, cbeg, cend));
         // 3. gather every per-sphere array (src/dem_simulation_clump.cu:126-172)
-        KC(launch_gather<R>(st, N, nwh, perm, so, sn));
+        KC(launch_gather<R>(st, dp, nwh, perm, so, sn));
         scur ^= 1;
         SphereBuf<R>& s = S[scur];
         // 4. neighbour counts, overflow check (src/dem_grid.cu:466-552, 709-719)
@@ -391,8 +428,14 @@ class Engine : public EngineBase {
         KC(launch_neighbors(st, dp, true, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, Ln.foff, boff, Ln.fword, bent,
                             dstatus + 2, block));
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
+        // history-valid bits of the old near list back into the old pair words
+        KC(launch_near_sync(st, N, nfoff, nfl, Lo.fword, block));
         KC(launch_hist_remap<R>(st, N, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword, Lo.hist[p], block));
         lcur ^= 1;
+        // near lists of the new list (the contact kernel's candidates)
+        HIPC(grow_near(Ln.cap, bcap));
+        KC(launch_near_build(st, dp, s.pos, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfl, slot2near, bcnt, nboff,
+                             nbl, stmp, stotal + 2, block));
         nfront = tot[0];
         nback = tot[1];
         // 6. clump member lists in ascending sorted order (replaces the per-step
@@ -464,13 +507,18 @@ class Engine : public EngineBase {
             KC(launch_fill_f64(st, (double*)rec.pair_cpt, 4 * (long long)rec_cap, 0.0));
         }
         ListBuf<R> L;
-        L.foff = Lf[lcur].foff; L.fword = Lf[lcur].fword; L.boff = boff; L.bent = bent;
+        L.nfoff = nfoff; L.nf = nfl; L.nboff = nboff; L.nb = nbl;
         L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
         const int* halt = (const int*)dstatus;
         hipEvent_t e0 = prof_begin();
-        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, pene_partial, counters, (int*)(dstatus + 1), rec, record,
-                              block, 1, halt, (int)n));
+        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, cand, counters, rec, record, block, 1, halt, (int)n));
         prof_end(1, e0, n);
+        if (dp.nfaces > 0 || dp.pene_on) {
+            e0 = prof_begin();
+            KC(launch_walls<R>(st, dp, S[scur], C, cand, pst, pene_partial, counters, (int*)(dstatus + 1), rec, record,
+                               block, halt, (int)n));
+            prof_end(5, e0, n);
+        }
         if (dp.pene_on) {
             e0 = prof_begin();
             KC(launch_pene_finalize(st, dp, pst, pene_partial, nbk, (int)(n % oi), cache_f, cache_t, halt, (int)n));
@@ -700,7 +748,9 @@ class Engine : public EngineBase {
         HIPC(hipStreamSynchronize(st));
         if (!nfront) return 0;
         const FrontList& L = Lf[lcur];
-        const int q = (int)((nstep + 1) & 1);  // buffer written by the last step
+        KC(launch_near_sync(st, N, nfoff, nfl, L.fword, block));
+        HIPC(hipStreamSynchronize(st));
+        const int q = (int)(nstep & 1);  // step nstep-1 (parity q^1) wrote buffer q
         std::vector<uint32_t> w(nfront);
         std::vector<R> h((size_t)nfront * HSTRIDE);
         HIPC(hipMemcpy(w.data(), L.fword, sizeof(uint32_t) * nfront, hipMemcpyDeviceToHost));
@@ -877,6 +927,11 @@ class Engine : public EngineBase {
     unsigned long long* counters = nullptr;
     PeneState* pst = nullptr;
     double *pene_partial = nullptr, *cache_f = nullptr, *cache_t = nullptr, *ke_partial = nullptr, *dtri = nullptr;
+    CandBuf<R> cand = {nullptr, nullptr, nullptr};
+    int *nfoff = nullptr, *nboff = nullptr, *slot2near = nullptr;
+    uint2* nfl = nullptr;
+    int2* nbl = nullptr;
+    long long ncap_f = 0, ncap_b = 0;
     RecordBuf rec;
     bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;
     long long nstep = 0, nrebuild = 0;

@@ -22,6 +22,7 @@ struct SphereBuf {
     uint32_t* wflag;                  // bit f: wall/penetrator history f valid
     R* wh;                            // [MAX_FACES][N][HSTRIDE] wall/penetrator histories
     typename Vec4T<R>::type* sft;     // [2N] force, torque about the clump centre
+    F4* p32;                          // FP32 (x, y, z) - box centre, radius; w < 0: use FP64 only
 };
 
 // Per-clump arrays (FP64 state; snap is the contact kernel's copy in R).
@@ -38,11 +39,21 @@ struct ClumpBuf {
 // CSR back rows (i, owner j < i, owner slot).
 template <class R>
 struct ListBuf {
-    int* foff;
-    uint32_t* fword;
-    int* boff;
-    int2* bent;
-    R* hist[2];
+    int* nfoff;       // [N + 1] near front rows
+    uint2* nf;        // (front slot, pair word with the history-valid bits)
+    int* nboff;       // [N + 1] near back rows
+    int2* nb;         // (owner, index of the pair in the owner's near row)
+    R* hist[2];       // [front slots][HSTRIDE] ping-pong histories
+};
+
+// Wall/penetrator candidates of k_contacts, per block of k_contacts: the
+// count, then in slots [b * block, b * block + count) the sphere index and its
+// sphere-sphere force and torque sums.
+template <class R>
+struct CandBuf {
+    int* cnt;                         // [blocks]
+    int* idx;                         // [blocks * block]
+    typename Vec4T<R>::type* raw;     // [2 * blocks * block]
 };
 
 struct RecordBuf {
@@ -84,14 +95,22 @@ template <class R>
 int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int step,
                        int block);
 template <class R>
-int launch_gather(hipStream_t st, int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst);
+int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst);
 template <class R>
 int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
                       const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
+int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
+                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint2* nf, int* slot2near, int* nbcnt,
+                      int* nboff, int2* nb, int* stmp, int* stotal, int block);
+int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint2* nf, uint32_t* fword, int block);
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                    int block, int remap, const int* halt, int step);
+                    PeneState* pst, CandBuf<R> W, unsigned long long* ctr, RecordBuf rec, bool record, int block,
+                    int remap, const int* halt, int step);
+template <class R>
+int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, CandBuf<R> W, PeneState* pst,
+                 double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block,
+                 const int* halt, int step);
 template <class R>
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
                        const int* mend, const int* mem, int block, const int* halt, int step);

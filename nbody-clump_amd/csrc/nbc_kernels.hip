@@ -66,6 +66,31 @@ template <class R> __device__ __forceinline__ typename Vec4T<R>::type mk4(R x, R
 __device__ __forceinline__ V3<double> xyz(const D4& v) { return mk<double>(v.x, v.y, v.z); }
 __device__ __forceinline__ bool finite3(const V3<double>& v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
 
+// FP32 copy of a sphere position, relative to the box centre. Coordinates
+// farther than two half-extents from the centre, or not finite, get w = -1:
+// the contact kernel then skips the FP32 pre-filter for that sphere.
+__device__ __forceinline__ F4 pos32(const DevParams& P, const V3<double>& x, double r) {
+    const double dx = x.x - P.pc[0], dy = x.y - P.pc[1], dz = x.z - P.pc[2], lim = 2.0 * P.half;
+    F4 o;
+    o.x = (float)dx; o.y = (float)dy; o.z = (float)dz;
+    o.w = (fabs(dx) <= lim && fabs(dy) <= lim && fabs(dz) <= lim) ? (float)r : -1.0f;
+    return o;
+}
+
+// FP32 pre-filter of the sphere-sphere test Dn = ri + rj - |xj - xi| > 0.
+// Each coordinate of pos32 is within half * 2^-23 of the exact value, so the
+// FP32 distance differs from the FP64 one by less than
+// 2*sqrt(3)*half*2^-23 + a few ulp of the distance; f32_marg = 4*half*2^-23
+// and the relative slack 1e-5 cover both. Returns false only when the FP64
+// test would fail too.
+__device__ __forceinline__ bool maybe_touch(const DevParams& P, const F4& a, const F4& b) {
+    if (a.w < 0.0f || b.w < 0.0f) return true;
+    const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float rs = (a.w + b.w) * 1.00001f + P.f32_marg;
+    return !(d2 > rs * rs);
+}
+
 // ---------------------------------------------------------------------------
 // K1: clump half-kick, drift and attitude (FP64). Thread 0 of block 0 also
 // advances the penetrator (src/dem_integrate_penetrator.cu:3-41) and clears
@@ -175,6 +200,7 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         S.pos[i] = pn;
         S.cum[i] = cn;
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
+        S.p32[i] = pos32(P, up, po.w);
     }
     // the rebuild request (src/dem_clump.cu:132-134) records the step index:
     // later kernels of this step and all kernels of later steps stand down
@@ -375,11 +401,14 @@ __global__ void k_seg_bounds(const uint32_t* key, int n, int nkeys, int* beg, in
 // (src/dem_simulation_clump.cu:126-164); the displacement restarts at zero
 // (src/dem_simulation_clump.cu:172).
 template <class R>
-__global__ void k_gather(int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst) {
+__global__ void k_gather(DevParams P, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst) {
+    const int N = P.N;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int o = perm[i];
-    dst.pos[i] = src.pos[o];
+    const D4 po = src.pos[o];
+    dst.pos[i] = po;
+    dst.p32[i] = pos32(P, xyz(po), po.w);
     D4 z; z.x = 0.0; z.y = 0.0; z.z = 0.0; z.w = 0.0;
     dst.cum[i] = z;
     dst.loc[i] = src.loc[o];
@@ -499,6 +528,90 @@ __global__ void k_hist_remap(int N, int p, const int* perm, const int* nfoff, ui
             w |= VALID(p);
         }
         nfword[s] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near lists: the contact kernel's view of the Verlet list. A rebuild happens
+// before any sphere has moved more than Thresh from its position at the last
+// rebuild (the displacement test runs on every sphere every step), so a pair
+// whose gap at the rebuild is 2 * Thresh or more cannot touch until the next
+// rebuild, and an intra-clump pair never does. The near list keeps the other
+// front entries, in slot order, as (slot, pair word); between rebuilds it holds
+// the history-valid bits of its pairs (k_near_sync writes them back into the
+// pair words). The back near list keeps, in back-row order, (owner, index of
+// the pair in the owner's near row). P.near_add = 2 * Thresh (infinite when
+// Thresh is not a finite non-negative number: every pair is then near).
+__device__ __forceinline__ bool near_pair(const DevParams& P, const D4& a, const D4& b, uint32_t w) {
+    if (w & INTRA) return false;
+    if (w & (VALID(0) | VALID(1))) return true;
+    const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    const double reach = (a.w + b.w + P.near_add) * (1.0 + 1e-9);
+    return !(dx * dx + dy * dy + dz * dz >= reach * reach);
+}
+
+__global__ void k_near_count(DevParams P, const D4* pos, const int* foff, const uint32_t* fword, int* ncnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const D4 a = pos[i];
+    int c = 0;
+    for (int s = foff[i]; s < foff[i + 1]; ++s) {
+        const uint32_t w = fword[s];
+        c += near_pair(P, a, pos[w & JMASK], w) ? 1 : 0;
+    }
+    ncnt[i] = c;
+}
+
+__global__ void k_near_fill(DevParams P, const D4* pos, const int* foff, const uint32_t* fword, const int* nfoff,
+                            uint2* nf, int* slot2near) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    const D4 a = pos[i];
+    int o = nfoff[i];
+    for (int s = foff[i]; s < foff[i + 1]; ++s) {
+        const uint32_t w = fword[s];
+        if (near_pair(P, a, pos[w & JMASK], w)) {
+            nf[o] = make_uint2((unsigned)s, w);
+            slot2near[s] = o++;
+        } else {
+            slot2near[s] = -1;
+        }
+    }
+}
+
+__global__ void k_near_back_count(int N, const int* boff, const int2* bent, const int* slot2near, int* nbcnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c = 0;
+    for (int e = boff[i]; e < boff[i + 1]; ++e) {
+        const int2 be = bent[e];
+        c += (be.y >= 0 && slot2near[be.y] >= 0) ? 1 : 0;
+    }
+    nbcnt[i] = c;
+}
+
+__global__ void k_near_back_fill(int N, const int* boff, const int2* bent, const int* slot2near, const int* nboff,
+                                 int2* nb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int o = nboff[i];
+    for (int e = boff[i]; e < boff[i + 1]; ++e) {
+        const int2 be = bent[e];
+        if (be.y >= 0) {
+            const int q = slot2near[be.y];
+            if (q >= 0) nb[o++] = make_int2(be.x & ~BINTRA, q);
+        }
+    }
+}
+
+// history-valid bits of the near pairs back into the pair words
+__global__ void k_near_sync(int N, const int* nfoff, const uint2* nf, uint32_t* fword) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t V = VALID(0) | VALID(1);
+    for (int o = nfoff[i]; o < nfoff[i + 1]; ++o) {
+        const uint2 e = nf[o];
+        fword[e.x] = (fword[e.x] & ~V) | (e.y & V);
     }
 }
 
@@ -733,35 +846,71 @@ __device__ __forceinline__ V3<double> closest_on_tri(const V3<double>& p, const 
 
 // ---------------------------------------------------------------------------
 // K3: all contacts of sphere i, then its force and clump-torque contribution.
+// Candidate tests for k_walls: the exact FP64 face predicates of the wall
+// code below (a face is evaluated only when its predicate holds) and the
+// penetrator culls; a sphere failing all of them has no wall or penetrator
+// contact this step.
+__device__ __forceinline__ bool wall_candidate(const DevParams& P, const V3<double>& xi, double ri) {
+    const V3<double> RV = xi - mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
+    if (P.wall_type == NBC_WALL_CUBE) {
+        bool c = false;
+        for (int a = 0; a < 3; ++a) {
+            const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
+            const double Sd = P.wside[a];
+            c = c || (E > 0 && E + ri - Sd > 0) || (E < 0 && E - ri + Sd < 0);
+        }
+        return c;
+    }
+    const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+    const double ERR = Dot<double>(RV, Wv);
+    const V3<double> RVb = RV - ERR * Wv;
+    const double norm = Length<double>(RVb);
+    const double d_cap = fabs(ERR) - P.wheight + ri;
+    const double d_side = norm - P.wrad + ri;
+    return (d_cap >= 1e-14 && ERR != 0.0) || d_side >= 1e-14 || !(d_cap == d_cap) || !(d_side == d_side);
+}
+
+__device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneState* pst, const V3<double>& xi,
+                                               double ri) {
+    const V3<double> RV = xi - mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
+    if (P.pshape == NBC_PENE_SPHERE) return ri - (Length<double>(RV) - P.prad) > 0.0;
+    if (P.pshape == NBC_PENE_CONE) {
+        const double reach = P.pbound + ri;
+        return P.ntri > 0 && Dot<double>(RV, RV) < reach * reach;
+    }
+    const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
+    const double LatDx = P.cell[0];
+    const double ERR = Dot<double>(RV, pdir);
+    if (!(fabs(ERR) < (P.pheight + 0.80 * LatDx))) return false;
+    const double norm = Length<double>(RV - ERR * pdir);
+    return norm < (P.prad + 0.80 * LatDx);
+}
+
 template <class R, bool RECORD>
-__global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                                                  PeneState* pst, double* pene_partial,
-                                                  unsigned long long* counters, int* bad, RecordBuf rec, int remap,
-                                                  const int* halt, int n) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                                                  PeneState* pst, CandBuf<R> W, unsigned long long* counters,
+                                                  RecordBuf rec, int remap, const int* halt, int n) {
     typedef typename Vec4T<R>::type R4;
     if (*halt <= n) return;
-    // Per-wave queue of touching pairs: lanes scan their list rows, push the
-    // pairs that touch, and the queue is then evaluated one pair per lane so
-    // the force law runs on full waves. Each lane adds its own results in
-    // queue order, which is its slot order (front rows, then back rows).
+    // Per-wave queue of touching pairs: the wave scans its list segments, pushes
+    // the pairs that touch, and the queue is then evaluated one pair per lane
+    // so the force law runs on full waves. The lane of each sphere adds that
+    // sphere's results in queue order, which is its slot order (front rows,
+    // then back rows).
     constexpr int Q = 128;
     __shared__ int q_slot[4][Q];
     __shared__ int q_oth[4][Q];
     __shared__ int q_inf[4][Q];
     __shared__ R q_res[4][Q][6];
+    __shared__ int s_fo[4][65], s_bo[4][65];
+    __shared__ F4 s_p32[4][64];
     const int lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int i = lb * blockDim.x + threadIdx.x;
     const bool active = i < P.N;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int wbase = i - lane;
-    int n_ss = 0, n_w = 0;
-    V3<double> pf_sum = mk<double>(0.0, 0.0, 0.0), pt_sum = mk<double>(0.0, 0.0, 0.0);
-    D4 pi4;
-    pi4.x = pi4.y = pi4.z = pi4.w = 0.0;
-    if (active) pi4 = S.pos[i];
-    const V3<double> xi = xyz(pi4);
-    const double ri = pi4.w;
-    V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss, Fw = Fss, Tw = Fss, Fp = Fss, Tp = Fss;
+    int n_ss = 0;
+    V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss;
     R* hcur = L.hist[p];
     R* hnew = L.hist[p ^ 1];
     int qn = 0;  // wave-uniform queue length
@@ -774,7 +923,7 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                 const int side = inf >> 6;
                 const int me = wbase + (inf & 63);
                 const int oth = q_oth[wv][e];
-                const int s = q_slot[wv][e];
+                const int ne = q_slot[wv][e];  // index of the pair in the owner's near row
                 const int I = side ? oth : me, J = side ? me : oth;
                 const D4 pI = S.pos[I], pJ = S.pos[J];
                 const R4 kI = S.kin[I], kJ = S.kin[J];
@@ -785,7 +934,9 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                 const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
                 const double DistL = sqrt(DistS);
                 const double Dn = pI.w + pJ.w - DistL;
-                const uint32_t w = L.fword[s];
+                const uint2 nw = L.nf[ne];
+                const int s = (int)nw.x;  // front slot: history and record index
+                const uint32_t w = nw.y;
                 PairIn<R> in;
                 in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
                 in.rI = (R)pI.w; in.rJ = (R)pJ.w; in.mI = kI.w; in.mJ = kJ.w;
@@ -798,7 +949,7 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                 V3<R> f, t;
                 if (side == 0) {
                     store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
-                    if (!(w & VALID(p ^ 1))) L.fword[s] = w | VALID(p ^ 1);
+                    if (!(w & VALID(p ^ 1))) L.nf[ne].y = w | VALID(p ^ 1);
                     f = o.cfc;
                     t = o.ctc;
                     if (RECORD) {
@@ -819,69 +970,106 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (int e = 0; e < qn; ++e) {
-            if ((q_inf[wv][e] & 63) == lane) {
-                const R* rr = q_res[wv][e];
-                Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
-                Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
-            }
+        // Queue keys (owner lane, +64 for back entries) are non-decreasing, so
+        // each lane's front and back entries are two runs found by binary search.
+        auto lbound = [&](int key) {
+            int lo = 0;
+            for (int st = Q; st > 0; st >>= 1)
+                if (lo + st <= qn && q_inf[wv][lo + st - 1] < key) lo += st;
+            return lo;
+        };
+        const int f0 = lbound(lane), f1 = lbound(lane + 1), b0 = lbound(lane + 64), b1 = lbound(lane + 65);
+        for (int e = f0; e < f1; ++e) {
+            const R* rr = q_res[wv][e];
+            Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
+            Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
+        }
+        for (int e = b0; e < b1; ++e) {
+            const R* rr = q_res[wv][e];
+            Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
+            Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         qn = 0;
     };
 
-    // front rows: pairs this sphere owns (it writes their history)
+    // The wave's 64 spheres own the contiguous list segments
+    // [foff[wbase], foff[wbase + 64]) and [boff[wbase], boff[wbase + 64]); the
+    // wave walks each segment 64 entries at a time (coalesced list reads, equal
+    // work per lane) and finds the owner row of an entry by binary search over
+    // the row offsets held in LDS. Touching pairs are queued in entry order,
+    // i.e. per sphere in slot order, front rows before back rows.
     {
-        const int f0 = active ? L.foff[i] : 0, f1 = active ? L.foff[i + 1] : 0;
-        for (int k = 0;; ++k) {
-            const int s = f0 + k;
-            const bool has = s < f1;
-            if (__ballot(has) == 0ull) break;
+        const int ir = i < P.N ? i : P.N;
+        s_fo[wv][lane] = L.nfoff[ir];
+        s_bo[wv][lane] = L.nboff[ir];
+        if (lane == 63) {
+            const int ie = i + 1 < P.N ? i + 1 : P.N;
+            s_fo[wv][64] = L.nfoff[ie];
+            s_bo[wv][64] = L.nboff[ie];
+        }
+        F4 a = S.p32[i < P.N ? i : P.N - 1];  // clamped load, then a value select
+        if (!active) a.w = -1.0f;
+        s_p32[wv][lane] = a;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    // front entries: pairs the row's sphere owns (it writes their history)
+    {
+        const int F1 = s_fo[wv][64];
+        for (int c = s_fo[wv][0]; c < F1; c += 64) {
+            const int e = c + lane;
             bool touch = false;
-            int j = 0;
-            if (has) {
-                const uint32_t w = L.fword[s];
-                if (!(w & INTRA)) {
-                    j = (int)(w & JMASK);
+            int j = 0, r = 0;
+            if (e < F1) {
+                const uint32_t w = L.nf[e].y;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if (s_fo[wv][r + st] <= e) r += st;
+                j = (int)(w & JMASK);
+                if (maybe_touch(P, s_p32[wv][r], S.p32[j])) {
+                    const D4 pi4r = S.pos[wbase + r];
                     const D4 pj4 = S.pos[j];
-                    const V3<double> Dd = xyz(pj4) - xi;
+                    const V3<double> Dd = xyz(pj4) - xyz(pi4r);
                     const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-                    const double rs = ri + pj4.w;
-                    // clearly apart: the exact test would fail too (margin >> rounding)
+                    const double rs = pi4r.w + pj4.w;
                     if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
-                    if (!touch && (w & VALID(p ^ 1))) L.fword[s] = w & ~VALID(p ^ 1);
                 }
+                if (!touch && (w & VALID(p ^ 1))) L.nf[e].y = w & ~VALID(p ^ 1);
             }
             const uint64_t m = __ballot(touch);
             if (touch) {
                 const int pos = qn + __popcll(m & lanemask_lt());
-                q_slot[wv][pos] = s;
+                q_slot[wv][pos] = e;
                 q_oth[wv][pos] = j;
-                q_inf[wv][pos] = lane;
+                q_inf[wv][pos] = r;  // key: owner lane, front side
                 n_ss++;
             }
             qn += __popcll(m);
             if (qn > Q - 64) flush();
         }
     }
-    // back rows: the owner holds the history; the same pair is evaluated again
-    // from the same inputs and this sphere takes the reaction
+    // back entries: the owner holds the history; the same pair is evaluated
+    // again from the same inputs and the row's sphere takes the reaction
     {
-        const int b0 = active ? L.boff[i] : 0, b1 = active ? L.boff[i + 1] : 0;
-        for (int k = 0;; ++k) {
-            const int e = b0 + k;
-            const bool has = e < b1;
-            if (__ballot(has) == 0ull) break;
+        const int B1 = s_bo[wv][64];
+        for (int c = s_bo[wv][0]; c < B1; c += 64) {
+            const int e = c + lane;
             bool touch = false;
+            int r = 0;
             int2 be = make_int2(0, -1);
-            if (has) {
-                be = L.bent[e];
-                if (be.y >= 0 && !(be.x & BINTRA)) {
+            if (e < B1) {
+                be = L.nb[e];
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if (s_bo[wv][r + st] <= e) r += st;
+                if (maybe_touch(P, s_p32[wv][r], S.p32[be.x])) {
+                    const D4 pi4r = S.pos[wbase + r];
                     const D4 po4 = S.pos[be.x];
-                    const V3<double> Dd = xi - xyz(po4);
+                    const V3<double> Dd = xyz(pi4r) - xyz(po4);
                     const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-                    const double rs = po4.w + ri;
+                    const double rs = po4.w + pi4r.w;
                     if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
                 }
             }
@@ -890,290 +1078,402 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
                 const int pos = qn + __popcll(m & lanemask_lt());
                 q_slot[wv][pos] = be.y;
                 q_oth[wv][pos] = be.x;
-                q_inf[wv][pos] = lane | 64;
+                q_inf[wv][pos] = r | 64;  // key: owner lane, back side
             }
             qn += __popcll(m);
             if (qn > Q - 64) flush();
         }
     }
     if (qn > 0) flush();
+    // Wall and penetrator contacts are rare. A sphere that may have one, had
+    // one last step, or fails the input check of the wall kernel is handed to
+    // k_walls with its sphere-sphere sums; every other sphere finishes here.
+    bool cand = false;
     if (active) {
+        const D4 pi4 = S.pos[i];
+        const V3<double> xi = xyz(pi4);
+        const double ri = pi4.w;
         const R4 ki = S.kin[i];
-        const int ci_raw = S.cid[i];
-        const bool cid_ok = ci_raw >= 0 && ci_raw < P.Nc;
-        const int ci = cid_ok ? ci_raw : 0;
-        const R4 sv = C.snap[2 * ci], sw = C.snap[2 * ci + 1];
-        const V3<R> armi = Ld4<R>::xyz(ki), vi = Ld4<R>::xyz(sv), wi = Ld4<R>::xyz(sw);
-        const R mi = ki.w;
-        // walls and penetrator
-        uint32_t wf = S.wflag[i];
-        const uint32_t wf0 = wf;
-        const bool clump_ok = sv.w != R(0);
         if (P.nfaces > 0 || P.pene_on) {
-            const R4 li = S.loc[i];
-            const bool ok = cid_ok && clump_ok && finite3(xi) && isfinite((double)li.x) &&
-                            isfinite((double)li.y) && isfinite((double)li.z) && isfinite(ri) && ri > 0.0 &&
-                            isfinite((double)mi) && mi > R(0);
-            if (P.nfaces > 0 && !ok) atomicMin(bad, i);
-            const R Rr = (R)ri;
-            if (P.nfaces > 0 && ok) {
-                const V3<double> wpos = mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
-                const V3<R> wv = mk<R>((R)P.wvel[0], (R)P.wvel[1], (R)P.wvel[2]);
-                const V3<R> ww = mk<R>((R)P.wangv[0], (R)P.wangv[1], (R)P.wangv[2]);
-                const V3<double> RV = xi - wpos;
-                if (P.wall_type == NBC_WALL_CUBE) {
-                    // faces 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
-                    for (int f = 0; f < 6; ++f) {
-                        const int a = f >> 1;
-                        const bool up = (f & 1) == 0;
-                        const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
-                        const double Sd = P.wside[a];
-                        const bool cand = up ? (E > 0 && E + ri - Sd > 0) : (E < 0 && E - ri + Sd < 0);
-                        bool touch = false;
-                        if (cand) {
-                            const V3<double> ea = mk<double>(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
-                            const V3<double> RVb = RV - E * ea, RVt = E * ea;
-                            const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
-                            const V3<double> RVn = RV - RVo;
-                            const double sgn = up ? -1.0 : 1.0;
-                            const V3<double> nin = mk<double>(a == 0 ? sgn : 0.0, a == 1 ? sgn : 0.0, a == 2 ? sgn : 0.0);
-                            const double Ap = Dot<double>(RVn, nin);
-                            const double Dn = ri - Ap;
-                            if (Dn > 0.0) {
-                                touch = true;
-                                V3<R> H, Mt, Mr;
-                                R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
-                                if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
-                                else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
-                                WallOut<R> o;
-                                wall_force<R>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
-                                              cvt3<R>(RVo), vi, wi, wv, ww, H, Mt, Mr, o);
-                                Fw = Fw + o.nf + o.tf;
-                                Tw = Tw - (R)Ap * Cross<R>(cvt3<R>(nin), o.tf);
-                                Tw = Tw - o.rm - o.tm;
-                                store_hist<R>(hp, o.ht, o.htw, o.hr);
-                                n_w++;
-                            }
-                        }
-                        if (touch) wf |= 1u << f; else wf &= ~(1u << f);
-                    }
-                } else {
-                    // cylinder surfaces 0 bottom cap, 1 top cap, 2 side
-                    const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
-                    const double ERR = Dot<double>(RV, Wv);
-                    const V3<double> RVb = RV - ERR * Wv, RVt = ERR * Wv;
-                    const double norm = Length<double>(RVb);
-                    const double d_cap = fabs(ERR) - P.wheight + ri;
-                    const double d_side = norm - P.wrad + ri;
-                    for (int f = 0; f < 3; ++f) {
-                        bool cand;
-                        V3<double> RVo;
-                        if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
-                        else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
-                        else { cand = d_side >= 1e-14; RVo = (P.wrad / norm) * RVb + RVt; }
-                        bool touch = false;
-                        if (cand) {
-                            const V3<double> RVn = RV - RVo;
-                            const double Dn = ri - Length<double>(RVn);
-                            if (Dn > 0.0) {
-                                touch = true;
-                                const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
-                                const double DistL = sqrt(DistS);
-                                const R DistLr = (R)DistL;
-                                const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
-                                V3<R> H, Mt, Mr;
-                                R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
-                                if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
-                                else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
-                                WallOut<R> o;
-                                wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
-                                              vi, wi, wv, ww, H, Mt, Mr, o);
-                                Fw = Fw + o.nf + o.tf;
-                                Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
-                                Tw = Tw - o.rm - o.tm;
-                                store_hist<R>(hp, o.ht, o.htw, o.hr);
-                                n_w++;
-                            }
-                        }
-                        if (touch) wf |= 1u << f; else wf &= ~(1u << f);
-                    }
-                }
-                if (!isfinite((double)Fw.x) || !isfinite((double)Fw.y) || !isfinite((double)Fw.z) ||
-                    !isfinite((double)Tw.x) || !isfinite((double)Tw.y) || !isfinite((double)Tw.z))
-                    atomicMin(bad, i);
+            cand = S.wflag[i] != 0u;
+            if (!cand && P.nfaces > 0) {
+                const int c = S.cid[i];
+                const bool ok = c >= 0 && c < P.Nc && C.snap[2 * c].w != R(0) && finite3(xi) && isfinite(ri) &&
+                                ri > 0.0 && isfinite((double)ki.x) && isfinite((double)ki.y) &&
+                                isfinite((double)ki.z) && isfinite((double)ki.w) && ki.w > R(0);
+                cand = !ok || wall_candidate(P, xi, ri);
             }
-            if (P.pene_on) {
-                const int pf = P.pene_face;
-                const V3<double> pp = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
-                const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
-                const V3<double> RV = xi - pp;
-                bool touch = false;
-                V3<double> RVo = mk<double>(0.0, 0.0, 0.0), Dist = RVo, DistU64 = RVo;
-                double DistL = 0.0, Dn = 0.0;
-                bool have_u = false;
-                if (P.pshape == NBC_PENE_SPHERE) {
-                    // src/dem_sphere_intruder.cuh:27-50
-                    const double cd = Length<double>(RV);
-                    V3<double> nrm;
-                    if (cd > 1.0e-15) {
-                        nrm = (1.0 / cd) * RV;
-                    } else {
-                        const double t = sqrt(pdir.x * pdir.x + pdir.y * pdir.y + pdir.z * pdir.z);
-                        nrm = t > 1e-20 ? mk<double>(pdir.x / t, pdir.y / t, pdir.z / t) : mk<double>(0.0, 0.0, 0.0);
-                        if (Length<double>(nrm) < 1.0e-15) nrm = mk<double>(0.0, 0.0, 1.0);
-                    }
-                    const double gap = cd - P.prad;
-                    const double ov = ri - gap;
-                    if (ov > 0.0) {
-                        touch = true;
-                        RVo = P.prad * nrm;
-                        Dist = gap * nrm;
-                        DistL = gap;
-                        Dn = ov;
-                        DistU64 = nrm;
-                        have_u = true;
-                        if (P.pwait) atomicCAS(&pst->contact_state, 0, 1);
-                    }
-                } else if (P.pshape == NBC_PENE_CONE) {
-                    // spheres outside the mesh's bounding sphere cannot touch it
-                    const double reach = P.pbound + ri;
-                    if (Dot<double>(RV, RV) < reach * reach) {
-                    double best = 1e300;
-                    V3<double> bq = mk<double>(0.0, 0.0, 0.0);
-                    for (int t = 0; t < P.ntri; ++t) {
-                        const double* v = P.ptri + 9 * t;
-                        const V3<double> q = closest_on_tri(RV, mk<double>(v[0], v[1], v[2]), mk<double>(v[3], v[4], v[5]),
-                                                            mk<double>(v[6], v[7], v[8]));
-                        const V3<double> dq = RV - q;
-                        const double ds = Dot<double>(dq, dq);
-                        if (ds < best) { best = ds; bq = q; }
-                    }
-                    const V3<double> RVn = RV - bq;
-                    const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
-                    const double dl = sqrt(DistS);
-                    if (P.ntri > 0 && ri - dl > 0.0) {
-                        touch = true; RVo = bq; Dist = RVn; DistL = dl; Dn = ri - dl;
-                    }
-                    }
-                } else {
-                    // src/dem_contact_penetrator.cu:42-83
-                    const double LatDx = P.cell[0];
-                    const double ERR = Dot<double>(RV, pdir);
-                    if (fabs(ERR) < (P.pheight + 0.80 * LatDx)) {
-                        const V3<double> RVb = RV - ERR * pdir;
-                        const double norm = Length<double>(RVb);
-                        if (norm < (P.prad + 0.80 * LatDx)) {
-                            const V3<double> RVt = ERR * pdir;
-                            V3<double> o3;
-                            if (fabs(ERR) < P.pheight) o3 = (1.0 / norm) * P.prad * RVb + RVt;
-                            else if (norm < P.prad) o3 = (1.0 / fabs(ERR)) * P.pheight * RVt + RVb;
-                            else o3 = (1.0 / norm) * P.prad * RVb + (1.0 / fabs(ERR)) * P.pheight * RVt;
-                            const V3<double> RVn = RV - o3;
-                            const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
-                            const double dl = sqrt(DistS);
-                            if (ri - dl > 0.0) { touch = true; RVo = o3; Dist = RVn; DistL = dl; Dn = ri - dl; }
-                        }
-                    }
-                }
-                if (touch) {
-                    const R DistLr = (R)DistL;
-                    const V3<R> DistU = have_u ? cvt3<R>(DistU64) : (R(1.0) / DistLr) * cvt3<R>(Dist);
-                    R Rij = Rr;
-                    if (P.pshape == NBC_PENE_SPHERE) Rij = Rr * (R)P.prad / (Rr + (R)P.prad);
-                    R Mij = mi;
-                    if (P.pmotion == NBC_PENE_GRAVITY) Mij = mi * (R)P.pmass / (mi + (R)P.pmass);
-                    V3<R> H, Mt, Mr;
-                    R* hp = S.wh + ((size_t)pf * P.N + i) * HSTRIDE;
-                    if ((wf >> pf) & 1u) load_hist<R>(hp, H, Mt, Mr);
-                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
-                    const V3<R> pv = mk<R>((R)pst->vel[0], (R)pst->vel[1], (R)pst->vel[2]);
-                    const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
-                    const V3<R> RVor = cvt3<R>(RVo);
-                    WallOut<R> o;
-                    wall_force<R>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
-                                  pw, H, Mt, Mr, o);
-                    const V3<R> cf = mk<R>(R(0), R(0), R(0));
-                    Fp = o.nf + o.tf;
-                    Tp = -DistLr * Cross<R>(DistU, o.tf);
-                    V3<R> pTc = R(-1.0) * Cross<R>(RVor, (o.nf + o.tf + cf));
-                    Tp = Tp - o.rm - o.tm;
-                    pTc = pTc + o.rm + o.tm;
-                    store_hist<R>(hp, o.ht, o.htw, o.hr);
-                    pf_sum = mk<double>((double)Fp.x, (double)Fp.y, (double)Fp.z);
-                    pt_sum = mk<double>((double)pTc.x, (double)pTc.y, (double)pTc.z);
-                    n_w++;
-                    if (RECORD) {
-                        const V3<R> tn = Cross<R>(o.nf, o.arm);
-                        const V3<R> tt = Cross<R>(o.tf, o.arm) - o.rm - o.tm;
-                        D4 a; a.x = o.nf.x; a.y = o.nf.y; a.z = o.nf.z; a.w = 0.0;
-                        D4 b; b.x = o.tf.x; b.y = o.tf.y; b.z = o.tf.z; b.w = 0.0;
-                        D4 c; c.x = tn.x; c.y = tn.y; c.z = tn.z; c.w = 0.0;
-                        D4 d; d.x = tt.x; d.y = tt.y; d.z = tt.z; d.w = 0.0;
-                        rec.pene[4 * (size_t)i] = a; rec.pene[4 * (size_t)i + 1] = b;
-                        rec.pene[4 * (size_t)i + 2] = c; rec.pene[4 * (size_t)i + 3] = d;
-                    }
-                }
-                if (touch) wf |= 1u << pf; else wf &= ~(1u << pf);
+            if (!cand && P.pene_on) cand = pene_candidate(P, pst, xi, ri);
+        }
+        if (!cand) {
+            // totF/totT with zero wall and penetrator terms (src/dem_clump.cu:97-98)
+            // and the clump torque term T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
+            const V3<R> Z = mk<R>(R(0), R(0), R(0));
+            const V3<R> tF = Fss + Z + Z;
+            const V3<R> tT = Tss + Z + Z;
+            V3<R> r;
+            if (sizeof(R) == sizeof(double)) {
+                const D4 pc = C.pq[2 * S.cid[i]];
+                r = cvt3<R>(mk<double>(xi.x - pc.x, xi.y - pc.y, xi.z - pc.z));
+            } else {
+                r = Ld4<R>::xyz(ki);  // x_s - x_c = R(q) * local up to FP64 rounding
             }
-        }
-        if (wf != wf0) S.wflag[i] = wf;
-        // totF/totT (src/dem_clump.cu:97-98) and the clump torque term
-        // T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
-        const V3<R> tF = Fss + Fw + Fp;
-        const V3<R> tT = Tss + Tw + Tp;
-        V3<R> r;
-        if (sizeof(R) == sizeof(double)) {
-            const D4 pc = C.pq[2 * ci];
-            r = cvt3<R>(mk<double>(xi.x - pc.x, xi.y - pc.y, xi.z - pc.z));
-        } else {
-            r = armi;  // x_s - x_c = R(q) * local up to FP64 rounding
-        }
-        const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
-                               tT.z + (r.x * tF.y - r.y * tF.x));
-        S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
-        S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
-        if (RECORD) {
-            D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
-            D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
-            rec.tot[2 * (size_t)i] = a;
-            rec.tot[2 * (size_t)i + 1] = b;
-            D4 q[6];
-            const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
-            for (int k = 0; k < 6; ++k) {
-                q[k].x = parts[k].x; q[k].y = parts[k].y; q[k].z = parts[k].z; q[k].w = 0.0;
-                rec.parts[6 * (size_t)i + k] = q[k];
+            const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
+                                   tT.z + (r.x * tF.y - r.y * tF.x));
+            S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
+            S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+            if (RECORD) {
+                D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
+                D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
+                rec.tot[2 * (size_t)i] = a;
+                rec.tot[2 * (size_t)i + 1] = b;
+                const V3<R> parts[6] = {Fss, Tss, Z, Z, Z, Z};
+                for (int k = 0; k < 6; ++k) {
+                    D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
+                    rec.parts[6 * (size_t)i + k] = q;
+                }
             }
         }
     }
-    // block reductions: touching counts and FP64 penetrator partial sums
-    __shared__ double red[6][4];
-    __shared__ unsigned int cnt[2][4];
-    double v[6] = {pf_sum.x, pf_sum.y, pf_sum.z, pt_sum.x, pt_sum.y, pt_sum.z};
-    unsigned a0 = (unsigned)n_ss, a1 = (unsigned)n_w;
+    // block-ordered compaction of the candidates (ascending sphere index) and
+    // the touching count
+    __shared__ int s_wc[4];
+    __shared__ unsigned int s_cnt[4];
+    const uint64_t cm = __ballot(cand);
+    unsigned a0 = (unsigned)n_ss;
+    for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
+    if (lane == 0) {
+        s_wc[wv] = __popcll(cm);
+        s_cnt[wv] = a0;
+    }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    if (P.nfaces > 0 || P.pene_on) {
+        int base = 0;
+        for (int k = 0; k < wv; ++k) base += s_wc[k];
+        if (cand) {
+            const size_t o = (size_t)lb * blockDim.x + base + __popcll(cm & lanemask_lt());
+            W.idx[o] = i;
+            W.raw[2 * o] = mk4<R>(Fss.x, Fss.y, Fss.z, R(0));
+            W.raw[2 * o + 1] = mk4<R>(Tss.x, Tss.y, Tss.z, R(0));
+        }
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < nw; ++k) t += s_wc[k];
+            W.cnt[lb] = t;
+        }
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < nw; ++k) t += s_cnt[k];
+        if (t) atomicAdd(&counters[0], t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3 walls: wall and penetrator contacts of the candidates k_contacts compacted
+// per block, then the per-sphere totals and clump torque term
+// (src/dem_force_cube_wall_clump.cu:16-298, src/dem_force_cylind_wall.cu:152-237,
+// src/dem_contact_penetrator.cu:8-84, src/dem_force_penetrator_clump.cu:4-182,
+// src/dem_clump.cu:89-101, src/dem_clumpsphere.cu:86-107). One wave handles
+// the candidates of one block of k_contacts in sphere order (few candidates per
+// region, so one wave each keeps the launch to about one round of waves), and
+// the FP64 penetrator partial sums stay in region order.
+template <class R, bool RECORD>
+__global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, CandBuf<R> W, int nreg,
+                                               int rstride, PeneState* pst, double* pene_partial,
+                                               unsigned long long* counters, int* bad, RecordBuf rec, const int* halt,
+                                               int n) {
+    typedef typename Vec4T<R>::type R4;
+    if (*halt <= n) return;
+    // one wave per candidate region (= block of k_contacts)
+    const int lane = threadIdx.x & 63;
+    const int lb = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (lb >= nreg) return;
+    const int ncand = W.cnt[lb];
+    int n_w = 0;
+    V3<double> acc_f = mk<double>(0.0, 0.0, 0.0), acc_t = acc_f;
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int t = c0 + lane;
+        const bool active = t < ncand;
+        int i = 0;
+        V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss, Fw = Fss, Tw = Fss, Fp = Fss, Tp = Fss;
+        V3<double> pf_sum = mk<double>(0.0, 0.0, 0.0), pt_sum = pf_sum;
+        D4 pi4;
+        pi4.x = pi4.y = pi4.z = pi4.w = 0.0;
+        if (active) {
+            const size_t o = (size_t)lb * rstride + t;
+            i = W.idx[o];
+            Fss = Ld4<R>::xyz(W.raw[2 * o]);
+            Tss = Ld4<R>::xyz(W.raw[2 * o + 1]);
+            pi4 = S.pos[i];
+        }
+        const V3<double> xi = xyz(pi4);
+        const double ri = pi4.w;
+        if (active) {
+            const R4 ki = S.kin[i];
+            const int ci_raw = S.cid[i];
+            const bool cid_ok = ci_raw >= 0 && ci_raw < P.Nc;
+            const int ci = cid_ok ? ci_raw : 0;
+            const R4 sv = C.snap[2 * ci], sw = C.snap[2 * ci + 1];
+            const V3<R> armi = Ld4<R>::xyz(ki), vi = Ld4<R>::xyz(sv), wi = Ld4<R>::xyz(sw);
+            const R mi = ki.w;
+            // walls and penetrator
+            uint32_t wf = S.wflag[i];
+            const uint32_t wf0 = wf;
+            const bool clump_ok = sv.w != R(0);
+            if (P.nfaces > 0 || P.pene_on) {
+                const R4 li = S.loc[i];
+                const bool ok = cid_ok && clump_ok && finite3(xi) && isfinite((double)li.x) &&
+                                isfinite((double)li.y) && isfinite((double)li.z) && isfinite(ri) && ri > 0.0 &&
+                                isfinite((double)mi) && mi > R(0);
+                if (P.nfaces > 0 && !ok) atomicMin(bad, i);
+                const R Rr = (R)ri;
+                if (P.nfaces > 0 && ok) {
+                    const V3<double> wpos = mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
+                    const V3<R> wv = mk<R>((R)P.wvel[0], (R)P.wvel[1], (R)P.wvel[2]);
+                    const V3<R> ww = mk<R>((R)P.wangv[0], (R)P.wangv[1], (R)P.wangv[2]);
+                    const V3<double> RV = xi - wpos;
+                    if (P.wall_type == NBC_WALL_CUBE) {
+                        // faces 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
+                        for (int f = 0; f < 6; ++f) {
+                            const int a = f >> 1;
+                            const bool up = (f & 1) == 0;
+                            const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
+                            const double Sd = P.wside[a];
+                            const bool cand = up ? (E > 0 && E + ri - Sd > 0) : (E < 0 && E - ri + Sd < 0);
+                            bool touch = false;
+                            if (cand) {
+                                const V3<double> ea = mk<double>(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
+                                const V3<double> RVb = RV - E * ea, RVt = E * ea;
+                                const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
+                                const V3<double> RVn = RV - RVo;
+                                const double sgn = up ? -1.0 : 1.0;
+                                const V3<double> nin = mk<double>(a == 0 ? sgn : 0.0, a == 1 ? sgn : 0.0, a == 2 ? sgn : 0.0);
+                                const double Ap = Dot<double>(RVn, nin);
+                                const double Dn = ri - Ap;
+                                if (Dn > 0.0) {
+                                    touch = true;
+                                    V3<R> H, Mt, Mr;
+                                    R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
+                                    if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                                    WallOut<R> o;
+                                    wall_force<R>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
+                                                  cvt3<R>(RVo), vi, wi, wv, ww, H, Mt, Mr, o);
+                                    Fw = Fw + o.nf + o.tf;
+                                    Tw = Tw - (R)Ap * Cross<R>(cvt3<R>(nin), o.tf);
+                                    Tw = Tw - o.rm - o.tm;
+                                    store_hist<R>(hp, o.ht, o.htw, o.hr);
+                                    n_w++;
+                                }
+                            }
+                            if (touch) wf |= 1u << f; else wf &= ~(1u << f);
+                        }
+                    } else {
+                        // cylinder surfaces 0 bottom cap, 1 top cap, 2 side
+                        const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+                        const double ERR = Dot<double>(RV, Wv);
+                        const V3<double> RVb = RV - ERR * Wv, RVt = ERR * Wv;
+                        const double norm = Length<double>(RVb);
+                        const double d_cap = fabs(ERR) - P.wheight + ri;
+                        const double d_side = norm - P.wrad + ri;
+                        for (int f = 0; f < 3; ++f) {
+                            bool cand;
+                            V3<double> RVo;
+                            if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
+                            else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
+                            else { cand = d_side >= 1e-14; RVo = (P.wrad / norm) * RVb + RVt; }
+                            bool touch = false;
+                            if (cand) {
+                                const V3<double> RVn = RV - RVo;
+                                const double Dn = ri - Length<double>(RVn);
+                                if (Dn > 0.0) {
+                                    touch = true;
+                                    const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                                    const double DistL = sqrt(DistS);
+                                    const R DistLr = (R)DistL;
+                                    const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
+                                    V3<R> H, Mt, Mr;
+                                    R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
+                                    if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                                    WallOut<R> o;
+                                    wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
+                                                  vi, wi, wv, ww, H, Mt, Mr, o);
+                                    Fw = Fw + o.nf + o.tf;
+                                    Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
+                                    Tw = Tw - o.rm - o.tm;
+                                    store_hist<R>(hp, o.ht, o.htw, o.hr);
+                                    n_w++;
+                                }
+                            }
+                            if (touch) wf |= 1u << f; else wf &= ~(1u << f);
+                        }
+                    }
+                    if (!isfinite((double)Fw.x) || !isfinite((double)Fw.y) || !isfinite((double)Fw.z) ||
+                        !isfinite((double)Tw.x) || !isfinite((double)Tw.y) || !isfinite((double)Tw.z))
+                        atomicMin(bad, i);
+                }
+                if (P.pene_on) {
+                    const int pf = P.pene_face;
+                    const V3<double> pp = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
+                    const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
+                    const V3<double> RV = xi - pp;
+                    bool touch = false;
+                    V3<double> RVo = mk<double>(0.0, 0.0, 0.0), Dist = RVo, DistU64 = RVo;
+                    double DistL = 0.0, Dn = 0.0;
+                    bool have_u = false;
+                    if (P.pshape == NBC_PENE_SPHERE) {
+                        // src/dem_sphere_intruder.cuh:27-50
+                        const double cd = Length<double>(RV);
+                        V3<double> nrm;
+                        if (cd > 1.0e-15) {
+                            nrm = (1.0 / cd) * RV;
+                        } else {
+                            const double t = sqrt(pdir.x * pdir.x + pdir.y * pdir.y + pdir.z * pdir.z);
+                            nrm = t > 1e-20 ? mk<double>(pdir.x / t, pdir.y / t, pdir.z / t) : mk<double>(0.0, 0.0, 0.0);
+                            if (Length<double>(nrm) < 1.0e-15) nrm = mk<double>(0.0, 0.0, 1.0);
+                        }
+                        const double gap = cd - P.prad;
+                        const double ov = ri - gap;
+                        if (ov > 0.0) {
+                            touch = true;
+                            RVo = P.prad * nrm;
+                            Dist = gap * nrm;
+                            DistL = gap;
+                            Dn = ov;
+                            DistU64 = nrm;
+                            have_u = true;
+                            if (P.pwait) atomicCAS(&pst->contact_state, 0, 1);
+                        }
+                    } else if (P.pshape == NBC_PENE_CONE) {
+                        // spheres outside the mesh's bounding sphere cannot touch it
+                        const double reach = P.pbound + ri;
+                        if (Dot<double>(RV, RV) < reach * reach) {
+                        double best = 1e300;
+                        V3<double> bq = mk<double>(0.0, 0.0, 0.0);
+                        for (int t = 0; t < P.ntri; ++t) {
+                            const double* v = P.ptri + 9 * t;
+                            const V3<double> q = closest_on_tri(RV, mk<double>(v[0], v[1], v[2]), mk<double>(v[3], v[4], v[5]),
+                                                                mk<double>(v[6], v[7], v[8]));
+                            const V3<double> dq = RV - q;
+                            const double ds = Dot<double>(dq, dq);
+                            if (ds < best) { best = ds; bq = q; }
+                        }
+                        const V3<double> RVn = RV - bq;
+                        const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                        const double dl = sqrt(DistS);
+                        if (P.ntri > 0 && ri - dl > 0.0) {
+                            touch = true; RVo = bq; Dist = RVn; DistL = dl; Dn = ri - dl;
+                        }
+                        }
+                    } else {
+                        // src/dem_contact_penetrator.cu:42-83
+                        const double LatDx = P.cell[0];
+                        const double ERR = Dot<double>(RV, pdir);
+                        if (fabs(ERR) < (P.pheight + 0.80 * LatDx)) {
+                            const V3<double> RVb = RV - ERR * pdir;
+                            const double norm = Length<double>(RVb);
+                            if (norm < (P.prad + 0.80 * LatDx)) {
+                                const V3<double> RVt = ERR * pdir;
+                                V3<double> o3;
+                                if (fabs(ERR) < P.pheight) o3 = (1.0 / norm) * P.prad * RVb + RVt;
+                                else if (norm < P.prad) o3 = (1.0 / fabs(ERR)) * P.pheight * RVt + RVb;
+                                else o3 = (1.0 / norm) * P.prad * RVb + (1.0 / fabs(ERR)) * P.pheight * RVt;
+                                const V3<double> RVn = RV - o3;
+                                const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                                const double dl = sqrt(DistS);
+                                if (ri - dl > 0.0) { touch = true; RVo = o3; Dist = RVn; DistL = dl; Dn = ri - dl; }
+                            }
+                        }
+                    }
+                    if (touch) {
+                        const R DistLr = (R)DistL;
+                        const V3<R> DistU = have_u ? cvt3<R>(DistU64) : (R(1.0) / DistLr) * cvt3<R>(Dist);
+                        R Rij = Rr;
+                        if (P.pshape == NBC_PENE_SPHERE) Rij = Rr * (R)P.prad / (Rr + (R)P.prad);
+                        R Mij = mi;
+                        if (P.pmotion == NBC_PENE_GRAVITY) Mij = mi * (R)P.pmass / (mi + (R)P.pmass);
+                        V3<R> H, Mt, Mr;
+                        R* hp = S.wh + ((size_t)pf * P.N + i) * HSTRIDE;
+                        if ((wf >> pf) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                        else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                        const V3<R> pv = mk<R>((R)pst->vel[0], (R)pst->vel[1], (R)pst->vel[2]);
+                        const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
+                        const V3<R> RVor = cvt3<R>(RVo);
+                        WallOut<R> o;
+                        wall_force<R>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
+                                      pw, H, Mt, Mr, o);
+                        const V3<R> cf = mk<R>(R(0), R(0), R(0));
+                        Fp = o.nf + o.tf;
+                        Tp = -DistLr * Cross<R>(DistU, o.tf);
+                        V3<R> pTc = R(-1.0) * Cross<R>(RVor, (o.nf + o.tf + cf));
+                        Tp = Tp - o.rm - o.tm;
+                        pTc = pTc + o.rm + o.tm;
+                        store_hist<R>(hp, o.ht, o.htw, o.hr);
+                        pf_sum = mk<double>((double)Fp.x, (double)Fp.y, (double)Fp.z);
+                        pt_sum = mk<double>((double)pTc.x, (double)pTc.y, (double)pTc.z);
+                        n_w++;
+                        if (RECORD) {
+                            const V3<R> tn = Cross<R>(o.nf, o.arm);
+                            const V3<R> tt = Cross<R>(o.tf, o.arm) - o.rm - o.tm;
+                            D4 a; a.x = o.nf.x; a.y = o.nf.y; a.z = o.nf.z; a.w = 0.0;
+                            D4 b; b.x = o.tf.x; b.y = o.tf.y; b.z = o.tf.z; b.w = 0.0;
+                            D4 c; c.x = tn.x; c.y = tn.y; c.z = tn.z; c.w = 0.0;
+                            D4 d; d.x = tt.x; d.y = tt.y; d.z = tt.z; d.w = 0.0;
+                            rec.pene[4 * (size_t)i] = a; rec.pene[4 * (size_t)i + 1] = b;
+                            rec.pene[4 * (size_t)i + 2] = c; rec.pene[4 * (size_t)i + 3] = d;
+                        }
+                    }
+                    if (touch) wf |= 1u << pf; else wf &= ~(1u << pf);
+                }
+            }
+            if (wf != wf0) S.wflag[i] = wf;
+            // totF/totT (src/dem_clump.cu:97-98) and the clump torque term
+            // T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
+            const V3<R> tF = Fss + Fw + Fp;
+            const V3<R> tT = Tss + Tw + Tp;
+            V3<R> r;
+            if (sizeof(R) == sizeof(double)) {
+                const D4 pc = C.pq[2 * ci];
+                r = cvt3<R>(mk<double>(xi.x - pc.x, xi.y - pc.y, xi.z - pc.z));
+            } else {
+                r = armi;  // x_s - x_c = R(q) * local up to FP64 rounding
+            }
+            const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
+                                   tT.z + (r.x * tF.y - r.y * tF.x));
+            S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
+            S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+            if (RECORD) {
+                D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
+                D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
+                rec.tot[2 * (size_t)i] = a;
+                rec.tot[2 * (size_t)i + 1] = b;
+                D4 q[6];
+                const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
+                for (int k = 0; k < 6; ++k) {
+                    q[k].x = parts[k].x; q[k].y = parts[k].y; q[k].z = parts[k].z; q[k].w = 0.0;
+                    rec.parts[6 * (size_t)i + k] = q[k];
+                }
+            }
+        }
+
+        acc_f = acc_f + pf_sum;
+        acc_t = acc_t + pt_sum;
+    }
+    // wave reductions: touching count and the FP64 penetrator partial sums of
+    // this region (a fixed tree over the lanes)
+    double v[6] = {acc_f.x, acc_f.y, acc_f.z, acc_t.x, acc_t.y, acc_t.z};
+    unsigned a1 = (unsigned)n_w;
     for (int o = 32; o > 0; o >>= 1) {
         for (int k = 0; k < 6; ++k) v[k] += __shfl_down(v[k], o, 64);
-        a0 += __shfl_down(a0, o, 64);
         a1 += __shfl_down(a1, o, 64);
     }
     if (lane == 0) {
-        for (int k = 0; k < 6; ++k) red[k][wv] = v[k];
-        cnt[0][wv] = a0;
-        cnt[1][wv] = a1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        unsigned long long s0 = 0, s1 = 0;
-        for (int k = 0; k < nw; ++k) { s0 += cnt[0][k]; s1 += cnt[1][k]; }
-        if (s0) atomicAdd(&counters[0], s0);
-        if (s1) atomicAdd(&counters[1], s1);
-        if (P.pene_on) {
-            for (int k = 0; k < 6; ++k) {
-                double s = 0.0;
-                for (int q = 0; q < nw; ++q) s += red[k][q];
-                pene_partial[6 * (size_t)lb + k] = s;
-            }
-        }
+        if (a1) atomicAdd(&counters[1], (unsigned long long)a1);
+        if (P.pene_on)
+            for (int k = 0; k < 6; ++k) pene_partial[6 * (size_t)lb + k] = v[k];
     }
 }
 
@@ -1384,6 +1684,31 @@ int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* po
     return 0;
 }
 
+int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
+                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint2* nf, int* slot2near, int* nbcnt,
+                      int* nboff, int2* nb, int* stmp, int* stotal, int block) {
+    const int N = P.N;
+    const unsigned g = nblk(N, block);
+    int e;
+    if ((e = launch_fill_i32(st, ncnt + N, 1, 0))) return e;
+    if ((e = launch_fill_i32(st, nbcnt + N, 1, 0))) return e;
+    k_near_count<<<g, block, 0, st>>>(P, pos, foff, fword, ncnt);
+    LAUNCH_CHECK();
+    if ((e = launch_exclusive_scan(st, ncnt, nfoff, N + 1, stmp, stotal))) return e;
+    k_near_fill<<<g, block, 0, st>>>(P, pos, foff, fword, nfoff, nf, slot2near);
+    LAUNCH_CHECK();
+    k_near_back_count<<<g, block, 0, st>>>(N, boff, bent, slot2near, nbcnt);
+    LAUNCH_CHECK();
+    if ((e = launch_exclusive_scan(st, nbcnt, nboff, N + 1, stmp, stotal + 1))) return e;
+    k_near_back_fill<<<g, block, 0, st>>>(N, boff, bent, slot2near, nboff, nb);
+    LAUNCH_CHECK();
+    return 0;
+}
+int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint2* nf, uint32_t* fword, int block) {
+    k_near_sync<<<nblk(N, block), block, 0, st>>>(N, nfoff, nf, fword);
+    LAUNCH_CHECK();
+    return 0;
+}
 int launch_back_resolve(hipStream_t st, int N, const int* foff, const uint32_t* fword, const int* boff, int2* bent,
                         int block) {
     k_back_resolve<<<nblk(N, block), block, 0, st>>>(N, foff, fword, boff, bent);
@@ -1420,8 +1745,8 @@ int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
     return 0;
 }
 template <class R>
-int launch_gather(hipStream_t st, int N, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst) {
-    k_gather<R><<<nblk(N, 256), 256, 0, st>>>(N, nface, perm, src, dst);
+int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst) {
+    k_gather<R><<<nblk(P.N, 256), 256, 0, st>>>(P, nface, perm, src, dst);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1435,13 +1760,26 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
 int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                    int block, int remap, const int* halt, int step) {
+                    PeneState* pst, CandBuf<R> W, unsigned long long* ctr, RecordBuf rec, bool record, int block,
+                    int remap, const int* halt, int step) {
     const unsigned nb = nblk(P.N, block);
     if (record)
-        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap, halt, step);
+        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, W, ctr, rec, remap, halt, step);
     else
-        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, partial, ctr, bad, rec, remap, halt, step);
+        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, W, ctr, rec, remap, halt, step);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, CandBuf<R> W, PeneState* pst,
+                 double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block,
+                 const int* halt, int step) {
+    const int nreg = (int)nblk(P.N, block);  // candidate regions: one per k_contacts block
+    const unsigned nb = (unsigned)((nreg + 3) / 4);
+    if (record)
+        k_walls<R, true><<<nb, 256, 0, st>>>(P, S, C, W, nreg, block, pst, partial, ctr, bad, rec, halt, step);
+    else
+        k_walls<R, false><<<nb, 256, 0, st>>>(P, S, C, W, nreg, block, pst, partial, ctr, bad, rec, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1457,12 +1795,14 @@ int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
     template int launch_clump_before<R>(hipStream_t, const DevParams&, ClumpBuf<R>, PeneState*, unsigned long long*, \
                                         int, const int*, int);                                                       \
     template int launch_sphere_pose<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, int*, int, int);    \
-    template int launch_gather<R>(hipStream_t, int, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
+    template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
     template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
-                                    PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, int, int,       \
+                                    PeneState*, CandBuf<R>, unsigned long long*, RecordBuf, bool, int, int,          \
                                     const int*, int);                                                                \
+    template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, CandBuf<R>, PeneState*,   \
+                                 double*, unsigned long long*, int*, RecordBuf, bool, int, const int*, int);         \
     template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, const int*,         \
                                        const int*, const int*, int, const int*, int);
 INST(double)

@@ -173,3 +173,37 @@ def test_sqrt_div_rounding_on_device_matches_host():
     ref.step(3)
     assert_lists_equal(ctx, ref)
     assert ctx.stats().touching_ss == ref.stats().touching_ss
+
+
+@pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
+def test_fast_collisions_between_rebuilds(prec):
+    """Clumps thrown at each other at up to 0.5 m/s: pairs close from outside
+    the contact kernel's near range (gap 2 * Thresh at the last rebuild) and
+    collide between rebuilds. Touching sets, histories and states must follow
+    the oracle, which scans every list pair every step."""
+    banner(f"fast collisions between rebuilds, precision {prec}")
+    bed = nbc.generate_bed(1, 300, seed=8)
+    rng = np.random.default_rng(8)
+    bed.cvel[:] = rng.uniform(-0.5, 0.5, bed.cvel.shape)
+    bed.cangv[:] = rng.uniform(-50.0, 50.0, bed.cangv.shape)
+    bed.params.StepSize = 1e-5  # up to 5 um per step: a rebuild every ~50 steps
+    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
+    ctx, ref = pair_setup(bed, opt)
+    seen = 0
+    for k in range(20):
+        ctx.step(20)
+        ref.step(20)
+        sa, sb = ctx.stats(), ref.stats()
+        seen = max(seen, sb.touching_ss)
+        if prec == nbc.NBC_FP64:
+            assert (sa.touching_ss, sa.touching_wall, sa.rebuilds) == (sb.touching_ss, sb.touching_wall, sb.rebuilds), k
+    sb = ref.stats()
+    assert sb.rebuilds >= 4 and seen > 0
+    if prec == nbc.NBC_FP64:
+        assert_lists_equal(ctx, ref)
+        assert np.abs(ctx.spheres()["pos"] - ref.spheres()["pos"]).max() <= 1e-12
+        ha, hb = ctx.pair_history(), ref.pair_history()
+        assert relerr(ha, hb) <= FP64_TOL
+    else:
+        ca, cb = ctx.clumps(), ref.clumps()
+        assert np.abs(ca["pos"] - cb["pos"]).max() < 1e-7
