@@ -99,6 +99,8 @@ struct DevParams {
     double half;          // largest box half-extent
     float f32_marg;       // absolute slack of the FP32 distance pre-filter
     double near_add;      // 2 * Thresh: gap below which a pair enters the near list
+    double wl_slack;      // wall list: distance a sphere may move after the list is built (2 * Thresh)
+    double pene_move;     // wall list: penetrator travel allowed before the list is rebuilt
     int wall_type;        // NBC_WALL_*
     int nfaces;           // wall faces in use (6 cube, 3 cylinder, 0 none)
     int pene_on;          // isIntruder
@@ -116,8 +118,14 @@ struct DevParams {
 struct PeneState {
     double pos[3], vel[3], angv[3], quat[4];
     double totF[3], totT[3];
+    double lpos[3];       // position when the wall list was last built
     int contact_state;
     int pad;
 };
+
+// bits of the status word next to the halt step
+constexpr int ST_CLUMP_BAD = 1;  // a clump state is not finite (sticky)
+constexpr int ST_REBUILD = 2;    // a sphere moved more than Thresh: rebuild the lists
+constexpr int ST_WLIST = 4;      // the penetrator left its allowance: rebuild the wall list
 
 }  // namespace nbc

@@ -117,8 +117,13 @@ class Engine : public EngineBase {
         }
         dp.f32_marg = (float)(4.0 * dp.half * std::ldexp(1.0, -23));
         // pairs farther apart than 2 * Thresh at a rebuild cannot touch before the next one
-        dp.near_add = (std::isfinite(P.Thresh) && P.Thresh >= 0.0) ? 2.0 * P.Thresh
-                                                                 : std::numeric_limits<double>::infinity();
+        const bool thresh_ok = std::isfinite(P.Thresh) && P.Thresh >= 0.0;
+        const double inf = std::numeric_limits<double>::infinity();
+        dp.near_add = thresh_ok ? 2.0 * P.Thresh : inf;
+        // wall list: spheres move at most 2 * Thresh after a list build; the
+        // penetrator may move pene_move before the list is rebuilt
+        dp.wl_slack = thresh_ok ? 2.0 * P.Thresh * (1.0 + 1e-9) + 1e-12 : inf;
+        dp.pene_move = thresh_ok ? 8.0 * P.Thresh : inf;
         dp.wall_type = wall_type; dp.nfaces = nfaces;
         dp.pene_on = P.isIntruder ? 1 : 0;
         dp.pene_face = nfaces;
@@ -160,9 +165,11 @@ class Engine : public EngineBase {
         HIPC(dalloc(&counters, 2));
         HIPC(dalloc(&pst, 1));
         nbk = contacts_blocks(N, block);
-        HIPC(dalloc(&pene_partial, 6 * (size_t)nbk));
-        HIPC(dalloc(&cand.cnt, nbk)); HIPC(dalloc(&cand.idx, (size_t)nbk * block));
-        HIPC(dalloc(&cand.raw, 2 * (size_t)nbk * block));
+        HIPC(dalloc(&pene_partial, 6 * (((size_t)N + 63) / 64)));  // one per wave of k_walls
+        if (nfaces > 0 || dp.pene_on) {
+            HIPC(dalloc(&wb.list, N)); HIPC(dalloc(&wb.pos_of, N)); HIPC(dalloc(&wb.res, 4 * (size_t)N));
+            HIPC(dalloc(&wl_off, (size_t)N + 1));
+        }
         HIPC(dalloc(&cache_f, 3 * (size_t)oi)); HIPC(dalloc(&cache_t, 3 * (size_t)oi));
         HIPC(dalloc(&ke_partial, (Nc + 255) / 256));
         HIPC(hipHostMalloc((void**)&hstatus, 4 * sizeof(uint32_t), hipHostMallocDefault));
@@ -232,7 +239,7 @@ class Engine : public EngineBase {
         F(boff); F(bent); F(nfoff); F(nboff); F(nfl); F(nbl); F(slot2near);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
-        F(dstatus); F(counters); F(pst); F(pene_partial); F(cand.cnt); F(cand.idx); F(cand.raw); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
+        F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         if (hstatus) (void)hipHostFree(hstatus);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
@@ -337,6 +344,7 @@ class Engine : public EngineBase {
             if (!pe) return fail(NBC_E_BADINPUT, "nbc_upload: isIntruder is set but no penetrator was given");
             for (int a = 0; a < 3; ++a) {
                 ps.pos[a] = pe->pos[a]; ps.vel[a] = pe->vel[a]; ps.angv[a] = pe->angv[a]; dp.pdir[a] = pe->dir[a];
+                ps.lpos[a] = pe->pos[a];
             }
             for (int a = 0; a < 4; ++a) ps.quat[a] = pe->quat[a];
             ps.contact_state = pe->contact_state;
@@ -447,6 +455,8 @@ class Engine : public EngineBase {
         const int* cv = second ? val1 : val0;
         HIPC(hipMemcpyAsync(mem, cv, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
         KC(launch_seg_bounds(st, ck, N, Nc, mbeg, mend));
+        // 7. wall list of the new order
+        { int e = build_wall_list(); if (e) return e; }
         nrebuild++;
         prof_end(3, e0, -1);
         return 0;
@@ -482,7 +492,7 @@ class Engine : public EngineBase {
     }
 
     int read_status() {
-        HIPC(hipMemcpyAsync(hstatus, dstatus, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(hstatus, dstatus, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         return 0;
     }
@@ -490,7 +500,7 @@ class Engine : public EngineBase {
     // K1 + K2 of step n (src/dem_simulation_clump.cu:113-119)
     int enqueue_head(long long n) {
         hipEvent_t e0 = prof_begin();
-        KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (const int*)dstatus, (int)n));
+        KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (int*)dstatus, (int)n));
         KC(launch_sphere_pose<R>(st, dp, S[scur], C, (int*)dstatus, (int)n, block));
         prof_end(0, e0, n);
         return 0;
@@ -510,18 +520,21 @@ class Engine : public EngineBase {
         L.nfoff = nfoff; L.nf = nfl; L.nboff = nboff; L.nb = nbl;
         L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
         const int* halt = (const int*)dstatus;
-        hipEvent_t e0 = prof_begin();
-        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, cand, counters, rec, record, block, 1, halt, (int)n));
-        prof_end(1, e0, n);
-        if (dp.nfaces > 0 || dp.pene_on) {
+        int* bad = (int*)(dstatus + 1);
+        hipEvent_t e0;
+        if (wl_len > 0) {
             e0 = prof_begin();
-            KC(launch_walls<R>(st, dp, S[scur], C, cand, pst, pene_partial, counters, (int*)(dstatus + 1), rec, record,
-                               block, halt, (int)n));
+            KC(launch_walls<R>(st, dp, S[scur], C, wb, wl_len, pst, pene_partial, counters, bad, rec, record, halt,
+                               (int)n));
             prof_end(5, e0, n);
         }
+        e0 = prof_begin();
+        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, wb, counters, bad, rec, record, block, 1, halt, (int)n));
+        prof_end(1, e0, n);
         if (dp.pene_on) {
             e0 = prof_begin();
-            KC(launch_pene_finalize(st, dp, pst, pene_partial, nbk, (int)(n % oi), cache_f, cache_t, halt, (int)n));
+            KC(launch_pene_finalize(st, dp, pst, pene_partial, (wl_len + 63) / 64, (int)(n % oi), cache_f, cache_t,
+                                    halt, (int)n));
             prof_end(4, e0, n);
         }
         e0 = prof_begin();
@@ -531,7 +544,19 @@ class Engine : public EngineBase {
     }
 
     int clear_halt() {
-        KC(launch_fill_i32(st, (int*)dstatus, 1, HALT_NONE));
+        KC(launch_clear_halt(st, (int*)dstatus, HALT_NONE));
+        return 0;
+    }
+
+    // Wall list at the current positions and penetrator position.
+    int build_wall_list() {
+        if (!(nfaces > 0 || dp.pene_on)) return 0;
+        const SphereBuf<R>& s = S[scur];
+        KC(launch_wlist_build(st, dp, s.pos, s.wflag, pst, fcnt, wl_off, wb.list, wb.pos_of, stmp, stotal + 2, block));
+        int len = 0;
+        HIPC(hipMemcpyAsync(&len, wl_off + N, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        wl_len = len;
         return 0;
     }
 
@@ -569,9 +594,14 @@ class Engine : public EngineBase {
                 n += m;
                 continue;
             }
-            // steps n .. h-1 completed; step h stopped after k_sphere_pose
+            // steps n .. h-1 completed; step h stopped after k_sphere_pose for
+            // a rebuild, or for a new wall list only
             prof_drop(h);
-            if ((e = rebuild(h & 1))) return e;
+            if ((hstatus[3] & ST_REBUILD) || !(hstatus[3] & ST_WLIST)) {
+                if ((e = rebuild(h & 1))) return e;
+            } else {
+                if ((e = build_wall_list())) return e;
+            }
             if ((e = clear_halt())) return e;
             if ((e = enqueue_tail(h, rec && h == end - 1))) return e;
             n = (long long)h + 1;
@@ -922,12 +952,14 @@ class Engine : public EngineBase {
     int *val0 = nullptr, *val1 = nullptr, *hist = nullptr, *hscan = nullptr, *stmp = nullptr, *stotal = nullptr;
     int *perm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
     int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;
-    uint32_t* dstatus = nullptr;  // [0] halt step, [1] smallest bad sphere, [2] overflow
+    uint32_t* dstatus = nullptr;  // [0] halt step, [1] smallest bad sphere, [2] overflow, [3] clump state not finite
     uint32_t* hstatus = nullptr;
     unsigned long long* counters = nullptr;
     PeneState* pst = nullptr;
     double *pene_partial = nullptr, *cache_f = nullptr, *cache_t = nullptr, *ke_partial = nullptr, *dtri = nullptr;
-    CandBuf<R> cand = {nullptr, nullptr, nullptr};
+    WallBuf<R> wb = {nullptr, nullptr, nullptr};
+    int* wl_off = nullptr;
+    int wl_len = 0;  // spheres on the wall list
     int *nfoff = nullptr, *nboff = nullptr, *slot2near = nullptr;
     uint2* nfl = nullptr;
     int2* nbl = nullptr;

@@ -46,14 +46,13 @@ struct ListBuf {
     R* hist[2];       // [front slots][HSTRIDE] ping-pong histories
 };
 
-// Wall/penetrator candidates of k_contacts, per block of k_contacts: the
-// count, then in slots [b * block, b * block + count) the sphere index and its
-// sphere-sphere force and torque sums.
+// Wall list: spheres that may touch a wall or the penetrator until the list
+// is rebuilt, in sphere order, with their wall/penetrator results per step.
 template <class R>
-struct CandBuf {
-    int* cnt;                         // [blocks]
-    int* idx;                         // [blocks * block]
-    typename Vec4T<R>::type* raw;     // [2 * blocks * block]
+struct WallBuf {
+    int* list;                        // [len] sphere index
+    int* pos_of;                      // [N] index into list, or -1
+    typename Vec4T<R>::type* res;     // [4 * len] Fw, Tw, Fp, Tp
 };
 
 struct RecordBuf {
@@ -64,6 +63,7 @@ struct RecordBuf {
     D4* pair_cpt;  // [front slots] contact point (CacheCtp)
 };
 
+int launch_clear_halt(hipStream_t st, int* status, int none);
 int launch_fill_i32(hipStream_t st, int* a, long long n, int v);
 int launch_fill_u32(hipStream_t st, uint32_t* a, long long n, uint32_t v);
 int launch_fill_f64(hipStream_t st, double* a, long long n, double v);
@@ -90,7 +90,7 @@ int contacts_blocks(int N, int block);
 // kernel of that step, and every kernel of later steps, returns at entry.
 template <class R>
 int launch_clump_before(hipStream_t st, const DevParams& P, ClumpBuf<R> C, PeneState* pst, unsigned long long* ctr,
-                        int block, const int* halt, int step);
+                        int block, int* halt, int step);
 template <class R>
 int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int step,
                        int block);
@@ -105,12 +105,14 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const i
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint2* nf, uint32_t* fword, int block);
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    PeneState* pst, CandBuf<R> W, unsigned long long* ctr, RecordBuf rec, bool record, int block,
-                    int remap, const int* halt, int step);
+                    PeneState* pst, WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
+                    int block, int remap, const int* halt, int step);
 template <class R>
-int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, CandBuf<R> W, PeneState* pst,
-                 double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block,
+int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
+                 PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
                  const int* halt, int step);
+int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* wflag, PeneState* pst,
+                       int* cnt, int* off, int* list, int* pos_of, int* stmp, int* stotal, int block);
 template <class R>
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
                        const int* mend, const int* mem, int block, const int* halt, int step);

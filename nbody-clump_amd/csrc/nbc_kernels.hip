@@ -97,7 +97,7 @@ __device__ __forceinline__ bool maybe_touch(const DevParams& P, const F4& a, con
 // the step counters.
 template <class R>
 __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsigned long long* counters,
-                               const int* halt, int n) {
+                               int* halt, int n) {
     if (*halt < n) return;  // a previous step asked for a rebuild: replayed later
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -119,6 +119,14 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
                 pst->pos[0] = pst->pos[0] + Dt * pst->vel[0];
                 pst->pos[1] = pst->pos[1] + Dt * pst->vel[1];
                 pst->pos[2] = pst->pos[2] + Dt * pst->vel[2];
+            }
+            // the wall list covers the penetrator's reach plus pene_move:
+            // past that, stop this step after k_sphere_pose for a new list
+            const double dx = pst->pos[0] - pst->lpos[0], dy = pst->pos[1] - pst->lpos[1],
+                         dz = pst->pos[2] - pst->lpos[2];
+            if (!(dx * dx + dy * dy + dz * dz <= P.pene_move * P.pene_move)) {
+                atomicOr(&halt[3], ST_WLIST);
+                atomicMin(&halt[0], n);
             }
         }
     }
@@ -172,6 +180,7 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
     // first record flags a finite clump state (checked as in
     // src/dem_force_cube_wall_clump.cu:196-197)
     const bool ok = finite3(vel) && finite3(w) && isfinite(nx) && isfinite(ny) && isfinite(nz) && isfinite(nw);
+    if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);  // k_contacts then checks every sphere exactly
     C.snap[2 * i] = mk4<R>((R)vel.x, (R)vel.y, (R)vel.z, ok ? R(1) : R(0));
     C.snap[2 * i + 1] = mk4<R>((R)w.x, (R)w.y, (R)w.z, R(0));
 }
@@ -205,7 +214,10 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
     // the rebuild request (src/dem_clump.cu:132-134) records the step index:
     // later kernels of this step and all kernels of later steps stand down
     const uint64_t b = __ballot(over);
-    if (b != 0ull && lane_id() == (int)__ffsll((unsigned long long)b) - 1) atomicMin(halt, n);
+    if (b != 0ull && lane_id() == (int)__ffsll((unsigned long long)b) - 1) {
+        atomicOr(&halt[3], ST_REBUILD);
+        atomicMin(halt, n);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -846,50 +858,57 @@ __device__ __forceinline__ V3<double> closest_on_tri(const V3<double>& p, const 
 
 // ---------------------------------------------------------------------------
 // K3: all contacts of sphere i, then its force and clump-torque contribution.
-// Candidate tests for k_walls: the exact FP64 face predicates of the wall
-// code below (a face is evaluated only when its predicate holds) and the
-// penetrator culls; a sphere failing all of them has no wall or penetrator
-// contact this step.
-__device__ __forceinline__ bool wall_candidate(const DevParams& P, const V3<double>& xi, double ri) {
+// Candidate tests for k_walls. With sl = 0 they are the exact FP64 face
+// predicates of the wall code (a face is evaluated only when its predicate
+// holds) and the penetrator culls. Every predicate is a threshold on a
+// distance-like function of the position with Lipschitz constant at most
+// (1 + |dir|^2), so lowering the thresholds by sl * (1 + |dir|^2) gives a test
+// that holds for every position within sl of xi: k_contacts runs it on the
+// FP32 position copy. A sphere failing it has no wall or penetrator contact.
+__device__ __forceinline__ bool wall_candidate(const DevParams& P, const V3<double>& xi, double ri, double sl) {
     const V3<double> RV = xi - mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
     if (P.wall_type == NBC_WALL_CUBE) {
         bool c = false;
         for (int a = 0; a < 3; ++a) {
             const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
             const double Sd = P.wside[a];
-            c = c || (E > 0 && E + ri - Sd > 0) || (E < 0 && E - ri + Sd < 0);
+            c = c || (E > -sl && E + ri - Sd > -sl) || (E < sl && E - ri + Sd < sl);
         }
         return c;
     }
     const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+    const double s2 = sl * (1.0 + Dot<double>(Wv, Wv));
     const double ERR = Dot<double>(RV, Wv);
     const V3<double> RVb = RV - ERR * Wv;
     const double norm = Length<double>(RVb);
     const double d_cap = fabs(ERR) - P.wheight + ri;
     const double d_side = norm - P.wrad + ri;
-    return (d_cap >= 1e-14 && ERR != 0.0) || d_side >= 1e-14 || !(d_cap == d_cap) || !(d_side == d_side);
+    return (d_cap >= 1e-14 - s2 && (ERR != 0.0 || sl > 0.0)) || d_side >= 1e-14 - s2 || !(d_cap == d_cap) ||
+           !(d_side == d_side);
 }
 
 __device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneState* pst, const V3<double>& xi,
-                                               double ri) {
+                                               double ri, double sl) {
     const V3<double> RV = xi - mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
-    if (P.pshape == NBC_PENE_SPHERE) return ri - (Length<double>(RV) - P.prad) > 0.0;
+    if (P.pshape == NBC_PENE_SPHERE) return ri - (Length<double>(RV) - P.prad) > -sl;
     if (P.pshape == NBC_PENE_CONE) {
-        const double reach = P.pbound + ri;
+        const double reach = P.pbound + ri + sl;
         return P.ntri > 0 && Dot<double>(RV, RV) < reach * reach;
     }
     const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
+    const double s2 = sl * (1.0 + Dot<double>(pdir, pdir));
     const double LatDx = P.cell[0];
     const double ERR = Dot<double>(RV, pdir);
-    if (!(fabs(ERR) < (P.pheight + 0.80 * LatDx))) return false;
+    if (!(fabs(ERR) < (P.pheight + 0.80 * LatDx) + s2)) return false;
     const double norm = Length<double>(RV - ERR * pdir);
-    return norm < (P.prad + 0.80 * LatDx);
+    return norm < (P.prad + 0.80 * LatDx) + s2;
 }
 
 template <class R, bool RECORD>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                                                  PeneState* pst, CandBuf<R> W, unsigned long long* counters,
-                                                  RecordBuf rec, int remap, const int* halt, int n) {
+                                                  PeneState* pst, WallBuf<R> WL, unsigned long long* counters,
+                                                  int* bad, RecordBuf rec, int remap, const int* halt, int n) {
+    const int* status = halt + 3;  // ST_* bits
     typedef typename Vec4T<R>::type R4;
     if (*halt <= n) return;
     // Per-wave queue of touching pairs: the wave scans its list segments, pushes
@@ -1018,12 +1037,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     // front entries: pairs the row's sphere owns (it writes their history)
     {
         const int F1 = s_fo[wv][64];
-        for (int c = s_fo[wv][0]; c < F1; c += 64) {
+        int c = s_fo[wv][0];
+        // the next chunk's pair words are loaded one iteration ahead
+        uint32_t wnext = c + lane < F1 ? L.nf[c + lane].y : 0u;
+        for (; c < F1; c += 64) {
             const int e = c + lane;
             bool touch = false;
             int j = 0, r = 0;
+            const uint32_t wcur = wnext;
+            if (e + 64 < F1) wnext = L.nf[e + 64].y;
             if (e < F1) {
-                const uint32_t w = L.nf[e].y;
+                const uint32_t w = wcur;
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_fo[wv][r + st] <= e) r += st;
@@ -1054,13 +1078,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     // again from the same inputs and the row's sphere takes the reaction
     {
         const int B1 = s_bo[wv][64];
-        for (int c = s_bo[wv][0]; c < B1; c += 64) {
+        int c = s_bo[wv][0];
+        int2 bnext = c + lane < B1 ? L.nb[c + lane] : make_int2(0, -1);
+        for (; c < B1; c += 64) {
             const int e = c + lane;
             bool touch = false;
             int r = 0;
-            int2 be = make_int2(0, -1);
+            int2 be = bnext;
+            if (e + 64 < B1) bnext = L.nb[e + 64];
             if (e < B1) {
-                be = L.nb[e];
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_bo[wv][r + st] <= e) r += st;
@@ -1088,392 +1114,379 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     // Wall and penetrator contacts are rare. A sphere that may have one, had
     // one last step, or fails the input check of the wall kernel is handed to
     // k_walls with its sphere-sphere sums; every other sphere finishes here.
-    bool cand = false;
     if (active) {
-        const D4 pi4 = S.pos[i];
-        const V3<double> xi = xyz(pi4);
-        const double ri = pi4.w;
         const R4 ki = S.kin[i];
-        if (P.nfaces > 0 || P.pene_on) {
-            cand = S.wflag[i] != 0u;
-            if (!cand && P.nfaces > 0) {
+        int wk = -1;
+        if (P.nfaces > 0 || P.pene_on) wk = WL.pos_of[i];
+        if (P.nfaces > 0 && wk < 0) {
+            // input check of the wall kernel (src/dem_force_cube_wall_clump.cu:196-197)
+            // for spheres off the wall list: exact only when a cheap sign is off
+            const F4 q = s_p32[wv][lane];
+            if (!(q.w > 0.0f) || !isfinite((double)ki.x) || !isfinite((double)ki.y) || !isfinite((double)ki.z) ||
+                !isfinite((double)ki.w) || !(ki.w > R(0)) || (*status & ST_CLUMP_BAD)) {
+                const D4 pi4 = S.pos[i];
+                const R4 li = S.loc[i];
                 const int c = S.cid[i];
-                const bool ok = c >= 0 && c < P.Nc && C.snap[2 * c].w != R(0) && finite3(xi) && isfinite(ri) &&
-                                ri > 0.0 && isfinite((double)ki.x) && isfinite((double)ki.y) &&
-                                isfinite((double)ki.z) && isfinite((double)ki.w) && ki.w > R(0);
-                cand = !ok || wall_candidate(P, xi, ri);
+                const bool cid_ok = c >= 0 && c < P.Nc;
+                const bool ok = cid_ok && C.snap[2 * (cid_ok ? c : 0)].w != R(0) && finite3(xyz(pi4)) &&
+                                isfinite((double)li.x) && isfinite((double)li.y) && isfinite((double)li.z) &&
+                                isfinite(pi4.w) && pi4.w > 0.0 && isfinite((double)ki.w) && ki.w > R(0);
+                if (!ok) atomicMin(bad, i);
             }
-            if (!cand && P.pene_on) cand = pene_candidate(P, pst, xi, ri);
         }
-        if (!cand) {
-            // totF/totT with zero wall and penetrator terms (src/dem_clump.cu:97-98)
-            // and the clump torque term T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
-            const V3<R> Z = mk<R>(R(0), R(0), R(0));
-            const V3<R> tF = Fss + Z + Z;
-            const V3<R> tT = Tss + Z + Z;
-            V3<R> r;
-            if (sizeof(R) == sizeof(double)) {
-                const D4 pc = C.pq[2 * S.cid[i]];
-                r = cvt3<R>(mk<double>(xi.x - pc.x, xi.y - pc.y, xi.z - pc.z));
-            } else {
-                r = Ld4<R>::xyz(ki);  // x_s - x_c = R(q) * local up to FP64 rounding
-            }
-            const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
-                                   tT.z + (r.x * tF.y - r.y * tF.x));
-            S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
-            S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
-            if (RECORD) {
-                D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
-                D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
-                rec.tot[2 * (size_t)i] = a;
-                rec.tot[2 * (size_t)i + 1] = b;
-                const V3<R> parts[6] = {Fss, Tss, Z, Z, Z, Z};
-                for (int k = 0; k < 6; ++k) {
-                    D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
-                    rec.parts[6 * (size_t)i + k] = q;
-                }
+        // totF/totT (src/dem_clump.cu:97-98) and the clump torque term
+        // T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
+        const V3<R> Z = mk<R>(R(0), R(0), R(0));
+        V3<R> Fw = Z, Tw = Z, Fp = Z, Tp = Z;
+        if (wk >= 0) {
+            const R4* w4 = WL.res + 4 * (size_t)wk;
+            Fw = Ld4<R>::xyz(w4[0]);
+            Tw = Ld4<R>::xyz(w4[1]);
+            Fp = Ld4<R>::xyz(w4[2]);
+            Tp = Ld4<R>::xyz(w4[3]);
+        }
+        const V3<R> tF = Fss + Fw + Fp;
+        const V3<R> tT = Tss + Tw + Tp;
+        V3<R> r;
+        if (sizeof(R) == sizeof(double)) {
+            const D4 pi4 = S.pos[i];
+            const D4 pc = C.pq[2 * S.cid[i]];
+            r = cvt3<R>(mk<double>(pi4.x - pc.x, pi4.y - pc.y, pi4.z - pc.z));
+        } else {
+            r = Ld4<R>::xyz(ki);  // x_s - x_c = R(q) * local up to FP64 rounding
+        }
+        const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
+                               tT.z + (r.x * tF.y - r.y * tF.x));
+        S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
+        S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+        if (RECORD) {
+            D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
+            D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
+            rec.tot[2 * (size_t)i] = a;
+            rec.tot[2 * (size_t)i + 1] = b;
+            const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
+            for (int k = 0; k < 6; ++k) {
+                D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
+                rec.parts[6 * (size_t)i + k] = q;
             }
         }
     }
-    // block-ordered compaction of the candidates (ascending sphere index) and
-    // the touching count
-    __shared__ int s_wc[4];
+    // touching sphere-sphere count
     __shared__ unsigned int s_cnt[4];
-    const uint64_t cm = __ballot(cand);
     unsigned a0 = (unsigned)n_ss;
     for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
-    if (lane == 0) {
-        s_wc[wv] = __popcll(cm);
-        s_cnt[wv] = a0;
-    }
+    if (lane == 0) s_cnt[wv] = a0;
     __syncthreads();
-    const int nw = (blockDim.x + 63) >> 6;
-    if (P.nfaces > 0 || P.pene_on) {
-        int base = 0;
-        for (int k = 0; k < wv; ++k) base += s_wc[k];
-        if (cand) {
-            const size_t o = (size_t)lb * blockDim.x + base + __popcll(cm & lanemask_lt());
-            W.idx[o] = i;
-            W.raw[2 * o] = mk4<R>(Fss.x, Fss.y, Fss.z, R(0));
-            W.raw[2 * o + 1] = mk4<R>(Tss.x, Tss.y, Tss.z, R(0));
-        }
-        if (threadIdx.x == 0) {
-            int t = 0;
-            for (int k = 0; k < nw; ++k) t += s_wc[k];
-            W.cnt[lb] = t;
-        }
-    }
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int k = 0; k < nw; ++k) t += s_cnt[k];
+        for (int k = 0; k < ((int)blockDim.x + 63) >> 6; ++k) t += s_cnt[k];
         if (t) atomicAdd(&counters[0], t);
     }
 }
 
 // ---------------------------------------------------------------------------
-// K3 walls: wall and penetrator contacts of the candidates k_contacts compacted
-// per block, then the per-sphere totals and clump torque term
+// Wall and penetrator contacts of the spheres on the wall list
 // (src/dem_force_cube_wall_clump.cu:16-298, src/dem_force_cylind_wall.cu:152-237,
-// src/dem_contact_penetrator.cu:8-84, src/dem_force_penetrator_clump.cu:4-182,
-// src/dem_clump.cu:89-101, src/dem_clumpsphere.cu:86-107). One wave handles
-// the candidates of one block of k_contacts in sphere order (few candidates per
-// region, so one wave each keeps the launch to about one round of waves), and
-// the FP64 penetrator partial sums stay in region order.
+// src/dem_contact_penetrator.cu:8-84, src/dem_force_penetrator_clump.cu:4-182).
+// They depend only on each sphere's own state, so this runs before the
+// sphere-sphere kernel and leaves (Fw, Tw, Fp, Tp) per list entry for it. One
+// lane per list entry; the list is in sphere order, so the per-wave FP64
+// penetrator partial sums keep the order of the sphere index.
 template <class R, bool RECORD>
-__global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, CandBuf<R> W, int nreg,
-                                               int rstride, PeneState* pst, double* pene_partial,
-                                               unsigned long long* counters, int* bad, RecordBuf rec, const int* halt,
-                                               int n) {
+__global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
+                                               PeneState* pst, double* pene_partial, unsigned long long* counters,
+                                               int* bad, RecordBuf rec, const int* halt, int n) {
     typedef typename Vec4T<R>::type R4;
-    if (*halt <= n) return;
-    // one wave per candidate region (= block of k_contacts)
+    if (*halt <= n) return;  // a rebuild of step n comes first
     const int lane = threadIdx.x & 63;
-    const int lb = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (lb >= nreg) return;
-    const int ncand = W.cnt[lb];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gw = k >> 6;  // wave index: penetrator partial slot
+    const bool active = k < len;
     int n_w = 0;
-    V3<double> acc_f = mk<double>(0.0, 0.0, 0.0), acc_t = acc_f;
-    for (int c0 = 0; c0 < ncand; c0 += 64) {
-        const int t = c0 + lane;
-        const bool active = t < ncand;
-        int i = 0;
-        V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss, Fw = Fss, Tw = Fss, Fp = Fss, Tp = Fss;
-        V3<double> pf_sum = mk<double>(0.0, 0.0, 0.0), pt_sum = pf_sum;
-        D4 pi4;
-        pi4.x = pi4.y = pi4.z = pi4.w = 0.0;
-        if (active) {
-            const size_t o = (size_t)lb * rstride + t;
-            i = W.idx[o];
-            Fss = Ld4<R>::xyz(W.raw[2 * o]);
-            Tss = Ld4<R>::xyz(W.raw[2 * o + 1]);
-            pi4 = S.pos[i];
-        }
-        const V3<double> xi = xyz(pi4);
-        const double ri = pi4.w;
-        if (active) {
-            const R4 ki = S.kin[i];
-            const int ci_raw = S.cid[i];
-            const bool cid_ok = ci_raw >= 0 && ci_raw < P.Nc;
-            const int ci = cid_ok ? ci_raw : 0;
-            const R4 sv = C.snap[2 * ci], sw = C.snap[2 * ci + 1];
-            const V3<R> armi = Ld4<R>::xyz(ki), vi = Ld4<R>::xyz(sv), wi = Ld4<R>::xyz(sw);
-            const R mi = ki.w;
-            // walls and penetrator
-            uint32_t wf = S.wflag[i];
-            const uint32_t wf0 = wf;
-            const bool clump_ok = sv.w != R(0);
-            if (P.nfaces > 0 || P.pene_on) {
-                const R4 li = S.loc[i];
-                const bool ok = cid_ok && clump_ok && finite3(xi) && isfinite((double)li.x) &&
-                                isfinite((double)li.y) && isfinite((double)li.z) && isfinite(ri) && ri > 0.0 &&
-                                isfinite((double)mi) && mi > R(0);
-                if (P.nfaces > 0 && !ok) atomicMin(bad, i);
-                const R Rr = (R)ri;
-                if (P.nfaces > 0 && ok) {
-                    const V3<double> wpos = mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
-                    const V3<R> wv = mk<R>((R)P.wvel[0], (R)P.wvel[1], (R)P.wvel[2]);
-                    const V3<R> ww = mk<R>((R)P.wangv[0], (R)P.wangv[1], (R)P.wangv[2]);
-                    const V3<double> RV = xi - wpos;
-                    if (P.wall_type == NBC_WALL_CUBE) {
-                        // faces 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
-                        for (int f = 0; f < 6; ++f) {
-                            const int a = f >> 1;
-                            const bool up = (f & 1) == 0;
-                            const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
-                            const double Sd = P.wside[a];
-                            const bool cand = up ? (E > 0 && E + ri - Sd > 0) : (E < 0 && E - ri + Sd < 0);
-                            bool touch = false;
-                            if (cand) {
-                                const V3<double> ea = mk<double>(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
-                                const V3<double> RVb = RV - E * ea, RVt = E * ea;
-                                const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
-                                const V3<double> RVn = RV - RVo;
-                                const double sgn = up ? -1.0 : 1.0;
-                                const V3<double> nin = mk<double>(a == 0 ? sgn : 0.0, a == 1 ? sgn : 0.0, a == 2 ? sgn : 0.0);
-                                const double Ap = Dot<double>(RVn, nin);
-                                const double Dn = ri - Ap;
-                                if (Dn > 0.0) {
-                                    touch = true;
-                                    V3<R> H, Mt, Mr;
-                                    R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
-                                    if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
-                                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
-                                    WallOut<R> o;
-                                    wall_force<R>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
-                                                  cvt3<R>(RVo), vi, wi, wv, ww, H, Mt, Mr, o);
-                                    Fw = Fw + o.nf + o.tf;
-                                    Tw = Tw - (R)Ap * Cross<R>(cvt3<R>(nin), o.tf);
-                                    Tw = Tw - o.rm - o.tm;
-                                    store_hist<R>(hp, o.ht, o.htw, o.hr);
-                                    n_w++;
-                                }
+    int i = 0;
+    V3<R> Fw = mk<R>(R(0), R(0), R(0)), Tw = Fw, Fp = Fw, Tp = Fw;
+    V3<double> pf_sum = mk<double>(0.0, 0.0, 0.0), pt_sum = pf_sum;
+    D4 pi4;
+    pi4.x = pi4.y = pi4.z = pi4.w = 0.0;
+    if (active) {
+        i = WL.list[k];
+        pi4 = S.pos[i];
+    }
+    const V3<double> xi = xyz(pi4);
+    const double ri = pi4.w;
+    if (active) {
+        const R4 ki = S.kin[i];
+        const int ci_raw = S.cid[i];
+        const bool cid_ok = ci_raw >= 0 && ci_raw < P.Nc;
+        const int ci = cid_ok ? ci_raw : 0;
+        const R4 sv = C.snap[2 * ci], sw = C.snap[2 * ci + 1];
+        const V3<R> armi = Ld4<R>::xyz(ki), vi = Ld4<R>::xyz(sv), wi = Ld4<R>::xyz(sw);
+        const R mi = ki.w;
+        // walls and penetrator
+        uint32_t wf = S.wflag[i];
+        const uint32_t wf0 = wf;
+        const bool clump_ok = sv.w != R(0);
+        if (P.nfaces > 0 || P.pene_on) {
+            const R4 li = S.loc[i];
+            const bool ok = cid_ok && clump_ok && finite3(xi) && isfinite((double)li.x) &&
+                            isfinite((double)li.y) && isfinite((double)li.z) && isfinite(ri) && ri > 0.0 &&
+                            isfinite((double)mi) && mi > R(0);
+            if (P.nfaces > 0 && !ok) atomicMin(bad, i);
+            const R Rr = (R)ri;
+            if (P.nfaces > 0 && ok) {
+                const V3<double> wpos = mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
+                const V3<R> wv = mk<R>((R)P.wvel[0], (R)P.wvel[1], (R)P.wvel[2]);
+                const V3<R> ww = mk<R>((R)P.wangv[0], (R)P.wangv[1], (R)P.wangv[2]);
+                const V3<double> RV = xi - wpos;
+                if (P.wall_type == NBC_WALL_CUBE) {
+                    // faces 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
+                    for (int f = 0; f < 6; ++f) {
+                        const int a = f >> 1;
+                        const bool up = (f & 1) == 0;
+                        const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
+                        const double Sd = P.wside[a];
+                        const bool cand = up ? (E > 0 && E + ri - Sd > 0) : (E < 0 && E - ri + Sd < 0);
+                        bool touch = false;
+                        if (cand) {
+                            const V3<double> ea = mk<double>(a == 0 ? 1.0 : 0.0, a == 1 ? 1.0 : 0.0, a == 2 ? 1.0 : 0.0);
+                            const V3<double> RVb = RV - E * ea, RVt = E * ea;
+                            const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
+                            const V3<double> RVn = RV - RVo;
+                            const double sgn = up ? -1.0 : 1.0;
+                            const V3<double> nin = mk<double>(a == 0 ? sgn : 0.0, a == 1 ? sgn : 0.0, a == 2 ? sgn : 0.0);
+                            const double Ap = Dot<double>(RVn, nin);
+                            const double Dn = ri - Ap;
+                            if (Dn > 0.0) {
+                                touch = true;
+                                V3<R> H, Mt, Mr;
+                                R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
+                                if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                                else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                                WallOut<R> o;
+                                wall_force<R>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
+                                              cvt3<R>(RVo), vi, wi, wv, ww, H, Mt, Mr, o);
+                                Fw = Fw + o.nf + o.tf;
+                                Tw = Tw - (R)Ap * Cross<R>(cvt3<R>(nin), o.tf);
+                                Tw = Tw - o.rm - o.tm;
+                                store_hist<R>(hp, o.ht, o.htw, o.hr);
+                                n_w++;
                             }
-                            if (touch) wf |= 1u << f; else wf &= ~(1u << f);
                         }
-                    } else {
-                        // cylinder surfaces 0 bottom cap, 1 top cap, 2 side
-                        const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
-                        const double ERR = Dot<double>(RV, Wv);
-                        const V3<double> RVb = RV - ERR * Wv, RVt = ERR * Wv;
-                        const double norm = Length<double>(RVb);
-                        const double d_cap = fabs(ERR) - P.wheight + ri;
-                        const double d_side = norm - P.wrad + ri;
-                        for (int f = 0; f < 3; ++f) {
-                            bool cand;
-                            V3<double> RVo;
-                            if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
-                            else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
-                            else { cand = d_side >= 1e-14; RVo = (P.wrad / norm) * RVb + RVt; }
-                            bool touch = false;
-                            if (cand) {
-                                const V3<double> RVn = RV - RVo;
-                                const double Dn = ri - Length<double>(RVn);
-                                if (Dn > 0.0) {
-                                    touch = true;
-                                    const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
-                                    const double DistL = sqrt(DistS);
-                                    const R DistLr = (R)DistL;
-                                    const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
-                                    V3<R> H, Mt, Mr;
-                                    R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
-                                    if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
-                                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
-                                    WallOut<R> o;
-                                    wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
-                                                  vi, wi, wv, ww, H, Mt, Mr, o);
-                                    Fw = Fw + o.nf + o.tf;
-                                    Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
-                                    Tw = Tw - o.rm - o.tm;
-                                    store_hist<R>(hp, o.ht, o.htw, o.hr);
-                                    n_w++;
-                                }
-                            }
-                            if (touch) wf |= 1u << f; else wf &= ~(1u << f);
-                        }
+                        if (touch) wf |= 1u << f; else wf &= ~(1u << f);
                     }
-                    if (!isfinite((double)Fw.x) || !isfinite((double)Fw.y) || !isfinite((double)Fw.z) ||
-                        !isfinite((double)Tw.x) || !isfinite((double)Tw.y) || !isfinite((double)Tw.z))
-                        atomicMin(bad, i);
-                }
-                if (P.pene_on) {
-                    const int pf = P.pene_face;
-                    const V3<double> pp = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
-                    const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
-                    const V3<double> RV = xi - pp;
-                    bool touch = false;
-                    V3<double> RVo = mk<double>(0.0, 0.0, 0.0), Dist = RVo, DistU64 = RVo;
-                    double DistL = 0.0, Dn = 0.0;
-                    bool have_u = false;
-                    if (P.pshape == NBC_PENE_SPHERE) {
-                        // src/dem_sphere_intruder.cuh:27-50
-                        const double cd = Length<double>(RV);
-                        V3<double> nrm;
-                        if (cd > 1.0e-15) {
-                            nrm = (1.0 / cd) * RV;
-                        } else {
-                            const double t = sqrt(pdir.x * pdir.x + pdir.y * pdir.y + pdir.z * pdir.z);
-                            nrm = t > 1e-20 ? mk<double>(pdir.x / t, pdir.y / t, pdir.z / t) : mk<double>(0.0, 0.0, 0.0);
-                            if (Length<double>(nrm) < 1.0e-15) nrm = mk<double>(0.0, 0.0, 1.0);
-                        }
-                        const double gap = cd - P.prad;
-                        const double ov = ri - gap;
-                        if (ov > 0.0) {
-                            touch = true;
-                            RVo = P.prad * nrm;
-                            Dist = gap * nrm;
-                            DistL = gap;
-                            Dn = ov;
-                            DistU64 = nrm;
-                            have_u = true;
-                            if (P.pwait) atomicCAS(&pst->contact_state, 0, 1);
-                        }
-                    } else if (P.pshape == NBC_PENE_CONE) {
-                        // spheres outside the mesh's bounding sphere cannot touch it
-                        const double reach = P.pbound + ri;
-                        if (Dot<double>(RV, RV) < reach * reach) {
-                        double best = 1e300;
-                        V3<double> bq = mk<double>(0.0, 0.0, 0.0);
-                        for (int t = 0; t < P.ntri; ++t) {
-                            const double* v = P.ptri + 9 * t;
-                            const V3<double> q = closest_on_tri(RV, mk<double>(v[0], v[1], v[2]), mk<double>(v[3], v[4], v[5]),
-                                                                mk<double>(v[6], v[7], v[8]));
-                            const V3<double> dq = RV - q;
-                            const double ds = Dot<double>(dq, dq);
-                            if (ds < best) { best = ds; bq = q; }
-                        }
-                        const V3<double> RVn = RV - bq;
-                        const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
-                        const double dl = sqrt(DistS);
-                        if (P.ntri > 0 && ri - dl > 0.0) {
-                            touch = true; RVo = bq; Dist = RVn; DistL = dl; Dn = ri - dl;
-                        }
-                        }
-                    } else {
-                        // src/dem_contact_penetrator.cu:42-83
-                        const double LatDx = P.cell[0];
-                        const double ERR = Dot<double>(RV, pdir);
-                        if (fabs(ERR) < (P.pheight + 0.80 * LatDx)) {
-                            const V3<double> RVb = RV - ERR * pdir;
-                            const double norm = Length<double>(RVb);
-                            if (norm < (P.prad + 0.80 * LatDx)) {
-                                const V3<double> RVt = ERR * pdir;
-                                V3<double> o3;
-                                if (fabs(ERR) < P.pheight) o3 = (1.0 / norm) * P.prad * RVb + RVt;
-                                else if (norm < P.prad) o3 = (1.0 / fabs(ERR)) * P.pheight * RVt + RVb;
-                                else o3 = (1.0 / norm) * P.prad * RVb + (1.0 / fabs(ERR)) * P.pheight * RVt;
-                                const V3<double> RVn = RV - o3;
+                } else {
+                    // cylinder surfaces 0 bottom cap, 1 top cap, 2 side
+                    const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+                    const double ERR = Dot<double>(RV, Wv);
+                    const V3<double> RVb = RV - ERR * Wv, RVt = ERR * Wv;
+                    const double norm = Length<double>(RVb);
+                    const double d_cap = fabs(ERR) - P.wheight + ri;
+                    const double d_side = norm - P.wrad + ri;
+                    for (int f = 0; f < 3; ++f) {
+                        bool cand;
+                        V3<double> RVo;
+                        if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
+                        else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
+                        else { cand = d_side >= 1e-14; RVo = (P.wrad / norm) * RVb + RVt; }
+                        bool touch = false;
+                        if (cand) {
+                            const V3<double> RVn = RV - RVo;
+                            const double Dn = ri - Length<double>(RVn);
+                            if (Dn > 0.0) {
+                                touch = true;
                                 const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
-                                const double dl = sqrt(DistS);
-                                if (ri - dl > 0.0) { touch = true; RVo = o3; Dist = RVn; DistL = dl; Dn = ri - dl; }
+                                const double DistL = sqrt(DistS);
+                                const R DistLr = (R)DistL;
+                                const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
+                                V3<R> H, Mt, Mr;
+                                R* hp = S.wh + ((size_t)f * P.N + i) * HSTRIDE;
+                                if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                                else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                                WallOut<R> o;
+                                wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
+                                              vi, wi, wv, ww, H, Mt, Mr, o);
+                                Fw = Fw + o.nf + o.tf;
+                                Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
+                                Tw = Tw - o.rm - o.tm;
+                                store_hist<R>(hp, o.ht, o.htw, o.hr);
+                                n_w++;
                             }
                         }
+                        if (touch) wf |= 1u << f; else wf &= ~(1u << f);
                     }
-                    if (touch) {
-                        const R DistLr = (R)DistL;
-                        const V3<R> DistU = have_u ? cvt3<R>(DistU64) : (R(1.0) / DistLr) * cvt3<R>(Dist);
-                        R Rij = Rr;
-                        if (P.pshape == NBC_PENE_SPHERE) Rij = Rr * (R)P.prad / (Rr + (R)P.prad);
-                        R Mij = mi;
-                        if (P.pmotion == NBC_PENE_GRAVITY) Mij = mi * (R)P.pmass / (mi + (R)P.pmass);
-                        V3<R> H, Mt, Mr;
-                        R* hp = S.wh + ((size_t)pf * P.N + i) * HSTRIDE;
-                        if ((wf >> pf) & 1u) load_hist<R>(hp, H, Mt, Mr);
-                        else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
-                        const V3<R> pv = mk<R>((R)pst->vel[0], (R)pst->vel[1], (R)pst->vel[2]);
-                        const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
-                        const V3<R> RVor = cvt3<R>(RVo);
-                        WallOut<R> o;
-                        wall_force<R>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
-                                      pw, H, Mt, Mr, o);
-                        const V3<R> cf = mk<R>(R(0), R(0), R(0));
-                        Fp = o.nf + o.tf;
-                        Tp = -DistLr * Cross<R>(DistU, o.tf);
-                        V3<R> pTc = R(-1.0) * Cross<R>(RVor, (o.nf + o.tf + cf));
-                        Tp = Tp - o.rm - o.tm;
-                        pTc = pTc + o.rm + o.tm;
-                        store_hist<R>(hp, o.ht, o.htw, o.hr);
-                        pf_sum = mk<double>((double)Fp.x, (double)Fp.y, (double)Fp.z);
-                        pt_sum = mk<double>((double)pTc.x, (double)pTc.y, (double)pTc.z);
-                        n_w++;
-                        if (RECORD) {
-                            const V3<R> tn = Cross<R>(o.nf, o.arm);
-                            const V3<R> tt = Cross<R>(o.tf, o.arm) - o.rm - o.tm;
-                            D4 a; a.x = o.nf.x; a.y = o.nf.y; a.z = o.nf.z; a.w = 0.0;
-                            D4 b; b.x = o.tf.x; b.y = o.tf.y; b.z = o.tf.z; b.w = 0.0;
-                            D4 c; c.x = tn.x; c.y = tn.y; c.z = tn.z; c.w = 0.0;
-                            D4 d; d.x = tt.x; d.y = tt.y; d.z = tt.z; d.w = 0.0;
-                            rec.pene[4 * (size_t)i] = a; rec.pene[4 * (size_t)i + 1] = b;
-                            rec.pene[4 * (size_t)i + 2] = c; rec.pene[4 * (size_t)i + 3] = d;
+                }
+                if (!isfinite((double)Fw.x) || !isfinite((double)Fw.y) || !isfinite((double)Fw.z) ||
+                    !isfinite((double)Tw.x) || !isfinite((double)Tw.y) || !isfinite((double)Tw.z))
+                    atomicMin(bad, i);
+            }
+            if (P.pene_on) {
+                const int pf = P.pene_face;
+                const V3<double> pp = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
+                const V3<double> pdir = mk<double>(P.pdir[0], P.pdir[1], P.pdir[2]);
+                const V3<double> RV = xi - pp;
+                bool touch = false;
+                V3<double> RVo = mk<double>(0.0, 0.0, 0.0), Dist = RVo, DistU64 = RVo;
+                double DistL = 0.0, Dn = 0.0;
+                bool have_u = false;
+                if (P.pshape == NBC_PENE_SPHERE) {
+                    // src/dem_sphere_intruder.cuh:27-50
+                    const double cd = Length<double>(RV);
+                    V3<double> nrm;
+                    if (cd > 1.0e-15) {
+                        nrm = (1.0 / cd) * RV;
+                    } else {
+                        const double t = sqrt(pdir.x * pdir.x + pdir.y * pdir.y + pdir.z * pdir.z);
+                        nrm = t > 1e-20 ? mk<double>(pdir.x / t, pdir.y / t, pdir.z / t) : mk<double>(0.0, 0.0, 0.0);
+                        if (Length<double>(nrm) < 1.0e-15) nrm = mk<double>(0.0, 0.0, 1.0);
+                    }
+                    const double gap = cd - P.prad;
+                    const double ov = ri - gap;
+                    if (ov > 0.0) {
+                        touch = true;
+                        RVo = P.prad * nrm;
+                        Dist = gap * nrm;
+                        DistL = gap;
+                        Dn = ov;
+                        DistU64 = nrm;
+                        have_u = true;
+                        if (P.pwait) atomicCAS(&pst->contact_state, 0, 1);
+                    }
+                } else if (P.pshape == NBC_PENE_CONE) {
+                    // spheres outside the mesh's bounding sphere cannot touch it
+                    const double reach = P.pbound + ri;
+                    if (Dot<double>(RV, RV) < reach * reach) {
+                    double best = 1e300;
+                    V3<double> bq = mk<double>(0.0, 0.0, 0.0);
+                    for (int t = 0; t < P.ntri; ++t) {
+                        const double* v = P.ptri + 9 * t;
+                        const V3<double> q = closest_on_tri(RV, mk<double>(v[0], v[1], v[2]), mk<double>(v[3], v[4], v[5]),
+                                                            mk<double>(v[6], v[7], v[8]));
+                        const V3<double> dq = RV - q;
+                        const double ds = Dot<double>(dq, dq);
+                        if (ds < best) { best = ds; bq = q; }
+                    }
+                    const V3<double> RVn = RV - bq;
+                    const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                    const double dl = sqrt(DistS);
+                    if (P.ntri > 0 && ri - dl > 0.0) {
+                        touch = true; RVo = bq; Dist = RVn; DistL = dl; Dn = ri - dl;
+                    }
+                    }
+                } else {
+                    // src/dem_contact_penetrator.cu:42-83
+                    const double LatDx = P.cell[0];
+                    const double ERR = Dot<double>(RV, pdir);
+                    if (fabs(ERR) < (P.pheight + 0.80 * LatDx)) {
+                        const V3<double> RVb = RV - ERR * pdir;
+                        const double norm = Length<double>(RVb);
+                        if (norm < (P.prad + 0.80 * LatDx)) {
+                            const V3<double> RVt = ERR * pdir;
+                            V3<double> o3;
+                            if (fabs(ERR) < P.pheight) o3 = (1.0 / norm) * P.prad * RVb + RVt;
+                            else if (norm < P.prad) o3 = (1.0 / fabs(ERR)) * P.pheight * RVt + RVb;
+                            else o3 = (1.0 / norm) * P.prad * RVb + (1.0 / fabs(ERR)) * P.pheight * RVt;
+                            const V3<double> RVn = RV - o3;
+                            const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                            const double dl = sqrt(DistS);
+                            if (ri - dl > 0.0) { touch = true; RVo = o3; Dist = RVn; DistL = dl; Dn = ri - dl; }
                         }
                     }
-                    if (touch) wf |= 1u << pf; else wf &= ~(1u << pf);
                 }
-            }
-            if (wf != wf0) S.wflag[i] = wf;
-            // totF/totT (src/dem_clump.cu:97-98) and the clump torque term
-            // T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
-            const V3<R> tF = Fss + Fw + Fp;
-            const V3<R> tT = Tss + Tw + Tp;
-            V3<R> r;
-            if (sizeof(R) == sizeof(double)) {
-                const D4 pc = C.pq[2 * ci];
-                r = cvt3<R>(mk<double>(xi.x - pc.x, xi.y - pc.y, xi.z - pc.z));
-            } else {
-                r = armi;  // x_s - x_c = R(q) * local up to FP64 rounding
-            }
-            const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
-                                   tT.z + (r.x * tF.y - r.y * tF.x));
-            S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
-            S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
-            if (RECORD) {
-                D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
-                D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
-                rec.tot[2 * (size_t)i] = a;
-                rec.tot[2 * (size_t)i + 1] = b;
-                D4 q[6];
-                const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
-                for (int k = 0; k < 6; ++k) {
-                    q[k].x = parts[k].x; q[k].y = parts[k].y; q[k].z = parts[k].z; q[k].w = 0.0;
-                    rec.parts[6 * (size_t)i + k] = q[k];
+                if (touch) {
+                    const R DistLr = (R)DistL;
+                    const V3<R> DistU = have_u ? cvt3<R>(DistU64) : (R(1.0) / DistLr) * cvt3<R>(Dist);
+                    R Rij = Rr;
+                    if (P.pshape == NBC_PENE_SPHERE) Rij = Rr * (R)P.prad / (Rr + (R)P.prad);
+                    R Mij = mi;
+                    if (P.pmotion == NBC_PENE_GRAVITY) Mij = mi * (R)P.pmass / (mi + (R)P.pmass);
+                    V3<R> H, Mt, Mr;
+                    R* hp = S.wh + ((size_t)pf * P.N + i) * HSTRIDE;
+                    if ((wf >> pf) & 1u) load_hist<R>(hp, H, Mt, Mr);
+                    else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
+                    const V3<R> pv = mk<R>((R)pst->vel[0], (R)pst->vel[1], (R)pst->vel[2]);
+                    const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
+                    const V3<R> RVor = cvt3<R>(RVo);
+                    WallOut<R> o;
+                    wall_force<R>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
+                                  pw, H, Mt, Mr, o);
+                    const V3<R> cf = mk<R>(R(0), R(0), R(0));
+                    Fp = o.nf + o.tf;
+                    Tp = -DistLr * Cross<R>(DistU, o.tf);
+                    V3<R> pTc = R(-1.0) * Cross<R>(RVor, (o.nf + o.tf + cf));
+                    Tp = Tp - o.rm - o.tm;
+                    pTc = pTc + o.rm + o.tm;
+                    store_hist<R>(hp, o.ht, o.htw, o.hr);
+                    pf_sum = mk<double>((double)Fp.x, (double)Fp.y, (double)Fp.z);
+                    pt_sum = mk<double>((double)pTc.x, (double)pTc.y, (double)pTc.z);
+                    n_w++;
+                    if (RECORD) {
+                        const V3<R> tn = Cross<R>(o.nf, o.arm);
+                        const V3<R> tt = Cross<R>(o.tf, o.arm) - o.rm - o.tm;
+                        D4 a; a.x = o.nf.x; a.y = o.nf.y; a.z = o.nf.z; a.w = 0.0;
+                        D4 b; b.x = o.tf.x; b.y = o.tf.y; b.z = o.tf.z; b.w = 0.0;
+                        D4 c; c.x = tn.x; c.y = tn.y; c.z = tn.z; c.w = 0.0;
+                        D4 d; d.x = tt.x; d.y = tt.y; d.z = tt.z; d.w = 0.0;
+                        rec.pene[4 * (size_t)i] = a; rec.pene[4 * (size_t)i + 1] = b;
+                        rec.pene[4 * (size_t)i + 2] = c; rec.pene[4 * (size_t)i + 3] = d;
+                    }
                 }
+                if (touch) wf |= 1u << pf; else wf &= ~(1u << pf);
             }
         }
-
-        acc_f = acc_f + pf_sum;
-        acc_t = acc_t + pt_sum;
+        if (wf != wf0) S.wflag[i] = wf;
+        R4* o = WL.res + 4 * (size_t)k;
+        o[0] = mk4<R>(Fw.x, Fw.y, Fw.z, R(0));
+        o[1] = mk4<R>(Tw.x, Tw.y, Tw.z, R(0));
+        o[2] = mk4<R>(Fp.x, Fp.y, Fp.z, R(0));
+        o[3] = mk4<R>(Tp.x, Tp.y, Tp.z, R(0));
     }
     // wave reductions: touching count and the FP64 penetrator partial sums of
-    // this region (a fixed tree over the lanes)
-    double v[6] = {acc_f.x, acc_f.y, acc_f.z, acc_t.x, acc_t.y, acc_t.z};
+    // this wave's 64 entries (a fixed tree over the lanes)
+    double v[6] = {pf_sum.x, pf_sum.y, pf_sum.z, pt_sum.x, pt_sum.y, pt_sum.z};
     unsigned a1 = (unsigned)n_w;
     for (int o = 32; o > 0; o >>= 1) {
-        for (int k = 0; k < 6; ++k) v[k] += __shfl_down(v[k], o, 64);
+        for (int q = 0; q < 6; ++q) v[q] += __shfl_down(v[q], o, 64);
         a1 += __shfl_down(a1, o, 64);
     }
-    if (lane == 0) {
+    if (lane == 0 && (k - lane) < len) {
         if (a1) atomicAdd(&counters[1], (unsigned long long)a1);
         if (P.pene_on)
-            for (int k = 0; k < 6; ++k) pene_partial[6 * (size_t)lb + k] = v[k];
+            for (int q = 0; q < 6; ++q) pene_partial[6 * (size_t)gw + q] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wall list: spheres that may touch a wall or the penetrator before the list
+// is rebuilt. A sphere moves at most 2 * Thresh from its position at the
+// build (at most Thresh from the last rebuild, whichever came later), the
+// walls do not move and the penetrator moves at most pene_move, so the
+// candidate tests with that slack cover every contact until the next build.
+// Spheres with a live wall history or a position that is not finite are
+// listed too.
+__device__ __forceinline__ bool wall_listed(const DevParams& P, const PeneState* pst, const D4& x, uint32_t wf) {
+    const V3<double> xi = xyz(x);
+    if ((wf & 0xFFu) != 0u || !finite3(xi) || !isfinite(x.w)) return true;
+    if (P.nfaces > 0 && wall_candidate(P, xi, x.w, P.wl_slack)) return true;
+    return P.pene_on && pene_candidate(P, pst, xi, x.w, P.wl_slack + P.pene_move);
+}
+
+__global__ void k_wlist_count(DevParams P, const D4* pos, const uint32_t* wflag, const PeneState* pst, int* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    cnt[i] = wall_listed(P, pst, pos[i], wflag[i]) ? 1 : 0;
+}
+
+__global__ void k_wlist_fill(DevParams P, const D4* pos, const uint32_t* wflag, PeneState* pst, const int* off,
+                             int* list, int* pos_of) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+        for (int a = 0; a < 3; ++a) pst->lpos[a] = pst->pos[a];
+    if (i >= P.N) return;
+    if (wall_listed(P, pst, pos[i], wflag[i])) {
+        list[off[i]] = i;
+        pos_of[i] = off[i];
+    } else {
+        pos_of[i] = -1;
     }
 }
 
@@ -1521,18 +1534,34 @@ __global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, const 
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.Nc) return;
     const int b = mbeg[c], e = mend[c];
+    // clump state first: independent of the member sums
+    const D4 pr = C.prop[c];
+    const D4 vr = C.vw[2 * c], wr = C.vw[2 * c + 1];
     D4 Fr, Tr;
     if (b >= 0 && e > b) {
+        // members in ascending sorted order, four loads in flight at a time;
+        // the first member starts the sums, the rest are added in order
         typedef typename Vec4T<R>::type R4;
-        int s = mem[b];
-        R4 f = S.sft[2 * (size_t)s], t = S.sft[2 * (size_t)s + 1];
-        V3<double> F = mk<double>(f.x, f.y, f.z), T = mk<double>(t.x, t.y, t.z);
-        for (int k = b + 1; k < e; ++k) {
-            s = mem[k];
-            f = S.sft[2 * (size_t)s];
-            t = S.sft[2 * (size_t)s + 1];
-            F = F + mk<double>(f.x, f.y, f.z);
-            T = T + mk<double>(t.x, t.y, t.z);
+        V3<double> F = mk<double>(0.0, 0.0, 0.0), T = F;
+        for (int k0 = b; k0 < e; k0 += 4) {
+            int m[4];
+            R4 f[4], t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) m[u] = k0 + u < e ? mem[k0 + u] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (m[u] >= 0) {
+                    f[u] = S.sft[2 * (size_t)m[u]];
+                    t[u] = S.sft[2 * (size_t)m[u] + 1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (m[u] < 0) continue;
+                const V3<double> fu = mk<double>(f[u].x, f[u].y, f[u].z), tu = mk<double>(t[u].x, t[u].y, t[u].z);
+                if (k0 + u == b) { F = fu; T = tu; }
+                else { F = F + fu; T = T + tu; }
+            }
         }
         Fr.x = F.x; Fr.y = F.y; Fr.z = F.z; Fr.w = 0.0;
         Tr.x = T.x; Tr.y = T.y; Tr.z = T.z; Tr.w = 0.0;
@@ -1542,14 +1571,12 @@ __global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, const 
         Fr = C.ft[2 * c];
         Tr = C.ft[2 * c + 1];
     }
-    const D4 pr = C.prop[c];
     V3<double> F = xyz(Fr), T = xyz(Tr);
     F = (1.0 / pr.w) * F;
     F.z = F.z + P.Gravity;
     T.x = (1.0 / pr.x) * T.x;
     T.y = (1.0 / pr.y) * T.y;
     T.z = (1.0 / pr.z) * T.z;
-    const D4 vr = C.vw[2 * c], wr = C.vw[2 * c + 1];
     const V3<double> v = xyz(vr) + 0.5 * P.dt * F;
     const V3<double> w = xyz(wr) + 0.5 * P.dt * T;
     D4 vo; vo.x = v.x; vo.y = v.y; vo.z = v.z; vo.w = 0.0;
@@ -1586,6 +1613,15 @@ static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) 
         if (e_ != hipSuccess) return (int)e_;             \
     } while (0)
 
+__global__ void k_clear_halt(int* status, int none) {
+    status[0] = none;
+    status[3] &= ST_CLUMP_BAD;
+}
+int launch_clear_halt(hipStream_t st, int* status, int none) {
+    k_clear_halt<<<1, 1, 0, st>>>(status, none);
+    LAUNCH_CHECK();
+    return 0;
+}
 int launch_fill_i32(hipStream_t st, int* a, long long n, int v) {
     if (n <= 0) return 0;
     k_fill_i32<<<nblk(n, 256), 256, 0, st>>>(a, n, v);
@@ -1731,7 +1767,7 @@ int launch_pene_finalize(hipStream_t st, const DevParams& P, PeneState* pst, con
 
 template <class R>
 int launch_clump_before(hipStream_t st, const DevParams& P, ClumpBuf<R> C, PeneState* pst, unsigned long long* ctr,
-                        int block, const int* halt, int step) {
+                        int block, int* halt, int step) {
     const int n = P.Nc > 1 ? P.Nc : 1;
     k_clump_before<R><<<nblk(n, block), block, 0, st>>>(P, C, pst, ctr, halt, step);
     LAUNCH_CHECK();
@@ -1760,26 +1796,38 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
 int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    PeneState* pst, CandBuf<R> W, unsigned long long* ctr, RecordBuf rec, bool record, int block,
-                    int remap, const int* halt, int step) {
+                    PeneState* pst, WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
+                    int block, int remap, const int* halt, int step) {
     const unsigned nb = nblk(P.N, block);
     if (record)
-        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, W, ctr, rec, remap, halt, step);
+        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, WL, ctr, bad, rec, remap, halt, step);
     else
-        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, W, ctr, rec, remap, halt, step);
+        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, WL, ctr, bad, rec, remap, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
 template <class R>
-int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, CandBuf<R> W, PeneState* pst,
-                 double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block,
+int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
+                 PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
                  const int* halt, int step) {
-    const int nreg = (int)nblk(P.N, block);  // candidate regions: one per k_contacts block
-    const unsigned nb = (unsigned)((nreg + 3) / 4);
+    if (len <= 0) return 0;
+    const unsigned nb = nblk(len, 256);
     if (record)
-        k_walls<R, true><<<nb, 256, 0, st>>>(P, S, C, W, nreg, block, pst, partial, ctr, bad, rec, halt, step);
+        k_walls<R, true><<<nb, 256, 0, st>>>(P, S, C, WL, len, pst, partial, ctr, bad, rec, halt, step);
     else
-        k_walls<R, false><<<nb, 256, 0, st>>>(P, S, C, W, nreg, block, pst, partial, ctr, bad, rec, halt, step);
+        k_walls<R, false><<<nb, 256, 0, st>>>(P, S, C, WL, len, pst, partial, ctr, bad, rec, halt, step);
+    LAUNCH_CHECK();
+    return 0;
+}
+int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* wflag, PeneState* pst,
+                       int* cnt, int* off, int* list, int* pos_of, int* stmp, int* stotal, int block) {
+    const unsigned g = nblk(P.N, block);
+    int e;
+    if ((e = launch_fill_i32(st, cnt + P.N, 1, 0))) return e;
+    k_wlist_count<<<g, block, 0, st>>>(P, pos, wflag, pst, cnt);
+    LAUNCH_CHECK();
+    if ((e = launch_exclusive_scan(st, cnt, off, P.N + 1, stmp, stotal))) return e;
+    k_wlist_fill<<<g, block, 0, st>>>(P, pos, wflag, pst, off, list, pos_of);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1793,16 +1841,16 @@ int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
 
 #define INST(R)                                                                                                      \
     template int launch_clump_before<R>(hipStream_t, const DevParams&, ClumpBuf<R>, PeneState*, unsigned long long*, \
-                                        int, const int*, int);                                                       \
+                                        int, int*, int);                                                             \
     template int launch_sphere_pose<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, int*, int, int);    \
     template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
     template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
-                                    PeneState*, CandBuf<R>, unsigned long long*, RecordBuf, bool, int, int,          \
+                                    PeneState*, WallBuf<R>, unsigned long long*, int*, RecordBuf, bool, int, int,    \
                                     const int*, int);                                                                \
-    template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, CandBuf<R>, PeneState*,   \
-                                 double*, unsigned long long*, int*, RecordBuf, bool, int, const int*, int);         \
+    template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>, int,          \
+                                 PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, const int*, int);  \
     template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, const int*,         \
                                        const int*, const int*, int, const int*, int);
 INST(double)

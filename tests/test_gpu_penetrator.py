@@ -85,3 +85,37 @@ def test_gravity_after_contact_sphere_penetrator():
     assert pb["state"] == 2 and pa["state"] == 2
     assert np.abs(pa["pos"] - pb["pos"]).max() <= 1e-12
     assert np.abs(pa["vel"] - pb["vel"]).max() <= 1e-9 * max(np.abs(pb["vel"]).max(), 1e-30)
+
+
+@pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
+def test_fast_penetrator_wall_list_refresh(prec):
+    """A sphere penetrator driven through the bed at 4 m/s with a small Thresh:
+    it leaves its wall-list allowance (8 * Thresh) every few steps, so the
+    wall list is rebuilt many times between and at neighbour-list rebuilds.
+    The reaction history and the bed must follow the oracle."""
+    print(f"[case] fast sphere penetrator, wall list refreshes, precision {prec}", flush=True)
+    bed = nbc.generate_bed(1, 600, seed=4, container=(0.01, 0, 0, 0))
+    bed.params.isIntruder = 1
+    bed.params.OutputInterval = 500
+    bed.params.StepSize = 1e-6
+    bed.params.Thresh = 5e-6  # allowance 40 um: a new wall list every 10 steps
+    Rp = 0.002
+    # 1 mm above the bed: ~25 wall-list rebuilds over a resting bed, then contact
+    zc = sphere_clearance(bed, Rp) + 1e-3
+    pene = nbc.sphere_penetrator(Rp, (0.0, 0.0, zc), (0.0, 0.0, -4.0))
+    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
+    ctx = nbc.Context(bed.finalized_params(opt), opt)
+    ctx.upload(bed, bed.wall, pene)
+    ref = orc.Oracle(orc.finalize_params(bed), opt)
+    ref.upload(bed, bed.wall, pene)
+    ctx.step(500)
+    ref.step(500)
+    fa, fb = ctx.penetrator()["cache_f"], ref.penetrator()["cache_f"]
+    assert np.count_nonzero(fb[:, 2]) > 150, "penetrator out of contact"
+    scale = np.abs(fb).max()
+    if prec == nbc.NBC_FP64:
+        assert np.abs(fa - fb).max() <= 1e-9 * scale
+        assert np.abs(ctx.spheres()["pos"] - ref.spheres()["pos"]).max() <= 1e-12
+        assert ctx.stats().touching_wall == ref.stats().touching_wall
+    else:
+        assert np.abs(fa - fb).max() <= 2e-3 * scale
