@@ -9,8 +9,9 @@ of the whole bed (kinematics, rebuild when flagged, contacts, walls, clump
 reduction, integration) with all state resident in HBM.
 
 value: sphere-steps/s of the FP32-mixed form; the plain FP64 form runs the
-same steps beside it. roofline: the contact kernel (k_contacts), the dominant
-kernel, with algorithmic bytes per launch
+same steps beside it. roofline: the contact stage, the dominant part of the
+step (k_walls + k_pairs + k_sums, which together replace the reference's
+contact kernels), with algorithmic bytes per step
     N * 72 B + C_ss * 80 B + C_w * 72 B
 (per-array accounting of the force inputs/outputs; C_ss / C_w = touching
 sphere-sphere / wall contacts counted on the device each step), timed with HIP
@@ -49,7 +50,7 @@ def run_form(bed, precision, steps, warmup):
     out = dict(seconds=t1 - t0, k3_ms=k3_ms, k3_n=k3_n, rebuilds=s1.rebuilds - s0.rebuilds,
                touching_ss=0.5 * (s0.touching_ss + s1.touching_ss), touching_wall=0.5 * (s0.touching_wall + s1.touching_wall),
                front_pairs=s1.front_pairs)
-    for cls in (0, 2, 3, 4, 5):
+    for cls in (0, 2, 3, 4, 5, 6):
         out[f"cls{cls}_ms"], out[f"cls{cls}_n"] = ctx.profile_get(cls)
     ctx.close()
     return out
@@ -164,7 +165,11 @@ def main():
         print(f"[bench] fp64: {fp64}", file=sys.stderr, flush=True)
     N = bed.n
     value = N * a.steps / mixed["seconds"]
-    k3_avg = mixed["k3_ms"] / max(mixed["k3_n"], 1) * 1e-3
+    def avg(form, key):
+        return form[key + "_ms"] / max(form[key + "_n"], 1) * 1e-3
+
+    t_pairs, t_walls, t_sums = avg(mixed, "k3"), avg(mixed, "cls5"), avg(mixed, "cls6")
+    k3_avg = t_pairs + t_walls + t_sums
     k3_bytes = N * 72 + mixed["touching_ss"] * 80 + mixed["touching_wall"] * 72
     achieved = k3_bytes / k3_avg
     # whole-step algorithmic bytes (B_step, mixed): N*236 + Nc*336 + C_ss*80 + C_w*72
@@ -190,10 +195,13 @@ def main():
             "list_pairs_per_sphere": mixed["front_pairs"] / N, "rebuilds_in_timed_steps": mixed["rebuilds"],
         },
         "roofline": {
-            "kernel": "k_contacts",
+            "kernel": "contact stage: k_walls + k_pairs + k_sums",
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
             "avg_launch_us": k3_avg * 1e6, "algorithmic_bytes_per_launch": k3_bytes,
+            "avg_us": {"k_pairs": t_pairs * 1e6, "k_sums": t_sums * 1e6, "k_walls": t_walls * 1e6,
+                       "k_clump_before+k_sphere_pose": avg(mixed, "cls0") * 1e6,
+                       "k_clump_after": avg(mixed, "cls2") * 1e6},
         },
         "step_hbm_frac": step_bytes * a.steps / mixed["seconds"] / HBM_PEAK,
     }

@@ -294,8 +294,9 @@ int nbc_get_pene_exports(nbc_ctx* ctx, double* fn, double* ft, double* tn, doubl
 int nbc_get_pair_exports(nbc_ctx* ctx, double* fcn, double* cpt);
 
 /* ---- timing of library kernels on the library stream ----------------- */
-/* Kernel classes: 0 kinematics(K1+K2), 1 sphere-sphere contacts, 2 clump
- * reduction, 3 rebuild, 4 penetrator sum, 5 wall/penetrator contacts. */
+/* Kernel classes: 0 kinematics(K1+K2), 1 sphere-sphere pairs, 2 clump
+ * reduction, 3 rebuild, 4 penetrator sum, 5 wall/penetrator contacts,
+ * 6 per-sphere contact sums. */
 int nbc_profile_enable(nbc_ctx* ctx, int32_t on);
 int nbc_profile_get(nbc_ctx* ctx, int32_t kernel_class, double* total_ms, int64_t* launches);
 

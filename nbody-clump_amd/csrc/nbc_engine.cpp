@@ -2,12 +2,14 @@
 //
 // The loop body of src/dem_simulation_clump.cu:110-226 becomes, per step:
 //   K1 k_clump_before, K2 k_sphere_pose (sets the rebuild flag),
-//   one 8-byte status read-back, the rebuild sequence only when flagged or on
-//   the first step, K3 k_contacts, k_pene_finalize (penetrator runs only),
+//   k_walls (wall list), k_pairs + k_sums (sphere-sphere contacts and
+//   per-sphere totals), k_pene_finalize (penetrator runs only),
 //   K4 k_clump_after.
-// The status read-back also reports K3's invalid-input index from the
-// previous step, so the reference's fatal wall checks
-// (src/dem_force_cube_wall_clump.cu:322-331) surface as NBC_E_NONFINITE.
+// Steps are enqueued in batches; one status read-back per batch reports the
+// first step that needs a rebuild (or a new wall list), which is then done
+// and the step replayed from its contact kernels, and the invalid-input index
+// of the reference's fatal wall checks (src/dem_force_cube_wall_clump.cu:322-331),
+// which surfaces as NBC_E_NONFINITE.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -76,6 +78,10 @@ class Engine : public EngineBase {
         device = dev;
         HIPC(hipSetDevice(device));
         HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        // side stream (default priority) for the wall kernel, which overlaps k_pairs
+        HIPC(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        HIPC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         N = P.ParticleNum;
         Nc = P.ClumpNum;
         if (N <= 0 || Nc <= 0 || P.BranchLmtLen < 2 || P.CellNum <= 0)
@@ -86,7 +92,7 @@ class Engine : public EngineBase {
         block = O.block_size > 0 ? O.block_size : 256;
         if (block < 64 || block > 1024 || (block % 64) != 0)
             return fail(NBC_E_BADINPUT, "nbc_create: block_size must be a multiple of 64 in [64, 1024]");
-        if (block > 256) block = 256;  // k_contacts is launched with __launch_bounds__(256)
+        if (block > 256) block = 256;  // k_sums is launched with __launch_bounds__(256)
         rbits = O.radix_bits > 0 ? O.radix_bits : 8;
         if (rbits > 8) rbits = 8;
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
@@ -203,24 +209,22 @@ class Engine : public EngineBase {
         return hipSuccess;
     }
 
-    // near lists hold at most as many entries as the front / back lists
-    hipError_t grow_near(long long fcap, long long bc) {
-        if (fcap > ncap_f) {
-            dfree(nfl); dfree(slot2near);
-            nfl = nullptr; slot2near = nullptr; ncap_f = 0;
-            hipError_t e = dalloc(&nfl, (size_t)fcap);
-            if (e != hipSuccess) return e;
-            e = dalloc(&slot2near, (size_t)fcap);
-            if (e != hipSuccess) return e;
-            ncap_f = fcap;
-        }
-        if (bc > ncap_b) {
-            dfree(nbl);
-            nbl = nullptr; ncap_b = 0;
-            hipError_t e = dalloc(&nbl, (size_t)bc);
-            if (e != hipSuccess) return e;
-            ncap_b = bc;
-        }
+    // near pairs are a subset of the front list: every near array fits in the
+    // front-list capacity (one entry per near pair on each side)
+    hipError_t grow_near(long long fcap) {
+        if (fcap <= ncap) return hipSuccess;
+        dfree(nfw); dfree(nfs); dfree(nfi); dfree(nfb); dfree(stampB); dfree(slot2near); dfree(cF); dfree(cB);
+        nfw = nullptr; nfs = nfi = nfb = stampB = slot2near = nullptr; cF = cB = nullptr; ncap = 0;
+        hipError_t e;
+        if ((e = dalloc(&nfw, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&nfs, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&nfi, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&nfb, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&stampB, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&slot2near, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&cF, 2 * (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&cB, 2 * (size_t)fcap)) != hipSuccess) return e;
+        ncap = fcap;
         return hipSuccess;
     }
 
@@ -236,7 +240,7 @@ class Engine : public EngineBase {
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap);
-        F(boff); F(bent); F(nfoff); F(nboff); F(nfl); F(nbl); F(slot2near);
+        F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nfi); F(nfb); F(stampB); F(slot2near); F(cF); F(cB);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
@@ -244,8 +248,13 @@ class Engine : public EngineBase {
         if (hstatus) (void)hipHostFree(hstatus);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
+        if (st2) (void)hipStreamSynchronize(st2);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
-        st = nullptr;
+        st = st2 = nullptr;
+        ev_fork = ev_join = nullptr;
         device = -1;
     }
 
@@ -324,6 +333,8 @@ class Engine : public EngineBase {
         HIPC(hipMemset(Lf[1].foff, 0, sizeof(int) * ((size_t)N + 1)));
         HIPC(hipMemset(boff, 0, sizeof(int) * ((size_t)N + 1)));
         HIPC(hipMemset(nfoff, 0, sizeof(int) * ((size_t)N + 1)));
+        near_total = 0;
+        wl_len = 0;
         HIPC(hipMemset(nboff, 0, sizeof(int) * ((size_t)N + 1)));
         nfront = nback = 0;
         HIPC(hipMemset(perm, 0, sizeof(int) * N));
@@ -437,13 +448,15 @@ class Engine : public EngineBase {
                             dstatus + 2, block));
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
         // history-valid bits of the old near list back into the old pair words
-        KC(launch_near_sync(st, N, nfoff, nfl, Lo.fword, block));
+        KC(launch_near_sync(st, N, nfoff, nfw, nfs, Lo.fword, block));
         KC(launch_hist_remap<R>(st, N, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword, Lo.hist[p], block));
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
-        HIPC(grow_near(Ln.cap, bcap));
-        KC(launch_near_build(st, dp, s.pos, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfl, slot2near, bcnt, nboff,
-                             nbl, stmp, stotal + 2, block));
+        HIPC(grow_near(Ln.cap));
+        KC(launch_near_build(st, dp, s.pos, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfw, nfs, nfi, nfb, slot2near,
+                             bcnt, nboff, stampB, stmp, stotal + 2, block));
+        HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
         nfront = tot[0];
         nback = tot[1];
         // 6. clump member lists in ascending sorted order (replaces the per-step
@@ -517,20 +530,29 @@ class Engine : public EngineBase {
             KC(launch_fill_f64(st, (double*)rec.pair_cpt, 4 * (long long)rec_cap, 0.0));
         }
         ListBuf<R> L;
-        L.nfoff = nfoff; L.nf = nfl; L.nboff = nboff; L.nb = nbl;
+        L.nfoff = nfoff; L.nfw = nfw; L.nfs = nfs; L.nfi = nfi; L.nfb = nfb;
+        L.nboff = nboff; L.stampB = stampB; L.cF = cF; L.cB = cB;
         L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
         const int* halt = (const int*)dstatus;
         int* bad = (int*)(dstatus + 1);
         hipEvent_t e0;
         if (wl_len > 0) {
-            e0 = prof_begin();
-            KC(launch_walls<R>(st, dp, S[scur], C, wb, wl_len, pst, pene_partial, counters, bad, rec, record, halt,
+            // k_walls on the side stream, overlapping k_pairs; k_sums waits for both
+            HIPC(hipEventRecord(ev_fork, st));
+            HIPC(hipStreamWaitEvent(st2, ev_fork, 0));
+            e0 = prof_begin(st2);
+            KC(launch_walls<R>(st2, dp, S[scur], C, wb, wl_len, pst, pene_partial, counters, bad, rec, record, halt,
                                (int)n));
-            prof_end(5, e0, n);
+            prof_end(5, e0, n, st2);
+            HIPC(hipEventRecord(ev_join, st2));
         }
         e0 = prof_begin();
-        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, pst, wb, counters, bad, rec, record, block, 1, halt, (int)n));
+        KC(launch_pairs<R>(st, dp, S[scur], C, L, p, near_total, counters, rec, record, halt, (int)n));
         prof_end(1, e0, n);
+        if (wl_len > 0) HIPC(hipStreamWaitEvent(st, ev_join, 0));
+        e0 = prof_begin();
+        KC(launch_sums<R>(st, dp, S[scur], C, L, p, wb, bad, rec, record, block, 1, halt, (int)n));
+        prof_end(6, e0, n);
         if (dp.pene_on) {
             e0 = prof_begin();
             KC(launch_pene_finalize(st, dp, pst, pene_partial, (wl_len + 63) / 64, (int)(n % oi), cache_f, cache_t,
@@ -633,16 +655,16 @@ class Engine : public EngineBase {
 
     // ------------------------------------------------------------------
     // profiling with events on the library stream
-    hipEvent_t prof_begin() {
+    hipEvent_t prof_begin(hipStream_t s = nullptr) {
         if (!prof) return nullptr;
         hipEvent_t e = get_event();
-        (void)hipEventRecord(e, st);
+        (void)hipEventRecord(e, s ? s : st);
         return e;
     }
-    void prof_end(int cls, hipEvent_t b, long long step = -1) {
+    void prof_end(int cls, hipEvent_t b, long long step = -1, hipStream_t s = nullptr) {
         if (!prof || !b) return;
         hipEvent_t e = get_event();
-        (void)hipEventRecord(e, st);
+        (void)hipEventRecord(e, s ? s : st);
         pending.push_back({cls, step, b, e});
     }
     // forget the timings of launches that stood down after a halt at step h
@@ -778,7 +800,7 @@ class Engine : public EngineBase {
         HIPC(hipStreamSynchronize(st));
         if (!nfront) return 0;
         const FrontList& L = Lf[lcur];
-        KC(launch_near_sync(st, N, nfoff, nfl, L.fword, block));
+        KC(launch_near_sync(st, N, nfoff, nfw, nfs, L.fword, block));
         HIPC(hipStreamSynchronize(st));
         const int q = (int)(nstep & 1);  // step nstep-1 (parity q^1) wrote buffer q
         std::vector<uint32_t> w(nfront);
@@ -937,7 +959,8 @@ class Engine : public EngineBase {
     nbc_params P;
     nbc_options O;
     DevParams dp;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr, st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int N = 0, Nc = 0, block = 256, rbits = 8, wall_type = 0, nfaces = 0, nwh = 0, oi = 1, nbk = 1, batch = 64;
     int pair_cap_per = 16;
     SphereBuf<R> S[2] = {};
@@ -961,9 +984,11 @@ class Engine : public EngineBase {
     int* wl_off = nullptr;
     int wl_len = 0;  // spheres on the wall list
     int *nfoff = nullptr, *nboff = nullptr, *slot2near = nullptr;
-    uint2* nfl = nullptr;
-    int2* nbl = nullptr;
-    long long ncap_f = 0, ncap_b = 0;
+    uint32_t* nfw = nullptr;
+    int *nfs = nullptr, *nfi = nullptr, *nfb = nullptr, *stampB = nullptr;
+    R4 *cF = nullptr, *cB = nullptr;
+    long long ncap = 0;
+    int near_total = 0;  // near pairs (front and back entries alike)
     RecordBuf rec;
     bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;
     long long nstep = 0, nrebuild = 0;

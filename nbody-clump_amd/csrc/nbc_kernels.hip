@@ -1,17 +1,19 @@
 // HIP kernels of the clumped-sphere DEM timestep for gfx950 (MI355X).
 //
-// One steady-state step is four launches on the library stream:
+// One steady-state step is six launches on the library stream:
 //   k_clump_before   per clump   FP64 half-kick, drift, quaternion update
 //                                (src/dem_integrate_clumpsphere.cu:3-90)
 //   k_sphere_pose    per sphere  FP64 sphere pose, displacement, rebuild flag
 //                                (src/dem_clump.cu:113-138)
-//   k_contacts       per sphere  sphere-sphere, wall and penetrator contacts,
-//                                per-sphere force and clump torque
-//                                (src/dem_force_clump.cu:3-191,
-//                                src/dem_force_cube_wall_clump.cu:16-298,
+//   k_walls          per wall-list sphere  wall and penetrator contacts
+//                                (src/dem_force_cube_wall_clump.cu:16-298,
 //                                src/dem_force_cylind_wall.cu:152-237,
 //                                src/dem_contact_penetrator.cu:8-84,
-//                                src/dem_force_penetrator_clump.cu:4-182,
+//                                src/dem_force_penetrator_clump.cu:4-182)
+//   k_pairs          per near pair  sphere-sphere contact, evaluated once
+//                                (src/dem_force_clump.cu:3-161)
+//   k_sums           per sphere  ordered contact sums, totF/totT, clump torque
+//                                (src/dem_force_clump.cu:163-191,
 //                                src/dem_clump.cu:89-101,
 //                                src/dem_clumpsphere.cu:86-107)
 //   k_clump_after    per clump   deterministic member sum + second half-kick
@@ -21,14 +23,17 @@
 //
 // A rebuild (hash, stable LSD radix sort, cell bounds, gather, two-pass
 // neighbour search into CSR lists, back-slot resolution, history remap,
-// clump member lists) replaces src/dem_grid.cu:105-750.
+// clump member lists) replaces src/dem_grid.cu:105-750; it also builds the
+// near lists (the pairs that can touch before the next rebuild) and the wall
+// list (the spheres that can touch a wall or the penetrator).
 //
 // Every sphere-sphere pair is stored once (owner = lower sorted index) and
-// evaluated by both spheres' threads from the same inputs, so each thread
-// only writes its own sphere: no float atomics, results independent of
-// scheduling. The owner writes the new history into the other buffer of a
-// ping-pong pair; bits 30/31 of the pair word say whether each buffer holds a
-// valid history, which replaces the reference's per-step memsets of every slot.
+// evaluated once; its two contributions are stored per list entry and each
+// sphere adds its own in the reference's slot order, so there are no float
+// atomics and results do not depend on scheduling. The owner writes the new
+// history into the other buffer of a ping-pong pair; bits 30/31 of the pair
+// word say whether each buffer holds a valid history, which replaces the
+// reference's per-step memsets of every slot.
 #include <hip/hip_runtime.h>
 
 #include "nbc_device.h"
@@ -43,12 +48,16 @@ namespace nbc {
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
-// Logical block index that gives each XCD a contiguous run of blocks
-// (workgroups are dispatched round-robin over the 8 XCDs, each with its own L2).
-__device__ __forceinline__ int xcd_block(int b, int nb, int remap) {
-    if (!remap) return b;
-    const int nx = 8, q = nb / nx, r = nb % nx, x = b % nx, k = b / nx;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+// Logical block index giving each XCD runs of G consecutive blocks, the runs
+// of the 8 XCDs interleaved: neighbouring blocks (which share spheres) meet in
+// one L2, and every XCD still gets work from the whole bed.
+// The mapping permutes the first nb - nb % (8 G) blocks; the rest keep the
+// hardware order, so every logical block is run exactly once.
+__device__ __forceinline__ int xcd_group_block(int b, int nb, int G) {
+    const int nfull = nb - nb % (8 * G);
+    if (b >= nfull) return b;
+    const int x = b & 7, k = b >> 3;
+    return (k / G) * (8 * G) + x * G + (k % G);
 }
 
 template <class R> struct Ld4;
@@ -180,7 +189,7 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
     // first record flags a finite clump state (checked as in
     // src/dem_force_cube_wall_clump.cu:196-197)
     const bool ok = finite3(vel) && finite3(w) && isfinite(nx) && isfinite(ny) && isfinite(nz) && isfinite(nw);
-    if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);  // k_contacts then checks every sphere exactly
+    if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);  // k_sums then checks every sphere exactly
     C.snap[2 * i] = mk4<R>((R)vel.x, (R)vel.y, (R)vel.z, ok ? R(1) : R(0));
     C.snap[2 * i + 1] = mk4<R>((R)w.x, (R)w.y, (R)w.z, R(0));
 }
@@ -575,7 +584,7 @@ __global__ void k_near_count(DevParams P, const D4* pos, const int* foff, const 
 }
 
 __global__ void k_near_fill(DevParams P, const D4* pos, const int* foff, const uint32_t* fword, const int* nfoff,
-                            uint2* nf, int* slot2near) {
+                            uint32_t* nfw, int* nfs, int* nfi, int* nfb, int* slot2near) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const D4 a = pos[i];
@@ -583,7 +592,10 @@ __global__ void k_near_fill(DevParams P, const D4* pos, const int* foff, const u
     for (int s = foff[i]; s < foff[i + 1]; ++s) {
         const uint32_t w = fword[s];
         if (near_pair(P, a, pos[w & JMASK], w)) {
-            nf[o] = make_uint2((unsigned)s, w);
+            nfw[o] = w;
+            nfs[o] = s;
+            nfi[o] = i;
+            nfb[o] = -1;  // set by k_near_back_fill when the pair has a back entry
             slot2near[s] = o++;
         } else {
             slot2near[s] = -1;
@@ -603,7 +615,7 @@ __global__ void k_near_back_count(int N, const int* boff, const int2* bent, cons
 }
 
 __global__ void k_near_back_fill(int N, const int* boff, const int2* bent, const int* slot2near, const int* nboff,
-                                 int2* nb) {
+                                 int* nfb, int* stampB) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int o = nboff[i];
@@ -611,19 +623,22 @@ __global__ void k_near_back_fill(int N, const int* boff, const int2* bent, const
         const int2 be = bent[e];
         if (be.y >= 0) {
             const int q = slot2near[be.y];
-            if (q >= 0) nb[o++] = make_int2(be.x & ~BINTRA, q);
+            if (q >= 0) {
+                nfb[q] = o;
+                stampB[o++] = -1;
+            }
         }
     }
 }
 
 // history-valid bits of the near pairs back into the pair words
-__global__ void k_near_sync(int N, const int* nfoff, const uint2* nf, uint32_t* fword) {
+__global__ void k_near_sync(int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const uint32_t V = VALID(0) | VALID(1);
     for (int o = nfoff[i]; o < nfoff[i + 1]; ++o) {
-        const uint2 e = nf[o];
-        fword[e.x] = (fword[e.x] & ~V) | (e.y & V);
+        const int s = nfs[o];
+        fword[s] = (fword[s] & ~V) | (nfw[o] & V);
     }
 }
 
@@ -863,8 +878,9 @@ __device__ __forceinline__ V3<double> closest_on_tri(const V3<double>& p, const 
 // holds) and the penetrator culls. Every predicate is a threshold on a
 // distance-like function of the position with Lipschitz constant at most
 // (1 + |dir|^2), so lowering the thresholds by sl * (1 + |dir|^2) gives a test
-// that holds for every position within sl of xi: k_contacts runs it on the
-// FP32 position copy. A sphere failing it has no wall or penetrator contact.
+// that holds for every position within sl of xi (the wall list uses the
+// distance a sphere can move before the list is rebuilt). A sphere failing it
+// has no wall or penetrator contact.
 __device__ __forceinline__ bool wall_candidate(const DevParams& P, const V3<double>& xi, double ri, double sl) {
     const V3<double> RV = xi - mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
     if (P.wall_type == NBC_WALL_CUBE) {
@@ -904,58 +920,64 @@ __device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneSta
     return norm < (P.prad + 0.80 * LatDx) + s2;
 }
 
+// K3a: every touching near pair evaluated once (src/dem_force_clump.cu:54-161).
+// Waves walk the near front list flat, CH chunks of 64 entries each: the FP32
+// pre-filter and the FP64 touching test per entry, touching pairs queued in
+// LDS and evaluated one per lane. The owner's contribution goes to the pair's
+// front entry, the partner's (the reaction and its own torque term) to the
+// pair's back entry with the step number as stamp; k_sums adds them per
+// sphere in slot order. The owner writes the new history into the other
+// ping-pong buffer and sets or clears the pair's history-valid bit.
 template <class R, bool RECORD>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                                                  PeneState* pst, WallBuf<R> WL, unsigned long long* counters,
-                                                  int* bad, RecordBuf rec, int remap, const int* halt, int n) {
-    const int* status = halt + 3;  // ST_* bits
+__global__ __launch_bounds__(256) void k_pairs(
+    DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p, int total, unsigned long long* counters,
+    RecordBuf rec, const int* halt, int n) {
     typedef typename Vec4T<R>::type R4;
     if (*halt <= n) return;
-    // Per-wave queue of touching pairs: the wave scans its list segments, pushes
-    // the pairs that touch, and the queue is then evaluated one pair per lane
-    // so the force law runs on full waves. The lane of each sphere adds that
-    // sphere's results in queue order, which is its slot order (front rows,
-    // then back rows).
     constexpr int Q = 128;
-    __shared__ int q_slot[4][Q];
-    __shared__ int q_oth[4][Q];
-    __shared__ int q_inf[4][Q];
-    __shared__ R q_res[4][Q][6];
-    __shared__ int s_fo[4][65], s_bo[4][65];
-    __shared__ F4 s_p32[4][64];
-    const int lb = xcd_block(blockIdx.x, gridDim.x, remap);
-    const int i = lb * blockDim.x + threadIdx.x;
-    const bool active = i < P.N;
+    constexpr int CH = 4;
+    __shared__ int q_e[4][Q], q_i[4][Q], q_j[4][Q];
+    __shared__ uint32_t q_w[4][Q];
+    __shared__ unsigned int s_cnt[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int wbase = i - lane;
-    int n_ss = 0;
-    V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss;
+    const int lb = xcd_group_block(blockIdx.x, gridDim.x, 16);
+    const long long e0 = ((long long)lb * (blockDim.x >> 6) + wv) * (64 * CH);
     R* hcur = L.hist[p];
     R* hnew = L.hist[p ^ 1];
-    int qn = 0;  // wave-uniform queue length
+    int qn = 0, n_ss = 0;
 
     auto flush = [&]() {
         for (int r0 = 0; r0 < qn; r0 += 64) {
-            const int e = r0 + lane;
-            if (e < qn) {
-                const int inf = q_inf[wv][e];
-                const int side = inf >> 6;
-                const int me = wbase + (inf & 63);
-                const int oth = q_oth[wv][e];
-                const int ne = q_slot[wv][e];  // index of the pair in the owner's near row
-                const int I = side ? oth : me, J = side ? me : oth;
-                const D4 pI = S.pos[I], pJ = S.pos[J];
+            const int k = r0 + lane;
+            const int e = k < qn ? q_e[wv][k] : 0, I = k < qn ? q_i[wv][k] : 0, J = k < qn ? q_j[wv][k] : 0;
+            const uint32_t w = k < qn ? q_w[wv][k] : 0u;
+            // the FP64 touching test of the queued candidate (the positions are
+            // needed for the force anyway)
+            bool touch = false;
+            D4 pI, pJ;
+            pI.x = pI.y = pI.z = pI.w = 0.0;
+            pJ = pI;
+            double DistS = 0.0;
+            V3<double> Dd = mk<double>(0.0, 0.0, 0.0);
+            if (k < qn) {
+                pI = S.pos[I];
+                pJ = S.pos[J];
+                Dd = xyz(pJ) - xyz(pI);
+                DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+                const double rs = pI.w + pJ.w;
+                if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
+                if (!touch && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
+            }
+            if (touch) {
+                n_ss++;
                 const R4 kI = S.kin[I], kJ = S.kin[J];
                 const int cI = S.cid[I], cJ = S.cid[J];
                 const R4 svI = C.snap[2 * cI], swI = C.snap[2 * cI + 1];
                 const R4 svJ = C.snap[2 * cJ], swJ = C.snap[2 * cJ + 1];
-                const V3<double> Dd = xyz(pJ) - xyz(pI);
-                const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+                const int s = L.nfs[e];  // front slot: history and record index
+                const int b = L.nfb[e];  // the pair's back entry
                 const double DistL = sqrt(DistS);
                 const double Dn = pI.w + pJ.w - DistL;
-                const uint2 nw = L.nf[ne];
-                const int s = (int)nw.x;  // front slot: history and record index
-                const uint32_t w = nw.y;
                 PairIn<R> in;
                 in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
                 in.rI = (R)pI.w; in.rJ = (R)pJ.w; in.mI = kI.w; in.mJ = kJ.w;
@@ -965,60 +987,101 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
                 PairOut<R> o;
                 pair_force<R>(P, in, o);
-                V3<R> f, t;
-                if (side == 0) {
-                    store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
-                    if (!(w & VALID(p ^ 1))) L.nf[ne].y = w | VALID(p ^ 1);
-                    f = o.cfc;
-                    t = o.ctc;
-                    if (RECORD) {
-                        D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
-                        const V3<double> cpt = xyz(pI) + (double)o.Ap * mk<double>((double)o.DistU.x, (double)o.DistU.y,
-                                                                                  (double)o.DistU.z);
-                        D4 b; b.x = cpt.x; b.y = cpt.y; b.z = cpt.z; b.w = 0.0;
-                        rec.pair_fcn[s] = a;
-                        rec.pair_cpt[s] = b;
-                    }
-                } else {
-                    f = R(-1.0) * o.cfc;  // x - y == x + (-y) exactly: same sums as subtracting
-                    t = o.ctc2;
+                store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
+                if (!(w & VALID(p ^ 1))) L.nfw[e] = w | VALID(p ^ 1);
+                L.cF[2 * (size_t)e] = mk4<R>(o.cfc.x, o.cfc.y, o.cfc.z, R(0));
+                L.cF[2 * (size_t)e + 1] = mk4<R>(o.ctc.x, o.ctc.y, o.ctc.z, R(0));
+                if (b >= 0) {  // a pair without a back entry gives the partner no reaction
+                    const V3<R> f = R(-1.0) * o.cfc;  // x - y == x + (-y) exactly: same sums as subtracting
+                    L.cB[2 * (size_t)b] = mk4<R>(f.x, f.y, f.z, R(0));
+                    L.cB[2 * (size_t)b + 1] = mk4<R>(o.ctc2.x, o.ctc2.y, o.ctc2.z, R(0));
+                    L.stampB[b] = n;
                 }
-                R* rr = q_res[wv][e];
-                rr[0] = f.x; rr[1] = f.y; rr[2] = f.z; rr[3] = t.x; rr[4] = t.y; rr[5] = t.z;
+                if (RECORD) {
+                    D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
+                    const V3<double> cpt = xyz(pI) + (double)o.Ap * mk<double>((double)o.DistU.x, (double)o.DistU.y,
+                                                                              (double)o.DistU.z);
+                    D4 bq; bq.x = cpt.x; bq.y = cpt.y; bq.z = cpt.z; bq.w = 0.0;
+                    rec.pair_fcn[s] = a;
+                    rec.pair_cpt[s] = bq;
+                }
             }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // Queue keys (owner lane, +64 for back entries) are non-decreasing, so
-        // each lane's front and back entries are two runs found by binary search.
-        auto lbound = [&](int key) {
-            int lo = 0;
-            for (int st = Q; st > 0; st >>= 1)
-                if (lo + st <= qn && q_inf[wv][lo + st - 1] < key) lo += st;
-            return lo;
-        };
-        const int f0 = lbound(lane), f1 = lbound(lane + 1), b0 = lbound(lane + 64), b1 = lbound(lane + 65);
-        for (int e = f0; e < f1; ++e) {
-            const R* rr = q_res[wv][e];
-            Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
-            Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
-        }
-        for (int e = b0; e < b1; ++e) {
-            const R* rr = q_res[wv][e];
-            Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
-            Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         qn = 0;
     };
 
-    // The wave's 64 spheres own the contiguous list segments
-    // [foff[wbase], foff[wbase + 64]) and [boff[wbase], boff[wbase + 64]); the
-    // wave walks each segment 64 entries at a time (coalesced list reads, equal
-    // work per lane) and finds the owner row of an entry by binary search over
-    // the row offsets held in LDS. Touching pairs are queued in entry order,
-    // i.e. per sphere in slot order, front rows before back rows.
+    // the next chunk's list words are loaded one chunk ahead
+    uint32_t wnext = 0u;
+    int inext = 0;
+    if (e0 + lane < total) {
+        wnext = __builtin_nontemporal_load(L.nfw + e0 + lane);
+        inext = __builtin_nontemporal_load(L.nfi + e0 + lane);
+    }
+    for (int c = 0; c < CH; ++c) {
+        const long long cb = e0 + 64 * c;
+        if (cb >= total) break;  // wave-uniform
+        const int e = (int)cb + lane;
+        const uint32_t w = wnext;
+        const int i = inext;
+        if (c + 1 < CH && e + 64 < total) {
+            wnext = __builtin_nontemporal_load(L.nfw + e + 64);
+            inext = __builtin_nontemporal_load(L.nfi + e + 64);
+        }
+        const int j = (int)(w & JMASK);
+        bool cand = false;
+        if (e < total) {
+            cand = maybe_touch(P, S.p32[i], S.p32[j]);
+            if (!cand && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
+        }
+        const uint64_t m = __ballot(cand);
+        if (cand) {
+            const int pos = qn + __popcll(m & lanemask_lt());
+            q_e[wv][pos] = e;
+            q_i[wv][pos] = i;
+            q_j[wv][pos] = j;
+            q_w[wv][pos] = w;
+        }
+        qn += __popcll(m);
+        if (qn > Q - 64) flush();
+    }
+    if (qn > 0) flush();
+    unsigned a0 = (unsigned)n_ss;
+    for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
+    if (lane == 0) s_cnt[wv] = a0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < ((int)blockDim.x + 63) >> 6; ++k) t += s_cnt[k];
+        if (t) atomicAdd(&counters[0], t);
+    }
+}
+
+// K3b: per sphere, the touching pairs' contributions in the reference's order
+// (front slots ascending, then back entries in back-row order;
+// src/dem_force_clump.cu:163-191), the wall and penetrator terms, totF/totT
+// (src/dem_clump.cu:89-101) and the clump torque term T + (x_s - x_c) x F
+// (src/dem_clumpsphere.cu:86-107). A wave reads the contiguous near segments
+// of its 64 spheres 64 entries at a time (coalesced), queues the touching
+// entries' contributions in LDS with their owner lane, and each lane adds its
+// own in queue order.
+template <class R, bool RECORD>
+__global__ __launch_bounds__(256) void k_sums(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                                              WallBuf<R> WL, int* bad, RecordBuf rec, int remap, const int* halt,
+                                              int n) {
+    typedef typename Vec4T<R>::type R4;
+    if (*halt <= n) return;
+    const int* status = halt + 3;  // ST_* bits
+    constexpr int Q = 128;
+    __shared__ int q_key[4][Q];
+    __shared__ R q_val[4][Q][6];
+    __shared__ int s_fo[4][65], s_bo[4][65];
+    const int lb = remap ? xcd_group_block(blockIdx.x, gridDim.x, 16) : (int)blockIdx.x;
+    const int i = lb * blockDim.x + threadIdx.x;
+    const bool active = i < P.N;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss;
     {
         const int ir = i < P.N ? i : P.N;
         s_fo[wv][lane] = L.nfoff[ir];
@@ -1028,92 +1091,94 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             s_fo[wv][64] = L.nfoff[ie];
             s_bo[wv][64] = L.nboff[ie];
         }
-        F4 a = S.p32[i < P.N ? i : P.N - 1];  // clamped load, then a value select
-        if (!active) a.w = -1.0f;
-        s_p32[wv][lane] = a;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
-    // front entries: pairs the row's sphere owns (it writes their history)
+    int qn = 0;
+    // Queue keys (owner lane, +64 for back entries) are non-decreasing, so each
+    // lane's front and back entries are two runs found by binary search.
+    auto drain = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        auto lbound = [&](int key) {
+            int lo = 0;
+            for (int st = Q; st > 0; st >>= 1)
+                if (lo + st <= qn && q_key[wv][lo + st - 1] < key) lo += st;
+            return lo;
+        };
+        const int f0 = lbound(lane), f1 = lbound(lane + 1), b0 = lbound(lane + 64), b1 = lbound(lane + 65);
+        for (int k = f0; k < f1; ++k) {
+            const R* v = q_val[wv][k];
+            Fss = Fss + mk<R>(v[0], v[1], v[2]);
+            Tss = Tss + mk<R>(v[3], v[4], v[5]);
+        }
+        for (int k = b0; k < b1; ++k) {
+            const R* v = q_val[wv][k];
+            Fss = Fss + mk<R>(v[0], v[1], v[2]);
+            Tss = Tss + mk<R>(v[3], v[4], v[5]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        qn = 0;
+    };
+    auto push = [&](bool t, int key, const R4& f, const R4& tq) {
+        const uint64_t m = __ballot(t);
+        if (t) {
+            const int pos = qn + __popcll(m & lanemask_lt());
+            q_key[wv][pos] = key;
+            R* v = q_val[wv][pos];
+            v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = tq.x; v[4] = tq.y; v[5] = tq.z;
+        }
+        qn += __popcll(m);
+        if (qn > Q - 64) drain();
+    };
+    // front entries touch this step when the history-valid bit of the new
+    // buffer is set, back entries when stamped with the step number; each
+    // chunk's words are loaded one chunk ahead
     {
-        const int F1 = s_fo[wv][64];
-        int c = s_fo[wv][0];
-        // the next chunk's pair words are loaded one iteration ahead
-        uint32_t wnext = c + lane < F1 ? L.nf[c + lane].y : 0u;
-        for (; c < F1; c += 64) {
+        const int F0 = s_fo[wv][0], F1 = s_fo[wv][64], B0 = s_bo[wv][0], B1 = s_bo[wv][64];
+        uint32_t wnext = F0 + lane < F1 ? __builtin_nontemporal_load(L.nfw + F0 + lane) : 0u;
+        int snext = B0 + lane < B1 ? __builtin_nontemporal_load(L.stampB + B0 + lane) : -1;
+        for (int c = F0; c < F1; c += 64) {
             const int e = c + lane;
-            bool touch = false;
-            int j = 0, r = 0;
-            const uint32_t wcur = wnext;
-            if (e + 64 < F1) wnext = L.nf[e + 64].y;
-            if (e < F1) {
-                const uint32_t w = wcur;
+            const uint32_t w = wnext;
+            if (e + 64 < F1) wnext = __builtin_nontemporal_load(L.nfw + e + 64);
+            bool t = false;
+            int r = 0;
+            R4 f, tq;
+            f.x = f.y = f.z = f.w = R(0);
+            tq = f;
+            if (e < F1 && (w & VALID(p ^ 1))) {
+                t = true;
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_fo[wv][r + st] <= e) r += st;
-                j = (int)(w & JMASK);
-                if (maybe_touch(P, s_p32[wv][r], S.p32[j])) {
-                    const D4 pi4r = S.pos[wbase + r];
-                    const D4 pj4 = S.pos[j];
-                    const V3<double> Dd = xyz(pj4) - xyz(pi4r);
-                    const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-                    const double rs = pi4r.w + pj4.w;
-                    if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
-                }
-                if (!touch && (w & VALID(p ^ 1))) L.nf[e].y = w & ~VALID(p ^ 1);
+                f = L.cF[2 * (size_t)e];
+                tq = L.cF[2 * (size_t)e + 1];
             }
-            const uint64_t m = __ballot(touch);
-            if (touch) {
-                const int pos = qn + __popcll(m & lanemask_lt());
-                q_slot[wv][pos] = e;
-                q_oth[wv][pos] = j;
-                q_inf[wv][pos] = r;  // key: owner lane, front side
-                n_ss++;
-            }
-            qn += __popcll(m);
-            if (qn > Q - 64) flush();
+            push(t, r, f, tq);
         }
-    }
-    // back entries: the owner holds the history; the same pair is evaluated
-    // again from the same inputs and the row's sphere takes the reaction
-    {
-        const int B1 = s_bo[wv][64];
-        int c = s_bo[wv][0];
-        int2 bnext = c + lane < B1 ? L.nb[c + lane] : make_int2(0, -1);
-        for (; c < B1; c += 64) {
+        for (int c = B0; c < B1; c += 64) {
             const int e = c + lane;
-            bool touch = false;
+            const int sb = snext;
+            if (e + 64 < B1) snext = __builtin_nontemporal_load(L.stampB + e + 64);
+            bool t = false;
             int r = 0;
-            int2 be = bnext;
-            if (e + 64 < B1) bnext = L.nb[e + 64];
-            if (e < B1) {
+            R4 f, tq;
+            f.x = f.y = f.z = f.w = R(0);
+            tq = f;
+            if (e < B1 && sb == n) {
+                t = true;
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_bo[wv][r + st] <= e) r += st;
-                if (maybe_touch(P, s_p32[wv][r], S.p32[be.x])) {
-                    const D4 pi4r = S.pos[wbase + r];
-                    const D4 po4 = S.pos[be.x];
-                    const V3<double> Dd = xyz(pi4r) - xyz(po4);
-                    const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-                    const double rs = po4.w + pi4r.w;
-                    if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
-                }
+                f = L.cB[2 * (size_t)e];
+                tq = L.cB[2 * (size_t)e + 1];
             }
-            const uint64_t m = __ballot(touch);
-            if (touch) {
-                const int pos = qn + __popcll(m & lanemask_lt());
-                q_slot[wv][pos] = be.y;
-                q_oth[wv][pos] = be.x;
-                q_inf[wv][pos] = r | 64;  // key: owner lane, back side
-            }
-            qn += __popcll(m);
-            if (qn > Q - 64) flush();
+            push(t, r + 64, f, tq);
         }
     }
-    if (qn > 0) flush();
-    // Wall and penetrator contacts are rare. A sphere that may have one, had
-    // one last step, or fails the input check of the wall kernel is handed to
-    // k_walls with its sphere-sphere sums; every other sphere finishes here.
+    if (qn > 0) drain();
     if (active) {
         const R4 ki = S.kin[i];
         int wk = -1;
@@ -1121,7 +1186,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         if (P.nfaces > 0 && wk < 0) {
             // input check of the wall kernel (src/dem_force_cube_wall_clump.cu:196-197)
             // for spheres off the wall list: exact only when a cheap sign is off
-            const F4 q = s_p32[wv][lane];
+            const F4 q = S.p32[i];
             if (!(q.w > 0.0f) || !isfinite((double)ki.x) || !isfinite((double)ki.y) || !isfinite((double)ki.z) ||
                 !isfinite((double)ki.w) || !(ki.w > R(0)) || (*status & ST_CLUMP_BAD)) {
                 const D4 pi4 = S.pos[i];
@@ -1170,17 +1235,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
                 rec.parts[6 * (size_t)i + k] = q;
             }
         }
-    }
-    // touching sphere-sphere count
-    __shared__ unsigned int s_cnt[4];
-    unsigned a0 = (unsigned)n_ss;
-    for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
-    if (lane == 0) s_cnt[wv] = a0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int k = 0; k < ((int)blockDim.x + 63) >> 6; ++k) t += s_cnt[k];
-        if (t) atomicAdd(&counters[0], t);
     }
 }
 
@@ -1721,8 +1775,8 @@ int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* po
 }
 
 int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
-                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint2* nf, int* slot2near, int* nbcnt,
-                      int* nboff, int2* nb, int* stmp, int* stotal, int block) {
+                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs, int* nfi,
+                      int* nfb, int* slot2near, int* nbcnt, int* nboff, int* stampB, int* stmp, int* stotal, int block) {
     const int N = P.N;
     const unsigned g = nblk(N, block);
     int e;
@@ -1731,17 +1785,18 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const i
     k_near_count<<<g, block, 0, st>>>(P, pos, foff, fword, ncnt);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, ncnt, nfoff, N + 1, stmp, stotal))) return e;
-    k_near_fill<<<g, block, 0, st>>>(P, pos, foff, fword, nfoff, nf, slot2near);
+    k_near_fill<<<g, block, 0, st>>>(P, pos, foff, fword, nfoff, nfw, nfs, nfi, nfb, slot2near);
     LAUNCH_CHECK();
     k_near_back_count<<<g, block, 0, st>>>(N, boff, bent, slot2near, nbcnt);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, nbcnt, nboff, N + 1, stmp, stotal + 1))) return e;
-    k_near_back_fill<<<g, block, 0, st>>>(N, boff, bent, slot2near, nboff, nb);
+    k_near_back_fill<<<g, block, 0, st>>>(N, boff, bent, slot2near, nboff, nfb, stampB);
     LAUNCH_CHECK();
     return 0;
 }
-int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint2* nf, uint32_t* fword, int block) {
-    k_near_sync<<<nblk(N, block), block, 0, st>>>(N, nfoff, nf, fword);
+int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
+                     int block) {
+    k_near_sync<<<nblk(N, block), block, 0, st>>>(N, nfoff, nfw, nfs, fword);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1795,14 +1850,26 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
 }
 int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
-int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    PeneState* pst, WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                    int block, int remap, const int* halt, int step) {
+int launch_pairs(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p, int total,
+                 unsigned long long* ctr, RecordBuf rec, bool record, const int* halt, int step) {
+    if (total <= 0) return 0;
+    const long long waves = ((long long)total + 64 * 4 - 1) / (64 * 4);  // k_pairs: 4 chunks of 64 per wave
+    const unsigned nb = (unsigned)((waves + 3) / 4);
+    if (record)
+        k_pairs<R, true><<<nb, 256, 0, st>>>(P, S, C, L, p, total, ctr, rec, halt, step);
+    else
+        k_pairs<R, false><<<nb, 256, 0, st>>>(P, S, C, L, p, total, ctr, rec, halt, step);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_sums(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                WallBuf<R> WL, int* bad, RecordBuf rec, bool record, int block, int remap, const int* halt, int step) {
     const unsigned nb = nblk(P.N, block);
     if (record)
-        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, pst, WL, ctr, bad, rec, remap, halt, step);
+        k_sums<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, WL, bad, rec, remap, halt, step);
     else
-        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, pst, WL, ctr, bad, rec, remap, halt, step);
+        k_sums<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, WL, bad, rec, remap, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1846,9 +1913,10 @@ int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
     template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
-    template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
-                                    PeneState*, WallBuf<R>, unsigned long long*, int*, RecordBuf, bool, int, int,    \
-                                    const int*, int);                                                                \
+    template int launch_pairs<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int, int,     \
+                                 unsigned long long*, RecordBuf, bool, const int*, int);                             \
+    template int launch_sums<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int, WallBuf<R>, \
+                                int*, RecordBuf, bool, int, int, const int*, int);                                   \
     template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>, int,          \
                                  PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, const int*, int);  \
     template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, const int*,         \
