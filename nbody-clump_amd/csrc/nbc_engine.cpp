@@ -2,8 +2,8 @@
 //
 // The loop body of src/dem_simulation_clump.cu:110-226 becomes, per step:
 //   K1 k_clump_before, K2 k_sphere_pose (sets the rebuild flag),
-//   k_walls (wall list), k_pairs + k_sums (sphere-sphere contacts and
-//   per-sphere totals), k_pene_finalize (penetrator runs only),
+//   k_walls (wall list, side stream) beside k_contacts (sphere-sphere contacts
+//   and per-sphere totals), k_pene_finalize (penetrator runs only),
 //   K4 k_clump_after.
 // Steps are enqueued in batches; one status read-back per batch reports the
 // first step that needs a rebuild (or a new wall list), which is then done
@@ -78,7 +78,7 @@ class Engine : public EngineBase {
         device = dev;
         HIPC(hipSetDevice(device));
         HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        // side stream (default priority) for the wall kernel, which overlaps k_pairs
+        // side stream (default priority) for the wall kernel, which overlaps k_contacts
         HIPC(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
@@ -92,7 +92,7 @@ class Engine : public EngineBase {
         block = O.block_size > 0 ? O.block_size : 256;
         if (block < 64 || block > 1024 || (block % 64) != 0)
             return fail(NBC_E_BADINPUT, "nbc_create: block_size must be a multiple of 64 in [64, 1024]");
-        if (block > 256) block = 256;  // k_sums is launched with __launch_bounds__(256)
+        if (block > 256) block = 256;  // k_contacts is launched with __launch_bounds__(256)
         rbits = O.radix_bits > 0 ? O.radix_bits : 8;
         if (rbits > 8) rbits = 8;
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
@@ -168,7 +168,7 @@ class Engine : public EngineBase {
         HIPC(dalloc(&fcnt, (size_t)N + 1)); HIPC(dalloc(&bcnt, (size_t)N + 1));
         HIPC(dalloc(&mbeg, Nc)); HIPC(dalloc(&mend, Nc)); HIPC(dalloc(&mem, N));
         HIPC(dalloc(&dstatus, 4));
-        HIPC(dalloc(&counters, 2));
+        HIPC(dalloc(&counters, 2));  // touching pairs, touching wall contacts
         HIPC(dalloc(&pst, 1));
         nbk = contacts_blocks(N, block);
         HIPC(dalloc(&pene_partial, 6 * (((size_t)N + 63) / 64)));  // one per wave of k_walls
@@ -213,17 +213,15 @@ class Engine : public EngineBase {
     // front-list capacity (one entry per near pair on each side)
     hipError_t grow_near(long long fcap) {
         if (fcap <= ncap) return hipSuccess;
-        dfree(nfw); dfree(nfs); dfree(nfi); dfree(nfb); dfree(stampB); dfree(slot2near); dfree(cF); dfree(cB);
-        nfw = nullptr; nfs = nfi = nfb = stampB = slot2near = nullptr; cF = cB = nullptr; ncap = 0;
+        dfree(nfw); dfree(nfs); dfree(nbo); dfree(nbe); dfree(slot2near);
+        nfw = nullptr; nfs = nbo = nbe = slot2near = nullptr;
+        ncap = 0;
         hipError_t e;
         if ((e = dalloc(&nfw, (size_t)fcap)) != hipSuccess) return e;
         if ((e = dalloc(&nfs, (size_t)fcap)) != hipSuccess) return e;
-        if ((e = dalloc(&nfi, (size_t)fcap)) != hipSuccess) return e;
-        if ((e = dalloc(&nfb, (size_t)fcap)) != hipSuccess) return e;
-        if ((e = dalloc(&stampB, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&nbo, (size_t)fcap)) != hipSuccess) return e;
+        if ((e = dalloc(&nbe, (size_t)fcap)) != hipSuccess) return e;
         if ((e = dalloc(&slot2near, (size_t)fcap)) != hipSuccess) return e;
-        if ((e = dalloc(&cF, 2 * (size_t)fcap)) != hipSuccess) return e;
-        if ((e = dalloc(&cB, 2 * (size_t)fcap)) != hipSuccess) return e;
         ncap = fcap;
         return hipSuccess;
     }
@@ -240,7 +238,7 @@ class Engine : public EngineBase {
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap);
-        F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nfi); F(nfb); F(stampB); F(slot2near); F(cF); F(cB);
+        F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
@@ -453,8 +451,8 @@ class Engine : public EngineBase {
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));
-        KC(launch_near_build(st, dp, s.pos, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfw, nfs, nfi, nfb, slot2near,
-                             bcnt, nboff, stampB, stmp, stotal + 2, block));
+        KC(launch_near_build(st, dp, s.pos, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt,
+                             nboff, nbo, nbe, stmp, stotal + 2, block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         nfront = tot[0];
@@ -530,14 +528,13 @@ class Engine : public EngineBase {
             KC(launch_fill_f64(st, (double*)rec.pair_cpt, 4 * (long long)rec_cap, 0.0));
         }
         ListBuf<R> L;
-        L.nfoff = nfoff; L.nfw = nfw; L.nfs = nfs; L.nfi = nfi; L.nfb = nfb;
-        L.nboff = nboff; L.stampB = stampB; L.cF = cF; L.cB = cB;
+        L.nfoff = nfoff; L.nfw = nfw; L.nfs = nfs; L.nboff = nboff; L.nbo = nbo; L.nbe = nbe;
         L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
         const int* halt = (const int*)dstatus;
         int* bad = (int*)(dstatus + 1);
         hipEvent_t e0;
         if (wl_len > 0) {
-            // k_walls on the side stream, overlapping k_pairs; k_sums waits for both
+            // k_walls on the side stream, overlapping k_contacts
             HIPC(hipEventRecord(ev_fork, st));
             HIPC(hipStreamWaitEvent(st2, ev_fork, 0));
             e0 = prof_begin(st2);
@@ -547,12 +544,10 @@ class Engine : public EngineBase {
             HIPC(hipEventRecord(ev_join, st2));
         }
         e0 = prof_begin();
-        KC(launch_pairs<R>(st, dp, S[scur], C, L, p, near_total, counters, rec, record, halt, (int)n));
+        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, wb, counters, bad, rec, record, block, 1, halt, (int)n));
         prof_end(1, e0, n);
+        // the penetrator sum and k_clump_after (which finishes the wall-list spheres) wait for k_walls
         if (wl_len > 0) HIPC(hipStreamWaitEvent(st, ev_join, 0));
-        e0 = prof_begin();
-        KC(launch_sums<R>(st, dp, S[scur], C, L, p, wb, bad, rec, record, block, 1, halt, (int)n));
-        prof_end(6, e0, n);
         if (dp.pene_on) {
             e0 = prof_begin();
             KC(launch_pene_finalize(st, dp, pst, pene_partial, (wl_len + 63) / 64, (int)(n % oi), cache_f, cache_t,
@@ -560,7 +555,7 @@ class Engine : public EngineBase {
             prof_end(4, e0, n);
         }
         e0 = prof_begin();
-        KC(launch_clump_after<R>(st, dp, S[scur], C, mbeg, mend, mem, block, halt, (int)n));
+        KC(launch_clump_after<R>(st, dp, S[scur], C, wb, rec, record, mbeg, mend, mem, block, halt, (int)n));
         prof_end(2, e0, n);
         return 0;
     }
@@ -985,8 +980,7 @@ class Engine : public EngineBase {
     int wl_len = 0;  // spheres on the wall list
     int *nfoff = nullptr, *nboff = nullptr, *slot2near = nullptr;
     uint32_t* nfw = nullptr;
-    int *nfs = nullptr, *nfi = nullptr, *nfb = nullptr, *stampB = nullptr;
-    R4 *cF = nullptr, *cB = nullptr;
+    int *nfs = nullptr, *nbo = nullptr, *nbe = nullptr;
     long long ncap = 0;
     int near_total = 0;  // near pairs (front and back entries alike)
     RecordBuf rec;

@@ -42,12 +42,9 @@ struct ListBuf {
     int* nfoff;                       // [N + 1] near front rows
     uint32_t* nfw;                    // [near] pair word with the history-valid bits
     int* nfs;                         // [near] front slot (history, records)
-    int* nfi;                         // [near] owner sphere
-    int* nfb;                         // [near] the pair's back near entry
     int* nboff;                       // [N + 1] near back rows
-    int* stampB;                      // [near] step number when the back entry last touched
-    typename Vec4T<R>::type* cF;      // [2 near] owner's force, torque of the pair
-    typename Vec4T<R>::type* cB;      // [2 near] partner's force, torque (by back entry)
+    int* nbo;                         // [near] owner of the back entry's pair
+    int* nbe;                         // [near] the pair's index in the owner's near row
     R* hist[2];                       // [front slots][HSTRIDE] ping-pong histories
 };
 
@@ -105,16 +102,14 @@ template <class R>
 int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
                       const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
 int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
-                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs, int* nfi,
-                      int* nfb, int* slot2near, int* nbcnt, int* nboff, int* stampB, int* stmp, int* stotal, int block);
+                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs,
+                      int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal, int block);
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
                      int block);
 template <class R>
-int launch_pairs(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p, int total,
-                 unsigned long long* ctr, RecordBuf rec, bool record, const int* halt, int step);
-template <class R>
-int launch_sums(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                WallBuf<R> WL, int* bad, RecordBuf rec, bool record, int block, int remap, const int* halt, int step);
+int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                    WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block, int remap,
+                    const int* halt, int step);
 template <class R>
 int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
                  PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
@@ -122,7 +117,8 @@ int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R>
 int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* wflag, PeneState* pst,
                        int* cnt, int* off, int* list, int* pos_of, int* stmp, int* stotal, int block);
 template <class R>
-int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
-                       const int* mend, const int* mem, int block, const int* halt, int step);
+int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL,
+                       RecordBuf rec, bool record, const int* mbeg, const int* mend, const int* mem, int block,
+                       const int* halt, int step);
 
 }  // namespace nbc

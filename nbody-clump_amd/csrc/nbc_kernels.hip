@@ -1,6 +1,6 @@
 // HIP kernels of the clumped-sphere DEM timestep for gfx950 (MI355X).
 //
-// One steady-state step is six launches on the library stream:
+// One steady-state step is five launches (k_walls on a side stream):
 //   k_clump_before   per clump   FP64 half-kick, drift, quaternion update
 //                                (src/dem_integrate_clumpsphere.cu:3-90)
 //   k_sphere_pose    per sphere  FP64 sphere pose, displacement, rebuild flag
@@ -10,10 +10,8 @@
 //                                src/dem_force_cylind_wall.cu:152-237,
 //                                src/dem_contact_penetrator.cu:8-84,
 //                                src/dem_force_penetrator_clump.cu:4-182)
-//   k_pairs          per near pair  sphere-sphere contact, evaluated once
-//                                (src/dem_force_clump.cu:3-161)
-//   k_sums           per sphere  ordered contact sums, totF/totT, clump torque
-//                                (src/dem_force_clump.cu:163-191,
+//   k_contacts       per sphere  sphere-sphere contacts, totF/totT, clump torque
+//                                (src/dem_force_clump.cu:3-191,
 //                                src/dem_clump.cu:89-101,
 //                                src/dem_clumpsphere.cu:86-107)
 //   k_clump_after    per clump   deterministic member sum + second half-kick
@@ -28,9 +26,9 @@
 // list (the spheres that can touch a wall or the penetrator).
 //
 // Every sphere-sphere pair is stored once (owner = lower sorted index) and
-// evaluated once; its two contributions are stored per list entry and each
-// sphere adds its own in the reference's slot order, so there are no float
-// atomics and results do not depend on scheduling. The owner writes the new
+// evaluated by both spheres' waves from the same inputs, so each wave only
+// writes its own spheres: no float atomics, results independent of
+// scheduling. The owner writes the new
 // history into the other buffer of a ping-pong pair; bits 30/31 of the pair
 // word say whether each buffer holds a valid history, which replaces the
 // reference's per-step memsets of every slot.
@@ -189,7 +187,7 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
     // first record flags a finite clump state (checked as in
     // src/dem_force_cube_wall_clump.cu:196-197)
     const bool ok = finite3(vel) && finite3(w) && isfinite(nx) && isfinite(ny) && isfinite(nz) && isfinite(nw);
-    if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);  // k_sums then checks every sphere exactly
+    if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);  // k_contacts then checks every sphere exactly
     C.snap[2 * i] = mk4<R>((R)vel.x, (R)vel.y, (R)vel.z, ok ? R(1) : R(0));
     C.snap[2 * i + 1] = mk4<R>((R)w.x, (R)w.y, (R)w.z, R(0));
 }
@@ -584,7 +582,7 @@ __global__ void k_near_count(DevParams P, const D4* pos, const int* foff, const 
 }
 
 __global__ void k_near_fill(DevParams P, const D4* pos, const int* foff, const uint32_t* fword, const int* nfoff,
-                            uint32_t* nfw, int* nfs, int* nfi, int* nfb, int* slot2near) {
+                            uint32_t* nfw, int* nfs, int* slot2near) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const D4 a = pos[i];
@@ -594,8 +592,6 @@ __global__ void k_near_fill(DevParams P, const D4* pos, const int* foff, const u
         if (near_pair(P, a, pos[w & JMASK], w)) {
             nfw[o] = w;
             nfs[o] = s;
-            nfi[o] = i;
-            nfb[o] = -1;  // set by k_near_back_fill when the pair has a back entry
             slot2near[s] = o++;
         } else {
             slot2near[s] = -1;
@@ -615,7 +611,7 @@ __global__ void k_near_back_count(int N, const int* boff, const int2* bent, cons
 }
 
 __global__ void k_near_back_fill(int N, const int* boff, const int2* bent, const int* slot2near, const int* nboff,
-                                 int* nfb, int* stampB) {
+                                 int* nbo, int* nbe) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int o = nboff[i];
@@ -624,8 +620,8 @@ __global__ void k_near_back_fill(int N, const int* boff, const int2* bent, const
         if (be.y >= 0) {
             const int q = slot2near[be.y];
             if (q >= 0) {
-                nfb[q] = o;
-                stampB[o++] = -1;
+                nbo[o] = be.x & ~BINTRA;
+                nbe[o++] = q;
             }
         }
     }
@@ -920,167 +916,45 @@ __device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneSta
     return norm < (P.prad + 0.80 * LatDx) + s2;
 }
 
-// K3a: every touching near pair evaluated once (src/dem_force_clump.cu:54-161).
-// Waves walk the near front list flat, CH chunks of 64 entries each: the FP32
-// pre-filter and the FP64 touching test per entry, touching pairs queued in
-// LDS and evaluated one per lane. The owner's contribution goes to the pair's
-// front entry, the partner's (the reaction and its own torque term) to the
-// pair's back entry with the step number as stamp; k_sums adds them per
-// sphere in slot order. The owner writes the new history into the other
-// ping-pong buffer and sets or clears the pair's history-valid bit.
-template <class R, bool RECORD>
-__global__ __launch_bounds__(256) void k_pairs(
-    DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p, int total, unsigned long long* counters,
-    RecordBuf rec, const int* halt, int n) {
-    typedef typename Vec4T<R>::type R4;
-    if (*halt <= n) return;
-    constexpr int Q = 128;
-    constexpr int CH = 4;
-    __shared__ int q_e[4][Q], q_i[4][Q], q_j[4][Q];
-    __shared__ uint32_t q_w[4][Q];
-    __shared__ unsigned int s_cnt[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int lb = xcd_group_block(blockIdx.x, gridDim.x, 16);
-    const long long e0 = ((long long)lb * (blockDim.x >> 6) + wv) * (64 * CH);
-    R* hcur = L.hist[p];
-    R* hnew = L.hist[p ^ 1];
-    int qn = 0, n_ss = 0;
-
-    auto flush = [&]() {
-        for (int r0 = 0; r0 < qn; r0 += 64) {
-            const int k = r0 + lane;
-            const int e = k < qn ? q_e[wv][k] : 0, I = k < qn ? q_i[wv][k] : 0, J = k < qn ? q_j[wv][k] : 0;
-            const uint32_t w = k < qn ? q_w[wv][k] : 0u;
-            // the FP64 touching test of the queued candidate (the positions are
-            // needed for the force anyway)
-            bool touch = false;
-            D4 pI, pJ;
-            pI.x = pI.y = pI.z = pI.w = 0.0;
-            pJ = pI;
-            double DistS = 0.0;
-            V3<double> Dd = mk<double>(0.0, 0.0, 0.0);
-            if (k < qn) {
-                pI = S.pos[I];
-                pJ = S.pos[J];
-                Dd = xyz(pJ) - xyz(pI);
-                DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
-                const double rs = pI.w + pJ.w;
-                if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
-                if (!touch && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
-            }
-            if (touch) {
-                n_ss++;
-                const R4 kI = S.kin[I], kJ = S.kin[J];
-                const int cI = S.cid[I], cJ = S.cid[J];
-                const R4 svI = C.snap[2 * cI], swI = C.snap[2 * cI + 1];
-                const R4 svJ = C.snap[2 * cJ], swJ = C.snap[2 * cJ + 1];
-                const int s = L.nfs[e];  // front slot: history and record index
-                const int b = L.nfb[e];  // the pair's back entry
-                const double DistL = sqrt(DistS);
-                const double Dn = pI.w + pJ.w - DistL;
-                PairIn<R> in;
-                in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
-                in.rI = (R)pI.w; in.rJ = (R)pJ.w; in.mI = kI.w; in.mJ = kJ.w;
-                in.armI = Ld4<R>::xyz(kI); in.armJ = Ld4<R>::xyz(kJ);
-                in.vI = Ld4<R>::xyz(svI); in.wI = Ld4<R>::xyz(swI); in.vJ = Ld4<R>::xyz(svJ); in.wJ = Ld4<R>::xyz(swJ);
-                if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
-                else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
-                PairOut<R> o;
-                pair_force<R>(P, in, o);
-                store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
-                if (!(w & VALID(p ^ 1))) L.nfw[e] = w | VALID(p ^ 1);
-                L.cF[2 * (size_t)e] = mk4<R>(o.cfc.x, o.cfc.y, o.cfc.z, R(0));
-                L.cF[2 * (size_t)e + 1] = mk4<R>(o.ctc.x, o.ctc.y, o.ctc.z, R(0));
-                if (b >= 0) {  // a pair without a back entry gives the partner no reaction
-                    const V3<R> f = R(-1.0) * o.cfc;  // x - y == x + (-y) exactly: same sums as subtracting
-                    L.cB[2 * (size_t)b] = mk4<R>(f.x, f.y, f.z, R(0));
-                    L.cB[2 * (size_t)b + 1] = mk4<R>(o.ctc2.x, o.ctc2.y, o.ctc2.z, R(0));
-                    L.stampB[b] = n;
-                }
-                if (RECORD) {
-                    D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
-                    const V3<double> cpt = xyz(pI) + (double)o.Ap * mk<double>((double)o.DistU.x, (double)o.DistU.y,
-                                                                              (double)o.DistU.z);
-                    D4 bq; bq.x = cpt.x; bq.y = cpt.y; bq.z = cpt.z; bq.w = 0.0;
-                    rec.pair_fcn[s] = a;
-                    rec.pair_cpt[s] = bq;
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        qn = 0;
-    };
-
-    // the next chunk's list words are loaded one chunk ahead
-    uint32_t wnext = 0u;
-    int inext = 0;
-    if (e0 + lane < total) {
-        wnext = __builtin_nontemporal_load(L.nfw + e0 + lane);
-        inext = __builtin_nontemporal_load(L.nfi + e0 + lane);
-    }
-    for (int c = 0; c < CH; ++c) {
-        const long long cb = e0 + 64 * c;
-        if (cb >= total) break;  // wave-uniform
-        const int e = (int)cb + lane;
-        const uint32_t w = wnext;
-        const int i = inext;
-        if (c + 1 < CH && e + 64 < total) {
-            wnext = __builtin_nontemporal_load(L.nfw + e + 64);
-            inext = __builtin_nontemporal_load(L.nfi + e + 64);
-        }
-        const int j = (int)(w & JMASK);
-        bool cand = false;
-        if (e < total) {
-            cand = maybe_touch(P, S.p32[i], S.p32[j]);
-            if (!cand && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
-        }
-        const uint64_t m = __ballot(cand);
-        if (cand) {
-            const int pos = qn + __popcll(m & lanemask_lt());
-            q_e[wv][pos] = e;
-            q_i[wv][pos] = i;
-            q_j[wv][pos] = j;
-            q_w[wv][pos] = w;
-        }
-        qn += __popcll(m);
-        if (qn > Q - 64) flush();
-    }
-    if (qn > 0) flush();
-    unsigned a0 = (unsigned)n_ss;
-    for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
-    if (lane == 0) s_cnt[wv] = a0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int k = 0; k < ((int)blockDim.x + 63) >> 6; ++k) t += s_cnt[k];
-        if (t) atomicAdd(&counters[0], t);
-    }
-}
-
-// K3b: per sphere, the touching pairs' contributions in the reference's order
-// (front slots ascending, then back entries in back-row order;
-// src/dem_force_clump.cu:163-191), the wall and penetrator terms, totF/totT
+// K3: sphere-sphere contacts of 64 spheres per wave (src/dem_force_clump.cu:3-191),
+// then each sphere's totals with its wall and penetrator terms
 // (src/dem_clump.cu:89-101) and the clump torque term T + (x_s - x_c) x F
-// (src/dem_clumpsphere.cu:86-107). A wave reads the contiguous near segments
-// of its 64 spheres 64 entries at a time (coalesced), queues the touching
-// entries' contributions in LDS with their owner lane, and each lane adds its
-// own in queue order.
+// (src/dem_clumpsphere.cu:86-107). The wave's 64 spheres own the contiguous
+// near-list segments [nfoff[wbase], nfoff[wbase + 64]) and [nboff[wbase],
+// nboff[wbase + 64]); the wave walks each segment 64 entries at a time
+// (coalesced list reads, list words loaded one chunk ahead, equal work per
+// lane), finds an entry's row by binary search over the row offsets in LDS,
+// applies the FP32 pre-filter and queues the candidates in LDS. A flush takes
+// the queue one pair per lane: the FP64 touching test, then the force law.
+// Every pair is evaluated by both spheres' waves from the same inputs, so each
+// wave writes only its own spheres (no float atomics, no second kernel); the
+// owner writes the new history into the other ping-pong buffer and the pair's
+// history-valid bits. Each lane adds its sphere's results in queue order,
+// which is its slot order: front entries ascending, then back entries in
+// back-row order.
 template <class R, bool RECORD>
-__global__ __launch_bounds__(256) void k_sums(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                                              WallBuf<R> WL, int* bad, RecordBuf rec, int remap, const int* halt,
-                                              int n) {
+__global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                                                  WallBuf<R> WL, unsigned long long* counters, int* bad,
+                                                  RecordBuf rec, int remap, const int* halt, int n) {
     typedef typename Vec4T<R>::type R4;
     if (*halt <= n) return;
     const int* status = halt + 3;  // ST_* bits
     constexpr int Q = 128;
-    __shared__ int q_key[4][Q];
-    __shared__ R q_val[4][Q][6];
+    __shared__ int q_e[4][Q];    // front near index of the pair; -1 once found not touching
+    __shared__ int q_oth[4][Q];  // the other sphere
+    __shared__ int q_key[4][Q];  // owner lane, +64 for back entries (non-decreasing)
+    __shared__ R q_res[4][Q][6];
     __shared__ int s_fo[4][65], s_bo[4][65];
+    __shared__ F4 s_p32[4][64];
+    __shared__ unsigned int s_cnt[4];
     const int lb = remap ? xcd_group_block(blockIdx.x, gridDim.x, 16) : (int)blockIdx.x;
     const int i = lb * blockDim.x + threadIdx.x;
     const bool active = i < P.N;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int wbase = i - lane;
+    R* hcur = L.hist[p];
+    R* hnew = L.hist[p ^ 1];
+    int n_ss = 0;
     V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss;
     {
         const int ir = i < P.N ? i : P.N;
@@ -1091,16 +965,81 @@ __global__ __launch_bounds__(256) void k_sums(DevParams P, SphereBuf<R> S, Clump
             s_fo[wv][64] = L.nfoff[ie];
             s_bo[wv][64] = L.nboff[ie];
         }
+        F4 a = S.p32[i < P.N ? i : P.N - 1];  // clamped load, then a value select
+        if (!active) a.w = -1.0f;
+        s_p32[wv][lane] = a;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
-    int qn = 0;
-    // Queue keys (owner lane, +64 for back entries) are non-decreasing, so each
-    // lane's front and back entries are two runs found by binary search.
-    auto drain = [&]() {
+    int qn = 0;  // wave-uniform queue length
+
+    auto flush = [&]() __attribute__((always_inline)) {
+        for (int r0 = 0; r0 < qn; r0 += 64) {
+            const int k = r0 + lane;
+            if (k < qn) {
+                const int key = q_key[wv][k];
+                const int side = key >> 6;
+                const int me = wbase + (key & 63);
+                const int oth = q_oth[wv][k];
+                const int e = q_e[wv][k];
+                const int I = side ? oth : me, J = side ? me : oth;
+                const D4 pI = S.pos[I], pJ = S.pos[J];
+                const uint32_t w = L.nfw[e];
+                const V3<double> Dd = xyz(pJ) - xyz(pI);
+                const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
+                const double rs = pI.w + pJ.w;
+                bool touch = false;
+                if (!(DistS > rs * rs * (1.0 + 1e-9))) touch = rs - sqrt(DistS) > 0.0;
+                if (!touch) {
+                    if (side == 0 && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
+                    q_e[wv][k] = -1;
+                } else {
+                    if (side == 0) n_ss++;
+                    const R4 kI = S.kin[I], kJ = S.kin[J];
+                    const int cI = S.cid[I], cJ = S.cid[J];
+                    const R4 svI = C.snap[2 * cI], swI = C.snap[2 * cI + 1];
+                    const R4 svJ = C.snap[2 * cJ], swJ = C.snap[2 * cJ + 1];
+                    const int s = L.nfs[e];  // front slot: history and record index
+                    const double DistL = sqrt(DistS);
+                    const double Dn = pI.w + pJ.w - DistL;
+                    PairIn<R> in;
+                    in.Dist = cvt3<R>(Dd); in.DistS = (R)DistS; in.DistL = (R)DistL; in.Dn = (R)Dn;
+                    in.rI = (R)pI.w; in.rJ = (R)pJ.w; in.mI = kI.w; in.mJ = kJ.w;
+                    in.armI = Ld4<R>::xyz(kI); in.armJ = Ld4<R>::xyz(kJ);
+                    in.vI = Ld4<R>::xyz(svI); in.wI = Ld4<R>::xyz(swI); in.vJ = Ld4<R>::xyz(svJ);
+                    in.wJ = Ld4<R>::xyz(swJ);
+                    if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
+                    else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
+                    PairOut<R> o;
+                    pair_force<R>(P, in, o);
+                    V3<R> f, t;
+                    if (side == 0) {
+                        store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
+                        if (!(w & VALID(p ^ 1))) L.nfw[e] = w | VALID(p ^ 1);
+                        f = o.cfc;
+                        t = o.ctc;
+                        if (RECORD) {
+                            D4 a; a.x = (double)o.fcn.x; a.y = (double)o.fcn.y; a.z = (double)o.fcn.z; a.w = 0.0;
+                            const V3<double> cpt = xyz(pI) + (double)o.Ap * mk<double>((double)o.DistU.x,
+                                                                                      (double)o.DistU.y,
+                                                                                      (double)o.DistU.z);
+                            D4 bq; bq.x = cpt.x; bq.y = cpt.y; bq.z = cpt.z; bq.w = 0.0;
+                            rec.pair_fcn[s] = a;
+                            rec.pair_cpt[s] = bq;
+                        }
+                    } else {
+                        f = R(-1.0) * o.cfc;  // x - y == x + (-y) exactly: same sums as subtracting
+                        t = o.ctc2;
+                    }
+                    R* rr = q_res[wv][k];
+                    rr[0] = f.x; rr[1] = f.y; rr[2] = f.z; rr[3] = t.x; rr[4] = t.y; rr[5] = t.z;
+                }
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        auto lbound = [&](int key) {
+        // each lane's front and back entries are two runs of the sorted keys
+        auto lbound = [&](int key) __attribute__((always_inline)) {
             int lo = 0;
             for (int st = Q; st > 0; st >>= 1)
                 if (lo + st <= qn && q_key[wv][lo + st - 1] < key) lo += st;
@@ -1108,77 +1047,82 @@ __global__ __launch_bounds__(256) void k_sums(DevParams P, SphereBuf<R> S, Clump
         };
         const int f0 = lbound(lane), f1 = lbound(lane + 1), b0 = lbound(lane + 64), b1 = lbound(lane + 65);
         for (int k = f0; k < f1; ++k) {
-            const R* v = q_val[wv][k];
-            Fss = Fss + mk<R>(v[0], v[1], v[2]);
-            Tss = Tss + mk<R>(v[3], v[4], v[5]);
+            if (q_e[wv][k] < 0) continue;
+            const R* rr = q_res[wv][k];
+            Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
+            Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
         }
         for (int k = b0; k < b1; ++k) {
-            const R* v = q_val[wv][k];
-            Fss = Fss + mk<R>(v[0], v[1], v[2]);
-            Tss = Tss + mk<R>(v[3], v[4], v[5]);
+            if (q_e[wv][k] < 0) continue;
+            const R* rr = q_res[wv][k];
+            Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
+            Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         qn = 0;
     };
-    auto push = [&](bool t, int key, const R4& f, const R4& tq) {
-        const uint64_t m = __ballot(t);
-        if (t) {
+    auto push = [&](bool cand, int key, int e, int oth) __attribute__((always_inline)) {
+        const uint64_t m = __ballot(cand);
+        if (cand) {
             const int pos = qn + __popcll(m & lanemask_lt());
+            q_e[wv][pos] = e;
+            q_oth[wv][pos] = oth;
             q_key[wv][pos] = key;
-            R* v = q_val[wv][pos];
-            v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = tq.x; v[4] = tq.y; v[5] = tq.z;
         }
         qn += __popcll(m);
-        if (qn > Q - 64) drain();
+        if (qn > Q - 64) flush();
     };
-    // front entries touch this step when the history-valid bit of the new
-    // buffer is set, back entries when stamped with the step number; each
-    // chunk's words are loaded one chunk ahead
+
+    // front entries: pairs the row's sphere owns (it writes their history)
     {
-        const int F0 = s_fo[wv][0], F1 = s_fo[wv][64], B0 = s_bo[wv][0], B1 = s_bo[wv][64];
+        const int F0 = s_fo[wv][0], F1 = s_fo[wv][64];
         uint32_t wnext = F0 + lane < F1 ? __builtin_nontemporal_load(L.nfw + F0 + lane) : 0u;
-        int snext = B0 + lane < B1 ? __builtin_nontemporal_load(L.stampB + B0 + lane) : -1;
         for (int c = F0; c < F1; c += 64) {
             const int e = c + lane;
             const uint32_t w = wnext;
             if (e + 64 < F1) wnext = __builtin_nontemporal_load(L.nfw + e + 64);
-            bool t = false;
-            int r = 0;
-            R4 f, tq;
-            f.x = f.y = f.z = f.w = R(0);
-            tq = f;
-            if (e < F1 && (w & VALID(p ^ 1))) {
-                t = true;
+            bool cand = false;
+            int r = 0, j = 0;
+            if (e < F1) {
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_fo[wv][r + st] <= e) r += st;
-                f = L.cF[2 * (size_t)e];
-                tq = L.cF[2 * (size_t)e + 1];
+                j = (int)(w & JMASK);
+                cand = maybe_touch(P, s_p32[wv][r], S.p32[j]);
+                if (!cand && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
             }
-            push(t, r, f, tq);
+            push(cand, r, e, j);
+        }
+    }
+    // back entries: the owner holds the history; the same pair is evaluated
+    // again from the same inputs and the row's sphere takes the reaction
+    {
+        const int B0 = s_bo[wv][0], B1 = s_bo[wv][64];
+        int onext = 0, enext = 0;
+        if (B0 + lane < B1) {
+            onext = __builtin_nontemporal_load(L.nbo + B0 + lane);
+            enext = __builtin_nontemporal_load(L.nbe + B0 + lane);
         }
         for (int c = B0; c < B1; c += 64) {
             const int e = c + lane;
-            const int sb = snext;
-            if (e + 64 < B1) snext = __builtin_nontemporal_load(L.stampB + e + 64);
-            bool t = false;
+            const int o = onext, ne = enext;
+            if (e + 64 < B1) {
+                onext = __builtin_nontemporal_load(L.nbo + e + 64);
+                enext = __builtin_nontemporal_load(L.nbe + e + 64);
+            }
+            bool cand = false;
             int r = 0;
-            R4 f, tq;
-            f.x = f.y = f.z = f.w = R(0);
-            tq = f;
-            if (e < B1 && sb == n) {
-                t = true;
+            if (e < B1) {
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_bo[wv][r + st] <= e) r += st;
-                f = L.cB[2 * (size_t)e];
-                tq = L.cB[2 * (size_t)e + 1];
+                cand = maybe_touch(P, s_p32[wv][r], S.p32[o]);
             }
-            push(t, r + 64, f, tq);
+            push(cand, r + 64, ne, o);
         }
     }
-    if (qn > 0) drain();
+    if (qn > 0) flush();
     if (active) {
         const R4 ki = S.kin[i];
         int wk = -1;
@@ -1186,7 +1130,7 @@ __global__ __launch_bounds__(256) void k_sums(DevParams P, SphereBuf<R> S, Clump
         if (P.nfaces > 0 && wk < 0) {
             // input check of the wall kernel (src/dem_force_cube_wall_clump.cu:196-197)
             // for spheres off the wall list: exact only when a cheap sign is off
-            const F4 q = S.p32[i];
+            const F4 q = s_p32[wv][lane];
             if (!(q.w > 0.0f) || !isfinite((double)ki.x) || !isfinite((double)ki.y) || !isfinite((double)ki.z) ||
                 !isfinite((double)ki.w) || !(ki.w > R(0)) || (*status & ST_CLUMP_BAD)) {
                 const D4 pi4 = S.pos[i];
@@ -1199,42 +1143,52 @@ __global__ __launch_bounds__(256) void k_sums(DevParams P, SphereBuf<R> S, Clump
                 if (!ok) atomicMin(bad, i);
             }
         }
-        // totF/totT (src/dem_clump.cu:97-98) and the clump torque term
-        // T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
-        const V3<R> Z = mk<R>(R(0), R(0), R(0));
-        V3<R> Fw = Z, Tw = Z, Fp = Z, Tp = Z;
         if (wk >= 0) {
-            const R4* w4 = WL.res + 4 * (size_t)wk;
-            Fw = Ld4<R>::xyz(w4[0]);
-            Tw = Ld4<R>::xyz(w4[1]);
-            Fp = Ld4<R>::xyz(w4[2]);
-            Tp = Ld4<R>::xyz(w4[3]);
-        }
-        const V3<R> tF = Fss + Fw + Fp;
-        const V3<R> tT = Tss + Tw + Tp;
-        V3<R> r;
-        if (sizeof(R) == sizeof(double)) {
-            const D4 pi4 = S.pos[i];
-            const D4 pc = C.pq[2 * S.cid[i]];
-            r = cvt3<R>(mk<double>(pi4.x - pc.x, pi4.y - pc.y, pi4.z - pc.z));
+            // wall-list sphere: k_walls runs beside this kernel, so the raw sums
+            // go out (.w = 1 marks them) and k_clump_after adds the wall and
+            // penetrator terms and the clump torque term
+            S.sft[2 * (size_t)i] = mk4<R>(Fss.x, Fss.y, Fss.z, R(0));
+            S.sft[2 * (size_t)i + 1] = mk4<R>(Tss.x, Tss.y, Tss.z, R(1));
         } else {
-            r = Ld4<R>::xyz(ki);  // x_s - x_c = R(q) * local up to FP64 rounding
-        }
-        const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
-                               tT.z + (r.x * tF.y - r.y * tF.x));
-        S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
-        S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
-        if (RECORD) {
-            D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
-            D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
-            rec.tot[2 * (size_t)i] = a;
-            rec.tot[2 * (size_t)i + 1] = b;
-            const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
-            for (int k = 0; k < 6; ++k) {
-                D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
-                rec.parts[6 * (size_t)i + k] = q;
+            // totF/totT with zero wall and penetrator terms (src/dem_clump.cu:97-98)
+            // and the clump torque term T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
+            const V3<R> Z = mk<R>(R(0), R(0), R(0));
+            const V3<R> tF = Fss + Z + Z;
+            const V3<R> tT = Tss + Z + Z;
+            V3<R> r;
+            if (sizeof(R) == sizeof(double)) {
+                const D4 pi4 = S.pos[i];
+                const D4 pc = C.pq[2 * S.cid[i]];
+                r = cvt3<R>(mk<double>(pi4.x - pc.x, pi4.y - pc.y, pi4.z - pc.z));
+            } else {
+                r = Ld4<R>::xyz(ki);  // x_s - x_c = R(q) * local up to FP64 rounding
+            }
+            const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
+                                   tT.z + (r.x * tF.y - r.y * tF.x));
+            S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
+            S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+            if (RECORD) {
+                D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
+                D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
+                rec.tot[2 * (size_t)i] = a;
+                rec.tot[2 * (size_t)i + 1] = b;
+                const V3<R> parts[6] = {Fss, Tss, Z, Z, Z, Z};
+                for (int k = 0; k < 6; ++k) {
+                    D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
+                    rec.parts[6 * (size_t)i + k] = q;
+                }
             }
         }
+    }
+    // touching sphere-sphere count
+    unsigned a0 = (unsigned)n_ss;
+    for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
+    if (lane == 0) s_cnt[wv] = a0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < ((int)blockDim.x + 63) >> 6; ++k) t += s_cnt[k];
+        if (t) atomicAdd(&counters[0], t);
     }
 }
 
@@ -1581,9 +1535,46 @@ __global__ __launch_bounds__(256) void k_pene_finalize(DevParams P, PeneState* p
 // ---------------------------------------------------------------------------
 // K4: clump force/torque as the sum of its spheres in ascending sorted order
 // (FP64 accumulation), then the second half-kick.
+// A wall-list sphere's record holds its raw sphere-sphere sums (.w = 1 on the
+// torque): add the wall and penetrator terms of k_walls and the clump torque
+// term, in the order of the other spheres' totals.
 template <class R>
-__global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg, const int* mend,
-                              const int* mem, const int* halt, int n) {
+__device__ __forceinline__ void finish_listed(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
+                                              const WallBuf<R>& WL, const RecordBuf& rec, int record, int m, int c,
+                                              typename Vec4T<R>::type& f, typename Vec4T<R>::type& t) {
+    const typename Vec4T<R>::type* w4 = WL.res + 4 * (size_t)WL.pos_of[m];
+    const V3<R> Fss = Ld4<R>::xyz(f), Tss = Ld4<R>::xyz(t);
+    const V3<R> Fw = Ld4<R>::xyz(w4[0]), Tw = Ld4<R>::xyz(w4[1]), Fp = Ld4<R>::xyz(w4[2]), Tp = Ld4<R>::xyz(w4[3]);
+    const V3<R> tF = Fss + Fw + Fp;
+    const V3<R> tT = Tss + Tw + Tp;
+    V3<R> r;
+    if (sizeof(R) == sizeof(double)) {
+        const D4 pi4 = S.pos[m];
+        const D4 pc = C.pq[2 * c];
+        r = cvt3<R>(mk<double>(pi4.x - pc.x, pi4.y - pc.y, pi4.z - pc.z));
+    } else {
+        r = Ld4<R>::xyz(S.kin[m]);  // x_s - x_c = R(q) * local up to FP64 rounding
+    }
+    const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
+                           tT.z + (r.x * tF.y - r.y * tF.x));
+    f = mk4<R>(tF.x, tF.y, tF.z, R(0));
+    t = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+    if (record) {
+        D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
+        D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
+        rec.tot[2 * (size_t)m] = a;
+        rec.tot[2 * (size_t)m + 1] = b;
+        const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
+        for (int k = 0; k < 6; ++k) {
+            D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
+            rec.parts[6 * (size_t)m + k] = q;
+        }
+    }
+}
+
+template <class R>
+__global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, RecordBuf rec, int record,
+                              const int* mbeg, const int* mend, const int* mem, const int* halt, int n) {
     if (*halt <= n) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.Nc) return;
@@ -1612,6 +1603,7 @@ __global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, const 
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (m[u] < 0) continue;
+                if (t[u].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, m[u], c, f[u], t[u]);
                 const V3<double> fu = mk<double>(f[u].x, f[u].y, f[u].z), tu = mk<double>(t[u].x, t[u].y, t[u].z);
                 if (k0 + u == b) { F = fu; T = tu; }
                 else { F = F + fu; T = T + tu; }
@@ -1775,8 +1767,8 @@ int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* po
 }
 
 int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
-                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs, int* nfi,
-                      int* nfb, int* slot2near, int* nbcnt, int* nboff, int* stampB, int* stmp, int* stotal, int block) {
+                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs,
+                      int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal, int block) {
     const int N = P.N;
     const unsigned g = nblk(N, block);
     int e;
@@ -1785,12 +1777,12 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const i
     k_near_count<<<g, block, 0, st>>>(P, pos, foff, fword, ncnt);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, ncnt, nfoff, N + 1, stmp, stotal))) return e;
-    k_near_fill<<<g, block, 0, st>>>(P, pos, foff, fword, nfoff, nfw, nfs, nfi, nfb, slot2near);
+    k_near_fill<<<g, block, 0, st>>>(P, pos, foff, fword, nfoff, nfw, nfs, slot2near);
     LAUNCH_CHECK();
     k_near_back_count<<<g, block, 0, st>>>(N, boff, bent, slot2near, nbcnt);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, nbcnt, nboff, N + 1, stmp, stotal + 1))) return e;
-    k_near_back_fill<<<g, block, 0, st>>>(N, boff, bent, slot2near, nboff, nfb, stampB);
+    k_near_back_fill<<<g, block, 0, st>>>(N, boff, bent, slot2near, nboff, nbo, nbe);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1850,26 +1842,14 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
 }
 int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
-int launch_pairs(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p, int total,
-                 unsigned long long* ctr, RecordBuf rec, bool record, const int* halt, int step) {
-    if (total <= 0) return 0;
-    const long long waves = ((long long)total + 64 * 4 - 1) / (64 * 4);  // k_pairs: 4 chunks of 64 per wave
-    const unsigned nb = (unsigned)((waves + 3) / 4);
-    if (record)
-        k_pairs<R, true><<<nb, 256, 0, st>>>(P, S, C, L, p, total, ctr, rec, halt, step);
-    else
-        k_pairs<R, false><<<nb, 256, 0, st>>>(P, S, C, L, p, total, ctr, rec, halt, step);
-    LAUNCH_CHECK();
-    return 0;
-}
-template <class R>
-int launch_sums(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                WallBuf<R> WL, int* bad, RecordBuf rec, bool record, int block, int remap, const int* halt, int step) {
+int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
+                    WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block, int remap,
+                    const int* halt, int step) {
     const unsigned nb = nblk(P.N, block);
     if (record)
-        k_sums<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, WL, bad, rec, remap, halt, step);
+        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, WL, ctr, bad, rec, remap, halt, step);
     else
-        k_sums<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, WL, bad, rec, remap, halt, step);
+        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, WL, ctr, bad, rec, remap, halt, step);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1899,9 +1879,11 @@ int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const 
     return 0;
 }
 template <class R>
-int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, const int* mbeg,
-                       const int* mend, const int* mem, int block, const int* halt, int step) {
-    k_clump_after<R><<<nblk(P.Nc, block), block, 0, st>>>(P, S, C, mbeg, mend, mem, halt, step);
+int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL,
+                       RecordBuf rec, bool record, const int* mbeg, const int* mend, const int* mem, int block,
+                       const int* halt, int step) {
+    k_clump_after<R><<<nblk(P.Nc, block), block, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt,
+                                                          step);
     LAUNCH_CHECK();
     return 0;
 }
@@ -1913,14 +1895,13 @@ int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
     template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
-    template int launch_pairs<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int, int,     \
-                                 unsigned long long*, RecordBuf, bool, const int*, int);                             \
-    template int launch_sums<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int, WallBuf<R>, \
-                                int*, RecordBuf, bool, int, int, const int*, int);                                   \
+    template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
+                                    WallBuf<R>, unsigned long long*, int*, RecordBuf, bool, int, int, const int*,    \
+                                    int);                                                                            \
     template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>, int,          \
                                  PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, const int*, int);  \
-    template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, const int*,         \
-                                       const int*, const int*, int, const int*, int);
+    template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>,         \
+                                       RecordBuf, bool, const int*, const int*, const int*, int, const int*, int);
 INST(double)
 INST(float)
 
