@@ -74,6 +74,8 @@ enum { NBC_PENE_CONSTANT_VELOCITY = 0, NBC_PENE_GRAVITY = 1 };
  * src/dem_subsphere_io.cu:83-128 into Parameters/GlobalFlg,
  * src/dem_parameters.h:7-78) followed by the derived fields that
  * LoadParameters/SetMeshSize fill in. */
+#define NBC_MAX_LEVELS 8
+
 typedef struct nbc_params {
     int32_t ParticleNum;     /* NumberBody   */
     int32_t ClumpNum;
@@ -96,6 +98,17 @@ typedef struct nbc_params {
     uint32_t GridDim[3];
     int32_t CellNum;
     int32_t finalized;
+    /* hierarchical grid (nbc_options.hgrid_levels > 1; src/dem_hgrid.cu:205-242
+       with the level table generated, not read from grid.bt): level t holds
+       the spheres with LevelRmin[t] < r <= LevelRmin[t-1] (level 0 the
+       largest), cells of LevelCell[t] = 3 x the level's upper radius, cell
+       index LevelOff[t] + (cz*ny + cy)*nx + cx; CellNum = all level cells + 1,
+       the last cell holding out-of-box spheres. Levels = 1: uniform grid. */
+    int32_t Levels;
+    int32_t LevelOff[NBC_MAX_LEVELS];
+    int32_t LevelDim[NBC_MAX_LEVELS][3];
+    double LevelCell[NBC_MAX_LEVELS];
+    double LevelRmin[NBC_MAX_LEVELS];
 } nbc_params;
 
 /* Options that are not world.par keys (read from the optional nbc.par file by
@@ -107,6 +120,8 @@ typedef struct nbc_options {
     int32_t radix_bits;      /* LSD radix digit width for the cell sort (1..8) */
     int32_t pair_capacity;   /* front-pair slots reserved per sphere (average) */
     int32_t batch_steps;     /* steps enqueued per host round trip (default 64) */
+    int32_t hgrid_levels;    /* > 1: hierarchical grid with this many levels   */
+    int32_t reserved_i;
     double pene_frcoe;       /* penetrator friction; <= 0 means Frcoe          */
     double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
     double reserved[4];
@@ -190,7 +205,9 @@ int nbc_device_count(int32_t* out);
  * iniconfile receives the IniConFile value. Damping is derived. */
 int nbc_load_world_par(const char* path, nbc_params* out, char* iniconfile, size_t iniconfile_len);
 /* SetMeshSize: cell = 3*max radius (or opt->cell_size), origin -= cell,
- * size += 2*cell, GridDim = ceil(size/cell). opt may be NULL. */
+ * size += 2*cell, GridDim = ceil(size/cell). opt may be NULL. With
+ * opt->hgrid_levels > 1 it also fills the level table (see nbc_params); the
+ * margin and the uniform fields use the coarsest level's cell. */
 int nbc_finalize_params(nbc_params* p, const double* radius, int32_t n, const nbc_options* opt);
 /* Optional extension file nbc.par ("key = value"); missing file leaves defaults. */
 int nbc_load_options(const char* path, nbc_options* out);
@@ -219,7 +236,12 @@ int nbc_make_cone_mesh(double radius, double height, int32_t nseg, double* tri_o
  * kind 1 = C1: nclumps (default 200 000) clumps from 8 random templates of
  *          2-8 spheres, lognormal radii (median 1 mm, sigma 0.15, clipped to
  *          [0.6, 1.6] mm), sequential deposition into a cylinder of radius
- *          60 mm (container[0] overrides) whose height fits the bed.
+ *          60 mm (container[0] overrides) whose height fits the bed;
+ * kind 2 = C2: nclumps (default 571 429) clumps of 1-6 spheres from 32 unit
+ *          templates, each clump scaled so that radii are log-uniform over
+ *          [0.2, 2.0] mm, sequential deposition into a cube wall of section
+ *          150 x 150 mm (container[0] overrides) whose height fits the bed;
+ *          friction 0.6, meant for the hierarchical grid (3 levels).
  * The generator allocates the output arrays; release them with nbc_free_bed. */
 typedef struct nbc_bed_spec {
     int32_t kind;

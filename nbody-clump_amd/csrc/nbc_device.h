@@ -88,10 +88,19 @@ constexpr int HSTRIDE = 9;   // tangential(3) twisting(3) rolling(3) per history
 constexpr int MAX_FACES = 7; // cube 6 (or cylinder 3) + penetrator 1
 
 // ---- run parameters passed by value ----------------------------------------
+constexpr int MAX_LEVELS = 8;
+
 struct DevParams {
     int N, Nc, B, CellNum;
     uint32_t G[3];
     double origin[3], cell[3], icell[3], boxlo[3], boxup[3];
+    // hierarchical grid (L > 1): level 0 holds the largest spheres; cells are
+    // numbered level-major from loff[t]; the last cell is the out-of-box bucket
+    int L;
+    int loff[MAX_LEVELS];
+    int ldim[MAX_LEVELS][3];
+    double lcell[MAX_LEVELS];
+    double lrmin[MAX_LEVELS];  // a sphere's level is the first t with r > lrmin[t]
     double dt, Ymod, Prat, Frcoe, Beta, Cohesion, Gravity, Damping, Thresh;
     double pene_frcoe;
     double cKn, cKs, idt; // 2E/(3(1-nu^2)), 2E/(1+nu)/(2-nu), 1/dt (FP32 force path)

@@ -89,6 +89,20 @@ class Engine : public EngineBase {
                                         "(call nbc_finalize_params first)");
         if ((long long)N > (long long)JMASK)
             return fail(NBC_E_BADINPUT, "nbc_create: too many spheres for the 29-bit pair index");
+        if (P.Levels > 1) {
+            // the kernels index cells through this table: it must tile [0, CellNum - 1)
+            bool ok = P.Levels <= MAX_LEVELS;
+            long long off = 0;
+            for (int t = 0; ok && t < P.Levels; ++t) {
+                ok = P.LevelOff[t] == off && P.LevelCell[t] > 0.0;
+                long long c = 1;
+                for (int a = 0; a < 3; ++a) { ok = ok && P.LevelDim[t][a] > 0; c *= P.LevelDim[t][a]; }
+                off += c;
+            }
+            if (!ok || off + 1 != P.CellNum)
+                return fail(NBC_E_BADINPUT, "nbc_create: inconsistent hierarchical grid table (call "
+                                            "nbc_finalize_params with hgrid_levels)");
+        }
         block = O.block_size > 0 ? O.block_size : 256;
         if (block < 64 || block > 1024 || (block % 64) != 0)
             return fail(NBC_E_BADINPUT, "nbc_create: block_size must be a multiple of 64 in [64, 1024]");
@@ -109,6 +123,14 @@ class Engine : public EngineBase {
             dp.icell[a] = P.i_CellSize[a];
             dp.boxlo[a] = P.BoxLowLmt[a];
             dp.boxup[a] = P.BoxUpLmt[a];
+        }
+        dp.L = P.Levels > 1 ? P.Levels : 1;
+        for (int t = 0; t < MAX_LEVELS; ++t) {
+            const bool on = t < dp.L && dp.L > 1;
+            dp.loff[t] = on ? P.LevelOff[t] : 0;
+            for (int a = 0; a < 3; ++a) dp.ldim[t][a] = on ? P.LevelDim[t][a] : 0;
+            dp.lcell[t] = on ? P.LevelCell[t] : 0.0;
+            dp.lrmin[t] = on ? P.LevelRmin[t] : 0.0;
         }
         dp.dt = P.StepSize; dp.Ymod = P.Ymod; dp.Prat = P.Prat; dp.Frcoe = P.Frcoe; dp.Beta = P.Beta;
         dp.Cohesion = P.Cohesion; dp.Gravity = P.Gravity; dp.Damping = P.Damping; dp.Thresh = P.Thresh;
@@ -166,6 +188,10 @@ class Engine : public EngineBase {
         HIPC(dalloc(&cellkey, N)); HIPC(dalloc(&perm, N));
         HIPC(dalloc(&cbeg, P.CellNum)); HIPC(dalloc(&cend, P.CellNum));
         HIPC(dalloc(&fcnt, (size_t)N + 1)); HIPC(dalloc(&bcnt, (size_t)N + 1));
+        if (dp.L > 1) {
+            HIPC(dalloc(&xcnt, (size_t)N + 1)); HIPC(dalloc(&xoff, (size_t)N + 1));
+            HIPC(dalloc(&xbeg, (size_t)N)); HIPC(dalloc(&xend, (size_t)N));
+        }
         HIPC(dalloc(&mbeg, Nc)); HIPC(dalloc(&mend, Nc)); HIPC(dalloc(&mem, N));
         HIPC(dalloc(&dstatus, 4));
         HIPC(dalloc(&counters, 2));  // touching pairs, touching wall contacts
@@ -209,6 +235,25 @@ class Engine : public EngineBase {
         return hipSuccess;
     }
 
+    // cross-level entries of the hierarchical grid (key = owner j, value = i)
+    hipError_t grow_cross(long long cap) {
+        if (cap <= xcap) return hipSuccess;
+        dfree(xk0); dfree(xk1); dfree(xv0); dfree(xv1); dfree(xhist); dfree(xhscan); dfree(xstmp);
+        xk0 = xk1 = nullptr; xv0 = xv1 = xhist = xhscan = xstmp = nullptr;
+        xcap = 0;
+        const int n = (int)cap;
+        hipError_t e;
+        if ((e = dalloc(&xk0, (size_t)n)) != hipSuccess) return e;
+        if ((e = dalloc(&xk1, (size_t)n)) != hipSuccess) return e;
+        if ((e = dalloc(&xv0, (size_t)n)) != hipSuccess) return e;
+        if ((e = dalloc(&xv1, (size_t)n)) != hipSuccess) return e;
+        if ((e = dalloc(&xhist, (size_t)radix_tmp_ints(n, 8))) != hipSuccess) return e;
+        if ((e = dalloc(&xhscan, (size_t)radix_tmp_ints(n, 8))) != hipSuccess) return e;
+        if ((e = dalloc(&xstmp, (size_t)scan_tmp_ints(radix_tmp_ints(n, 8)))) != hipSuccess) return e;
+        xcap = cap;
+        return hipSuccess;
+    }
+
     // near pairs are a subset of the front list: every near array fits in the
     // front-list capacity (one entry per near pair on each side)
     hipError_t grow_near(long long fcap) {
@@ -243,6 +288,7 @@ class Engine : public EngineBase {
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
+        F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
         if (hstatus) (void)hipHostFree(hstatus);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
@@ -417,8 +463,35 @@ class Engine : public EngineBase {
         KC(launch_fill_i32(st, fcnt + N, 1, 0));
         KC(launch_fill_i32(st, bcnt + N, 1, 0));
         KC(launch_fill_u32(st, dstatus + 2, 1, 0u));
-        KC(launch_neighbors(st, dp, false, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, nullptr, nullptr, nullptr,
-                            nullptr, dstatus + 2, block));
+        const uint32_t* xkey = nullptr;
+        const int* xsrt = nullptr;
+        if (dp.L > 1) {
+            // hierarchical grid: counts, cross entries (j, i) sorted by j, then
+            // the front count of each owner
+            KC(launch_fill_i32(st, xcnt + N, 1, 0));
+            KC(launch_hneighbors(st, dp, 0, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, nullptr, nullptr,
+                                 nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, dstatus + 2,
+                                 block));
+            KC(launch_exclusive_scan(st, xcnt, xoff, N + 1, stmp, stotal + 3));
+            int X = 0;
+            HIPC(hipMemcpyAsync(&X, stotal + 3, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            if (X < 0) return fail(NBC_E_NEIGHBOR_OVERFLOW, "cross-level pair list exceeds 2^31 entries");
+            HIPC(grow_cross(std::max<long long>(X, 1)));
+            KC(launch_hneighbors(st, dp, 1, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, xoff, xk0, xv0,
+                                 nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, dstatus + 2, block));
+            int sec = 0;
+            if (X > 0)
+                KC(launch_radix_sort(st, xk0, xv0, xk1, xv1, X, bits_for(N - 1), rbits, xhist, xhscan, xstmp,
+                                     stotal + 3, &sec));
+            xkey = sec ? xk1 : xk0;
+            xsrt = sec ? xv1 : xv0;
+            KC(launch_seg_bounds(st, xkey, X, N, xbeg, xend));
+            KC(launch_hfront_total(st, N, P.BranchLmtLen / 2, fcnt, bcnt, xbeg, xend, dstatus + 2, block));
+        } else {
+            KC(launch_neighbors(st, dp, false, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, nullptr, nullptr,
+                                nullptr, nullptr, dstatus + 2, block));
+        }
         KC(launch_exclusive_scan(st, fcnt, Ln.foff, N + 1, stmp, stotal));
         KC(launch_exclusive_scan(st, bcnt, boff, N + 1, stmp, stotal + 1));
         int tot[2];
@@ -442,8 +515,12 @@ class Engine : public EngineBase {
         }
         if ((long long)tot[1] > bcap) HIPC(grow_back((long long)(tot[1] * 1.25) + 1024));
         // 5. fill rows, resolve back slots, remap histories (src/dem_grid.cu:561-647)
-        KC(launch_neighbors(st, dp, true, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, Ln.foff, boff, Ln.fword, bent,
-                            dstatus + 2, block));
+        if (dp.L > 1)
+            KC(launch_hneighbors(st, dp, 2, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, xoff, nullptr,
+                                 nullptr, Ln.foff, boff, xbeg, xend, xsrt, Ln.fword, bent, dstatus + 2, block));
+        else
+            KC(launch_neighbors(st, dp, true, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, Ln.foff, boff, Ln.fword,
+                                bent, dstatus + 2, block));
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
         // history-valid bits of the old near list back into the old pair words
         KC(launch_near_sync(st, N, nfoff, nfw, nfs, Lo.fword, block));
@@ -967,6 +1044,12 @@ class Engine : public EngineBase {
     int2* bent = nullptr;
     long long bcap = 0, nfront = 0, nback = 0, rec_cap = 0;
     uint32_t *key0 = nullptr, *key1 = nullptr, *cellkey = nullptr;
+    // hierarchical grid: per-sphere cross counts/offsets, per-owner segment
+    // bounds, cross entries and their sort scratch
+    int *xcnt = nullptr, *xoff = nullptr, *xbeg = nullptr, *xend = nullptr;
+    uint32_t *xk0 = nullptr, *xk1 = nullptr;
+    int *xv0 = nullptr, *xv1 = nullptr, *xhist = nullptr, *xhscan = nullptr, *xstmp = nullptr;
+    long long xcap = 0;
     int *val0 = nullptr, *val1 = nullptr, *hist = nullptr, *hscan = nullptr, *stmp = nullptr, *stotal = nullptr;
     int *perm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
     int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;

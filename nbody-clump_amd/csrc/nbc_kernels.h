@@ -101,6 +101,13 @@ int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm
 template <class R>
 int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
                       const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
+// Hierarchical grid search, phase 0 = counts, 1 = cross entries, 2 = rows.
+int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* pos, const uint32_t* cell,
+                      const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, int* xcnt,
+                      const int* xoff, uint32_t* xkey, int* xval, const int* foff, const int* boff, const int* xbeg,
+                      const int* xend, const int* xsrt, uint32_t* fword, int2* bent, unsigned* overflow, int block);
+int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* bcnt, const int* xbeg,
+                        const int* xend, unsigned* overflow, int block);
 int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
                       const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs,
                       int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal, int block);

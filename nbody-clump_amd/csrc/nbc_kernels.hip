@@ -235,12 +235,43 @@ __device__ __forceinline__ int clamp_u(uint32_t G, int v) {
     return (int)(u < hi ? u : hi);
 }
 
+// Level of a sphere in the hierarchical grid: the first t with r > lrmin[t]
+// (the last level takes the rest).
+__device__ __forceinline__ int level_of(const DevParams& P, double r) {
+    int t = 0;
+    while (t < P.L - 1 && !(r > P.lrmin[t])) ++t;
+    return t;
+}
+
+// Cell coordinate of x on a level of cell size s (src/dem_hgrid.cu:225-227).
+__device__ __forceinline__ int hcell(double x, double o, double s) { return (int)floor((x - o) / s); }
+
 __global__ void k_hash(DevParams P, const D4* pos, uint32_t* key, int* val) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const D4 p = pos[i];
     val[i] = i;
     int idx;
+    if (P.L > 1) {
+        // hierarchical grid, src/dem_hgrid.cu:205-242; the out-of-box bucket
+        // is the last cell (CellNum - 1) and each coordinate is clamped into
+        // its level's box
+        if (p.z > P.boxup[2] || p.y > P.boxup[1] || p.x > P.boxup[0] || p.z < P.boxlo[2] || p.y < P.boxlo[1] ||
+            p.x < P.boxlo[0]) {
+            idx = P.CellNum - 1;
+        } else {
+            const int t = level_of(P, p.w);
+            const double s = P.lcell[t];
+            const int* nd = P.ldim[t];
+            int cx = hcell(p.x, P.origin[0], s), cy = hcell(p.y, P.origin[1], s), cz = hcell(p.z, P.origin[2], s);
+            cx = cx < 0 ? 0 : (cx > nd[0] - 1 ? nd[0] - 1 : cx);
+            cy = cy < 0 ? 0 : (cy > nd[1] - 1 ? nd[1] - 1 : cy);
+            cz = cz < 0 ? 0 : (cz > nd[2] - 1 ? nd[2] - 1 : cz);
+            idx = P.loff[t] + (cz * nd[1] + cy) * nd[0] + cx;
+        }
+        key[i] = (uint32_t)idx;
+        return;
+    }
     if (p.z >= P.boxup[2] || p.y >= P.boxup[1] || p.x >= P.boxup[0] || p.z < P.boxlo[2] || p.y < P.boxlo[1] ||
         p.x < P.boxlo[0]) {
         idx = P.CellNum - 1;
@@ -503,6 +534,98 @@ __global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, co
         bcnt[i] = ni;
         if (ng > half || ni > half) atomicOr(overflow, 1u);
     }
+}
+
+// Hierarchical search, src/dem_hgrid_contlist.cu:57-161 with deterministic
+// rows. A sphere of level t scans levels t..0 (27 cells each, z outer / x
+// inner, ascending j, cut-off 1.5 (ri + rj)). Same level: j > i front, j < i
+// back. Coarser level: j < i (cells are numbered level-major, so coarser
+// spheres sort first); i lists j in its back row and emits the cross entry
+// (j, i), which becomes a front entry of j. PH 0 counts (same-level fronts,
+// backs, cross entries), PH 1 writes the cross entries at xoff[i], PH 2 writes
+// the rows: same-level fronts in discovery order, then the cross partners
+// xval[xbeg[i] .. xend[i]) (ascending i after the stable sort by j); backs in
+// reverse discovery order.
+template <int PH>
+__global__ void k_hneighbors(DevParams P, const D4* pos, const uint32_t* cell, const int* cbeg, const int* cend,
+                             const int* cid, int* fcnt, int* bcnt, int* xcnt, const int* xoff, uint32_t* xkey,
+                             int* xval, const int* foff, const int* boff, const int* xbeg, const int* xend,
+                             const int* xsrt, uint32_t* fword, int2* bent, unsigned* overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    if ((int)cell[i] == P.CellNum - 1) {
+        if (PH == 0) { fcnt[i] = 0; bcnt[i] = 0; xcnt[i] = 0; }
+        return;
+    }
+    const D4 pi = pos[i];
+    const double ri = pi.w;
+    const int my = level_of(P, ri);
+    int ng = 0, ni = 0, nx = 0;
+    int fb = 0, nf = 0, bb = 0, nbi = 0, xb = 0, xn = 0, ci = 0;
+    if (PH == 1) { xb = xoff[i]; xn = xcnt[i]; }
+    if (PH == 2) { fb = foff[i]; nf = foff[i + 1] - fb; bb = boff[i]; nbi = bcnt[i]; ci = cid[i]; }
+    for (int T = my; T >= 0; --T) {
+        const double s = P.lcell[T];
+        const int nxd = P.ldim[T][0], nyd = P.ldim[T][1], nzd = P.ldim[T][2];
+        const int cix = hcell(pi.x, P.origin[0], s), ciy = hcell(pi.y, P.origin[1], s);
+        const int ciz = hcell(pi.z, P.origin[2], s);
+        for (int cz = -1; cz <= 1; cz++)
+            for (int cy = -1; cy <= 1; cy++)
+                for (int cx = -1; cx <= 1; cx++) {
+                    const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
+                    if (jx < 0 || jx >= nxd || jy < 0 || jy >= nyd || jz < 0 || jz >= nzd) continue;
+                    const int cidx = P.loff[T] + (jz * nyd + jy) * nxd + jx;
+                    const int start = cbeg[cidx];
+                    if (start == -1) continue;
+                    const int end = cend[cidx];
+                    for (int j = start; j < end; j++) {
+                        if (j == i) continue;
+                        const D4 pj = pos[j];
+                        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                        const double ds = dx * dx + dy * dy + dz * dz;
+                        const double lim = 1.5 * (ri + pj.w);
+                        if (!(ds < lim * lim)) continue;
+                        if (T == my && i < j) {
+                            if (PH == 2 && ng < nf) fword[fb + ng] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+                            ng++;
+                        } else {
+                            if (PH == 2 && ni < nbi) bent[bb + (nbi - 1 - ni)] = make_int2(j, -1);
+                            ni++;
+                            if (T != my) {
+                                if (PH == 1 && nx < xn) { xkey[xb + nx] = (uint32_t)j; xval[xb + nx] = i; }
+                                nx++;
+                            }
+                        }
+                    }
+                }
+    }
+    if (PH == 0) {
+        fcnt[i] = ng;
+        bcnt[i] = ni;
+        xcnt[i] = nx;
+    }
+    if (PH == 2) {
+        const int b = xbeg[i];
+        if (b >= 0) {
+            const int e = xend[i];
+            for (int k = b; k < e && ng < nf; ++k, ++ng) {
+                const int j = xsrt[k];
+                fword[fb + ng] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+            }
+        }
+    }
+}
+
+// Front count = same-level fronts + cross entries owned by i; both halves of
+// a row hold at most BranchLmtLen / 2 entries.
+__global__ void k_hfront_total(int N, int half, int* fcnt, const int* bcnt, const int* xbeg, const int* xend,
+                               unsigned* overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int b = xbeg[i];
+    const int f = fcnt[i] + (b >= 0 ? xend[i] - b : 0);
+    fcnt[i] = f;
+    if (f > half || bcnt[i] > half) atomicOr(overflow, 1u);
 }
 
 // Owner slot of every back entry: src/dem_grid.cu:591-599 (Aref of the back
@@ -1747,7 +1870,7 @@ int launch_seg_bounds(hipStream_t st, const uint32_t* key, int n, int nkeys, int
     int e = launch_fill_i32(st, beg, nkeys, -1);
     if (e) return e;
     e = launch_fill_i32(st, end, nkeys, -1);
-    if (e) return e;
+    if (e || n <= 0) return e;
     k_seg_bounds<<<nblk(n, 256), 256, 0, st>>>(key, n, nkeys, beg, end);
     LAUNCH_CHECK();
     return 0;
@@ -1762,6 +1885,31 @@ int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* po
     else
         k_neighbors<false><<<nblk(P.N, block), block, 0, st>>>(P, pos, cell, cbeg, cend, cid, fcnt, bcnt, foff, boff,
                                                                fword, bent, overflow);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* pos, const uint32_t* cell,
+                      const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, int* xcnt,
+                      const int* xoff, uint32_t* xkey, int* xval, const int* foff, const int* boff, const int* xbeg,
+                      const int* xend, const int* xsrt, uint32_t* fword, int2* bent, unsigned* overflow, int block) {
+    const unsigned g = nblk(P.N, block);
+    if (phase == 0)
+        k_hneighbors<0><<<g, block, 0, st>>>(P, pos, cell, cbeg, cend, cid, fcnt, bcnt, xcnt, xoff, xkey, xval, foff,
+                                             boff, xbeg, xend, xsrt, fword, bent, overflow);
+    else if (phase == 1)
+        k_hneighbors<1><<<g, block, 0, st>>>(P, pos, cell, cbeg, cend, cid, fcnt, bcnt, xcnt, xoff, xkey, xval, foff,
+                                             boff, xbeg, xend, xsrt, fword, bent, overflow);
+    else
+        k_hneighbors<2><<<g, block, 0, st>>>(P, pos, cell, cbeg, cend, cid, fcnt, bcnt, xcnt, xoff, xkey, xval, foff,
+                                             boff, xbeg, xend, xsrt, fword, bent, overflow);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* bcnt, const int* xbeg,
+                        const int* xend, unsigned* overflow, int block) {
+    k_hfront_total<<<nblk(N, block), block, 0, st>>>(N, half, fcnt, bcnt, xbeg, xend, overflow);
     LAUNCH_CHECK();
     return 0;
 }

@@ -241,6 +241,48 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
         ctpl.resize(nc);
         for (int k = 0; k < nc; ++k) ctpl[k] = (int)(g() % 8);
         P.StepSize = 5e-7; P.EndStep = 10000; P.OutputInterval = 1000; P.BranchLmtLen = 128; P.Thresh = 0.25e-3;
+    } else if (spec->kind == 2) {
+        // C2: 32 unit templates of 1..6 spheres (relative radii U[0.8, 1]),
+        // each clump scaled by s log-uniform over [0.25, 2] mm, so radii fall
+        // in [0.2, 2.0] mm with a log-uniform spread
+        if (nc <= 0) nc = 571429;
+        std::vector<Template> unit;
+        for (int t = 0; t < 32; ++t) {
+            const int ns = 1 + t % 6;
+            std::vector<V> ps;
+            std::vector<double> rs;
+            for (int q = 0; q < ns; ++q) {
+                const double r = 0.8 + 0.2 * U(g);
+                if (q == 0) { ps.push_back({0, 0, 0}); rs.push_back(r); continue; }
+                const int par = (int)(g() % q);
+                double qq[4];
+                rand_quat(g, qq);
+                double m[3][3];
+                ivdcm(qq, m);
+                ps.push_back(ps[par] + (0.8 * 0.5 * (r + rs[par])) * mapv(m, V{1, 0, 0}));
+                rs.push_back(r);
+            }
+            unit.push_back(make_template(ps, rs, 1.0));
+        }
+        const double lo = std::log(0.25e-3), hi = std::log(2.0e-3);
+        ctpl.resize(nc);
+        for (int k = 0; k < nc; ++k) {
+            const Template& u = unit[g() % 32];
+            const double sc = std::exp(lo + (hi - lo) * U(g));
+            Template t;
+            t.mass = f32(u.mass * rho * sc * sc * sc);
+            for (int a = 0; a < 3; ++a) t.I[a] = u.I[a] * rho * sc * sc * sc * sc * sc;
+            for (size_t q = 0; q < u.off.size(); ++q) {
+                t.off.push_back({f32(u.off[q].x * sc), f32(u.off[q].y * sc), f32(u.off[q].z * sc)});
+                t.r.push_back(f32(u.r[q] * sc));
+                t.rbound = std::max(t.rbound, std::sqrt(dot(t.off.back(), t.off.back())) + t.r.back());
+            }
+            tpl.push_back(t);
+            ctpl[k] = k;
+        }
+        P.Frcoe = 0.6;
+        P.GridLevels = 3;
+        P.StepSize = 2e-7; P.EndStep = 5000; P.OutputInterval = 1000; P.BranchLmtLen = 256; P.Thresh = 0.08e-3;
     } else {
         nbc::set_global_error("nbc_generate_bed: unknown kind");
         return NBC_E_BADINPUT;
@@ -299,20 +341,24 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
         P.WorldOrigin[0] = 0; P.WorldOrigin[1] = 0; P.WorldOrigin[2] = 0;
         P.WorldSize[0] = Lx; P.WorldSize[1] = Ly; P.WorldSize[2] = Lz;
     } else {
-        // C1: sequential deposition into a cylinder (axis z, bottom cap at z = 0).
+        // C1 / C2: sequential deposition into a cylinder (C1, axis z, centred
+        // on x = y = 0) or a square section [0, L]^2 (C2), floor at z = 0.
         // Every clump drops at a random (x, y) and stops at its first contact
         // with the floor or an earlier sphere, so the bed is contact-connected
         // without initial overlaps.
-        const double Rc = spec->container[0] > 0 ? spec->container[0] : 0.06;
+        const bool cyl = spec->kind == 1;
+        const double Rc = spec->container[0] > 0 ? spec->container[0] : (cyl ? 0.06 : 0.15);
+        const double ext = cyl ? 2.0 * Rc : Rc;  // extent of the section along x and y
+        const double org = cyl ? -Rc : 0.0;      // its lower x / y
         double rmax = 0;
         for (const Template& t : tpl)
             for (double r : t.r) rmax = std::max(rmax, r);
         const double cs = 2.0 * rmax;
-        const int nco = (int)std::ceil(2.0 * Rc / cs) + 1;
+        const int nco = (int)std::ceil(ext / cs) + 1;
         std::vector<std::vector<Placed>> col((size_t)nco * nco);
         auto colidx = [&](double x, double y, int& cx, int& cy) {
-            cx = std::min(nco - 1, std::max(0, (int)std::floor((x + Rc) / cs)));
-            cy = std::min(nco - 1, std::max(0, (int)std::floor((y + Rc) / cs)));
+            cx = std::min(nco - 1, std::max(0, (int)std::floor((x - org) / cs)));
+            cy = std::min(nco - 1, std::max(0, (int)std::floor((y - org) / cs)));
         };
         std::vector<V> off;
         double zmax = 0;
@@ -326,7 +372,7 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
                 off[s] = mapv(m, t.off[s]);
                 rxy = std::max(rxy, std::sqrt(off[s].x * off[s].x + off[s].y * off[s].y) + t.r[s]);
             }
-            const double rr = Rc - rxy - 1e-6;
+            const double rr = (cyl ? Rc : 0.5 * Rc) - rxy - 1e-6;
             auto landing = [&](double x, double y) {
                 double best = -1e300;
                 for (size_t s = 0; s < t.off.size(); ++s) {
@@ -351,7 +397,12 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
             double x = 0, y = 0, best = 1e300;
             for (int tries = 0; tries < kDropTries; ++tries) {
                 double cx_, cy_;
-                do { cx_ = (2 * U(g) - 1) * rr; cy_ = (2 * U(g) - 1) * rr; } while (cx_ * cx_ + cy_ * cy_ > rr * rr);
+                if (cyl) {
+                    do { cx_ = (2 * U(g) - 1) * rr; cy_ = (2 * U(g) - 1) * rr; } while (cx_ * cx_ + cy_ * cy_ > rr * rr);
+                } else {
+                    cx_ = 0.5 * Rc + (2 * U(g) - 1) * rr;
+                    cy_ = 0.5 * Rc + (2 * U(g) - 1) * rr;
+                }
                 const double h = landing(cx_, cy_);
                 if (h < best) { best = h; x = cx_; y = cy_; }
             }
@@ -367,14 +418,22 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
             }
         }
         const double H = 0.5 * (zmax + 4.0 * rmax);  // half height: top cap above the bed
-        W.type = NBC_WALL_CYLINDER;
-        W.radius = Rc;
-        W.height = H;
-        W.dir[2] = 1.0;
-        W.prop[0] = W.prop[1] = W.prop[2] = W.prop[3] = 1.0;
-        W.pos[0] = 0; W.pos[1] = 0; W.pos[2] = H;
-        P.WorldOrigin[0] = -Rc; P.WorldOrigin[1] = -Rc; P.WorldOrigin[2] = 0;
-        P.WorldSize[0] = 2 * Rc; P.WorldSize[1] = 2 * Rc; P.WorldSize[2] = 2 * H;
+        if (!cyl) {
+            W.type = NBC_WALL_CUBE;
+            W.side[0] = 0.5 * Rc; W.side[1] = 0.5 * Rc; W.side[2] = H;
+            W.pos[0] = 0.5 * Rc; W.pos[1] = 0.5 * Rc; W.pos[2] = H;
+            P.WorldOrigin[0] = 0; P.WorldOrigin[1] = 0; P.WorldOrigin[2] = 0;
+            P.WorldSize[0] = Rc; P.WorldSize[1] = Rc; P.WorldSize[2] = 2 * H;
+        } else {
+            W.type = NBC_WALL_CYLINDER;
+            W.radius = Rc;
+            W.height = H;
+            W.dir[2] = 1.0;
+            W.prop[0] = W.prop[1] = W.prop[2] = W.prop[3] = 1.0;
+            W.pos[0] = 0; W.pos[1] = 0; W.pos[2] = H;
+            P.WorldOrigin[0] = -Rc; P.WorldOrigin[1] = -Rc; P.WorldOrigin[2] = 0;
+            P.WorldSize[0] = 2 * Rc; P.WorldSize[1] = 2 * Rc; P.WorldSize[2] = 2 * H;
+        }
     }
     for (int a = 0; a < 3; ++a) {
         P.BoxLowLmt[a] = P.WorldOrigin[a];
@@ -514,6 +573,7 @@ extern "C" int nbc_save_bed_text(const char* dir, const nbc_bed_out* b, const nb
     {
         std::ofstream f(d + "/nbc.par");
         f << "WallType = " << (W.type == NBC_WALL_CYLINDER ? "cylinder" : "cube") << "\n";
+        if (b->params.GridLevels > 1) f << "HGridLevels = " << (int)b->params.GridLevels << "\n";
     }
     (void)cmd;
     return 0;

@@ -140,6 +140,38 @@ extern "C" int nbc_finalize_params(nbc_params* p, const double* radius, int32_t 
     const double cells = (double)p->GridDim[0] * p->GridDim[1] * p->GridDim[2];
     if (cells > 2.0e9) return err(NBC_E_BADINPUT, "nbc_finalize_params: grid too large");
     p->CellNum = (int32_t)(p->GridDim[0] * p->GridDim[1] * p->GridDim[2]);
+    p->Levels = 1;
+    const int L = opt ? opt->hgrid_levels : 0;
+    if (L > 1) {
+        // hierarchical grid: geometric radius boundaries between the smallest
+        // and the largest radius, level 0 the largest spheres; every level
+        // shares the origin set above (margin = the coarsest cell)
+        if (L > NBC_MAX_LEVELS) return err(NBC_E_BADINPUT, "nbc_finalize_params: at most 8 grid levels");
+        if (opt->cell_size > 0.0) return err(NBC_E_BADINPUT, "nbc_finalize_params: CellSize and HGridLevels exclude each other");
+        double rmax = 0.0, rmin = 1e300;
+        for (int i = 0; i < n; ++i) {
+            rmax = std::max(rmax, radius[i]);
+            rmin = std::min(rmin, radius[i]);
+        }
+        const double q = std::pow(rmin / rmax, 1.0 / L);
+        double total = 0.0;
+        double upper = rmax;
+        for (int t = 0; t < L; ++t) {
+            p->LevelRmin[t] = t + 1 < L ? rmax * std::pow(q, t + 1) : -1.0;
+            p->LevelCell[t] = 3.0 * upper;
+            p->LevelOff[t] = (int32_t)total;
+            double c = 1.0;
+            for (int a = 0; a < 3; ++a) {
+                p->LevelDim[t][a] = (int32_t)std::ceil(p->WorldSize[a] / p->LevelCell[t]);
+                c *= p->LevelDim[t][a];
+            }
+            total += c;
+            upper = p->LevelRmin[t];
+        }
+        if (total + 1.0 > 2.0e9) return err(NBC_E_BADINPUT, "nbc_finalize_params: hierarchical grid too large");
+        p->Levels = L;
+        p->CellNum = (int32_t)total + 1;  // + the out-of-box bucket
+    }
     p->finalized = 1;
     return 0;
 }
@@ -175,6 +207,7 @@ extern "C" int nbc_load_options(const char* path, nbc_options* out) {
         else if (k == "PairCapacity") out->pair_capacity = std::atoi(v.c_str());
         else if (k == "PeneFrcoe") out->pene_frcoe = std::atof(v.c_str());
         else if (k == "CellSize") out->cell_size = std::atof(v.c_str());
+        else if (k == "HGridLevels") out->hgrid_levels = std::atoi(v.c_str());
         else return err(NBC_E_PARAM, "nbc.par: unknown key " + k);
     }
     return 0;

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libnbc.so")
 
 NBC_OK, NBC_E_BADINPUT, NBC_E_NEIGHBOR_OVERFLOW, NBC_E_NONFINITE, NBC_E_HIP, NBC_E_NODEVICE, NBC_E_IO, NBC_E_PARAM = range(8)
 NBC_WALL_NONE, NBC_WALL_CUBE, NBC_WALL_CYLINDER = 0, 1, 2
+NBC_MAX_LEVELS = 8
 NBC_FP64, NBC_MIXED = 0, 1
 NBC_PENE_CYLINDER, NBC_PENE_SPHERE, NBC_PENE_CONE = 0, 1, 2
 NBC_PENE_CONSTANT_VELOCITY, NBC_PENE_GRAVITY = 0, 1
@@ -36,6 +37,9 @@ class Params(C.Structure):
         ("isQuaternion", C.c_int32),
         ("Damping", C.c_double), ("CellSize", D3), ("i_CellSize", D3), ("GridDim", C.c_uint32 * 3),
         ("CellNum", C.c_int32), ("finalized", C.c_int32),
+        ("Levels", C.c_int32), ("LevelOff", C.c_int32 * NBC_MAX_LEVELS),
+        ("LevelDim", (C.c_int32 * 3) * NBC_MAX_LEVELS), ("LevelCell", C.c_double * NBC_MAX_LEVELS),
+        ("LevelRmin", C.c_double * NBC_MAX_LEVELS),
     ]
 
 
@@ -43,6 +47,7 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("wall_type", C.c_int32), ("block_size", C.c_int32),
         ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
+        ("hgrid_levels", C.c_int32), ("reserved_i", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("reserved", C.c_double * 4),
     ]
 
@@ -279,10 +284,11 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0, batch_steps=0):
+                 cell_size=0.0, batch_steps=0, hgrid_levels=0):
     o = Options()
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
     o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
+    o.hgrid_levels = hgrid_levels
     return o
 
 

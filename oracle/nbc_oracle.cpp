@@ -137,7 +137,11 @@ static int fail(orc_ctx* c, int code, const std::string& m) {
 
 // ---------------------------------------------------------------------------
 // Mesh set-up: src/dem_subsphere_io.cu:143-145 (Damping) and :197-230.
-extern "C" int orc_finalize_params(nbc_params* p, const double* radius, int n, double cell_override) {
+// Hierarchical grid table (no reference counterpart: the reference reads it
+// from grid.bt, src/dem_io.cu:204-248): L levels split [rmin, rmax]
+// geometrically, level 0 the largest spheres, cell = 3 x the level's largest
+// admissible radius, one origin for all levels.
+extern "C" int orc_finalize_params(nbc_params* p, const double* radius, int n, double cell_override, int levels) {
     double lnCOR = std::log(p->NormRes);
     p->Damping = 2.0 * std::sqrt(5.0 / 6.0) * lnCOR / std::sqrt(lnCOR * lnCOR + 3.14159265359 * 3.14159265359);
     double MeshSize = 0.0;
@@ -153,6 +157,30 @@ extern "C" int orc_finalize_params(nbc_params* p, const double* radius, int n, d
         p->GridDim[a] = (uint32_t)std::ceil(p->WorldSize[a] / p->CellSize[a]);
     }
     p->CellNum = (int32_t)(p->GridDim[0] * p->GridDim[1] * p->GridDim[2]);
+    p->Levels = 1;
+    if (levels > 1) {
+        double rmin = radius[0], rmax = radius[0];
+        for (int i = 1; i < n; ++i) {
+            if (radius[i] < rmin) rmin = radius[i];
+            if (radius[i] > rmax) rmax = radius[i];
+        }
+        const double ratio = std::pow(rmin / rmax, 1.0 / levels);
+        int off = 0;
+        for (int t = 0; t < levels; ++t) {
+            const double top = t == 0 ? rmax : p->LevelRmin[t - 1];
+            p->LevelRmin[t] = (t == levels - 1) ? -1.0 : rmax * std::pow(ratio, t + 1);
+            p->LevelCell[t] = 3.0 * top;
+            p->LevelOff[t] = off;
+            int cells = 1;
+            for (int a = 0; a < 3; ++a) {
+                p->LevelDim[t][a] = (int32_t)std::ceil(p->WorldSize[a] / p->LevelCell[t]);
+                cells *= p->LevelDim[t][a];
+            }
+            off += cells;
+        }
+        p->Levels = levels;
+        p->CellNum = off + 1;
+    }
     p->finalized = 1;
     return 0;
 }
@@ -373,11 +401,38 @@ static int cell_hash(const nbc_params& P, const d3& p) {
     return idx;
 }
 
+// Hierarchical grid (src/dem_hgrid.cu:205-242): level 0 holds the largest
+// spheres; a sphere's level is the first t with r > LevelRmin[t]. Corrected
+// semantics: out-of-box spheres go to the last cell (CellNum - 1, never
+// searched) instead of CellNum, and the per-level index is clamped into the
+// level's box so no index leaves its level.
+static int sphere_level(const nbc_params& P, double r) {
+    int t = 0;
+    while (t < P.Levels - 1 && !(r > P.LevelRmin[t])) ++t;
+    return t;
+}
+
+static int cell_hash_h(const nbc_params& P, const d3& p, double r) {
+    if (p.z > P.BoxUpLmt[2] || p.y > P.BoxUpLmt[1] || p.x > P.BoxUpLmt[0] || p.z < P.BoxLowLmt[2] ||
+        p.y < P.BoxLowLmt[1] || p.x < P.BoxLowLmt[0])
+        return P.CellNum - 1;
+    const int t = sphere_level(P, r);
+    const double sz = P.LevelCell[t];
+    const double x[3] = {p.x, p.y, p.z};
+    int ci[3];
+    for (int a = 0; a < 3; ++a) {
+        int v = (int)std::floor((x[a] - P.WorldOrigin[a]) / sz);
+        ci[a] = v < 0 ? 0 : (v > P.LevelDim[t][a] - 1 ? P.LevelDim[t][a] - 1 : v);
+    }
+    return P.LevelOff[t] + (ci[2] * P.LevelDim[t][1] + ci[1]) * P.LevelDim[t][0] + ci[0];
+}
+
 static void neighbor_search(orc_ctx* c) {
     const int N = c->N;
     std::vector<int> cel(N);
+    const bool hg = c->P.Levels > 1;
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < N; ++i) cel[i] = cell_hash(c->P, c->Pos[i]);
+    for (int i = 0; i < N; ++i) cel[i] = hg ? cell_hash_h(c->P, c->Pos[i], c->Radius[i]) : cell_hash(c->P, c->Pos[i]);
     // Stable sort of (cell, index): CUB SortPairs is a stable LSD radix sort
     // over the identity values written by GetParticleCellIndex_cuk.
     std::vector<int> order(N);
@@ -453,6 +508,82 @@ static int create_anei(orc_ctx* c) {
     return overflow;
 }
 
+// Hierarchical search, src/dem_hgrid_contlist.cu:57-161: a sphere of level t
+// searches levels t..0 (27 cells each, z outer / x inner, cut-off
+// 1.5 (ri + rj)). Same level: split by index as in the uniform grid. Coarser
+// level T < t: the partner j has the lower sorted index (cells are numbered
+// level-major), so i lists j in its back row and j lists i in its front row.
+// Corrected semantics (the reference appends with atomicAdd into a row its
+// owner is writing without one): j's own same-level fronts come first, then its
+// finer partners in ascending index; both halves are checked for overflow.
+static int create_anei_h(orc_ctx* c) {
+    const int N = c->N, B = c->B, half = B / 2;
+    const nbc_params& P = c->P;
+    int overflow = 0;
+    std::vector<std::vector<std::pair<int, int>>> cross(orc_get_threads());
+#pragma omp parallel reduction(| : overflow)
+    {
+#ifdef _OPENMP
+        std::vector<std::pair<int, int>>& mine = cross[omp_get_thread_num()];
+#else
+        std::vector<std::pair<int, int>>& mine = cross[0];
+#endif
+#pragma omp for schedule(dynamic, 256)
+        for (int index = 0; index < N; ++index) {
+            if (c->part_cel[index] == P.CellNum - 1) continue;
+            const d3 pi = c->Pos[index];
+            const double ri = c->Radius[index];
+            const int my = sphere_level(P, ri);
+            int* row = &c->Anei[(size_t)index * B];
+            int ng = 0, ni = 0;
+            for (int T = my; T >= 0; --T) {
+                const double sz = P.LevelCell[T];
+                const int* nd = P.LevelDim[T];
+                const int cix = (int)std::floor((pi.x - P.WorldOrigin[0]) / sz);
+                const int ciy = (int)std::floor((pi.y - P.WorldOrigin[1]) / sz);
+                const int ciz = (int)std::floor((pi.z - P.WorldOrigin[2]) / sz);
+                for (int cz = -1; cz <= 1; cz++)
+                    for (int cy = -1; cy <= 1; cy++)
+                        for (int cx = -1; cx <= 1; cx++) {
+                            const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
+                            if (jx < 0 || jx >= nd[0] || jy < 0 || jy >= nd[1] || jz < 0 || jz >= nd[2]) continue;
+                            const int cidx = P.LevelOff[T] + (jz * nd[1] + jy) * nd[0] + jx;
+                            const int start = c->cell_beg[cidx];
+                            if (start == -1) continue;
+                            const int end = c->cell_end[cidx];
+                            for (int j = start; j < end; j++) {
+                                if (j == index) continue;
+                                const d3 pj = c->Pos[j];
+                                double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                                double ds = dx * dx + dy * dy + dz * dz;
+                                double lim = 1.5 * (ri + c->Radius[j]);
+                                if (!(ds < lim * lim)) continue;
+                                if (T == my && index < j) {
+                                    if (ng < half) row[ng++] = j;
+                                    else overflow = 1;
+                                } else {
+                                    if (ni < half) { row[B - 1 - ni] = j; ni++; }
+                                    else overflow = 1;
+                                    if (T != my) mine.emplace_back(j, index);
+                                }
+                            }
+                        }
+            }
+            c->Ng[index] = ng;
+            c->Ni[index] = ni;
+        }
+    }
+    std::vector<std::pair<int, int>> all;
+    for (auto& v : cross) all.insert(all.end(), v.begin(), v.end());
+    std::sort(all.begin(), all.end());
+    for (const auto& e : all) {
+        const int j = e.first;
+        if (c->Ng[j] < half) c->Anei[(size_t)j * B + c->Ng[j]++] = e.second;
+        else overflow = 1;
+    }
+    return overflow;
+}
+
 // src/dem_grid.cu:680-750 plus the sort/gather of src/dem_simulation_clump.cu:126-175.
 static int rebuild(orc_ctx* c) {
     const int N = c->N, B = c->B;
@@ -481,7 +612,7 @@ static int rebuild(orc_ctx* c) {
     std::fill(c->Lpb.begin(), c->Lpb.end(), -1);
     std::fill(c->Anei.begin(), c->Anei.end(), -1);
     std::fill(c->Aref.begin(), c->Aref.end(), -1);
-    if (create_anei(c)) {
+    if (c->P.Levels > 1 ? create_anei_h(c) : create_anei(c)) {
         char m[256];
         std::snprintf(m, sizeof m,
                       "FATAL: Neighbor list overflow (BranchLmtLen=%d). Simulation stopped before using the "

@@ -18,7 +18,7 @@ import nbc  # noqa: E402  (struct definitions only)
 P = C.c_void_p
 I32 = C.c_int32
 _SIGS = {
-    "orc_finalize_params": ([C.POINTER(nbc.Params), P, I32, C.c_double], C.c_int),
+    "orc_finalize_params": ([C.POINTER(nbc.Params), P, I32, C.c_double, I32], C.c_int),
     "orc_create": ([C.POINTER(nbc.Params), C.POINTER(nbc.Options)], P),
     "orc_destroy": ([P], None),
     "orc_last_error": ([P], C.c_char_p),
@@ -70,10 +70,10 @@ def threads():
     return lib().orc_get_threads()
 
 
-def finalize_params(bed, cell_size=0.0):
+def finalize_params(bed, cell_size=0.0, levels=0):
     p = nbc.Params()
     C.memmove(C.byref(p), C.byref(bed.params), C.sizeof(nbc.Params))
-    rc = lib().orc_finalize_params(C.byref(p), ptr(bed.radius), bed.n, cell_size)
+    rc = lib().orc_finalize_params(C.byref(p), ptr(bed.radius), bed.n, cell_size, levels)
     assert rc == 0
     return p
 
