@@ -56,6 +56,7 @@ def c2_small():
 def test_hgrid_first_step(c2_small, prec):
     banner(f"hgrid first step, precision {prec}, N={c2_small.n}")
     ctx, ref = setup(c2_small, prec)
+    ctx.set_record(True)
     ctx.step(1)
     ref.step(1)
     assert_lists_equal(ctx, ref)
