@@ -78,6 +78,62 @@ def settle(bed, steps, dt):
     return out
 
 
+def run_c2(a):
+    """Polydisperse bed on the hierarchical grid (not the headline line): the
+    deposited C2 bed (1.93M spheres in 571K clumps of 1-6, radii log-uniform
+    0.2-2.0 mm, cube section 150 mm), FP32-mixed, 3-level grid next to the
+    uniform grid on the same bed; steady steps and forced rebuilds timed
+    separately."""
+    import nbc
+
+    if a.bed_cache and os.path.exists(a.bed_cache):
+        bed = nbc.Bed.load(a.bed_cache)
+    else:
+        t0 = time.perf_counter()
+        bed = nbc.generate_bed(2, a.nclumps if a.nclumps != 200000 else 0, seed=1)
+        print(f"[bench] C2 bed: {bed.n} spheres, {bed.nc} clumps in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if a.bed_cache:
+            bed.save(a.bed_cache)
+    out = {}
+    for levels in (3, 0):
+        opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CUBE, hgrid_levels=levels)
+        ctx = nbc.Context(bed.finalized_params(opt), opt)
+        ctx.upload(bed, bed.wall)
+        ctx.step(a.warmup)
+        s0 = ctx.stats()
+        ctx.profile(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.steps)
+        t1 = time.perf_counter()
+        s1 = ctx.stats()
+        k3_ms, k3_n = ctx.profile_get(1)
+        r_ms0, r_n0 = ctx.profile_get(3)
+        for _ in range(3):
+            ctx.rebuild()
+        ctx.sync()
+        r_ms, r_n = ctx.profile_get(3)
+        key = f"hgrid{levels}" if levels else "uniform"
+        out[key] = dict(sphere_steps_per_s=bed.n * a.steps / (t1 - t0), ms_per_step=(t1 - t0) / a.steps * 1e3,
+                        k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
+                        rebuild_ms=(r_ms - r_ms0) / max(r_n - r_n0, 1), touching_ss_per_sphere=s1.touching_ss / bed.n,
+                        list_pairs_per_sphere=s1.front_pairs / bed.n)
+        print(f"[bench] C2 {key}: {out[key]}", file=sys.stderr, flush=True)
+        ctx.close()
+    h = out["hgrid3"]
+    print(json.dumps({
+        "metric": "sphere-steps/s, C2 polydisperse bed on the hierarchical grid (FP32-mixed)",
+        "value": h["sphere_steps_per_s"], "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator, seed 1)",
+        "config": {"workload": "C2 bed: 571,429 clumps of 1-6 spheres, radii log-uniform 0.2-2.0 mm, cube section "
+                               "150 mm, mu 0.6, dt 2e-7 s, 3-level hierarchical grid, as deposited",
+                   "n_spheres": bed.n, "n_clumps": bed.nc},
+        "forms": out,
+    }))
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of known size: reading its FETCH_SIZE and
     WRITE_SIZE counters gives the factor that converts those counters into HBM
@@ -127,6 +183,8 @@ def main():
     ap.add_argument("--settle-dt", type=float, default=5e-6)
     ap.add_argument("--bed-cache", default=None,
                     help="load the settled bed from this .npz if it exists, else build it and save it there")
+    ap.add_argument("--workload", choices=("c1", "c2"), default="c1",
+                    help="c1: the headline bed (default); c2: polydisperse bed on the hierarchical grid")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -142,6 +200,9 @@ def main():
     if nbc.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.workload == "c2":
+        run_c2(a)
+        return
     if a.bed_cache and os.path.exists(a.bed_cache):
         bed = nbc.Bed.load(a.bed_cache)
         print(f"[bench] bed loaded from {a.bed_cache}: {bed.n} spheres, {bed.nc} clumps", file=sys.stderr, flush=True)

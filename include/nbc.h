@@ -275,6 +275,26 @@ int nbc_save_bed_text(const char* dir, const nbc_bed_out* bed, const nbc_penetra
 /* ---- device context ---------------------------------------------------- */
 int nbc_create(nbc_ctx** out, const nbc_params* params, const nbc_options* opt, int32_t device);
 int nbc_destroy(nbc_ctx* ctx);
+/* Batch of independent beds advanced by one context: every step is one set of
+ * kernel launches over all beds. params points to nbeds finalized parameter
+ * blocks; beds share the material, StepSize, Thresh, BranchLmtLen, Update,
+ * OutputInterval and isWallForce, and each keeps its own uniform grid, box and
+ * wall (all walls of one type, opt->wall_type); no hierarchical grid and no
+ * penetrator. The spheres and clumps of bed b occupy the index ranges
+ * [sphere_off[b], sphere_off[b+1]) and [clump_off[b], clump_off[b+1]) of the
+ * getters (nbc_get_batch_ranges; sorting never moves a sphere out of its
+ * bed's range); file indices and clump ids are global (offset by bed). A
+ * rebuild flagged by one bed rebuilds every bed.
+ * No counterpart in the reference, which runs one bed per process
+ * (src/main.cpp:7-8). */
+int nbc_create_batch(nbc_ctx** out, int32_t nbeds, const nbc_params* params, const nbc_options* opt,
+                     int32_t device);
+/* spheres, clumps and walls are arrays of nbeds (walls may be NULL when
+ * isWallForce == 0); each bed's clump ids are local to the bed. */
+int nbc_upload_batch(nbc_ctx* ctx, const nbc_spheres_in* spheres, const nbc_clumps_in* clumps,
+                     const nbc_wall* walls);
+/* nbeds + 1 offsets each (either pointer may be NULL). */
+int nbc_get_batch_ranges(nbc_ctx* ctx, int32_t* sphere_off, int32_t* clump_off);
 /* Copy host state in (file order) and reset histories; wall may be NULL when
  * isWallForce == 0; pene may be NULL when isIntruder == 0. */
 int nbc_upload(nbc_ctx* ctx, const nbc_spheres_in* s, const nbc_clumps_in* c,

@@ -72,6 +72,32 @@ int nbc_create(nbc_ctx** out, const nbc_params* params, const nbc_options* opt, 
     return 0;
 }
 
+int nbc_create_batch(nbc_ctx** out, int32_t nbeds, const nbc_params* params, const nbc_options* opt,
+                     int32_t device) {
+    if (!out || !params || nbeds < 1) { g_err = "nbc_create_batch: null argument"; return NBC_E_BADINPUT; }
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        g_err = "nbc_create_batch: no HIP device available";
+        return NBC_E_NODEVICE;
+    }
+    if (device < 0 || device >= n) { g_err = "nbc_create_batch: device index out of range"; return NBC_E_NODEVICE; }
+    nbc_options o;
+    if (opt) o = *opt; else std::memset(&o, 0, sizeof(o));
+    nbc_ctx* c = new (std::nothrow) nbc_ctx();
+    if (!c) { g_err = "nbc_create_batch: out of host memory"; return NBC_E_BADINPUT; }
+    c->eng = nbc::make_engine(o.precision);
+    int e = c->eng->init_batch(params, nbeds, o, device);
+    if (e) {
+        g_err = c->eng->err;
+        delete c->eng;
+        delete c;
+        return e;
+    }
+    *out = c;
+    return 0;
+}
+
 int nbc_destroy(nbc_ctx* ctx) {
     if (!ctx) return 0;
     delete ctx->eng;
@@ -86,6 +112,15 @@ int nbc_upload(nbc_ctx* ctx, const nbc_spheres_in* s, const nbc_clumps_in* c, co
                const nbc_penetrator* pene) {
     ENG(ctx);
     return ctx_err(ctx, ctx->eng->upload(s, c, wall, pene));
+}
+int nbc_upload_batch(nbc_ctx* ctx, const nbc_spheres_in* spheres, const nbc_clumps_in* clumps,
+                     const nbc_wall* walls) {
+    ENG(ctx);
+    return ctx_err(ctx, ctx->eng->upload_batch(spheres, clumps, walls));
+}
+int nbc_get_batch_ranges(nbc_ctx* ctx, int32_t* sphere_off, int32_t* clump_off) {
+    ENG(ctx);
+    return ctx_err(ctx, ctx->eng->get_batch_ranges(sphere_off, clump_off));
 }
 int nbc_step(nbc_ctx* ctx, int32_t nsteps) {
     ENG(ctx);

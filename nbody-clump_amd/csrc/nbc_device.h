@@ -90,6 +90,21 @@ constexpr int MAX_FACES = 7; // cube 6 (or cylinder 3) + penetrator 1
 // ---- run parameters passed by value ----------------------------------------
 constexpr int MAX_LEVELS = 8;
 
+// Container wall: cube (half-sides `side`) or cylinder (radius, half height,
+// axis `dir`), centre `pos`, rigid-body velocity and angular velocity.
+struct WallGeo {
+    double side[3], rad, height, dir[3], pos[3], vel[3], angv[3];
+};
+
+// One bed of a batch: its uniform grid (cells cell_off .. cell_off + cell_num - 1
+// of the shared cell arrays, the last one taking out-of-box spheres), box and wall.
+struct BedGeo {
+    double origin[3], cell[3], icell[3], boxlo[3], boxup[3];
+    uint32_t G[3];
+    int cell_off, cell_num, pad;
+    WallGeo W;
+};
+
 struct DevParams {
     int N, Nc, B, CellNum;
     uint32_t G[3];
@@ -114,8 +129,12 @@ struct DevParams {
     int nfaces;           // wall faces in use (6 cube, 3 cylinder, 0 none)
     int pene_on;          // isIntruder
     int pene_face;        // history slot of the penetrator (= nfaces)
-    // wall
-    double wside[3], wrad, wheight, wdir[3], wpos[3], wvel[3], wangv[3];
+    // wall (single bed)
+    WallGeo W;
+    // batch of beds (nbeds > 1): per-bed grid, box and wall; bid[i] = bed of sphere i
+    int nbeds;
+    const BedGeo* beds;
+    const int* bid;
     // penetrator (static part)
     int pshape, pmotion, pwait, ntri;
     double prad, pheight, pdir[3], pmass;

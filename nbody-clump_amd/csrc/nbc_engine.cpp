@@ -289,6 +289,7 @@ class Engine : public EngineBase {
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
+        F(dbeds); F(dbid);
         if (hstatus) (void)hipHostFree(hstatus);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
@@ -302,9 +303,163 @@ class Engine : public EngineBase {
         device = -1;
     }
 
+    static WallGeo wall_geo(const nbc_wall& w) {
+        WallGeo g;
+        for (int a = 0; a < 3; ++a) {
+            g.side[a] = w.side[a]; g.dir[a] = w.dir[a]; g.pos[a] = w.pos[a];
+            g.vel[a] = w.vel[a]; g.angv[a] = w.angv[a];
+        }
+        g.rad = w.radius; g.height = w.height;
+        return g;
+    }
+
+    // ------------------------------------------------------------------
+    // Batch of beds: one context whose sphere and clump arrays are the beds
+    // concatenated in bed order. Every bed keeps its own uniform grid (a range
+    // of the shared cell arrays), box and wall; sorting by cell keeps each
+    // bed's spheres in its own index range. A rebuild flagged by any bed
+    // rebuilds every bed.
+    int init_batch(const nbc_params* beds, int nb, const nbc_options& o, int dev) override {
+        if (!beds || nb < 1) return fail(NBC_E_BADINPUT, "nbc_create_batch: no beds");
+        nbc_params all = beds[0];
+        long long ns = 0, nc = 0, cells = 0;
+        soff.assign(1, 0);
+        coff.assign(1, 0);
+        for (int b = 0; b < nb; ++b) {
+            const nbc_params& q = beds[b];
+            if (!q.finalized) return fail(NBC_E_BADINPUT, "nbc_create_batch: call nbc_finalize_params on every bed");
+            if (q.Levels > 1 || q.isIntruder)
+                return fail(NBC_E_BADINPUT, "nbc_create_batch: beds use the uniform grid and no penetrator");
+            if (q.ParticleNum <= 0 || q.ClumpNum <= 0 || q.CellNum <= 0)
+                return fail(NBC_E_BADINPUT, "nbc_create_batch: empty bed");
+            const bool same = q.StepSize == all.StepSize && q.Ymod == all.Ymod && q.Prat == all.Prat &&
+                              q.Frcoe == all.Frcoe && q.NormRes == all.NormRes && q.Beta == all.Beta &&
+                              q.Cohesion == all.Cohesion && q.Gravity == all.Gravity && q.Thresh == all.Thresh &&
+                              q.BranchLmtLen == all.BranchLmtLen && q.isWallForce == all.isWallForce &&
+                              q.Damping == all.Damping && q.Update == all.Update &&
+                              q.OutputInterval == all.OutputInterval;
+            if (!same)
+                return fail(NBC_E_BADINPUT, "nbc_create_batch: beds must share material, StepSize, Thresh, "
+                                            "BranchLmtLen, Update, OutputInterval and isWallForce");
+            ns += q.ParticleNum;
+            nc += q.ClumpNum;
+            cells += q.CellNum;
+            if (ns > (1ll << 30) || nc > (1ll << 30) || cells > 2000000000ll)
+                return fail(NBC_E_BADINPUT, "nbc_create_batch: batch too large");
+            soff.push_back((int)ns);
+            coff.push_back((int)nc);
+            for (int a = 0; a < 3; ++a) {
+                all.BoxLowLmt[a] = std::min(all.BoxLowLmt[a], q.BoxLowLmt[a]);
+                all.BoxUpLmt[a] = std::max(all.BoxUpLmt[a], q.BoxUpLmt[a]);
+            }
+        }
+        all.ParticleNum = (int32_t)ns;
+        all.ClumpNum = (int32_t)nc;
+        all.CellNum = (int32_t)cells;
+        int e = init(all, o, dev);
+        if (e) return e;
+        nbeds = nb;
+        hbeds.assign(nb, BedGeo{});
+        int off = 0;
+        for (int b = 0; b < nb; ++b) {
+            const nbc_params& q = beds[b];
+            BedGeo& g = hbeds[b];
+            for (int a = 0; a < 3; ++a) {
+                g.origin[a] = q.WorldOrigin[a]; g.cell[a] = q.CellSize[a]; g.icell[a] = q.i_CellSize[a];
+                g.boxlo[a] = q.BoxLowLmt[a]; g.boxup[a] = q.BoxUpLmt[a]; g.G[a] = q.GridDim[a];
+            }
+            g.cell_off = off;
+            g.cell_num = q.CellNum;
+            off += q.CellNum;
+        }
+        HIPC(dalloc(&dbeds, (size_t)nb));
+        HIPC(dalloc(&dbid, (size_t)N));
+        std::vector<int> bid(N);
+        for (int b = 0; b < nb; ++b)
+            for (int i = soff[b]; i < soff[b + 1]; ++i) bid[i] = b;
+        HIPC(hipMemcpy(dbid, bid.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(dbeds, hbeds.data(), sizeof(BedGeo) * nb, hipMemcpyHostToDevice));
+        dp.nbeds = nb;
+        dp.beds = dbeds;
+        dp.bid = dbid;
+        return 0;
+    }
+
+    int upload_batch(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w) override {
+        if (nbeds < 1 || soff.empty()) return fail(NBC_E_BADINPUT, "nbc_upload_batch: context is not a batch");
+        if (!s || !c || (nfaces > 0 && !w)) return fail(NBC_E_BADINPUT, "nbc_upload_batch: null argument");
+        std::vector<int32_t> cid(N);
+        std::vector<double> pos(3 * (size_t)N), loc(3 * (size_t)N), mass(N), rad(N);
+        std::vector<double> cp(3 * (size_t)Nc), cv(3 * (size_t)Nc), cq(4 * (size_t)Nc), cw(3 * (size_t)Nc),
+            pr(4 * (size_t)Nc);
+        for (int b = 0; b < nbeds; ++b) {
+            const int n = soff[b + 1] - soff[b], m = coff[b + 1] - coff[b];
+            if (s[b].n != n || c[b].n != m)
+                return fail(NBC_E_BADINPUT, "nbc_upload_batch: sphere/clump counts differ from the bed parameters");
+            for (int k = 0; k < n; ++k) {
+                const int i = soff[b] + k;
+                const int q = s[b].clump_id[k];
+                cid[i] = (q >= 0 && q < m) ? q + coff[b] : -1;  // -1 is rejected by upload
+                for (int a = 0; a < 3; ++a) {
+                    pos[3 * (size_t)i + a] = s[b].pos[3 * (size_t)k + a];
+                    loc[3 * (size_t)i + a] = s[b].local[3 * (size_t)k + a];
+                }
+                mass[i] = s[b].mass[k];
+                rad[i] = s[b].radius[k];
+            }
+            for (int k = 0; k < m; ++k) {
+                const size_t j = (size_t)coff[b] + k;
+                for (int a = 0; a < 3; ++a) {
+                    cp[3 * j + a] = c[b].pos[3 * (size_t)k + a];
+                    cv[3 * j + a] = c[b].vel[3 * (size_t)k + a];
+                    cw[3 * j + a] = c[b].angv[3 * (size_t)k + a];
+                }
+                for (int a = 0; a < 4; ++a) {
+                    cq[4 * j + a] = c[b].quat[4 * (size_t)k + a];
+                    pr[4 * j + a] = c[b].prop[4 * (size_t)k + a];
+                }
+            }
+            if (nfaces > 0) {
+                bool okw = true;
+                for (int a = 0; a < 3; ++a)
+                    okw = okw && std::isfinite(w[b].pos[a]) && std::isfinite(w[b].vel[a]) && std::isfinite(w[b].angv[a]);
+                if (wall_type == NBC_WALL_CUBE)
+                    for (int a = 0; a < 3; ++a) okw = okw && std::isfinite(w[b].side[a]) && w[b].side[a] > 0.0;
+                else
+                    okw = okw && std::isfinite(w[b].radius) && std::isfinite(w[b].height) && w[b].radius > 0.0 &&
+                          w[b].height > 0.0;
+                if (!okw)
+                    return fail(NBC_E_NONFINITE, "FATAL: Invalid input or nonfinite wall force/torque at sorted "
+                                                 "particle index 0. Simulation stopped.");
+                hbeds[b].W = wall_geo(w[b]);
+            }
+        }
+        HIPC(hipMemcpy(dbeds, hbeds.data(), sizeof(BedGeo) * nbeds, hipMemcpyHostToDevice));
+        nbc_spheres_in sa;
+        std::memset(&sa, 0, sizeof(sa));
+        sa.n = N; sa.clump_id = cid.data(); sa.pos = pos.data(); sa.local = loc.data(); sa.mass = mass.data();
+        sa.radius = rad.data();
+        nbc_clumps_in ca;
+        std::memset(&ca, 0, sizeof(ca));
+        ca.n = Nc; ca.pos = cp.data(); ca.vel = cv.data(); ca.quat = cq.data(); ca.angv = cw.data(); ca.prop = pr.data();
+        batch_upload = true;
+        const int e = upload(&sa, &ca, nfaces > 0 ? &w[0] : nullptr, nullptr);
+        batch_upload = false;
+        return e;
+    }
+
+    int get_batch_ranges(int32_t* so, int32_t* co) override {
+        for (size_t b = 0; b < soff.size(); ++b) {
+            if (so) so[b] = soff[b];
+            if (co) co[b] = coff[b];
+        }
+        return 0;
+    }
+
     // ------------------------------------------------------------------
     int upload(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w, const nbc_penetrator* pe) override {
         HIPC(hipSetDevice(device));
+        if (nbeds > 1 && !batch_upload) return fail(NBC_E_BADINPUT, "nbc_upload: a batch context takes nbc_upload_batch");
         if (!s || !c || s->n != N || c->n != Nc)
             return fail(NBC_E_BADINPUT, "nbc_upload: sphere/clump counts differ from the context parameters");
         for (int i = 0; i < N; ++i)
@@ -327,11 +482,7 @@ class Engine : public EngineBase {
             if (!okw)
                 return fail(NBC_E_NONFINITE, "FATAL: Invalid input or nonfinite wall force/torque at sorted particle "
                                              "index 0. Simulation stopped.");
-            for (int a = 0; a < 3; ++a) {
-                dp.wside[a] = w->side[a]; dp.wdir[a] = w->dir[a]; dp.wpos[a] = w->pos[a];
-                dp.wvel[a] = w->vel[a]; dp.wangv[a] = w->angv[a];
-            }
-            dp.wrad = w->radius; dp.wheight = w->height;
+            dp.W = wall_geo(*w);
         }
         std::vector<D4> pos(N), cum(N);
         std::vector<R4> loc(N), kin(N);
@@ -1050,6 +1201,14 @@ class Engine : public EngineBase {
     uint32_t *xk0 = nullptr, *xk1 = nullptr;
     int *xv0 = nullptr, *xv1 = nullptr, *xhist = nullptr, *xhscan = nullptr, *xstmp = nullptr;
     long long xcap = 0;
+    // batch of beds: sphere / clump offsets per bed (nbeds + 1), per-bed grid
+    // and wall table (host copy and device copy), bed of each sphere
+    int nbeds = 1;
+    bool batch_upload = false;
+    std::vector<int> soff, coff;
+    std::vector<BedGeo> hbeds;
+    BedGeo* dbeds = nullptr;
+    int* dbid = nullptr;
     int *val0 = nullptr, *val1 = nullptr, *hist = nullptr, *hscan = nullptr, *stmp = nullptr, *stotal = nullptr;
     int *perm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
     int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;

@@ -14,6 +14,10 @@ class EngineBase {
     virtual int init(const nbc_params& p, const nbc_options& o, int device) = 0;
     virtual int upload(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w,
                        const nbc_penetrator* pe) = 0;
+    // batch of independent beds (uniform grids, no penetrator)
+    virtual int init_batch(const nbc_params* beds, int nbeds, const nbc_options& o, int device) = 0;
+    virtual int upload_batch(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w) = 0;
+    virtual int get_batch_ranges(int32_t* sphere_off, int32_t* clump_off) = 0;
     virtual int step(int nsteps) = 0;
     virtual int force_rebuild() = 0;
     virtual int set_record(int on) = 0;

@@ -235,6 +235,24 @@ __device__ __forceinline__ int clamp_u(uint32_t G, int v) {
     return (int)(u < hi ? u : hi);
 }
 
+// Uniform-grid cell of p, src/dem_grid.cu:155-202 (out-of-box spheres go to
+// the last cell).
+__device__ __forceinline__ int uniform_cell(const double* O, const double* cs, const uint32_t* G, const double* lo,
+                                            const double* up, int cell_num, const D4& p) {
+    if (p.z >= up[2] || p.y >= up[1] || p.x >= up[0] || p.z < lo[2] || p.y < lo[1] || p.x < lo[0])
+        return cell_num - 1;
+    int cx = (int)floor((p.x - O[0]) / cs[0]);
+    int cy = (int)floor((p.y - O[1]) / cs[1]);
+    int cz = (int)floor((p.z - O[2]) / cs[2]);
+    cx = clamp_u(G[0], cx);
+    cy = clamp_u(G[1], cy);
+    cz = clamp_u(G[2], cz);
+    int idx = (int)((uint32_t)cz * G[1] * G[0] + (uint32_t)cy * G[0] + (uint32_t)cx);
+    if (idx < 0) idx = 0;
+    if (idx >= cell_num) idx = cell_num - 1;
+    return idx;
+}
+
 // Level of a sphere in the hierarchical grid: the first t with r > lrmin[t]
 // (the last level takes the rest).
 __device__ __forceinline__ int level_of(const DevParams& P, double r) {
@@ -272,19 +290,12 @@ __global__ void k_hash(DevParams P, const D4* pos, uint32_t* key, int* val) {
         key[i] = (uint32_t)idx;
         return;
     }
-    if (p.z >= P.boxup[2] || p.y >= P.boxup[1] || p.x >= P.boxup[0] || p.z < P.boxlo[2] || p.y < P.boxlo[1] ||
-        p.x < P.boxlo[0]) {
-        idx = P.CellNum - 1;
+    if (P.nbeds > 1) {
+        // batch: the bed's own grid, numbered from its first cell
+        const BedGeo& g = P.beds[P.bid[i]];
+        idx = g.cell_off + uniform_cell(g.origin, g.cell, g.G, g.boxlo, g.boxup, g.cell_num, p);
     } else {
-        int cx = (int)floor((p.x - P.origin[0]) / P.cell[0]);
-        int cy = (int)floor((p.y - P.origin[1]) / P.cell[1]);
-        int cz = (int)floor((p.z - P.origin[2]) / P.cell[2]);
-        cx = clamp_u(P.G[0], cx);
-        cy = clamp_u(P.G[1], cy);
-        cz = clamp_u(P.G[2], cz);
-        idx = (int)((uint32_t)cz * P.G[1] * P.G[0] + (uint32_t)cy * P.G[0] + (uint32_t)cx);
-        if (idx < 0) idx = 0;
-        if (idx >= P.CellNum) idx = P.CellNum - 1;
+        idx = uniform_cell(P.origin, P.cell, P.G, P.boxlo, P.boxup, P.CellNum, p);
     }
     key[i] = (uint32_t)idx;
 }
@@ -489,15 +500,26 @@ __global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, co
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const int half = P.B / 2;
-    if ((int)cell[i] == P.CellNum - 1) {
+    // grid of the sphere's bed (batch) or the single grid
+    const double* O = P.origin;
+    const double* ic = P.icell;
+    const uint32_t* G = P.G;
+    int coff = 0, last = P.CellNum - 1;
+    if (P.nbeds > 1) {
+        const BedGeo& g = P.beds[P.bid[i]];
+        O = g.origin; ic = g.icell; G = g.G;
+        coff = g.cell_off;
+        last = g.cell_off + g.cell_num - 1;
+    }
+    if ((int)cell[i] == last) {
         if (!FILL) { fcnt[i] = 0; bcnt[i] = 0; }
         return;
     }
     const D4 pi = pos[i];
     const double ri = pi.w;
-    const int cix = (int)floor((pi.x - P.origin[0]) * P.icell[0]);
-    const int ciy = (int)floor((pi.y - P.origin[1]) * P.icell[1]);
-    const int ciz = (int)floor((pi.z - P.origin[2]) * P.icell[2]);
+    const int cix = (int)floor((pi.x - O[0]) * ic[0]);
+    const int ciy = (int)floor((pi.y - O[1]) * ic[1]);
+    const int ciz = (int)floor((pi.z - O[2]) * ic[2]);
     int ng = 0, ni = 0;
     int fb = 0, bb = 0, nbi = 0, ci = 0;
     if (FILL) { fb = foff[i]; bb = boff[i]; nbi = bcnt[i]; ci = cid[i]; }
@@ -505,10 +527,10 @@ __global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, co
         for (int cy = -1; cy <= 1; cy++)
             for (int cx = -1; cx <= 1; cx++) {
                 const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
-                if (jx < 0 || (uint32_t)jx >= P.G[0] || jy < 0 || (uint32_t)jy >= P.G[1] || jz < 0 ||
-                    (uint32_t)jz >= P.G[2])
+                if (jx < 0 || (uint32_t)jx >= G[0] || jy < 0 || (uint32_t)jy >= G[1] || jz < 0 ||
+                    (uint32_t)jz >= G[2])
                     continue;
-                const int cidx = (int)(((uint32_t)jz * P.G[1] + (uint32_t)jy) * P.G[0] + (uint32_t)jx);
+                const int cidx = coff + (int)(((uint32_t)jz * G[1] + (uint32_t)jy) * G[0] + (uint32_t)jx);
                 const int start = cbeg[cidx];
                 if (start == -1) continue;
                 const int end = cend[cidx];
@@ -1000,24 +1022,25 @@ __device__ __forceinline__ V3<double> closest_on_tri(const V3<double>& p, const 
 // that holds for every position within sl of xi (the wall list uses the
 // distance a sphere can move before the list is rebuilt). A sphere failing it
 // has no wall or penetrator contact.
-__device__ __forceinline__ bool wall_candidate(const DevParams& P, const V3<double>& xi, double ri, double sl) {
-    const V3<double> RV = xi - mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
+__device__ __forceinline__ bool wall_candidate(const DevParams& P, const WallGeo& W, const V3<double>& xi, double ri,
+                                               double sl) {
+    const V3<double> RV = xi - mk<double>(W.pos[0], W.pos[1], W.pos[2]);
     if (P.wall_type == NBC_WALL_CUBE) {
         bool c = false;
         for (int a = 0; a < 3; ++a) {
             const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
-            const double Sd = P.wside[a];
+            const double Sd = W.side[a];
             c = c || (E > -sl && E + ri - Sd > -sl) || (E < sl && E - ri + Sd < sl);
         }
         return c;
     }
-    const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+    const V3<double> Wv = mk<double>(W.dir[0], W.dir[1], W.dir[2]);
     const double s2 = sl * (1.0 + Dot<double>(Wv, Wv));
     const double ERR = Dot<double>(RV, Wv);
     const V3<double> RVb = RV - ERR * Wv;
     const double norm = Length<double>(RVb);
-    const double d_cap = fabs(ERR) - P.wheight + ri;
-    const double d_side = norm - P.wrad + ri;
+    const double d_cap = fabs(ERR) - W.height + ri;
+    const double d_side = norm - W.rad + ri;
     return (d_cap >= 1e-14 - s2 && (ERR != 0.0 || sl > 0.0)) || d_side >= 1e-14 - s2 || !(d_cap == d_cap) ||
            !(d_side == d_side);
 }
@@ -1365,9 +1388,10 @@ __global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, Clum
             if (P.nfaces > 0 && !ok) atomicMin(bad, i);
             const R Rr = (R)ri;
             if (P.nfaces > 0 && ok) {
-                const V3<double> wpos = mk<double>(P.wpos[0], P.wpos[1], P.wpos[2]);
-                const V3<R> wv = mk<R>((R)P.wvel[0], (R)P.wvel[1], (R)P.wvel[2]);
-                const V3<R> ww = mk<R>((R)P.wangv[0], (R)P.wangv[1], (R)P.wangv[2]);
+                const WallGeo& W = P.nbeds > 1 ? P.beds[P.bid[i]].W : P.W;
+                const V3<double> wpos = mk<double>(W.pos[0], W.pos[1], W.pos[2]);
+                const V3<R> wv = mk<R>((R)W.vel[0], (R)W.vel[1], (R)W.vel[2]);
+                const V3<R> ww = mk<R>((R)W.angv[0], (R)W.angv[1], (R)W.angv[2]);
                 const V3<double> RV = xi - wpos;
                 if (P.wall_type == NBC_WALL_CUBE) {
                     // faces 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
@@ -1375,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, Clum
                         const int a = f >> 1;
                         const bool up = (f & 1) == 0;
                         const double E = a == 0 ? RV.x : (a == 1 ? RV.y : RV.z);
-                        const double Sd = P.wside[a];
+                        const double Sd = W.side[a];
                         const bool cand = up ? (E > 0 && E + ri - Sd > 0) : (E < 0 && E - ri + Sd < 0);
                         bool touch = false;
                         if (cand) {
@@ -1407,18 +1431,18 @@ __global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, Clum
                     }
                 } else {
                     // cylinder surfaces 0 bottom cap, 1 top cap, 2 side
-                    const V3<double> Wv = mk<double>(P.wdir[0], P.wdir[1], P.wdir[2]);
+                    const V3<double> Wv = mk<double>(W.dir[0], W.dir[1], W.dir[2]);
                     const double ERR = Dot<double>(RV, Wv);
                     const V3<double> RVb = RV - ERR * Wv, RVt = ERR * Wv;
                     const double norm = Length<double>(RVb);
-                    const double d_cap = fabs(ERR) - P.wheight + ri;
-                    const double d_side = norm - P.wrad + ri;
+                    const double d_cap = fabs(ERR) - W.height + ri;
+                    const double d_side = norm - W.rad + ri;
                     for (int f = 0; f < 3; ++f) {
                         bool cand;
                         V3<double> RVo;
-                        if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
-                        else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (P.wheight / fabs(ERR)) * RVt + RVb; }
-                        else { cand = d_side >= 1e-14; RVo = (P.wrad / norm) * RVb + RVt; }
+                        if (f == 0) { cand = d_cap >= 1e-14 && ERR < 0; RVo = (W.height / fabs(ERR)) * RVt + RVb; }
+                        else if (f == 1) { cand = d_cap >= 1e-14 && ERR > 0; RVo = (W.height / fabs(ERR)) * RVt + RVb; }
+                        else { cand = d_side >= 1e-14; RVo = (W.rad / norm) * RVb + RVt; }
                         bool touch = false;
                         if (cand) {
                             const V3<double> RVn = RV - RVo;
@@ -1594,17 +1618,18 @@ __global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, Clum
 // candidate tests with that slack cover every contact until the next build.
 // Spheres with a live wall history or a position that is not finite are
 // listed too.
-__device__ __forceinline__ bool wall_listed(const DevParams& P, const PeneState* pst, const D4& x, uint32_t wf) {
+__device__ __forceinline__ bool wall_listed(const DevParams& P, const PeneState* pst, const D4& x, uint32_t wf,
+                                            int i) {
     const V3<double> xi = xyz(x);
     if ((wf & 0xFFu) != 0u || !finite3(xi) || !isfinite(x.w)) return true;
-    if (P.nfaces > 0 && wall_candidate(P, xi, x.w, P.wl_slack)) return true;
+    if (P.nfaces > 0 && wall_candidate(P, P.nbeds > 1 ? P.beds[P.bid[i]].W : P.W, xi, x.w, P.wl_slack)) return true;
     return P.pene_on && pene_candidate(P, pst, xi, x.w, P.wl_slack + P.pene_move);
 }
 
 __global__ void k_wlist_count(DevParams P, const D4* pos, const uint32_t* wflag, const PeneState* pst, int* cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
-    cnt[i] = wall_listed(P, pst, pos[i], wflag[i]) ? 1 : 0;
+    cnt[i] = wall_listed(P, pst, pos[i], wflag[i], i) ? 1 : 0;
 }
 
 __global__ void k_wlist_fill(DevParams P, const D4* pos, const uint32_t* wflag, PeneState* pst, const int* off,
@@ -1613,7 +1638,7 @@ __global__ void k_wlist_fill(DevParams P, const D4* pos, const uint32_t* wflag, 
     if (i == 0)
         for (int a = 0; a < 3; ++a) pst->lpos[a] = pst->pos[a];
     if (i >= P.N) return;
-    if (wall_listed(P, pst, pos[i], wflag[i])) {
+    if (wall_listed(P, pst, pos[i], wflag[i], i)) {
         list[off[i]] = i;
         pos_of[i] = off[i];
     } else {

@@ -130,6 +130,9 @@ _SIGS = {
     "nbc_free_bed": ([C.POINTER(BedOut)], None),
     "nbc_save_bed_text": ([C.c_char_p, C.POINTER(BedOut), C.POINTER(Penetrator)], C.c_int),
     "nbc_create": ([C.POINTER(P), C.POINTER(Params), C.POINTER(Options), I32], C.c_int),
+    "nbc_create_batch": ([C.POINTER(P), I32, C.POINTER(Params), C.POINTER(Options), I32], C.c_int),
+    "nbc_upload_batch": ([P, C.POINTER(SpheresIn), C.POINTER(ClumpsIn), C.POINTER(Wall)], C.c_int),
+    "nbc_get_batch_ranges": ([P, P, P], C.c_int),
     "nbc_destroy": ([P], C.c_int),
     "nbc_upload": ([P, C.POINTER(SpheresIn), C.POINTER(ClumpsIn), C.POINTER(Wall), C.POINTER(Penetrator)], C.c_int),
     "nbc_step": ([P, I32], C.c_int),
@@ -445,6 +448,47 @@ class Context:
         ms, n = C.c_double(), C.c_int64()
         self._c(lib().nbc_profile_get(self.h, cls, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class BatchContext(Context):
+    """Device context holding several independent beds, advanced together.
+
+    Getters return all beds concatenated; `ranges()` gives each bed's sphere
+    and clump index ranges (file indices and clump ids are offset by bed)."""
+
+    def __init__(self, params_list, opt=None, device=0):
+        n = len(params_list)
+        arr = (Params * n)()
+        for b, q in enumerate(params_list):
+            C.memmove(C.byref(arr[b]), C.byref(q), C.sizeof(Params))
+        self.h = C.c_void_p()
+        check(lib().nbc_create_batch(C.byref(self.h), n, arr, C.byref(opt) if opt else None, device))
+        self.nbeds = n
+        self.bed_params = arr
+        p = Params()
+        C.memmove(C.byref(p), C.byref(arr[0]), C.sizeof(Params))
+        p.ParticleNum = sum(q.ParticleNum for q in arr)
+        p.ClumpNum = sum(q.ClumpNum for q in arr)
+        p.CellNum = sum(q.CellNum for q in arr)
+        self.params = p
+        self.N = p.ParticleNum
+        self.Nc = p.ClumpNum
+
+    def upload(self, beds, walls=True):
+        assert len(beds) == self.nbeds
+        n = self.nbeds
+        s_arr, c_arr, w_arr = (SpheresIn * n)(), (ClumpsIn * n)(), (Wall * n)()
+        for b, bed in enumerate(beds):
+            s_arr[b], c_arr[b] = bed.spheres_in(), bed.clumps_in()
+            if walls and bed.wall is not None:
+                C.memmove(C.byref(w_arr[b]), C.byref(bed.wall), C.sizeof(Wall))
+        self._beds = list(beds)  # the structs point into the beds' arrays
+        self._c(lib().nbc_upload_batch(self.h, s_arr, c_arr, w_arr if walls else None))
+
+    def ranges(self):
+        so, co = np.zeros(self.nbeds + 1, np.int32), np.zeros(self.nbeds + 1, np.int32)
+        self._c(lib().nbc_get_batch_ranges(self.h, ptr(so), ptr(co)))
+        return so, co
 
 
 def download_bed(ctx, template):

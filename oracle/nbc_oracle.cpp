@@ -128,6 +128,7 @@ struct orc_ctx {
     int count = 0;
     long long touching_ss = 0, touching_wall = 0;
     bool first = true;
+    bool rebuild_next = false;
 };
 
 static int fail(orc_ctx* c, int code, const std::string& m) {
@@ -1291,7 +1292,8 @@ static int step_once(orc_ctx* c) {
     update_pos(c);
     if (c->P.isIntruder) pene_integrate_before(c);
     const int nf = c->wall_type == NBC_WALL_CUBE ? 6 : (c->wall_type == NBC_WALL_CYLINDER ? 3 : 0);
-    if (c->Update != 0 || c->first) {
+    if (c->Update != 0 || c->first || c->rebuild_next) {
+        c->rebuild_next = false;
         int e = rebuild(c);
         if (e) return e;
     } else {
@@ -1338,6 +1340,11 @@ extern "C" int orc_step(orc_ctx* c, int n) {
     }
     return 0;
 }
+
+// The next step rebuilds whatever the displacements say: a batch of beds that
+// share one device context rebuilds all of them when any one needs it, and
+// this is how a single-bed oracle follows it.
+extern "C" void orc_request_rebuild(orc_ctx* c) { c->rebuild_next = true; }
 
 extern "C" int orc_rebuild(orc_ctx* c) {
     int e = rebuild(c);

@@ -19,6 +19,7 @@ P = C.c_void_p
 I32 = C.c_int32
 _SIGS = {
     "orc_finalize_params": ([C.POINTER(nbc.Params), P, I32, C.c_double, I32], C.c_int),
+    "orc_request_rebuild": ([P], None),
     "orc_create": ([C.POINTER(nbc.Params), C.POINTER(nbc.Options)], P),
     "orc_destroy": ([P], None),
     "orc_last_error": ([P], C.c_char_p),
@@ -114,6 +115,9 @@ class Oracle:
 
     def rebuild(self):
         self._c(lib().orc_rebuild(self.h))
+
+    def request_rebuild(self):
+        lib().orc_request_rebuild(self.h)
 
     def set_record(self, on=True):
         pass
