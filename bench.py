@@ -134,6 +134,92 @@ def run_c2(a):
     }))
 
 
+def c4_beds(nbeds, seed0=1000, threads=16):
+    """nbeds C1-style beds, clump counts uniform in [2000, 60000], cylinder
+    radius scaled with count (60 mm at 200,000 clumps), seed seed0 + bed;
+    generated in parallel threads (the generator releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    import nbc
+
+    rng = np.random.default_rng(seed0)
+    counts = rng.integers(2000, 60001, nbeds)
+
+    def one(b):
+        R = 0.06 * (counts[b] / 200000.0) ** (1.0 / 3.0)
+        return nbc.generate_bed(1, int(counts[b]), seed=seed0 + b, container=(R, 0, 0, 0))
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, range(nbeds)))
+
+
+def run_c4(a):
+    """Batched beds (not the headline line): nbeds independent C1-style beds
+    advanced by one context (one launch set per step), FP32-mixed and FP64;
+    next to it the first 16 beds stepped as separate single-bed contexts."""
+    import nbc
+
+    t0 = time.perf_counter()
+    beds = c4_beds(a.nbeds)
+    N = sum(b.n for b in beds)
+    print(f"[bench] C4: {a.nbeds} beds, {N} spheres, {sum(b.nc for b in beds)} clumps in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    out = {}
+    for prec, name in ((nbc.NBC_MIXED, "batch_mixed"), (nbc.NBC_FP64, "batch_fp64")):
+        opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
+        ctx = nbc.BatchContext([b.finalized_params(opt) for b in beds], opt)
+        ctx.upload(beds)
+        ctx.step(a.warmup)
+        s0 = ctx.stats()
+        ctx.profile(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.steps)
+        t1 = time.perf_counter()
+        s1 = ctx.stats()
+        k3_ms, k3_n = ctx.profile_get(1)
+        out[name] = dict(sphere_steps_per_s=N * a.steps / (t1 - t0), ms_per_step=(t1 - t0) / a.steps * 1e3,
+                         k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
+                         touching_ss_per_sphere=s1.touching_ss / N)
+        print(f"[bench] C4 {name}: {out[name]}", file=sys.stderr, flush=True)
+        ctx.close()
+    # the same kind of beds one context each, stepped one after another
+    sub = beds[:16]
+    ns = sum(b.n for b in sub)
+    opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER)
+    ctxs = []
+    for b in sub:
+        c = nbc.Context(b.finalized_params(opt), opt)
+        c.upload(b, b.wall)
+        c.step(a.warmup)
+        ctxs.append(c)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(a.steps)
+    for c in ctxs:
+        c.sync()
+    t1 = time.perf_counter()
+    out["separate_contexts_mixed_16_beds"] = dict(sphere_steps_per_s=ns * a.steps / (t1 - t0), n_spheres=ns)
+    print(f"[bench] C4 separate: {out['separate_contexts_mixed_16_beds']}", file=sys.stderr, flush=True)
+    for c in ctxs:
+        c.close()
+    m = out["batch_mixed"]
+    print(json.dumps({
+        "metric": "sphere-steps/s, C4 batched beds (FP32-mixed, one context)",
+        "value": m["sphere_steps_per_s"], "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator)",
+        "config": {"workload": f"C4: {a.nbeds} C1-style beds, clump counts uniform in [2000, 60000], cylinder "
+                               "radius scaled with count, as deposited, dt 5e-7 s",
+                   "n_spheres": N, "n_beds": a.nbeds},
+        "forms": out,
+    }))
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of known size: reading its FETCH_SIZE and
     WRITE_SIZE counters gives the factor that converts those counters into HBM
@@ -183,8 +269,10 @@ def main():
     ap.add_argument("--settle-dt", type=float, default=5e-6)
     ap.add_argument("--bed-cache", default=None,
                     help="load the settled bed from this .npz if it exists, else build it and save it there")
-    ap.add_argument("--workload", choices=("c1", "c2"), default="c1",
-                    help="c1: the headline bed (default); c2: polydisperse bed on the hierarchical grid")
+    ap.add_argument("--workload", choices=("c1", "c2", "c4"), default="c1",
+                    help="c1: the headline bed (default); c2: polydisperse bed on the hierarchical grid; "
+                         "c4: batched beds")
+    ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -202,6 +290,9 @@ def main():
         sys.exit(1)
     if a.workload == "c2":
         run_c2(a)
+        return
+    if a.workload == "c4":
+        run_c4(a)
         return
     if a.bed_cache and os.path.exists(a.bed_cache):
         bed = nbc.Bed.load(a.bed_cache)
