@@ -14,7 +14,9 @@ kernel, with algorithmic bytes per launch
     N * 72 B + C_ss * 80 B + C_w * 72 B
 (per-array accounting of the force inputs/outputs; C_ss / C_w = touching
 sphere-sphere / wall contacts counted on the device each step), timed with HIP
-events on the library stream. cpu_baseline: the FP64 oracle (the reference's
+events on the library stream over a second timed region that follows the
+throughput region (the events themselves add gaps between kernels, so
+`value` comes from a region without them). cpu_baseline: the FP64 oracle (the reference's
 slot-padded dataflow restated in C++ with OpenMP) on a bounded sample of the
 same bed.
 """
@@ -30,7 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "nbody-clump_amd"))
 HBM_PEAK = 8.0e12
 
 
-def run_form(bed, precision, steps, warmup):
+def run_form(bed, precision, steps, warmup, prof_steps):
     import nbc
 
     opt = nbc.make_options(precision=precision, wall_type=nbc.NBC_WALL_CYLINDER)
@@ -39,16 +41,24 @@ def run_form(bed, precision, steps, warmup):
     ctx.upload(bed, bed.wall)
     ctx.step(warmup)
     s0 = ctx.stats()
-    ctx.profile(True)
     ctx.sync()
     t0 = time.perf_counter()
     ctx.step(steps)  # returns after a stream synchronize
     t1 = time.perf_counter()
     s1 = ctx.stats()
+    # kernel durations: HIP events around every kernel class, recorded on the
+    # stream each kernel runs on, over a second timed region of the same run
+    # (the events add gaps between kernels, so the throughput region runs
+    # without them)
+    ctx.profile(True)
+    ctx.step(prof_steps)
+    ctx.sync()
+    s2 = ctx.stats()
     k3_ms, k3_n = ctx.profile_get(1)
     out = dict(seconds=t1 - t0, k3_ms=k3_ms, k3_n=k3_n, rebuilds=s1.rebuilds - s0.rebuilds,
-               touching_ss=0.5 * (s0.touching_ss + s1.touching_ss), touching_wall=0.5 * (s0.touching_wall + s1.touching_wall),
-               front_pairs=s1.front_pairs)
+               prof_rebuilds=s2.rebuilds - s1.rebuilds,
+               touching_ss=0.5 * (s1.touching_ss + s2.touching_ss), touching_wall=0.5 * (s1.touching_wall + s2.touching_wall),
+               front_pairs=s2.front_pairs)
     for cls in (0, 2, 3, 4, 5, 6):
         out[f"cls{cls}_ms"], out[f"cls{cls}_n"] = ctx.profile_get(cls)
     ctx.close()
@@ -273,6 +283,8 @@ def main():
                     help="c1: the headline bed (default); c2: polydisperse bed on the hierarchical grid; "
                          "c4: batched beds")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
+    ap.add_argument("--prof-steps", type=int, default=300,
+                    help="steps of the second timed region, with HIP events around every kernel")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -308,11 +320,11 @@ def main():
               file=sys.stderr, flush=True)
         if a.bed_cache:
             bed.save(a.bed_cache)
-    mixed = run_form(bed, nbc.NBC_MIXED, a.steps, a.warmup)
+    mixed = run_form(bed, nbc.NBC_MIXED, a.steps, a.warmup, a.prof_steps)
     print(f"[bench] mixed: {mixed}", file=sys.stderr, flush=True)
     fp64 = None
     if not a.no_fp64:
-        fp64 = run_form(bed, nbc.NBC_FP64, a.steps, a.warmup)
+        fp64 = run_form(bed, nbc.NBC_FP64, a.steps, a.warmup, a.prof_steps)
         print(f"[bench] fp64: {fp64}", file=sys.stderr, flush=True)
     N = bed.n
     value = N * a.steps / mixed["seconds"]

@@ -163,8 +163,8 @@ class Engine : public EngineBase {
             HIPC(dalloc(&s.pos, N)); HIPC(dalloc(&s.cum, N)); HIPC(dalloc(&s.loc, N)); HIPC(dalloc(&s.kin, N));
             HIPC(dalloc(&s.cid, N)); HIPC(dalloc(&s.fid, N)); HIPC(dalloc(&s.wflag, N));
             HIPC(dalloc(&s.wh, (size_t)std::max(nwh, 1) * N * HSTRIDE));
-            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.p32, N)); }
-            else { s.sft = S[0].sft; s.p32 = S[0].p32; }
+            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.p32, N)); HIPC(dalloc(&s.slot, N)); }
+            else { s.sft = S[0].sft; s.p32 = S[0].p32; s.slot = S[0].slot; }
         }
         HIPC(dalloc(&C.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C.prop, Nc));
         HIPC(dalloc(&C.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C.snap, 2 * (size_t)Nc));
@@ -279,7 +279,7 @@ class Engine : public EngineBase {
         for (int b = 0; b < 2; ++b) {
             SphereBuf<R>& s = S[b];
             F(s.pos); F(s.cum); F(s.loc); F(s.kin); F(s.cid); F(s.fid); F(s.wflag); F(s.wh);
-            if (b == 0) { F(s.sft); F(s.p32); }
+            if (b == 0) { F(s.sft); F(s.p32); F(s.slot); }
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap);
@@ -577,6 +577,7 @@ class Engine : public EngineBase {
         HIPC(hipMemset(cache_t, 0, sizeof(double) * 3 * oi));
         init_update = P.Update != 0.0;
         first = true;
+        k1_next = false;
         nstep = 0;
         nrebuild = 0;
         count = 0;
@@ -694,6 +695,7 @@ class Engine : public EngineBase {
         const int* cv = second ? val1 : val0;
         HIPC(hipMemcpyAsync(mem, cv, sizeof(int) * N, hipMemcpyDeviceToDevice, st));
         KC(launch_seg_bounds(st, ck, N, Nc, mbeg, mend));
+        KC(launch_member_slot(st, N, mem, s.slot));
         // 7. wall list of the new order
         { int e = build_wall_list(); if (e) return e; }
         nrebuild++;
@@ -739,14 +741,17 @@ class Engine : public EngineBase {
     // K1 + K2 of step n (src/dem_simulation_clump.cu:113-119)
     int enqueue_head(long long n) {
         hipEvent_t e0 = prof_begin();
-        KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (int*)dstatus, (int)n));
+        if (!k1_next) KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (int*)dstatus, (int)n));
+        k1_next = false;
         KC(launch_sphere_pose<R>(st, dp, S[scur], C, (int*)dstatus, (int)n, block));
         prof_end(0, e0, n);
         return 0;
     }
 
     // K3, penetrator sum, K4 of step n (src/dem_simulation_clump.cu:178-225)
-    int enqueue_tail(long long n, bool record) {
+    // With `next`, k_clump_after also runs K1 of step n + 1 and the next
+    // enqueue_head skips it.
+    int enqueue_tail(long long n, bool record, bool next) {
         const int p = (int)(n & 1);
         if (record) {
             int e = ensure_record();
@@ -783,8 +788,10 @@ class Engine : public EngineBase {
             prof_end(4, e0, n);
         }
         e0 = prof_begin();
-        KC(launch_clump_after<R>(st, dp, S[scur], C, wb, rec, record, mbeg, mend, mem, block, halt, (int)n));
+        KC(launch_clump_after<R>(st, dp, S[scur], C, wb, rec, record, mbeg, mend, mem, block, (int*)dstatus, (int)n,
+                                 next, pst, counters));
         prof_end(2, e0, n);
+        k1_next = next;
         return 0;
     }
 
@@ -823,14 +830,14 @@ class Engine : public EngineBase {
                 if ((e = rebuild((int)(n & 1)))) return e;
                 if ((e = clear_halt())) return e;
                 first = init_update = false;
-                if ((e = enqueue_tail(n, rec && n == end - 1))) return e;
+                if ((e = enqueue_tail(n, rec && n == end - 1, n + 1 < end))) return e;
                 ++n;
                 continue;
             }
             const long long m = std::min<long long>(batch, end - n);
             for (long long k = 0; k < m; ++k) {
                 if ((e = enqueue_head(n + k))) return e;
-                if ((e = enqueue_tail(n + k, rec && n + k == end - 1))) return e;
+                if ((e = enqueue_tail(n + k, rec && n + k == end - 1, n + k + 1 < end))) return e;
             }
             if ((e = read_status())) return e;
             if ((e = check_status())) return e;
@@ -840,15 +847,17 @@ class Engine : public EngineBase {
                 continue;
             }
             // steps n .. h-1 completed; step h stopped after k_sphere_pose for
-            // a rebuild, or for a new wall list only
+            // a rebuild, or for a new wall list only (the K1 of step h + 1
+            // fused into the K4 of step h did not run either)
             prof_drop(h);
+            k1_next = false;
             if ((hstatus[3] & ST_REBUILD) || !(hstatus[3] & ST_WLIST)) {
                 if ((e = rebuild(h & 1))) return e;
             } else {
                 if ((e = build_wall_list())) return e;
             }
             if ((e = clear_halt())) return e;
-            if ((e = enqueue_tail(h, rec && h == end - 1))) return e;
+            if ((e = enqueue_tail(h, rec && h == end - 1, (long long)h + 1 < end))) return e;
             n = (long long)h + 1;
         }
         nstep = end;
@@ -1227,6 +1236,7 @@ class Engine : public EngineBase {
     int near_total = 0;  // near pairs (front and back entries alike)
     RecordBuf rec;
     bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;
+    bool k1_next = false;  // the last enqueued k_clump_after also ran K1 of the next step
     long long nstep = 0, nrebuild = 0;
     int count = 0;
     double prof_ms[8] = {0};

@@ -21,7 +21,8 @@ struct SphereBuf {
     int* fid;                         // file index (part_fix)
     uint32_t* wflag;                  // bit f: wall/penetrator history f valid
     R* wh;                            // [MAX_FACES][N][HSTRIDE] wall/penetrator histories
-    typename Vec4T<R>::type* sft;     // [2N] force, torque about the clump centre
+    typename Vec4T<R>::type* sft;     // [2N] force, torque about the clump centre, in member-slot order
+    int* slot;                        // member slot of the sphere (its index in the clump member order)
     F4* p32;                          // FP32 (x, y, z) - box centre, radius; w < 0: use FP64 only
 };
 
@@ -124,8 +125,10 @@ int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R>
 int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* wflag, PeneState* pst,
                        int* cnt, int* off, int* list, int* pos_of, int* stmp, int* stotal, int block);
 template <class R>
+// K4 of step `step`; with `next`, K1 of step + 1 runs in the same kernel.
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL,
                        RecordBuf rec, bool record, const int* mbeg, const int* mend, const int* mem, int block,
-                       const int* halt, int step);
+                       int* halt, int step, bool next, PeneState* pst, unsigned long long* counters);
+int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot);
 
 }  // namespace nbc

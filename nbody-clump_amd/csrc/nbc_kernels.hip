@@ -102,54 +102,55 @@ __device__ __forceinline__ bool maybe_touch(const DevParams& P, const F4& a, con
 // K1: clump half-kick, drift and attitude (FP64). Thread 0 of block 0 also
 // advances the penetrator (src/dem_integrate_penetrator.cu:3-41) and clears
 // the step counters.
-template <class R>
-__global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsigned long long* counters,
-                               int* halt, int n) {
-    if (*halt < n) return;  // a previous step asked for a rebuild: replayed later
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        counters[0] = 0ull;
-        counters[1] = 0ull;
-        if (P.pene_on) {
-            const double Dt = P.dt;
-            if (P.pmotion == NBC_PENE_GRAVITY && (!P.pwait || pst->contact_state != 0)) {
-                V3<double> F = mk<double>(pst->totF[0], pst->totF[1], pst->totF[2]);
-                V3<double> pf = (-1.0 / P.pmass) * F;
-                pf.z = pf.z + P.Gravity;
-                V3<double> pos = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
-                V3<double> vel = mk<double>(pst->vel[0], pst->vel[1], pst->vel[2]);
-                pos = pos + Dt * vel + 0.5 * Dt * Dt * pf;
-                vel = vel + 0.5 * Dt * pf;
-                pst->pos[0] = pos.x; pst->pos[1] = pos.y; pst->pos[2] = pos.z;
-                pst->vel[0] = vel.x; pst->vel[1] = vel.y; pst->vel[2] = vel.z;
-            } else {
-                pst->pos[0] = pst->pos[0] + Dt * pst->vel[0];
-                pst->pos[1] = pst->pos[1] + Dt * pst->vel[1];
-                pst->pos[2] = pst->pos[2] + Dt * pst->vel[2];
-            }
-            // the wall list covers the penetrator's reach plus pene_move:
-            // past that, stop this step after k_sphere_pose for a new list
-            const double dx = pst->pos[0] - pst->lpos[0], dy = pst->pos[1] - pst->lpos[1],
-                         dz = pst->pos[2] - pst->lpos[2];
-            if (!(dx * dx + dy * dy + dz * dz <= P.pene_move * P.pene_move)) {
-                atomicOr(&halt[3], ST_WLIST);
-                atomicMin(&halt[0], n);
-            }
-        }
-    }
-    if (i >= P.Nc) return;
+// Thread 0 of K1: clears the step counters and advances the penetrator
+// (src/dem_integrate_penetrator.cu:3-41) for step n.
+__device__ __forceinline__ void k1_thread0(const DevParams& P, PeneState* pst, unsigned long long* counters,
+                                           int* halt, int n) {
+    counters[0] = 0ull;
+    counters[1] = 0ull;
+    if (!P.pene_on) return;
     const double Dt = P.dt;
-    const D4 pr = C.prop[i];
-    D4 Fr = C.ft[2 * i], Tr = C.ft[2 * i + 1];
-    D4 pc = C.pq[2 * i], qc = C.pq[2 * i + 1];
-    D4 vr = C.vw[2 * i], wr = C.vw[2 * i + 1];
-    V3<double> pos = xyz(pc), vel = xyz(vr), w = xyz(wr);
-    V3<double> F = xyz(Fr), T = xyz(Tr);
+    if (P.pmotion == NBC_PENE_GRAVITY && (!P.pwait || pst->contact_state != 0)) {
+        V3<double> F = mk<double>(pst->totF[0], pst->totF[1], pst->totF[2]);
+        V3<double> pf = (-1.0 / P.pmass) * F;
+        pf.z = pf.z + P.Gravity;
+        V3<double> pos = mk<double>(pst->pos[0], pst->pos[1], pst->pos[2]);
+        V3<double> vel = mk<double>(pst->vel[0], pst->vel[1], pst->vel[2]);
+        pos = pos + Dt * vel + 0.5 * Dt * Dt * pf;
+        vel = vel + 0.5 * Dt * pf;
+        pst->pos[0] = pos.x; pst->pos[1] = pos.y; pst->pos[2] = pos.z;
+        pst->vel[0] = vel.x; pst->vel[1] = vel.y; pst->vel[2] = vel.z;
+    } else {
+        pst->pos[0] = pst->pos[0] + Dt * pst->vel[0];
+        pst->pos[1] = pst->pos[1] + Dt * pst->vel[1];
+        pst->pos[2] = pst->pos[2] + Dt * pst->vel[2];
+    }
+    // the wall list covers the penetrator's reach plus pene_move: past that,
+    // stop this step after k_sphere_pose for a new list
+    const double dx = pst->pos[0] - pst->lpos[0], dy = pst->pos[1] - pst->lpos[1], dz = pst->pos[2] - pst->lpos[2];
+    if (!(dx * dx + dy * dy + dz * dz <= P.pene_move * P.pene_move)) {
+        atomicOr(&halt[3], ST_WLIST);
+        atomicMin(&halt[0], n);
+    }
+}
+
+// Clump force and torque scaled to accelerations (src/dem_integrate_clumpsphere.cu:12-20).
+__device__ __forceinline__ void scale_ft(const DevParams& P, const D4& pr, V3<double>& F, V3<double>& T) {
     F = (1.0 / pr.w) * F;
     F.z = F.z + P.Gravity;
     T.x = (1.0 / pr.x) * T.x;
     T.y = (1.0 / pr.y) * T.y;
     T.z = (1.0 / pr.z) * T.z;
+}
+
+// K1 of one clump: half-kick, drift and attitude (FP64) from the scaled force
+// and torque of the previous step.
+template <class R>
+__device__ __forceinline__ void clump_drift(const DevParams& P, const ClumpBuf<R>& C, int i, const D4& pc,
+                                            const D4& qc, V3<double> vel, V3<double> w, const V3<double>& F,
+                                            const V3<double>& T, int* halt) {
+    const double Dt = P.dt;
+    V3<double> pos = xyz(pc);
     V3<double> dist = Dt * vel + 0.5 * Dt * Dt * F;
     pos = pos + dist;
     vel = vel + 0.5 * Dt * F;
@@ -190,6 +191,25 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
     if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);  // k_contacts then checks every sphere exactly
     C.snap[2 * i] = mk4<R>((R)vel.x, (R)vel.y, (R)vel.z, ok ? R(1) : R(0));
     C.snap[2 * i + 1] = mk4<R>((R)w.x, (R)w.y, (R)w.z, R(0));
+}
+
+// ---------------------------------------------------------------------------
+// K1: clump half-kick, drift and attitude (FP64). Thread 0 of block 0 also
+// advances the penetrator and clears the step counters.
+template <class R>
+__global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsigned long long* counters,
+                               int* halt, int n) {
+    if (*halt < n) return;  // a previous step asked for a rebuild: replayed later
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) k1_thread0(P, pst, counters, halt, n);
+    if (i >= P.Nc) return;
+    const D4 pr = C.prop[i];
+    const D4 Fr = C.ft[2 * i], Tr = C.ft[2 * i + 1];
+    const D4 pc = C.pq[2 * i], qc = C.pq[2 * i + 1];
+    const D4 vr = C.vw[2 * i], wr = C.vw[2 * i + 1];
+    V3<double> F = xyz(Fr), T = xyz(Tr);
+    scale_ft(P, pr, F, T);
+    clump_drift<R>(P, C, i, pc, qc, xyz(vr), xyz(wr), F, T, halt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1271,6 +1291,7 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
     if (qn > 0) flush();
     if (active) {
         const R4 ki = S.kin[i];
+        const size_t so = (size_t)S.slot[i];  // k_clump_after reads the totals in member order
         int wk = -1;
         if (P.nfaces > 0 || P.pene_on) wk = WL.pos_of[i];
         if (P.nfaces > 0 && wk < 0) {
@@ -1293,8 +1314,8 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
             // wall-list sphere: k_walls runs beside this kernel, so the raw sums
             // go out (.w = 1 marks them) and k_clump_after adds the wall and
             // penetrator terms and the clump torque term
-            S.sft[2 * (size_t)i] = mk4<R>(Fss.x, Fss.y, Fss.z, R(0));
-            S.sft[2 * (size_t)i + 1] = mk4<R>(Tss.x, Tss.y, Tss.z, R(1));
+            S.sft[2 * so] = mk4<R>(Fss.x, Fss.y, Fss.z, R(0));
+            S.sft[2 * so + 1] = mk4<R>(Tss.x, Tss.y, Tss.z, R(1));
         } else {
             // totF/totT with zero wall and penetrator terms (src/dem_clump.cu:97-98)
             // and the clump torque term T + (x_s - x_c) x F (src/dem_clumpsphere.cu:86-107)
@@ -1311,8 +1332,8 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
             }
             const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
                                    tT.z + (r.x * tF.y - r.y * tF.x));
-            S.sft[2 * (size_t)i] = mk4<R>(tF.x, tF.y, tF.z, R(0));
-            S.sft[2 * (size_t)i + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
+            S.sft[2 * so] = mk4<R>(tF.x, tF.y, tF.z, R(0));
+            S.sft[2 * so + 1] = mk4<R>(Tq.x, Tq.y, Tq.z, R(0));
             if (RECORD) {
                 D4 a; a.x = tF.x; a.y = tF.y; a.z = tF.z; a.w = 0.0;
                 D4 b; b.x = tT.x; b.y = tT.y; b.z = tT.z; b.w = 0.0;
@@ -1720,38 +1741,43 @@ __device__ __forceinline__ void finish_listed(const DevParams& P, const SphereBu
     }
 }
 
-template <class R>
-__global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, RecordBuf rec, int record,
-                              const int* mbeg, const int* mend, const int* mem, const int* halt, int n) {
+template <class R, bool NEXT>
+__global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, RecordBuf rec, int record,
+                              const int* mbeg, const int* mend, const int* mem, int* halt, int n, PeneState* pst,
+                              unsigned long long* counters) {
     if (*halt <= n) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    // NEXT: K1 of step n + 1 follows in the same thread (it returns when K4 of
+    // step n does, at a halt at or before n)
+    if (NEXT && c == 0) k1_thread0(P, pst, counters, halt, n + 1);
     if (c >= P.Nc) return;
     const int b = mbeg[c], e = mend[c];
     // clump state first: independent of the member sums
     const D4 pr = C.prop[c];
     const D4 vr = C.vw[2 * c], wr = C.vw[2 * c + 1];
+    D4 pc, qc;
+    if (NEXT) { pc = C.pq[2 * c]; qc = C.pq[2 * c + 1]; }
     D4 Fr, Tr;
     if (b >= 0 && e > b) {
-        // members in ascending sorted order, four loads in flight at a time;
+        // members in ascending sorted order; k_contacts wrote each sphere's
+        // totals at its member slot, so the clump's records are contiguous;
         // the first member starts the sums, the rest are added in order
         typedef typename Vec4T<R>::type R4;
+        constexpr int U = sizeof(R) == sizeof(double) ? 2 : 4;  // member records in flight
         V3<double> F = mk<double>(0.0, 0.0, 0.0), T = F;
-        for (int k0 = b; k0 < e; k0 += 4) {
-            int m[4];
-            R4 f[4], t[4];
+        for (int k0 = b; k0 < e; k0 += U) {
+            R4 f[U], t[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) m[u] = k0 + u < e ? mem[k0 + u] : -1;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (m[u] >= 0) {
-                    f[u] = S.sft[2 * (size_t)m[u]];
-                    t[u] = S.sft[2 * (size_t)m[u] + 1];
+            for (int u = 0; u < U; ++u) {
+                if (k0 + u < e) {
+                    f[u] = S.sft[2 * (size_t)(k0 + u)];
+                    t[u] = S.sft[2 * (size_t)(k0 + u) + 1];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (m[u] < 0) continue;
-                if (t[u].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, m[u], c, f[u], t[u]);
+            for (int u = 0; u < U; ++u) {
+                if (k0 + u >= e) continue;
+                if (t[u].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, mem[k0 + u], c, f[u], t[u]);
                 const V3<double> fu = mk<double>(f[u].x, f[u].y, f[u].z), tu = mk<double>(t[u].x, t[u].y, t[u].z);
                 if (k0 + u == b) { F = fu; T = tu; }
                 else { F = F + fu; T = T + tu; }
@@ -1766,17 +1792,25 @@ __global__ void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBu
         Tr = C.ft[2 * c + 1];
     }
     V3<double> F = xyz(Fr), T = xyz(Tr);
-    F = (1.0 / pr.w) * F;
-    F.z = F.z + P.Gravity;
-    T.x = (1.0 / pr.x) * T.x;
-    T.y = (1.0 / pr.y) * T.y;
-    T.z = (1.0 / pr.z) * T.z;
+    scale_ft(P, pr, F, T);
     const V3<double> v = xyz(vr) + 0.5 * P.dt * F;
     const V3<double> w = xyz(wr) + 0.5 * P.dt * T;
+    if (NEXT) {
+        // K1 of step n + 1 reads exactly this force, torque and velocity
+        clump_drift<R>(P, C, c, pc, qc, v, w, F, T, halt);
+        return;
+    }
     D4 vo; vo.x = v.x; vo.y = v.y; vo.z = v.z; vo.w = 0.0;
     D4 wo; wo.x = w.x; wo.y = w.y; wo.z = w.z; wo.w = 0.0;
     C.vw[2 * c] = vo;
     C.vw[2 * c + 1] = wo;
+}
+
+// Member slot of every sphere: the sphere's position in the clump member
+// order, where k_contacts writes its totals for k_clump_after.
+__global__ void k_member_slot(int N, const int* mem, int* slot) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) slot[mem[k]] = k;
 }
 
 // Kinetic energy partial sums (FP64), one value per block.
@@ -2054,9 +2088,20 @@ int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const 
 template <class R>
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL,
                        RecordBuf rec, bool record, const int* mbeg, const int* mend, const int* mem, int block,
-                       const int* halt, int step) {
-    k_clump_after<R><<<nblk(P.Nc, block), block, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt,
-                                                          step);
+                       int* halt, int step, bool next, PeneState* pst, unsigned long long* counters) {
+    const unsigned g = nblk(P.Nc, block);
+    if (next)
+        k_clump_after<R, true><<<g, block, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt, step, pst,
+                                                    counters);
+    else
+        k_clump_after<R, false><<<g, block, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt, step,
+                                                     pst, counters);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
+    k_member_slot<<<nblk(N, 256), 256, 0, st>>>(N, mem, slot);
     LAUNCH_CHECK();
     return 0;
 }
@@ -2074,7 +2119,8 @@ int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
     template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>, int,          \
                                  PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, const int*, int);  \
     template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>,         \
-                                       RecordBuf, bool, const int*, const int*, const int*, int, const int*, int);
+                                       RecordBuf, bool, const int*, const int*, const int*, int, int*, int, bool,     \
+                                       PeneState*, unsigned long long*);
 INST(double)
 INST(float)
 
