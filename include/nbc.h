@@ -124,7 +124,9 @@ typedef struct nbc_options {
     int32_t reserved_i;
     double pene_frcoe;       /* penetrator friction; <= 0 means Frcoe          */
     double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
-    double reserved[4];
+    double near_skin;        /* near-list skin as a fraction of Thresh (0: 0.25; >= 1: gap 2*Thresh,
+                                refreshed at rebuilds only) */
+    double reserved[3];
 } nbc_options;
 
 /* ---- boundaries -------------------------------------------------------- */
@@ -191,6 +193,8 @@ typedef struct nbc_stats {
     double pene_force[3];    /* reaction force on the penetrator (-sum F)    */
     double pene_torque[3];
     double max_cum_disp;     /* reserved */
+    int64_t near_refreshes;  /* near-list refreshes between rebuilds since upload */
+    int64_t near_pairs;      /* pairs on the contact kernel's near list         */
 } nbc_stats;
 
 typedef struct nbc_ctx nbc_ctx;
@@ -336,9 +340,10 @@ int nbc_get_pene_exports(nbc_ctx* ctx, double* fn, double* ft, double* tn, doubl
 int nbc_get_pair_exports(nbc_ctx* ctx, double* fcn, double* cpt);
 
 /* ---- timing of library kernels on the library stream ----------------- */
-/* Kernel classes: 0 kinematics(K1+K2), 1 sphere-sphere pairs, 2 clump
- * reduction, 3 rebuild, 4 penetrator sum, 5 wall/penetrator contacts,
- * 6 per-sphere contact sums. */
+/* Kernel classes: 0 sphere kinematics (K2; K1 of the first step of a call),
+ * 1 sphere-sphere contacts, 2 clump reduction and second half-kick (with K1
+ * of the next step inside the call), 3 rebuild, 4 penetrator sum, 5 wall and
+ * penetrator contacts (side stream), 6 near-list refresh between rebuilds. */
 int nbc_profile_enable(nbc_ctx* ctx, int32_t on);
 int nbc_profile_get(nbc_ctx* ctx, int32_t kernel_class, double* total_ms, int64_t* launches);
 

@@ -122,7 +122,7 @@ struct DevParams {
     double pc[3];         // box centre: origin of the FP32 position copy
     double half;          // largest box half-extent
     float f32_marg;       // absolute slack of the FP32 distance pre-filter
-    double near_add;      // 2 * Thresh: gap below which a pair enters the near list
+    double near_delta;    // near-list skin: a sphere's slack is min(2a + delta, Thresh + a), a = |cum| at the build
     double wl_slack;      // wall list: distance a sphere may move after the list is built (2 * Thresh)
     double pene_move;     // wall list: penetrator travel allowed before the list is rebuilt
     int wall_type;        // NBC_WALL_*
@@ -155,5 +155,6 @@ struct PeneState {
 constexpr int ST_CLUMP_BAD = 1;  // a clump state is not finite (sticky)
 constexpr int ST_REBUILD = 2;    // a sphere moved more than Thresh: rebuild the lists
 constexpr int ST_WLIST = 4;      // the penetrator left its allowance: rebuild the wall list
+constexpr int ST_NEAR = 8;       // a sphere left its near-list allowance: rebuild the near lists
 
 }  // namespace nbc

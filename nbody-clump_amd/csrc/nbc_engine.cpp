@@ -147,7 +147,10 @@ class Engine : public EngineBase {
         // pairs farther apart than 2 * Thresh at a rebuild cannot touch before the next one
         const bool thresh_ok = std::isfinite(P.Thresh) && P.Thresh >= 0.0;
         const double inf = std::numeric_limits<double>::infinity();
-        dp.near_add = thresh_ok ? 2.0 * P.Thresh : inf;
+        // near-list skin delta (a fraction of Thresh): smaller lists for the
+        // contact kernel, refreshed when a sphere leaves its allowance
+        const double skin = O.near_skin > 0.0 ? std::min(O.near_skin, 1.0) : 0.25;
+        dp.near_delta = thresh_ok ? skin * P.Thresh : inf;
         // wall list: spheres move at most 2 * Thresh after a list build; the
         // penetrator may move pene_move before the list is rebuilt
         dp.wl_slack = thresh_ok ? 2.0 * P.Thresh * (1.0 + 1e-9) + 1e-12 : inf;
@@ -176,6 +179,7 @@ class Engine : public EngineBase {
         HIPC(dalloc(&boff, (size_t)N + 1));
         bcap = 0; bent = nullptr;
         HIPC(dalloc(&nfoff, (size_t)N + 1)); HIPC(dalloc(&nboff, (size_t)N + 1));
+        HIPC(dalloc(&nslack, (size_t)N));
         HIPC(grow_front(0, (long long)N * pair_cap_per));
         HIPC(grow_front(1, (long long)N * pair_cap_per));
         HIPC(grow_back((long long)N * pair_cap_per));
@@ -289,7 +293,7 @@ class Engine : public EngineBase {
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
-        F(dbeds); F(dbid);
+        F(dbeds); F(dbid); F(nslack);
         if (hstatus) (void)hipHostFree(hstatus);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
@@ -580,6 +584,7 @@ class Engine : public EngineBase {
         k1_next = false;
         nstep = 0;
         nrebuild = 0;
+        nnear = 0;
         count = 0;
         uploaded = true;
         HIPC(hipDeviceSynchronize());
@@ -680,7 +685,8 @@ class Engine : public EngineBase {
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));
-        KC(launch_near_build(st, dp, s.pos, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt,
+        KC(launch_near_build(st, dp, s.pos, s.cum, nslack, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfw, nfs,
+                             slot2near, bcnt,
                              nboff, nbo, nbe, stmp, stotal + 2, block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
@@ -800,6 +806,22 @@ class Engine : public EngineBase {
         return 0;
     }
 
+    // Near lists of the current Verlet list at the current positions (a
+    // sphere left its near-list allowance between rebuilds).
+    int near_refresh() {
+        hipEvent_t e0 = prof_begin();
+        FrontList& L = Lf[lcur];
+        KC(launch_near_sync(st, N, nfoff, nfw, nfs, L.fword, block));
+        SphereBuf<R>& s = S[scur];
+        KC(launch_near_build(st, dp, s.pos, s.cum, nslack, L.foff, L.fword, boff, bent, fcnt, nfoff, nfw, nfs,
+                             slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2, block));
+        HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        nnear++;
+        prof_end(6, e0, -1);
+        return 0;
+    }
+
     // Wall list at the current positions and penetrator position.
     int build_wall_list() {
         if (!(nfaces > 0 || dp.pene_on)) return 0;
@@ -851,10 +873,12 @@ class Engine : public EngineBase {
             // fused into the K4 of step h did not run either)
             prof_drop(h);
             k1_next = false;
-            if ((hstatus[3] & ST_REBUILD) || !(hstatus[3] & ST_WLIST)) {
+            const uint32_t why = hstatus[3];
+            if ((why & ST_REBUILD) || !(why & (ST_WLIST | ST_NEAR))) {
                 if ((e = rebuild(h & 1))) return e;
             } else {
-                if ((e = build_wall_list())) return e;
+                if ((why & ST_NEAR) && (e = near_refresh())) return e;
+                if ((why & ST_WLIST) && (e = build_wall_list())) return e;
             }
             if ((e = clear_halt())) return e;
             if ((e = enqueue_tail(h, rec && h == end - 1, (long long)h + 1 < end))) return e;
@@ -1171,6 +1195,8 @@ class Engine : public EngineBase {
         s->back_entries = nback;
         s->touching_ss = (int64_t)ctr[0];
         s->touching_wall = (int64_t)ctr[1];
+        s->near_refreshes = nnear;
+        s->near_pairs = near_total;
         s->kinetic_energy = ke;
         for (int a = 0; a < 3; ++a) { s->pene_force[a] = -ps.totF[a]; s->pene_torque[a] = ps.totT[a]; }
         return 0;
@@ -1237,7 +1263,8 @@ class Engine : public EngineBase {
     RecordBuf rec;
     bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;
     bool k1_next = false;  // the last enqueued k_clump_after also ran K1 of the next step
-    long long nstep = 0, nrebuild = 0;
+    long long nstep = 0, nrebuild = 0, nnear = 0;
+    double* nslack = nullptr;  // per-sphere near-list slack of the last near build
     int count = 0;
     double prof_ms[8] = {0};
     int64_t prof_n[8] = {0};

@@ -109,9 +109,12 @@ int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* p
                       const int* xend, const int* xsrt, uint32_t* fword, int2* bent, unsigned* overflow, int block);
 int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* bcnt, const int* xbeg,
                         const int* xend, unsigned* overflow, int block);
-int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
-                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs,
-                      int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal, int block);
+// Near lists of the current list at the current positions; also sets each
+// sphere's slack and its near-list request threshold (cum.w).
+int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, double* slack, const int* foff,
+                      const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw,
+                      int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal,
+                      int block);
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
                      int block);
 template <class R>

@@ -218,7 +218,7 @@ template <class R>
 __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int n) {
     if (*halt < n) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool over = false;
+    bool over = false, near = false;
     if (i < P.N) {
         const int c = S.cid[i];
         const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
@@ -231,8 +231,10 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         const D4 cu = S.cum[i];
         V3<double> cum = xyz(cu) + up - xyz(po);
         over = Length<double>(cum) > P.Thresh;
+        // the sphere left its near-list allowance (threshold kept in cum.w)
+        near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > cu.w;
         D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
-        D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = 0.0;
+        D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = cu.w;
         S.pos[i] = pn;
         S.cum[i] = cn;
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
@@ -243,6 +245,11 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
     const uint64_t b = __ballot(over);
     if (b != 0ull && lane_id() == (int)__ffsll((unsigned long long)b) - 1) {
         atomicOr(&halt[3], ST_REBUILD);
+        atomicMin(halt, n);
+    }
+    const uint64_t bn = __ballot(near);
+    if (b == 0ull && bn != 0ull && lane_id() == (int)__ffsll((unsigned long long)bn) - 1) {
+        atomicOr(&halt[3], ST_NEAR);
         atomicMin(halt, n);
     }
 }
@@ -716,45 +723,79 @@ __global__ void k_hist_remap(int N, int p, const int* perm, const int* nfoff, ui
 }
 
 // ---------------------------------------------------------------------------
-// Near lists: the contact kernel's view of the Verlet list. A rebuild happens
-// before any sphere has moved more than Thresh from its position at the last
-// rebuild (the displacement test runs on every sphere every step), so a pair
-// whose gap at the rebuild is 2 * Thresh or more cannot touch until the next
-// rebuild, and an intra-clump pair never does. The near list keeps the other
-// front entries, in slot order, as (slot, pair word); between rebuilds it holds
+// Near lists: the contact kernel's view of the Verlet list. Each sphere gets a
+// slack at the near-list build: with a = |cum| (its displacement since the
+// last rebuild) and delta the near skin, slack = 2a + delta, and k_sphere_pose
+// asks for a new near list (ST_NEAR) once |cum| exceeds a + delta, so the
+// sphere has moved at most |cum| + a <= slack since the build. When
+// Thresh + a is smaller, the slack is Thresh + a and no request is needed
+// (|cum| <= Thresh until the next rebuild). A pair whose gap at the build is at
+// least the sum of the two slacks therefore cannot touch before the next near
+// build, and an intra-clump pair never does. The near list keeps the other
+// front entries, in slot order, as (slot, pair word); between builds it holds
 // the history-valid bits of its pairs (k_near_sync writes them back into the
 // pair words). The back near list keeps, in back-row order, (owner, index of
-// the pair in the owner's near row). P.near_add = 2 * Thresh (infinite when
-// Thresh is not a finite non-negative number: every pair is then near).
-__device__ __forceinline__ bool near_pair(const DevParams& P, const D4& a, const D4& b, uint32_t w) {
+// the pair in the owner's near row). With Thresh not a finite non-negative
+// number every slack is infinite: every pair is near.
+__device__ __forceinline__ bool near_pair(const D4& a, const D4& b, uint32_t w, double sa, double sb) {
     if (w & INTRA) return false;
     if (w & (VALID(0) | VALID(1))) return true;
     const double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
-    const double reach = (a.w + b.w + P.near_add) * (1.0 + 1e-9);
+    const double reach = (a.w + b.w + sa + sb) * (1.0 + 1e-9);
     return !(dx * dx + dy * dy + dz * dz >= reach * reach);
 }
 
-__global__ void k_near_count(DevParams P, const D4* pos, const int* foff, const uint32_t* fword, int* ncnt) {
+// Slack of every sphere and its near-list request threshold (|cum|^2 above
+// cum.w asks for a new near list), written into cum.w.
+__global__ void k_near_slack(DevParams P, D4* cum, double* slack) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    D4 c = cum[i];
+    const double a = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+    const double d = P.near_delta;
+    double sl, trig;
+    if (!(a == a) || !(d < 1e300)) {
+        sl = __longlong_as_double(0x7ff0000000000000ll);  // no bound: every pair is near
+        trig = sl;
+    } else if (2.0 * a + d <= P.Thresh + a) {
+        sl = (2.0 * a + d) * (1.0 + 1e-9) + 1e-15;
+        const double t = (a + d) * (1.0 - 1e-9);
+        trig = t * t;
+    } else {
+        sl = (P.Thresh + a) * (1.0 + 1e-9) + 1e-15;
+        trig = __longlong_as_double(0x7ff0000000000000ll);
+    }
+    slack[i] = sl;
+    c.w = trig;
+    cum[i] = c;
+}
+
+__global__ void k_near_count(DevParams P, const D4* pos, const double* slack, const int* foff, const uint32_t* fword,
+                             int* ncnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const D4 a = pos[i];
+    const double sa = slack[i];
     int c = 0;
     for (int s = foff[i]; s < foff[i + 1]; ++s) {
         const uint32_t w = fword[s];
-        c += near_pair(P, a, pos[w & JMASK], w) ? 1 : 0;
+        const int j = (int)(w & JMASK);
+        c += near_pair(a, pos[j], w, sa, slack[j]) ? 1 : 0;
     }
     ncnt[i] = c;
 }
 
-__global__ void k_near_fill(DevParams P, const D4* pos, const int* foff, const uint32_t* fword, const int* nfoff,
-                            uint32_t* nfw, int* nfs, int* slot2near) {
+__global__ void k_near_fill(DevParams P, const D4* pos, const double* slack, const int* foff, const uint32_t* fword,
+                            const int* nfoff, uint32_t* nfw, int* nfs, int* slot2near) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const D4 a = pos[i];
+    const double sa = slack[i];
     int o = nfoff[i];
     for (int s = foff[i]; s < foff[i + 1]; ++s) {
         const uint32_t w = fword[s];
-        if (near_pair(P, a, pos[w & JMASK], w)) {
+        const int j = (int)(w & JMASK);
+        if (near_pair(a, pos[j], w, sa, slack[j])) {
             nfw[o] = w;
             nfs[o] = s;
             slot2near[s] = o++;
@@ -1973,18 +2014,21 @@ int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* b
     return 0;
 }
 
-int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, const int* foff, const uint32_t* fword,
-                      const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw, int* nfs,
-                      int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal, int block) {
+int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, double* slack, const int* foff,
+                      const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw,
+                      int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal,
+                      int block) {
     const int N = P.N;
     const unsigned g = nblk(N, block);
     int e;
     if ((e = launch_fill_i32(st, ncnt + N, 1, 0))) return e;
     if ((e = launch_fill_i32(st, nbcnt + N, 1, 0))) return e;
-    k_near_count<<<g, block, 0, st>>>(P, pos, foff, fword, ncnt);
+    k_near_slack<<<g, block, 0, st>>>(P, cum, slack);
+    LAUNCH_CHECK();
+    k_near_count<<<g, block, 0, st>>>(P, pos, slack, foff, fword, ncnt);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, ncnt, nfoff, N + 1, stmp, stotal))) return e;
-    k_near_fill<<<g, block, 0, st>>>(P, pos, foff, fword, nfoff, nfw, nfs, slot2near);
+    k_near_fill<<<g, block, 0, st>>>(P, pos, slack, foff, fword, nfoff, nfw, nfs, slot2near);
     LAUNCH_CHECK();
     k_near_back_count<<<g, block, 0, st>>>(N, boff, bent, slot2near, nbcnt);
     LAUNCH_CHECK();

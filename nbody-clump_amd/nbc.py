@@ -48,7 +48,8 @@ class Options(C.Structure):
         ("precision", C.c_int32), ("wall_type", C.c_int32), ("block_size", C.c_int32),
         ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
         ("hgrid_levels", C.c_int32), ("reserved_i", C.c_int32),
-        ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("reserved", C.c_double * 4),
+        ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
+        ("reserved", C.c_double * 3),
     ]
 
 
@@ -90,7 +91,7 @@ class Stats(C.Structure):
         ("step", C.c_int64), ("rebuilds", C.c_int64), ("front_pairs", C.c_int64),
         ("back_entries", C.c_int64), ("touching_ss", C.c_int64), ("touching_wall", C.c_int64),
         ("kinetic_energy", C.c_double), ("pene_force", D3), ("pene_torque", D3),
-        ("max_cum_disp", C.c_double),
+        ("max_cum_disp", C.c_double), ("near_refreshes", C.c_int64), ("near_pairs", C.c_int64),
     ]
 
 
@@ -287,11 +288,12 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0, batch_steps=0, hgrid_levels=0):
+                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0):
     o = Options()
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
     o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
     o.hgrid_levels = hgrid_levels
+    o.near_skin = near_skin
     return o
 
 

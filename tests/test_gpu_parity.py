@@ -199,6 +199,8 @@ def test_fast_collisions_between_rebuilds(prec):
             assert (sa.touching_ss, sa.touching_wall, sa.rebuilds) == (sb.touching_ss, sb.touching_wall, sb.rebuilds), k
     sb = ref.stats()
     assert sb.rebuilds >= 4 and seen > 0
+    # the contact kernel's near lists were refreshed between rebuilds too
+    assert ctx.stats().near_refreshes > 0
     if prec == nbc.NBC_FP64:
         assert_lists_equal(ctx, ref)
         assert np.abs(ctx.spheres()["pos"] - ref.spheres()["pos"]).max() <= 1e-12
