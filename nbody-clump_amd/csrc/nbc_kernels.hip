@@ -678,19 +678,12 @@ __global__ void k_hfront_total(int N, int half, int* fcnt, const int* bcnt, cons
 }
 
 // Owner slot of every back entry: src/dem_grid.cu:591-599 (Aref of the back
-// rows), -1 when the owner's own search did not list the pair.
-__global__ void k_back_resolve(int N, const int* foff, const uint32_t* fword, const int* boff, int2* bent) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    for (int e = boff[i]; e < boff[i + 1]; ++e) {
-        const int o = bent[e].x;
-        int slot = -1;
-        for (int s = foff[o]; s < foff[o + 1]; ++s)
-            if ((int)(fword[s] & JMASK) == i) slot = s;
-        bent[e].y = slot;
-        if (slot >= 0 && (fword[slot] & INTRA)) bent[e].x = o | BINTRA;
-    }
-}
+// rows), -1 when the owner's own search did not list the pair. A wave takes
+// the back rows of 64 consecutive spheres (one contiguous segment), one entry
+// per lane, and each lane scans its owner's front row (pairs are listed once,
+// so the first match is the only one).
+__global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, const uint32_t* fword, const int* boff,
+                                                      int2* bent);
 
 // History inheritance, src/dem_grid.cu:603-647: each new front pair looks up
 // its pair in the old list of the lower old index and copies the history of
@@ -770,66 +763,193 @@ __global__ void k_near_slack(DevParams P, D4* cum, double* slack) {
     cum[i] = c;
 }
 
-__global__ void k_near_count(DevParams P, const D4* pos, const double* slack, const int* foff, const uint32_t* fword,
-                             int* ncnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.N) return;
-    const D4 a = pos[i];
-    const double sa = slack[i];
-    int c = 0;
-    for (int s = foff[i]; s < foff[i + 1]; ++s) {
-        const uint32_t w = fword[s];
-        const int j = (int)(w & JMASK);
-        c += near_pair(a, pos[j], w, sa, slack[j]) ? 1 : 0;
-    }
-    ncnt[i] = c;
+// The near-list passes work like the contact kernel: a wave takes 64
+// consecutive spheres, whose rows form one contiguous segment of the CSR
+// list, and reads it 64 entries at a time (coalesced); an entry finds its row
+// by binary search over the 65 row offsets kept in LDS.
+__device__ __forceinline__ int row_of(const int* fo, int e) {
+    int r = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1)
+        if (fo[r + st] <= e) r += st;
+    return r;
 }
 
-__global__ void k_near_fill(DevParams P, const D4* pos, const double* slack, const int* foff, const uint32_t* fword,
-                            const int* nfoff, uint32_t* nfw, int* nfs, int* slot2near) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.N) return;
-    const D4 a = pos[i];
-    const double sa = slack[i];
-    int o = nfoff[i];
-    for (int s = foff[i]; s < foff[i + 1]; ++s) {
-        const uint32_t w = fword[s];
-        const int j = (int)(w & JMASK);
-        if (near_pair(a, pos[j], w, sa, slack[j])) {
-            nfw[o] = w;
-            nfs[o] = s;
-            slot2near[s] = o++;
-        } else {
-            slot2near[s] = -1;
+// Loads the row offsets of the wave's spheres i0 .. i0+63 into fo[0..64].
+__device__ __forceinline__ void load_rows(int* fo, const int* off, int i0, int N) {
+    const int lane = lane_id();
+    fo[lane] = off[min(i0 + lane, N)];
+    if (lane == 0) fo[64] = off[min(i0 + 64, N)];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, const uint32_t* fword, const int* boff,
+                                                      int2* bent) {
+    __shared__ int s_fo[4][65];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= N) return;
+    load_rows(s_fo[wv], boff, i0, N);
+    const int end = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
+        const int e = k0 + lane;
+        if (e >= end) continue;
+        const int i = i0 + row_of(s_fo[wv], e);
+        const int o = bent[e].x;
+        int slot = -1;
+        const int se = foff[o + 1];
+        for (int t = foff[o]; t < se; ++t)
+            if ((int)(fword[t] & JMASK) == i) { slot = t; break; }
+        int2 be;
+        be.x = (slot >= 0 && (fword[slot] & INTRA)) ? (o | BINTRA) : o;
+        be.y = slot;
+        bent[e] = be;
+    }
+}
+
+// Pass 1: near flag of every front entry (into flag[]) and near count per sphere.
+__global__ __launch_bounds__(256) void k_near_scan(DevParams P, const D4* pos, const double* slack, const int* foff,
+                                                   const uint32_t* fword, int* ncnt, int* flag) {
+    __shared__ int s_fo[4][65];
+    __shared__ int s_cnt[4][64];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= P.N) return;
+    const int i = i0 + lane;
+    const bool act = i < P.N;
+    const D4 a = act ? pos[i] : D4{0.0, 0.0, 0.0, 0.0};
+    const double sa = act ? slack[i] : 0.0;
+    s_cnt[wv][lane] = 0;
+    load_rows(s_fo[wv], foff, i0, P.N);
+    const int end = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
+        const int e = k0 + lane;
+        int r = 0;
+        bool nr = false;
+        uint32_t w = 0;
+        int j = 0;
+        if (e < end) {
+            r = row_of(s_fo[wv], e);
+            w = fword[e];
+            j = (int)(w & JMASK);
+        }
+        // the row's own centre and slack come from the lane that owns the row
+        const double ax = __shfl(a.x, r), ay = __shfl(a.y, r), az = __shfl(a.z, r), aw = __shfl(a.w, r);
+        const double asl = __shfl(sa, r);
+        if (e < end) {
+            D4 ar; ar.x = ax; ar.y = ay; ar.z = az; ar.w = aw;
+            nr = near_pair(ar, pos[j], w, asl, slack[j]);
+            flag[e] = nr ? 1 : 0;
+            if (nr) atomicAdd(&s_cnt[wv][r], 1);
         }
     }
+    wave_sync();
+    if (act) ncnt[i] = s_cnt[wv][lane];
 }
 
-__global__ void k_near_back_count(int N, const int* boff, const int2* bent, const int* slot2near, int* nbcnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int c = 0;
-    for (int e = boff[i]; e < boff[i + 1]; ++e) {
-        const int2 be = bent[e];
-        c += (be.y >= 0 && slot2near[be.y] >= 0) ? 1 : 0;
-    }
-    nbcnt[i] = c;
-}
-
-__global__ void k_near_back_fill(int N, const int* boff, const int2* bent, const int* slot2near, const int* nboff,
-                                 int* nbo, int* nbe) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int o = nboff[i];
-    for (int e = boff[i]; e < boff[i + 1]; ++e) {
-        const int2 be = bent[e];
-        if (be.y >= 0) {
-            const int q = slot2near[be.y];
-            if (q >= 0) {
-                nbo[o] = be.x & ~BINTRA;
-                nbe[o++] = q;
+// Pass 2: near entries in slot order at nfoff[i] + (near entries before them
+// in the row); slot2near (which held the flags) becomes the near index or -1.
+__global__ __launch_bounds__(256) void k_near_put(int N, const int* foff, const uint32_t* fword, const int* nfoff,
+                                                  uint32_t* nfw, int* nfs, int* slot2near) {
+    __shared__ int s_fo[4][65];
+    __shared__ int s_carry[4][64];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= N) return;
+    const int i = i0 + lane;
+    const int no = nfoff[min(i, N)];
+    s_carry[wv][lane] = 0;
+    load_rows(s_fo[wv], foff, i0, N);
+    const int end = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
+        const int e = k0 + lane;
+        int r = 0, f = 0;
+        if (e < end) {
+            r = row_of(s_fo[wv], e);
+            f = slot2near[e];
+        }
+        const uint64_t m = __ballot(f != 0);
+        const int lo = max(s_fo[wv][r] - k0, 0);  // first lane of this row in the chunk
+        const uint64_t below = lanemask_lt() & ~((1ull << lo) - 1ull);
+        const int rank = __popcll(m & below);
+        const int base = __shfl(no, r) + s_carry[wv][r];
+        if (e < end) {
+            if (f) {
+                const int o = base + rank;
+                nfw[o] = fword[e];
+                nfs[o] = e;
+                slot2near[e] = o;
+            } else {
+                slot2near[e] = -1;
             }
         }
+        wave_sync();
+        if (e < end && f) atomicAdd(&s_carry[wv][r], 1);
+        wave_sync();
+    }
+}
+
+// Back near lists: a back entry is near when its pair is on the owner's near
+// row. Pass 1 counts, pass 2 writes (owner, near index) in back-row order.
+__global__ __launch_bounds__(256) void k_nback_scan(int N, const int* boff, const int2* bent, const int* slot2near,
+                                                    int* nbcnt) {
+    __shared__ int s_fo[4][65];
+    __shared__ int s_cnt[4][64];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= N) return;
+    s_cnt[wv][lane] = 0;
+    load_rows(s_fo[wv], boff, i0, N);
+    const int end = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
+        const int e = k0 + lane;
+        if (e < end) {
+            const int r = row_of(s_fo[wv], e);
+            const int2 be = bent[e];
+            if (be.y >= 0 && slot2near[be.y] >= 0) atomicAdd(&s_cnt[wv][r], 1);
+        }
+    }
+    wave_sync();
+    if (i0 + lane < N) nbcnt[i0 + lane] = s_cnt[wv][lane];
+}
+
+__global__ __launch_bounds__(256) void k_nback_put(int N, const int* boff, const int2* bent, const int* slot2near,
+                                                   const int* nboff, int* nbo, int* nbe) {
+    __shared__ int s_fo[4][65];
+    __shared__ int s_carry[4][64];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= N) return;
+    const int no = nboff[min(i0 + lane, N)];
+    s_carry[wv][lane] = 0;
+    load_rows(s_fo[wv], boff, i0, N);
+    const int end = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
+        const int e = k0 + lane;
+        int r = 0, q = -1, own = 0;
+        if (e < end) {
+            r = row_of(s_fo[wv], e);
+            const int2 be = bent[e];
+            if (be.y >= 0) q = slot2near[be.y];
+            own = be.x & ~BINTRA;
+        }
+        const uint64_t m = __ballot(q >= 0);
+        const int lo = max(s_fo[wv][r] - k0, 0);
+        const uint64_t below = lanemask_lt() & ~((1ull << lo) - 1ull);
+        const int o = __shfl(no, r) + s_carry[wv][r] + __popcll(m & below);
+        if (q >= 0) {
+            nbo[o] = own;
+            nbe[o] = q;
+        }
+        wave_sync();
+        if (q >= 0) atomicAdd(&s_carry[wv][r], 1);
+        wave_sync();
     }
 }
 
@@ -2025,15 +2145,16 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum
     if ((e = launch_fill_i32(st, nbcnt + N, 1, 0))) return e;
     k_near_slack<<<g, block, 0, st>>>(P, cum, slack);
     LAUNCH_CHECK();
-    k_near_count<<<g, block, 0, st>>>(P, pos, slack, foff, fword, ncnt);
+    const unsigned gw = nblk(N, 256);  // 4 waves of 64 spheres per block
+    k_near_scan<<<gw, 256, 0, st>>>(P, pos, slack, foff, fword, ncnt, slot2near);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, ncnt, nfoff, N + 1, stmp, stotal))) return e;
-    k_near_fill<<<g, block, 0, st>>>(P, pos, slack, foff, fword, nfoff, nfw, nfs, slot2near);
+    k_near_put<<<gw, 256, 0, st>>>(N, foff, fword, nfoff, nfw, nfs, slot2near);
     LAUNCH_CHECK();
-    k_near_back_count<<<g, block, 0, st>>>(N, boff, bent, slot2near, nbcnt);
+    k_nback_scan<<<gw, 256, 0, st>>>(N, boff, bent, slot2near, nbcnt);
     LAUNCH_CHECK();
     if ((e = launch_exclusive_scan(st, nbcnt, nboff, N + 1, stmp, stotal + 1))) return e;
-    k_near_back_fill<<<g, block, 0, st>>>(N, boff, bent, slot2near, nboff, nbo, nbe);
+    k_nback_put<<<gw, 256, 0, st>>>(N, boff, bent, slot2near, nboff, nbo, nbe);
     LAUNCH_CHECK();
     return 0;
 }
@@ -2045,7 +2166,8 @@ int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nf
 }
 int launch_back_resolve(hipStream_t st, int N, const int* foff, const uint32_t* fword, const int* boff, int2* bent,
                         int block) {
-    k_back_resolve<<<nblk(N, block), block, 0, st>>>(N, foff, fword, boff, bent);
+    (void)block;
+    k_back_resolve<<<nblk(N, 256), 256, 0, st>>>(N, foff, fword, boff, bent);
     LAUNCH_CHECK();
     return 0;
 }
