@@ -112,7 +112,7 @@ class Engine : public EngineBase {
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
         nfaces = wall_type == NBC_WALL_CUBE ? 6 : (wall_type == NBC_WALL_CYLINDER ? 3 : 0);
         pair_cap_per = O.pair_capacity > 0 ? O.pair_capacity : 16;
-        batch = O.batch_steps > 0 ? O.batch_steps : 64;
+        batch = O.batch_steps > 0 ? O.batch_steps : 4;
         // device parameters
         std::memset(&dp, 0, sizeof(dp));
         dp.N = N; dp.Nc = Nc; dp.B = P.BranchLmtLen; dp.CellNum = P.CellNum;
@@ -209,6 +209,8 @@ class Engine : public EngineBase {
         HIPC(dalloc(&cache_f, 3 * (size_t)oi)); HIPC(dalloc(&cache_t, 3 * (size_t)oi));
         HIPC(dalloc(&ke_partial, (Nc + 255) / 256));
         HIPC(hipHostMalloc((void**)&hstatus, 4 * sizeof(uint32_t), hipHostMallocDefault));
+        HIPC(hipHostMalloc((void**)&hring, 4 * sizeof(uint32_t) * kRing, hipHostMallocDefault));
+        for (int k = 0; k < kRing; ++k) HIPC(hipEventCreateWithFlags(&ev_stat[k], hipEventDisableTiming));
         std::memset(&rec, 0, sizeof(rec));
         return 0;
     }
@@ -295,6 +297,9 @@ class Engine : public EngineBase {
         F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
         F(dbeds); F(dbid); F(nslack);
         if (hstatus) (void)hipHostFree(hstatus);
+        if (hring) (void)hipHostFree(hring);
+        for (int k = 0; k < kRing; ++k)
+            if (ev_stat[k]) (void)hipEventDestroy(ev_stat[k]);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
         if (st2) (void)hipStreamSynchronize(st2);
@@ -834,10 +839,34 @@ class Engine : public EngineBase {
         return 0;
     }
 
-    // Steps are enqueued in batches without host round trips. When a step
-    // needs a rebuild, the device halt step stops the rest of the batch at
-    // that step's contact kernel; the host then rebuilds and replays from
-    // there, so rebuilds happen at exactly the reference's steps.
+    // Steps are enqueued in chunks of `batch` steps without host round trips.
+    // When a step needs a rebuild (or new near or wall lists), the device halt
+    // step stops every later kernel at entry; the host then rebuilds and
+    // replays from that step's contact kernel, so rebuilds happen at exactly
+    // the reference's steps. The status words are copied back after every
+    // chunk and read while up to kDepth chunks are in flight, so a halt wastes
+    // at most a few no-op steps and the GPU never waits for the host.
+    struct Chunk {
+        int slot;
+    };
+
+    // Rebuild, near-list refresh or wall list for the step h that stopped,
+    // then the rest of step h. Requires an idle stream and fresh hstatus.
+    int handle_halt(long long h, long long end, bool rec) {
+        int e;
+        prof_drop(h);
+        k1_next = false;  // the K1 fused into the stopped K4 did not run
+        const uint32_t why = hstatus[3];
+        if ((why & ST_REBUILD) || !(why & (ST_WLIST | ST_NEAR))) {
+            if ((e = rebuild((int)(h & 1)))) return e;
+        } else {
+            if ((why & ST_NEAR) && (e = near_refresh())) return e;
+            if ((why & ST_WLIST) && (e = build_wall_list())) return e;
+        }
+        if ((e = clear_halt())) return e;
+        return enqueue_tail(h, rec && h == end - 1, h + 1 < end);
+    }
+
     int step(int nsteps) override {
         if (!uploaded) return fail(NBC_E_BADINPUT, "nbc_step: no state uploaded");
         HIPC(hipSetDevice(device));
@@ -845,8 +874,10 @@ class Engine : public EngineBase {
         const long long end = nstep + nsteps;
         const bool rec = record_next;
         int e;
-        while (n < end) {
-            if (first || init_update) {
+        std::vector<Chunk> q;  // chunks in flight, oldest first
+        int next_slot = 0;
+        while (n < end || !q.empty()) {
+            if (n < end && (first || init_update)) {
                 // the first step always rebuilds (src/dem_simulation_clump.cu:124)
                 if ((e = enqueue_head(n))) return e;
                 if ((e = rebuild((int)(n & 1)))) return e;
@@ -856,33 +887,33 @@ class Engine : public EngineBase {
                 ++n;
                 continue;
             }
-            const long long m = std::min<long long>(batch, end - n);
-            for (long long k = 0; k < m; ++k) {
-                if ((e = enqueue_head(n + k))) return e;
-                if ((e = enqueue_tail(n + k, rec && n + k == end - 1, n + k + 1 < end))) return e;
-            }
-            if ((e = read_status())) return e;
-            if ((e = check_status())) return e;
-            const int h = (int)hstatus[0];
-            if (h == HALT_NONE) {
+            if (n < end && (int)q.size() < kDepth) {
+                const long long m = std::min<long long>(batch, end - n);
+                for (long long k = 0; k < m; ++k) {
+                    if ((e = enqueue_head(n + k))) return e;
+                    if ((e = enqueue_tail(n + k, rec && n + k == end - 1, n + k + 1 < end))) return e;
+                }
+                const int slot = next_slot;
+                next_slot = (next_slot + 1) % kRing;
+                HIPC(hipMemcpyAsync(hring + 4 * slot, dstatus, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                HIPC(hipEventRecord(ev_stat[slot], st));
+                q.push_back(Chunk{slot});
                 n += m;
                 continue;
             }
-            // steps n .. h-1 completed; step h stopped after k_sphere_pose for
-            // a rebuild, or for a new wall list only (the K1 of step h + 1
-            // fused into the K4 of step h did not run either)
-            prof_drop(h);
-            k1_next = false;
-            const uint32_t why = hstatus[3];
-            if ((why & ST_REBUILD) || !(why & (ST_WLIST | ST_NEAR))) {
-                if ((e = rebuild(h & 1))) return e;
-            } else {
-                if ((why & ST_NEAR) && (e = near_refresh())) return e;
-                if ((why & ST_WLIST) && (e = build_wall_list())) return e;
-            }
-            if ((e = clear_halt())) return e;
-            if ((e = enqueue_tail(h, rec && h == end - 1, (long long)h + 1 < end))) return e;
-            n = (long long)h + 1;
+            // the oldest chunk in flight
+            const Chunk c = q.front();
+            q.erase(q.begin());
+            HIPC(hipEventSynchronize(ev_stat[c.slot]));
+            const uint32_t* sw = hring + 4 * c.slot;
+            if (sw[0] == (uint32_t)HALT_NONE && sw[1] == (uint32_t)HALT_NONE) continue;
+            // a halt or a bad sphere: let the queued (no-op) chunks finish
+            q.clear();
+            if ((e = read_status())) return e;
+            if ((e = check_status())) return e;
+            const long long h = (long long)(int)hstatus[0];
+            if ((e = handle_halt(h, end, rec))) return e;
+            n = h + 1;
         }
         nstep = end;
         if ((e = read_status())) return e;
@@ -1249,6 +1280,12 @@ class Engine : public EngineBase {
     int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;
     uint32_t* dstatus = nullptr;  // [0] halt step, [1] smallest bad sphere, [2] overflow, [3] clump state not finite
     uint32_t* hstatus = nullptr;
+    // status words copied back after every chunk of steps (pinned ring) and
+    // the events that mark each copy complete
+    static constexpr int kDepth = 3;            // chunks in flight
+    static constexpr int kRing = kDepth + 1;
+    uint32_t* hring = nullptr;
+    hipEvent_t ev_stat[kRing] = {};
     unsigned long long* counters = nullptr;
     PeneState* pst = nullptr;
     double *pene_partial = nullptr, *cache_f = nullptr, *cache_t = nullptr, *ke_partial = nullptr, *dtri = nullptr;
