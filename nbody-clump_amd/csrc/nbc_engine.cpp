@@ -1,15 +1,17 @@
 // Device context and step orchestration behind the nbc C ABI.
 //
 // The loop body of src/dem_simulation_clump.cu:110-226 becomes, per step:
-//   K1 k_clump_before, K2 k_sphere_pose (sets the rebuild flag),
-//   k_walls (wall list, side stream) beside k_contacts (sphere-sphere contacts
-//   and per-sphere totals), k_pene_finalize (penetrator runs only),
-//   K4 k_clump_after.
-// Steps are enqueued in batches; one status read-back per batch reports the
-// first step that needs a rebuild (or a new wall list), which is then done
-// and the step replayed from its contact kernels, and the invalid-input index
-// of the reference's fatal wall checks (src/dem_force_cube_wall_clump.cu:322-331),
-// which surfaces as NBC_E_NONFINITE.
+//   K2 k_sphere_pose (sets the rebuild / near-list flags),
+//   k_contacts (sphere-sphere contacts and per-sphere totals; the wall-list
+//   spheres' wall and penetrator contacts in extra blocks of the same launch),
+//   k_pene_finalize (penetrator runs only),
+//   K4 k_clump_after, which also runs K1 (k_clump_before) of the next step
+//   inside one nbc_step call.
+// Steps are enqueued in chunks; status words copied back after each chunk
+// report the first step that needs a rebuild (or new near or wall lists),
+// which is then done and the step replayed from its contact kernel, and the
+// invalid-input index of the reference's fatal wall checks
+// (src/dem_force_cube_wall_clump.cu:322-331), which surfaces as NBC_E_NONFINITE.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,10 +80,6 @@ class Engine : public EngineBase {
         device = dev;
         HIPC(hipSetDevice(device));
         HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        // side stream (default priority) for the wall kernel, which overlaps k_contacts
-        HIPC(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
-        HIPC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        HIPC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         N = P.ParticleNum;
         Nc = P.ClumpNum;
         if (N <= 0 || Nc <= 0 || P.BranchLmtLen < 2 || P.CellNum <= 0)
@@ -302,13 +300,8 @@ class Engine : public EngineBase {
             if (ev_stat[k]) (void)hipEventDestroy(ev_stat[k]);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
         ev_pool.clear();
-        if (st2) (void)hipStreamSynchronize(st2);
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
-        st = st2 = nullptr;
-        ev_fork = ev_join = nullptr;
+        st = nullptr;
         device = -1;
     }
 
@@ -776,22 +769,10 @@ class Engine : public EngineBase {
         L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
         const int* halt = (const int*)dstatus;
         int* bad = (int*)(dstatus + 1);
-        hipEvent_t e0;
-        if (wl_len > 0) {
-            // k_walls on the side stream, overlapping k_contacts
-            HIPC(hipEventRecord(ev_fork, st));
-            HIPC(hipStreamWaitEvent(st2, ev_fork, 0));
-            e0 = prof_begin(st2);
-            KC(launch_walls<R>(st2, dp, S[scur], C, wb, wl_len, pst, pene_partial, counters, bad, rec, record, halt,
-                               (int)n));
-            prof_end(5, e0, n, st2);
-            HIPC(hipEventRecord(ev_join, st2));
-        }
-        e0 = prof_begin();
-        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, wb, counters, bad, rec, record, block, 1, halt, (int)n));
+        hipEvent_t e0 = prof_begin();
+        KC(launch_contacts<R>(st, dp, S[scur], C, L, p, wb, wl_len, pst, pene_partial, counters, bad, rec, record, 1,
+                              halt, (int)n));
         prof_end(1, e0, n);
-        // the penetrator sum and k_clump_after (which finishes the wall-list spheres) wait for k_walls
-        if (wl_len > 0) HIPC(hipStreamWaitEvent(st, ev_join, 0));
         if (dp.pene_on) {
             e0 = prof_begin();
             KC(launch_pene_finalize(st, dp, pst, pene_partial, (wl_len + 63) / 64, (int)(n % oi), cache_f, cache_t,
@@ -1248,8 +1229,7 @@ class Engine : public EngineBase {
     nbc_params P;
     nbc_options O;
     DevParams dp;
-    hipStream_t st = nullptr, st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t st = nullptr;
     int N = 0, Nc = 0, block = 256, rbits = 8, wall_type = 0, nfaces = 0, nwh = 0, oi = 1, nbk = 1, batch = 64;
     int pair_cap_per = 16;
     SphereBuf<R> S[2] = {};

@@ -117,14 +117,13 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum
                       int block);
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
                      int block);
+// Sphere-sphere contacts of every sphere and, in the same launch, the wall and
+// penetrator contacts of the wlen spheres on the wall list.
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block, int remap,
-                    const int* halt, int step);
-template <class R>
-int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
-                 PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                 const int* halt, int step);
+                    WallBuf<R> WL, int wlen, PeneState* pst, double* partial, unsigned long long* ctr, int* bad,
+                    RecordBuf rec, bool record, int remap, const int* halt, int step);
+
 int launch_wlist_build(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* wflag, PeneState* pst,
                        int* cnt, int* off, int* list, int* pos_of, int* stmp, int* stotal, int block);
 template <class R>

@@ -1,22 +1,25 @@
 // HIP kernels of the clumped-sphere DEM timestep for gfx950 (MI355X).
 //
-// One steady-state step is five launches (k_walls on a side stream):
-//   k_clump_before   per clump   FP64 half-kick, drift, quaternion update
-//                                (src/dem_integrate_clumpsphere.cu:3-90)
-//   k_sphere_pose    per sphere  FP64 sphere pose, displacement, rebuild flag
-//                                (src/dem_clump.cu:113-138)
-//   k_walls          per wall-list sphere  wall and penetrator contacts
+// One steady-state step is three launches:
+//   k_sphere_pose    per sphere  FP64 sphere pose, displacement, rebuild and
+//                                near-list flags (src/dem_clump.cu:113-138)
+//   k_contacts_f32 / k_contacts_f64
+//                    per sphere  sphere-sphere contacts, totF/totT, clump torque
+//                                (src/dem_force_clump.cu:3-191,
+//                                src/dem_clump.cu:89-101,
+//                                src/dem_clumpsphere.cu:86-107); extra blocks
+//                                of the same launch take the wall-list spheres'
+//                                wall and penetrator contacts
 //                                (src/dem_force_cube_wall_clump.cu:16-298,
 //                                src/dem_force_cylind_wall.cu:152-237,
 //                                src/dem_contact_penetrator.cu:8-84,
 //                                src/dem_force_penetrator_clump.cu:4-182)
-//   k_contacts       per sphere  sphere-sphere contacts, totF/totT, clump torque
-//                                (src/dem_force_clump.cu:3-191,
-//                                src/dem_clump.cu:89-101,
-//                                src/dem_clumpsphere.cu:86-107)
 //   k_clump_after    per clump   deterministic member sum + second half-kick
 //                                (src/dem_clumpsphere.cu:110-151,
-//                                src/dem_integrate_clumpsphere.cu:110-141)
+//                                src/dem_integrate_clumpsphere.cu:110-141),
+//                                then the next step's half-kick, drift and
+//                                quaternion update (k_clump_before,
+//                                src/dem_integrate_clumpsphere.cu:3-90)
 // plus k_pene_finalize (one block) when a penetrator is present.
 //
 // A rebuild (hash, stable LSD radix sort, cell bounds, gather, two-pass
@@ -1260,11 +1263,11 @@ __device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneSta
 // which is its slot order: front entries ascending, then back entries in
 // back-row order.
 template <class R, bool RECORD>
-__global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                                                  WallBuf<R> WL, unsigned long long* counters, int* bad,
-                                                  RecordBuf rec, int remap, const int* halt, int n) {
+__device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
+                                              const ListBuf<R>& L, int p, const WallBuf<R>& WL,
+                                              unsigned long long* counters, int* bad, const RecordBuf& rec, int remap,
+                                              const int* halt, int vblock, int nvblocks) {
     typedef typename Vec4T<R>::type R4;
-    if (*halt <= n) return;
     const int* status = halt + 3;  // ST_* bits
     constexpr int Q = 128;
     __shared__ int q_e[4][Q];    // front near index of the pair; -1 once found not touching
@@ -1274,7 +1277,7 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
     __shared__ int s_fo[4][65], s_bo[4][65];
     __shared__ F4 s_p32[4][64];
     __shared__ unsigned int s_cnt[4];
-    const int lb = remap ? xcd_group_block(blockIdx.x, gridDim.x, 16) : (int)blockIdx.x;
+    const int lb = remap ? xcd_group_block(vblock, nvblocks, 16) : vblock;
     const int i = lb * blockDim.x + threadIdx.x;
     const bool active = i < P.N;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1529,13 +1532,12 @@ __global__ __launch_bounds__(256) void k_contacts(DevParams P, SphereBuf<R> S, C
 // lane per list entry; the list is in sphere order, so the per-wave FP64
 // penetrator partial sums keep the order of the sphere index.
 template <class R, bool RECORD>
-__global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
-                                               PeneState* pst, double* pene_partial, unsigned long long* counters,
-                                               int* bad, RecordBuf rec, const int* halt, int n) {
+__device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
+                                           const WallBuf<R>& WL, int len, PeneState* pst, double* pene_partial,
+                                           unsigned long long* counters, int* bad, const RecordBuf& rec, int vblock) {
     typedef typename Vec4T<R>::type R4;
-    if (*halt <= n) return;  // a rebuild of step n comes first
     const int lane = threadIdx.x & 63;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = vblock * 256 + threadIdx.x;
     const int gw = k >> 6;  // wave index: penetrator partial slot
     const bool active = k < len;
     int n_w = 0;
@@ -1790,6 +1792,43 @@ __global__ __launch_bounds__(256) void k_walls(DevParams P, SphereBuf<R> S, Clum
         if (P.pene_on)
             for (int q = 0; q < 6; ++q) pene_partial[6 * (size_t)gw + q] = v[q];
     }
+}
+
+// K3 of a step: the sphere-sphere contacts, and, when the wall list is not
+// empty, the wall and penetrator contacts of the listed spheres in the same
+// launch (the first nwb blocks take the wall list; both parts only read what
+// the other writes after k_clump_after). The FP32 form asks for five waves per
+// SIMD, which the contact blocks need and the rarer wall blocks pay for with
+// a few spills.
+template <class R, bool RECORD>
+__device__ __forceinline__ void step3_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
+                                           const ListBuf<R>& L, int p, const WallBuf<R>& WL, int len, int nwb,
+                                           PeneState* pst, double* pene_partial, unsigned long long* counters,
+                                           int* bad, const RecordBuf& rec, int remap, const int* halt, int n) {
+    if (*halt <= n) return;  // a rebuild of step n comes first
+    if ((int)blockIdx.x < nwb)
+        walls_body<R, RECORD>(P, S, C, WL, len, pst, pene_partial, counters, bad, rec, blockIdx.x);
+    else
+        contacts_body<R, RECORD>(P, S, C, L, p, WL, counters, bad, rec, remap, halt, (int)blockIdx.x - nwb,
+                                 (int)gridDim.x - nwb);
+}
+
+template <bool RECORD>
+__global__ __launch_bounds__(256, 5) void k_contacts_f32(DevParams P, SphereBuf<float> S, ClumpBuf<float> C,
+                                                         ListBuf<float> L, int p, WallBuf<float> WL, int len, int nwb,
+                                                         PeneState* pst, double* pene_partial,
+                                                         unsigned long long* counters, int* bad, RecordBuf rec,
+                                                         int remap, const int* halt, int n) {
+    step3_body<float, RECORD>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
+}
+
+template <bool RECORD>
+__global__ __launch_bounds__(256) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
+                                                      ListBuf<double> L, int p, WallBuf<double> WL, int len, int nwb,
+                                                      PeneState* pst, double* pene_partial,
+                                                      unsigned long long* counters, int* bad, RecordBuf rec,
+                                                      int remap, const int* halt, int n) {
+    step3_body<double, RECORD>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -2216,26 +2255,25 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
 int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
-                    WallBuf<R> WL, unsigned long long* ctr, int* bad, RecordBuf rec, bool record, int block, int remap,
-                    const int* halt, int step) {
-    const unsigned nb = nblk(P.N, block);
-    if (record)
-        k_contacts<R, true><<<nb, block, 0, st>>>(P, S, C, L, p, WL, ctr, bad, rec, remap, halt, step);
-    else
-        k_contacts<R, false><<<nb, block, 0, st>>>(P, S, C, L, p, WL, ctr, bad, rec, remap, halt, step);
-    LAUNCH_CHECK();
-    return 0;
-}
-template <class R>
-int launch_walls(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, int len,
-                 PeneState* pst, double* partial, unsigned long long* ctr, int* bad, RecordBuf rec, bool record,
-                 const int* halt, int step) {
-    if (len <= 0) return 0;
-    const unsigned nb = nblk(len, 256);
-    if (record)
-        k_walls<R, true><<<nb, 256, 0, st>>>(P, S, C, WL, len, pst, partial, ctr, bad, rec, halt, step);
-    else
-        k_walls<R, false><<<nb, 256, 0, st>>>(P, S, C, WL, len, pst, partial, ctr, bad, rec, halt, step);
+                    WallBuf<R> WL, int wlen, PeneState* pst, double* partial, unsigned long long* ctr, int* bad,
+                    RecordBuf rec, bool record, int remap, const int* halt, int step) {
+    const int nwb = wlen > 0 ? (int)nblk(wlen, 256) : 0;
+    const unsigned nb = nblk(P.N, 256) + (unsigned)nwb;
+    if constexpr (sizeof(R) == sizeof(float)) {
+        if (record)
+            k_contacts_f32<true><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
+                                                     halt, step);
+        else
+            k_contacts_f32<false><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
+                                                      halt, step);
+    } else {
+        if (record)
+            k_contacts_f64<true><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
+                                                     halt, step);
+        else
+            k_contacts_f64<false><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
+                                                      halt, step);
+    }
     LAUNCH_CHECK();
     return 0;
 }
@@ -2280,10 +2318,8 @@ int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
     template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
-                                    WallBuf<R>, unsigned long long*, int*, RecordBuf, bool, int, int, const int*,    \
-                                    int);                                                                            \
-    template int launch_walls<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>, int,          \
-                                 PeneState*, double*, unsigned long long*, int*, RecordBuf, bool, const int*, int);  \
+                                    WallBuf<R>, int, PeneState*, double*, unsigned long long*, int*, RecordBuf,      \
+                                    bool, int, const int*, int);                                                     \
     template int launch_clump_after<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, WallBuf<R>,         \
                                        RecordBuf, bool, const int*, const int*, const int*, int, int*, int, bool,     \
                                        PeneState*, unsigned long long*);
