@@ -9,7 +9,8 @@ of the whole bed (kinematics, rebuild when flagged, contacts, walls, clump
 reduction, integration) with all state resident in HBM.
 
 value: sphere-steps/s of the FP32-mixed form; the plain FP64 form runs the
-same steps beside it. roofline: the contact kernel (k_contacts), the dominant
+same steps beside it. roofline: the contact kernel (k_contacts_f32, which also
+takes the wall-list spheres' wall contacts in extra blocks), the dominant
 kernel, with algorithmic bytes per launch
     N * 72 B + C_ss * 80 B + C_w * 72 B
 (per-array accounting of the force inputs/outputs; C_ss / C_w = touching
@@ -51,7 +52,7 @@ def run_form(bed, precision, steps, warmup, prof_steps):
     # (the events add gaps between kernels, so the throughput region runs
     # without them)
     ctx.profile(True)
-    ctx.step(prof_steps)
+    ctx.step(max(prof_steps, 1))
     ctx.sync()
     s2 = ctx.stats()
     k3_ms, k3_n = ctx.profile_get(1)
@@ -231,6 +232,21 @@ def run_c4(a):
     }))
 
 
+def profiled_traffic():
+    """HBM bytes per k_contacts_f32 launch on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by what the
+    counter reads for a copy of known size). None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_contacts_f32_c1_mixed.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of known size: reading its FETCH_SIZE and
     WRITE_SIZE counters gives the factor that converts those counters into HBM
@@ -360,9 +376,11 @@ def main():
             "near_entries_per_sphere": mixed["near_pairs"] / N, "near_refreshes_in_timed_steps": mixed["near_refreshes"],
         },
         "roofline": {
-            "kernel": "k_contacts",
+            "kernel": "k_contacts_f32",
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
+            "frac": achieved / HBM_PEAK, "traffic": profiled_traffic(),
+            "traffic_source": "profiles/k_contacts_f32_c1_mixed.md (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                              "copy-calibrated)",
             "avg_launch_us": k3_avg * 1e6, "algorithmic_bytes_per_launch": k3_bytes,
             "avg_us": {"k_contacts (+ wall-list blocks)": t_contacts * 1e6,
                        "k_sphere_pose": avg(mixed, "cls0") * 1e6,
