@@ -146,6 +146,52 @@ def run_c2(a):
     }))
 
 
+def run_c0(a):
+    """The reference's small CPU-runnable case (not the headline line): 2,000
+    clumps of 3 collinear spheres (6,000 spheres) in a 40 x 40 x 80 mm cube,
+    dt 1e-6 s, FP64 and FP32-mixed on the GPU next to the FP64 oracle on the
+    host cores."""
+    import nbc
+
+    bed = nbc.generate_bed(0, 2000, seed=1)
+    out = {}
+    for prec, name in ((nbc.NBC_FP64, "fp64"), (nbc.NBC_MIXED, "mixed")):
+        opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CUBE)
+        ctx = nbc.Context(bed.finalized_params(opt), opt)
+        ctx.upload(bed, bed.wall)
+        ctx.step(a.warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.steps)
+        t1 = time.perf_counter()
+        st = ctx.stats()
+        out[name] = dict(sphere_steps_per_s=bed.n * a.steps / (t1 - t0), us_per_step=(t1 - t0) / a.steps * 1e6,
+                         rebuilds=st.rebuilds, touching_ss=st.touching_ss)
+        print(f"[bench] C0 {name}: {out[name]}", file=sys.stderr, flush=True)
+        ctx.close()
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+
+        orc.set_threads(os.cpu_count() or 1)
+        o = orc.Oracle(orc.finalize_params(bed), nbc.make_options(wall_type=nbc.NBC_WALL_CUBE))
+        o.upload(bed, bed.wall)
+        o.step(a.warmup)
+        n = min(a.steps, 2000)
+        t0 = time.perf_counter()
+        o.step(n)
+        out["cpu_oracle_fp64"] = dict(sphere_steps_per_s=bed.n * n / (time.perf_counter() - t0), cores=orc.threads())
+        o.close()
+    m = out["mixed"]
+    print(json.dumps({
+        "metric": "sphere-steps/s, C0 small bed (FP32-mixed)", "value": m["sphere_steps_per_s"],
+        "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": m["us_per_step"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator, seed 1)",
+        "config": {"workload": "C0: 2,000 clumps of 3 collinear spheres, cube 40x40x80 mm, dt 1e-6 s",
+                   "n_spheres": bed.n}, "forms": out}))
+
+
 def c4_beds(nbeds, seed0=1000, threads=16):
     """nbeds C1-style beds, clump counts uniform in [2000, 60000], cylinder
     radius scaled with count (60 mm at 200,000 clumps), seed seed0 + bed;
@@ -296,9 +342,9 @@ def main():
     ap.add_argument("--settle-dt", type=float, default=5e-6)
     ap.add_argument("--bed-cache", default=None,
                     help="load the settled bed from this .npz if it exists, else build it and save it there")
-    ap.add_argument("--workload", choices=("c1", "c2", "c4"), default="c1",
-                    help="c1: the headline bed (default); c2: polydisperse bed on the hierarchical grid; "
-                         "c4: batched beds")
+    ap.add_argument("--workload", choices=("c0", "c1", "c2", "c4"), default="c1",
+                    help="c1: the headline bed (default); c0: the small cube bed; c2: polydisperse bed on the "
+                         "hierarchical grid; c4: batched beds")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
     ap.add_argument("--prof-steps", type=int, default=300,
                     help="steps of the second timed region, with HIP events around every kernel")
@@ -322,6 +368,9 @@ def main():
         return
     if a.workload == "c4":
         run_c4(a)
+        return
+    if a.workload == "c0":
+        run_c0(a)
         return
     if a.bed_cache and os.path.exists(a.bed_cache):
         bed = nbc.Bed.load(a.bed_cache)

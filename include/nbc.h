@@ -119,9 +119,10 @@ typedef struct nbc_options {
     int32_t block_size;      /* threads per block for per-sphere kernels (64..1024) */
     int32_t radix_bits;      /* LSD radix digit width for the cell sort (1..8) */
     int32_t pair_capacity;   /* front-pair slots reserved per sphere (average) */
-    int32_t batch_steps;     /* steps enqueued per host round trip (default 64) */
+    int32_t batch_steps;     /* steps per chunk between status checks (default 16 with graphs, else 4) */
     int32_t hgrid_levels;    /* > 1: hierarchical grid with this many levels   */
-    int32_t reserved_i;
+    int32_t graphs;          /* steady chunks as HIP graph replays: 0 auto (beds up to 262144
+                                spheres), 1 on, -1 off */
     double pene_frcoe;       /* penetrator friction; <= 0 means Frcoe          */
     double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
     double near_skin;        /* near-list skin as a fraction of Thresh (0: 0.25; >= 1: gap 2*Thresh,
@@ -195,6 +196,7 @@ typedef struct nbc_stats {
     double max_cum_disp;     /* reserved */
     int64_t near_refreshes;  /* near-list refreshes between rebuilds since upload */
     int64_t near_pairs;      /* pairs on the contact kernel's near list         */
+    int64_t graph_replays;   /* chunks of steps run as HIP graph replays since upload */
 } nbc_stats;
 
 typedef struct nbc_ctx nbc_ctx;
@@ -346,6 +348,29 @@ int nbc_get_pair_exports(nbc_ctx* ctx, double* fcn, double* cpt);
  * penetrator contacts (side stream), 6 near-list refresh between rebuilds. */
 int nbc_profile_enable(nbc_ctx* ctx, int32_t on);
 int nbc_profile_get(nbc_ctx* ctx, int32_t kernel_class, double* total_ms, int64_t* launches);
+
+/* ---- autotuner --------------------------------------------------------- */
+/* Picks options for one bed by timing candidates on the device: coordinate
+ * descent over near_skin, batch_steps, radix_bits and block_size (none of them
+ * changes any result), then, when asked, the cell size (1, 1.25 or 1.5 times
+ * 3*max radius) and the hierarchical grid (off, 2 or 3 levels), which change
+ * the sphere order and so the FP64 sums by rounding. Each candidate gets its
+ * own context: upload, warm-up steps (the first rebuild included), then
+ * `steps` timed steps. params are world.par values NOT yet finalized (every
+ * candidate finalizes its own copy); base gives precision, wall type and
+ * penetrator friction. best receives the fastest options; entries (may be
+ * NULL) receives up to cap measured candidates and *n_entries their count.
+ * No counterpart in the reference, whose launch shapes are fixed. */
+#define NBC_TUNE_CELL 1
+#define NBC_TUNE_HGRID 2
+typedef struct nbc_tune_entry {
+    nbc_options opt;
+    double ms_per_step;      /* < 0: the candidate failed (e.g. neighbour overflow) */
+} nbc_tune_entry;
+int nbc_autotune(const nbc_params* params, const nbc_options* base, const nbc_spheres_in* s,
+                 const nbc_clumps_in* c, const nbc_wall* wall, const nbc_penetrator* pene, int32_t device,
+                 int32_t steps, int32_t flags, nbc_options* best, nbc_tune_entry* entries, int32_t cap,
+                 int32_t* n_entries);
 
 /* ---- binary writers (src/dem_binary_io.h layout, native little-endian) */
 int nbc_write_particles_bin(const char* dir, int32_t step, int32_t n, const int32_t* sorted_id,

@@ -135,6 +135,9 @@ struct DevParams {
     int nbeds;
     const BedGeo* beds;
     const int* bid;
+    // graph replays: the step kernels' step argument is an offset from *sbase
+    // (read when the kernel runs); nullptr: the argument is the step itself
+    const int* sbase;
     // penetrator (static part)
     int pshape, pmotion, pwait, ntri;
     double prad, pheight, pdir[3], pmass;

@@ -110,7 +110,11 @@ class Engine : public EngineBase {
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
         nfaces = wall_type == NBC_WALL_CUBE ? 6 : (wall_type == NBC_WALL_CYLINDER ? 3 : 0);
         pair_cap_per = O.pair_capacity > 0 ? O.pair_capacity : 16;
-        batch = O.batch_steps > 0 ? O.batch_steps : 4;
+        // small beds are launch-bound: their steady chunks replay a captured
+        // graph of 16 steps; large beds keep 4-step eager chunks (a halt then
+        // wastes fewer no-op steps)
+        use_graphs = O.graphs > 0 || (O.graphs == 0 && P.ParticleNum <= 262144);
+        batch = O.batch_steps > 0 ? O.batch_steps : (use_graphs ? 16 : 4);
         // device parameters
         std::memset(&dp, 0, sizeof(dp));
         dp.N = N; dp.Nc = Nc; dp.B = P.BranchLmtLen; dp.CellNum = P.CellNum;
@@ -207,7 +211,8 @@ class Engine : public EngineBase {
         HIPC(dalloc(&cache_f, 3 * (size_t)oi)); HIPC(dalloc(&cache_t, 3 * (size_t)oi));
         HIPC(dalloc(&ke_partial, (Nc + 255) / 256));
         HIPC(hipHostMalloc((void**)&hstatus, 4 * sizeof(uint32_t), hipHostMallocDefault));
-        HIPC(hipHostMalloc((void**)&hring, 4 * sizeof(uint32_t) * kRing, hipHostMallocDefault));
+        HIPC(hipHostMalloc((void**)&hring, 4 * sizeof(uint32_t) * (kRing + 1), hipHostMallocDefault));
+        HIPC(dalloc(&dsbase, 1));
         for (int k = 0; k < kRing; ++k) HIPC(hipEventCreateWithFlags(&ev_stat[k], hipEventDisableTiming));
         std::memset(&rec, 0, sizeof(rec));
         return 0;
@@ -296,6 +301,9 @@ class Engine : public EngineBase {
         F(dbeds); F(dbid); F(nslack);
         if (hstatus) (void)hipHostFree(hstatus);
         if (hring) (void)hipHostFree(hring);
+        for (int k = 0; k < 2; ++k)
+            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+        F(dsbase);
         for (int k = 0; k < kRing; ++k)
             if (ev_stat[k]) (void)hipEventDestroy(ev_stat[k]);
         for (auto& ev : ev_pool) (void)hipEventDestroy(ev);
@@ -580,9 +588,11 @@ class Engine : public EngineBase {
         init_update = P.Update != 0.0;
         first = true;
         k1_next = false;
+        gen++;
         nstep = 0;
         nrebuild = 0;
         nnear = 0;
+        nreplay = 0;
         count = 0;
         uploaded = true;
         HIPC(hipDeviceSynchronize());
@@ -592,6 +602,7 @@ class Engine : public EngineBase {
     // ------------------------------------------------------------------
     // Neighbour-list rebuild for step parity p.
     int rebuild(int p) {
+        gen++;
         SphereBuf<R>& so = S[scur];
         SphereBuf<R>& sn = S[scur ^ 1];
         const double t0 = prof ? 1.0 : 0.0;
@@ -708,6 +719,7 @@ class Engine : public EngineBase {
     }
 
     int ensure_record() {
+        gen++;
         if (!rec.tot) {
             HIPC(dalloc(&rec.tot, 2 * (size_t)N));
             HIPC(dalloc(&rec.parts, 6 * (size_t)N));
@@ -745,9 +757,9 @@ class Engine : public EngineBase {
     // K1 + K2 of step n (src/dem_simulation_clump.cu:113-119)
     int enqueue_head(long long n) {
         hipEvent_t e0 = prof_begin();
-        if (!k1_next) KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (int*)dstatus, (int)n));
+        if (!k1_next) KC(launch_clump_before<R>(st, dp, C, pst, counters, block, (int*)dstatus, karg(n)));
         k1_next = false;
-        KC(launch_sphere_pose<R>(st, dp, S[scur], C, (int*)dstatus, (int)n, block));
+        KC(launch_sphere_pose<R>(st, dp, S[scur], C, (int*)dstatus, karg(n), block));
         prof_end(0, e0, n);
         return 0;
     }
@@ -771,16 +783,16 @@ class Engine : public EngineBase {
         int* bad = (int*)(dstatus + 1);
         hipEvent_t e0 = prof_begin();
         KC(launch_contacts<R>(st, dp, S[scur], C, L, p, wb, wl_len, pst, pene_partial, counters, bad, rec, record, 1,
-                              halt, (int)n));
+                              halt, karg(n)));
         prof_end(1, e0, n);
         if (dp.pene_on) {
             e0 = prof_begin();
             KC(launch_pene_finalize(st, dp, pst, pene_partial, (wl_len + 63) / 64, (int)(n % oi), cache_f, cache_t,
-                                    halt, (int)n));
+                                    halt, karg(n)));
             prof_end(4, e0, n);
         }
         e0 = prof_begin();
-        KC(launch_clump_after<R>(st, dp, S[scur], C, wb, rec, record, mbeg, mend, mem, block, (int*)dstatus, (int)n,
+        KC(launch_clump_after<R>(st, dp, S[scur], C, wb, rec, record, mbeg, mend, mem, block, (int*)dstatus, karg(n),
                                  next, pst, counters));
         prof_end(2, e0, n);
         k1_next = next;
@@ -795,6 +807,7 @@ class Engine : public EngineBase {
     // Near lists of the current Verlet list at the current positions (a
     // sphere left its near-list allowance between rebuilds).
     int near_refresh() {
+        gen++;
         hipEvent_t e0 = prof_begin();
         FrontList& L = Lf[lcur];
         KC(launch_near_sync(st, N, nfoff, nfw, nfs, L.fword, block));
@@ -810,6 +823,7 @@ class Engine : public EngineBase {
 
     // Wall list at the current positions and penetrator position.
     int build_wall_list() {
+        gen++;
         if (!(nfaces > 0 || dp.pene_on)) return 0;
         const SphereBuf<R>& s = S[scur];
         KC(launch_wlist_build(st, dp, s.pos, s.wflag, pst, fcnt, wl_off, wb.list, wb.pos_of, stmp, stotal + 2, block));
@@ -828,8 +842,46 @@ class Engine : public EngineBase {
     // chunk and read while up to kDepth chunks are in flight, so a halt wastes
     // at most a few no-op steps and the GPU never waits for the host.
     struct Chunk {
-        int slot;
+        int slot;  // event
+        int sw;    // status copy
     };
+
+    // Steady chunk of m steps from step n as a graph replay (captured on first
+    // use for this parity and generation): every step's K1 is fused into the
+    // previous step's k_clump_after, the last one included.
+    int launch_chunk_graph(long long n, long long m) {
+        const int par = (int)(n & 1);
+        if (!gexec[par] || ggen[par] != gen) {
+            if (gexec[par]) { (void)hipGraphExecDestroy(gexec[par]); gexec[par] = nullptr; }
+            DevParams saved = dp;
+            dp.sbase = dsbase;
+            graph_base = n;
+            HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int e = 0;
+            for (long long k = 0; k < m && !e; ++k) {
+                if (!e) e = enqueue_head(n + k);
+                if (!e) e = enqueue_tail(n + k, false, true);
+            }
+            hipError_t ce = e ? hipSuccess
+                              : hipMemcpyAsync(hring + 4 * kRing, dstatus, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+            hipGraph_t g = nullptr;
+            hipError_t ee = hipStreamEndCapture(st, &g);
+            graph_base = -1;
+            dp = saved;
+            if (e) { if (g) (void)hipGraphDestroy(g); return e; }
+            HIPC(ce);
+            HIPC(ee);
+            hipError_t ie = hipGraphInstantiate(&gexec[par], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPC(ie);
+            ggen[par] = gen;
+        }
+        KC(launch_set_int(st, dsbase, (int)n));
+        HIPC(hipGraphLaunch(gexec[par], st));
+        k1_next = true;
+        nreplay++;
+        return 0;
+    }
 
     // Rebuild, near-list refresh or wall list for the step h that stopped,
     // then the rest of step h. Requires an idle stream and fresh hstatus.
@@ -870,15 +922,21 @@ class Engine : public EngineBase {
             }
             if (n < end && (int)q.size() < kDepth) {
                 const long long m = std::min<long long>(batch, end - n);
-                for (long long k = 0; k < m; ++k) {
-                    if ((e = enqueue_head(n + k))) return e;
-                    if ((e = enqueue_tail(n + k, rec && n + k == end - 1, n + k + 1 < end))) return e;
-                }
                 const int slot = next_slot;
                 next_slot = (next_slot + 1) % kRing;
-                HIPC(hipMemcpyAsync(hring + 4 * slot, dstatus, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                int sw = slot;  // ring slot holding this chunk's status copy
+                if (use_graphs && !prof && !dp.pene_on && k1_next && m == batch && n + m < end) {
+                    if ((e = launch_chunk_graph(n, m))) return e;
+                    sw = kRing;
+                } else {
+                    for (long long k = 0; k < m; ++k) {
+                        if ((e = enqueue_head(n + k))) return e;
+                        if ((e = enqueue_tail(n + k, rec && n + k == end - 1, n + k + 1 < end))) return e;
+                    }
+                    HIPC(hipMemcpyAsync(hring + 4 * slot, dstatus, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                }
                 HIPC(hipEventRecord(ev_stat[slot], st));
-                q.push_back(Chunk{slot});
+                q.push_back(Chunk{slot, sw});
                 n += m;
                 continue;
             }
@@ -886,7 +944,7 @@ class Engine : public EngineBase {
             const Chunk c = q.front();
             q.erase(q.begin());
             HIPC(hipEventSynchronize(ev_stat[c.slot]));
-            const uint32_t* sw = hring + 4 * c.slot;
+            const uint32_t* sw = hring + 4 * c.sw;
             if (sw[0] == (uint32_t)HALT_NONE && sw[1] == (uint32_t)HALT_NONE) continue;
             // a halt or a bad sphere: let the queued (no-op) chunks finish
             q.clear();
@@ -1209,6 +1267,7 @@ class Engine : public EngineBase {
         s->touching_wall = (int64_t)ctr[1];
         s->near_refreshes = nnear;
         s->near_pairs = near_total;
+        s->graph_replays = nreplay;
         s->kinetic_energy = ke;
         for (int a = 0; a < 3; ++a) { s->pene_force[a] = -ps.totF[a]; s->pene_torque[a] = ps.totT[a]; }
         return 0;
@@ -1262,6 +1321,20 @@ class Engine : public EngineBase {
     uint32_t* hstatus = nullptr;
     // status words copied back after every chunk of steps (pinned ring) and
     // the events that mark each copy complete
+    // graph replays of steady chunks: one executable graph per parity of the
+    // chunk's first step, captured at generation ggen[par] of everything its
+    // kernel arguments point to (gen moves at every upload, rebuild, near or
+    // wall list build); the step base of a replay is written to dsbase by a
+    // one-thread kernel before it, and its status copy goes to the extra
+    // ring slot kRing (status words only accumulate until the host clears
+    // them, so a later chunk's copy is as good as this chunk's)
+    bool use_graphs = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    long long ggen[2] = {-1, -1};
+    long long gen = 0;
+    long long graph_base = -1;  // >= 0 while capturing: kernel step arguments are offsets from it
+    int* dsbase = nullptr;
+    int karg(long long n) const { return (int)(graph_base >= 0 ? n - graph_base : n); }
     static constexpr int kDepth = 3;            // chunks in flight
     static constexpr int kRing = kDepth + 1;
     uint32_t* hring = nullptr;
@@ -1280,7 +1353,7 @@ class Engine : public EngineBase {
     RecordBuf rec;
     bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;
     bool k1_next = false;  // the last enqueued k_clump_after also ran K1 of the next step
-    long long nstep = 0, nrebuild = 0, nnear = 0;
+    long long nstep = 0, nrebuild = 0, nnear = 0, nreplay = 0;
     double* nslack = nullptr;  // per-sphere near-list slack of the last near build
     int count = 0;
     double prof_ms[8] = {0};

@@ -66,6 +66,7 @@ struct RecordBuf {
     D4* pair_cpt;  // [front slots] contact point (CacheCtp)
 };
 
+int launch_set_int(hipStream_t st, int* a, int v);  // one-thread store (no memset: safe in graphs)
 int launch_clear_halt(hipStream_t st, int* status, int none);
 int launch_fill_i32(hipStream_t st, int* a, long long n, int v);
 int launch_fill_u32(hipStream_t st, uint32_t* a, long long n, uint32_t v);

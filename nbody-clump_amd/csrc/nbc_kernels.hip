@@ -47,6 +47,9 @@ namespace nbc {
 // ---------------------------------------------------------------------------
 // small helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+// Step index of a step kernel: the argument itself, or an offset from the base
+// a graph replay reads from device memory.
+__device__ __forceinline__ int step_of(const DevParams& P, int n) { return P.sbase ? *P.sbase + n : n; }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
 // Logical block index giving each XCD runs of G consecutive blocks, the runs
@@ -218,7 +221,8 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
 // ---------------------------------------------------------------------------
 // K2: sphere pose from the clump pose, cumulative displacement, rebuild flag.
 template <class R>
-__global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int n) {
+__global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int n_arg) {
+    const int n = step_of(P, n_arg);
     if (*halt < n) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false, near = false;
@@ -1804,8 +1808,8 @@ template <class R, bool RECORD>
 __device__ __forceinline__ void step3_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
                                            const ListBuf<R>& L, int p, const WallBuf<R>& WL, int len, int nwb,
                                            PeneState* pst, double* pene_partial, unsigned long long* counters,
-                                           int* bad, const RecordBuf& rec, int remap, const int* halt, int n) {
-    if (*halt <= n) return;  // a rebuild of step n comes first
+                                           int* bad, const RecordBuf& rec, int remap, const int* halt, int n_arg) {
+    if (*halt <= step_of(P, n_arg)) return;  // a rebuild of step n comes first
     if ((int)blockIdx.x < nwb)
         walls_body<R, RECORD>(P, S, C, WL, len, pst, pene_partial, counters, bad, rec, blockIdx.x);
     else
@@ -1943,8 +1947,9 @@ __device__ __forceinline__ void finish_listed(const DevParams& P, const SphereBu
 
 template <class R, bool NEXT>
 __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, RecordBuf rec, int record,
-                              const int* mbeg, const int* mend, const int* mem, int* halt, int n, PeneState* pst,
+                              const int* mbeg, const int* mend, const int* mem, int* halt, int n_arg, PeneState* pst,
                               unsigned long long* counters) {
+    const int n = step_of(P, n_arg);
     if (*halt <= n) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     // NEXT: K1 of step n + 1 follows in the same thread (it returns when K4 of
@@ -2041,10 +2046,18 @@ static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) 
         if (e_ != hipSuccess) return (int)e_;             \
     } while (0)
 
+__global__ void k_set_int(int* a, int v) { *a = v; }
+
 __global__ void k_clear_halt(int* status, int none) {
     status[0] = none;
     status[3] &= ST_CLUMP_BAD;
 }
+int launch_set_int(hipStream_t st, int* a, int v) {
+    k_set_int<<<1, 1, 0, st>>>(a, v);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_clear_halt(hipStream_t st, int* status, int none) {
     k_clear_halt<<<1, 1, 0, st>>>(status, none);
     LAUNCH_CHECK();

@@ -1,0 +1,108 @@
+// Autotuner: times candidate options for one bed through the public C ABI.
+//
+// Coordinate descent: each parameter in turn is set to every candidate value
+// with the others at their best so far; a candidate is a fresh context
+// (finalize, create, upload), warm-up steps including the first rebuild, then
+// `steps` timed steps closed by a stream synchronize.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/nbc.h"
+
+namespace nbc {
+void set_global_error(const std::string& m);
+}
+
+namespace {
+
+struct Bed {
+    const nbc_params* params;
+    const nbc_spheres_in* s;
+    const nbc_clumps_in* c;
+    const nbc_wall* wall;
+    const nbc_penetrator* pene;
+    int device;
+    int steps;
+};
+
+// ms per step of one candidate, < 0 when it fails
+double run_candidate(const Bed& b, const nbc_options& o) {
+    nbc_params p = *b.params;
+    if (nbc_finalize_params(&p, b.s->radius, b.s->n, &o) != 0) return -1.0;
+    nbc_ctx* ctx = nullptr;
+    if (nbc_create(&ctx, &p, &o, b.device) != 0) return -1.0;
+    double ms = -1.0;
+    const int warm = b.steps / 10 > 20 ? b.steps / 10 : 20;
+    if (nbc_upload(ctx, b.s, b.c, b.wall, b.pene) == 0 && nbc_step(ctx, warm) == 0 && nbc_synchronize(ctx) == 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (nbc_step(ctx, b.steps) == 0 && nbc_synchronize(ctx) == 0) {
+            const auto t1 = std::chrono::steady_clock::now();
+            ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / b.steps;
+        }
+    }
+    nbc_destroy(ctx);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" int nbc_autotune(const nbc_params* params, const nbc_options* base, const nbc_spheres_in* s,
+                            const nbc_clumps_in* c, const nbc_wall* wall, const nbc_penetrator* pene, int32_t device,
+                            int32_t steps, int32_t flags, nbc_options* best, nbc_tune_entry* entries, int32_t cap,
+                            int32_t* n_entries) {
+    if (!params || !s || !c || !best) {
+        nbc::set_global_error("nbc_autotune: null argument");
+        return NBC_E_BADINPUT;
+    }
+    if (params->finalized) {
+        nbc::set_global_error("nbc_autotune: pass world.par values before nbc_finalize_params");
+        return NBC_E_BADINPUT;
+    }
+    int32_t ndev = 0;
+    if (nbc_device_count(&ndev) != 0 || ndev <= 0) return NBC_E_NODEVICE;
+    Bed b{params, s, c, wall, pene, device, steps > 0 ? steps : 300};
+    nbc_options cur;
+    if (base) cur = *base; else std::memset(&cur, 0, sizeof(cur));
+    double rmax = 0.0;
+    for (int i = 0; i < s->n; ++i) rmax = std::max(rmax, s->radius[i]);
+    int count = 0;
+    double best_ms = -1.0;
+    auto record = [&](const nbc_options& o, double ms) {
+        if (entries && count < cap) { entries[count].opt = o; entries[count].ms_per_step = ms; }
+        ++count;
+    };
+    auto sweep = [&](auto set, const std::vector<double>& values) {
+        nbc_options pick = cur;
+        double pick_ms = best_ms;
+        for (double v : values) {
+            nbc_options o = cur;
+            set(o, v);
+            const double ms = run_candidate(b, o);
+            record(o, ms);
+            if (ms > 0.0 && (pick_ms < 0.0 || ms < pick_ms)) { pick = o; pick_ms = ms; }
+        }
+        cur = pick;
+        best_ms = pick_ms;
+    };
+    sweep([](nbc_options& o, double v) { o.near_skin = v; }, {0.125, 0.25, 0.5, 1.0});
+    sweep([](nbc_options& o, double v) { o.batch_steps = (int32_t)v; }, {2, 4, 8, 16});
+    sweep([](nbc_options& o, double v) { o.radix_bits = (int32_t)v; }, {4, 6, 8});
+    sweep([](nbc_options& o, double v) { o.block_size = (int32_t)v; }, {128, 256});
+    if (flags & NBC_TUNE_CELL)
+        sweep([rmax](nbc_options& o, double v) { o.cell_size = v * 3.0 * rmax; o.hgrid_levels = 0; },
+              {1.0, 1.25, 1.5});
+    if (flags & NBC_TUNE_HGRID)
+        sweep([](nbc_options& o, double v) { o.hgrid_levels = (int32_t)v; if (v > 1) o.cell_size = 0.0; },
+              {0, 2, 3});
+    if (n_entries) *n_entries = count;
+    if (best_ms < 0.0) {
+        nbc::set_global_error("nbc_autotune: no candidate ran");
+        return NBC_E_BADINPUT;
+    }
+    *best = cur;
+    return 0;
+}

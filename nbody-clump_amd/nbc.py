@@ -47,10 +47,17 @@ class Options(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("wall_type", C.c_int32), ("block_size", C.c_int32),
         ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
-        ("hgrid_levels", C.c_int32), ("reserved_i", C.c_int32),
+        ("hgrid_levels", C.c_int32), ("graphs", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
         ("reserved", C.c_double * 3),
     ]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("opt", Options), ("ms_per_step", C.c_double)]
+
+
+NBC_TUNE_CELL, NBC_TUNE_HGRID = 1, 2
 
 
 class Wall(C.Structure):
@@ -92,6 +99,7 @@ class Stats(C.Structure):
         ("back_entries", C.c_int64), ("touching_ss", C.c_int64), ("touching_wall", C.c_int64),
         ("kinetic_energy", C.c_double), ("pene_force", D3), ("pene_torque", D3),
         ("max_cum_disp", C.c_double), ("near_refreshes", C.c_int64), ("near_pairs", C.c_int64),
+        ("graph_replays", C.c_int64),
     ]
 
 
@@ -159,6 +167,9 @@ _SIGS = {
     "nbc_write_yi_bin": ([C.c_char_p, I32, I32, P, P, P, P, P], C.c_int),
     "nbc_write_contacts_bin": ([C.c_char_p, I32, I32, I32, P, P, P], C.c_int),
     "nbc_run_simulation_clump": ([C.c_char_p, I32], C.c_int),
+    "nbc_autotune": ([C.POINTER(Params), C.POINTER(Options), C.POINTER(SpheresIn), C.POINTER(ClumpsIn),
+                      C.POINTER(Wall), C.POINTER(Penetrator), I32, I32, I32, C.POINTER(Options),
+                      C.POINTER(TuneEntry), I32, C.POINTER(I32)], C.c_int),
 }
 
 _lib = None
@@ -288,12 +299,13 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0):
+                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0):
     o = Options()
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
     o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
     o.hgrid_levels = hgrid_levels
     o.near_skin = near_skin
+    o.graphs = graphs
     return o
 
 
@@ -491,6 +503,19 @@ class BatchContext(Context):
         so, co = np.zeros(self.nbeds + 1, np.int32), np.zeros(self.nbeds + 1, np.int32)
         self._c(lib().nbc_get_batch_ranges(self.h, ptr(so), ptr(co)))
         return so, co
+
+
+def autotune(bed, base=None, wall=None, pene=None, steps=300, flags=0, device=0):
+    """Fastest options for this bed (nbc_autotune): returns (options, [(options, ms_per_step), ...])."""
+    s, c = bed.spheres_in(), bed.clumps_in()
+    best = Options()
+    cap = 64
+    ent = (TuneEntry * cap)()
+    n = I32()
+    check(lib().nbc_autotune(C.byref(bed.params), C.byref(base) if base else None, C.byref(s), C.byref(c),
+                             C.byref(wall) if wall else None, C.byref(pene) if pene else None, device, steps, flags,
+                             C.byref(best), ent, cap, C.byref(n)))
+    return best, [(ent[k].opt, ent[k].ms_per_step) for k in range(min(n.value, cap))]
 
 
 def download_bed(ctx, template):

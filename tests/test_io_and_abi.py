@@ -227,3 +227,18 @@ def test_save_bed_text_roundtrip(tmp_path):
     pos, loc, m, r = np.zeros((a.n, 3)), np.zeros((a.n, 3)), np.zeros(a.n), np.zeros(a.n)
     assert nbc.lib().nbc_load_phase(str(tmp_path / ini).encode(), a.n, *[nbc.ptr(x) for x in (cid, pos, loc, m, r)]) == 0
     assert np.array_equal(pos, a.pos) and np.array_equal(r, a.radius) and np.array_equal(cid, a.clump_id)
+
+
+def test_autotune_argument_checks():
+    bed = nbc.generate_bed(0, 20, seed=2)
+    s, c = bed.spheres_in(), bed.clumps_in()
+    best = nbc.Options()
+    n = C.c_int32()
+    p = bed.finalized_params()
+    rc = nbc.lib().nbc_autotune(C.byref(p), None, C.byref(s), C.byref(c), None, None, 0, 10, 0, C.byref(best), None, 0,
+                                C.byref(n))
+    assert rc == nbc.NBC_E_BADINPUT and "before nbc_finalize_params" in nbc.lib().nbc_last_error(None).decode()
+    if nbc.device_count() == 0:
+        rc = nbc.lib().nbc_autotune(C.byref(bed.params), None, C.byref(s), C.byref(c), None, None, 0, 10, 0,
+                                    C.byref(best), None, 0, C.byref(n))
+        assert rc == nbc.NBC_E_NODEVICE
