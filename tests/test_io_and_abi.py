@@ -242,3 +242,34 @@ def test_autotune_argument_checks():
         rc = nbc.lib().nbc_autotune(C.byref(bed.params), None, C.byref(s), C.byref(c), None, None, 0, 10, 0,
                                     C.byref(best), None, 0, C.byref(n))
         assert rc == nbc.NBC_E_NODEVICE
+
+
+def test_nbc_par_keys(tmp_path):
+    f = tmp_path / "nbc.par"
+    f.write_text("Precision = mixed\nWallType = cylinder\nBlockSize = 128\nRadixBits = 6\nPairCapacity = 20\n"
+                 "PeneFrcoe = 0.4\nCellSize = 0.006\nHGridLevels = 3\nNearSkin = 0.5\nGraphs = -1\n"
+                 "BatchSteps = 8\n")
+    o = nbc.Options()
+    assert nbc.lib().nbc_load_options(str(f).encode(), C.byref(o)) == 0
+    assert (o.precision, o.wall_type, o.block_size, o.radix_bits, o.pair_capacity) == \
+        (nbc.NBC_MIXED, nbc.NBC_WALL_CYLINDER, 128, 6, 20)
+    assert (o.pene_frcoe, o.cell_size, o.hgrid_levels, o.near_skin, o.graphs, o.batch_steps) == \
+        (0.4, 0.006, 3, 0.5, -1, 8)
+    o2 = nbc.Options()
+    assert nbc.lib().nbc_load_options(str(tmp_path / "absent.par").encode(), C.byref(o2)) == 0
+    assert o2.precision == nbc.NBC_FP64 and o2.hgrid_levels == 0
+
+
+def test_c2_bed_text_asks_for_the_hierarchical_grid(tmp_path):
+    spec = nbc.BedSpec(2, 60, 1, nbc.D4(0.01, 0, 0, 0))
+    out = nbc.BedOut()
+    assert nbc.lib().nbc_generate_bed(C.byref(spec), C.byref(out)) == 0
+    try:
+        assert nbc.lib().nbc_save_bed_text(str(tmp_path).encode(), C.byref(out), None) == 0
+    finally:
+        nbc.lib().nbc_free_bed(C.byref(out))
+    o = nbc.Options()
+    assert nbc.lib().nbc_load_options(str(tmp_path / "nbc.par").encode(), C.byref(o)) == 0
+    assert o.hgrid_levels == 3 and o.wall_type == nbc.NBC_WALL_CUBE
+    rc, p, _ = load_params(str(tmp_path / "world.par"))
+    assert rc == 0 and p.GridLevels == 3 and p.Frcoe == 0.6
