@@ -205,7 +205,7 @@ class Engine : public EngineBase {
         nbk = contacts_blocks(N, block);
         HIPC(dalloc(&pene_partial, 6 * (((size_t)N + 63) / 64)));  // one per wave of k_walls
         if (nfaces > 0 || dp.pene_on) {
-            HIPC(dalloc(&wb.list, N)); HIPC(dalloc(&wb.pos_of, N)); HIPC(dalloc(&wb.res, 4 * (size_t)N));
+            HIPC(dalloc(&wb.list, N)); HIPC(dalloc(&wb.pos_of, N)); HIPC(dalloc(&wb.res, 5 * (size_t)N));
             HIPC(dalloc(&wl_off, (size_t)N + 1));
         }
         HIPC(dalloc(&cache_f, 3 * (size_t)oi)); HIPC(dalloc(&cache_t, 3 * (size_t)oi));

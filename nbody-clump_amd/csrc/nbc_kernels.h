@@ -55,7 +55,7 @@ template <class R>
 struct WallBuf {
     int* list;                        // [len] sphere index
     int* pos_of;                      // [N] index into list, or -1
-    typename Vec4T<R>::type* res;     // [4 * len] Fw, Tw, Fp, Tp
+    typename Vec4T<R>::type* res;     // [5 * N] Fw, Tw, Fp, Tp, clump torque arm, at the member slot
 };
 
 struct RecordBuf {

@@ -1777,11 +1777,21 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
             }
         }
         if (wf != wf0) S.wflag[i] = wf;
-        R4* o = WL.res + 4 * (size_t)k;
+        // results at the sphere's member slot, with the clump torque arm, so
+        // that k_clump_after finds them next to the sphere's contact totals
+        R4* o = WL.res + 5 * (size_t)S.slot[i];
         o[0] = mk4<R>(Fw.x, Fw.y, Fw.z, R(0));
         o[1] = mk4<R>(Tw.x, Tw.y, Tw.z, R(0));
         o[2] = mk4<R>(Fp.x, Fp.y, Fp.z, R(0));
         o[3] = mk4<R>(Tp.x, Tp.y, Tp.z, R(0));
+        V3<R> arm;
+        if (sizeof(R) == sizeof(double)) {
+            const D4 pcl = C.pq[2 * ci];
+            arm = cvt3<R>(mk<double>(pi4.x - pcl.x, pi4.y - pcl.y, pi4.z - pcl.z));
+        } else {
+            arm = Ld4<R>::xyz(ki);  // x_s - x_c = R(q) * local up to FP64 rounding
+        }
+        o[4] = mk4<R>(arm.x, arm.y, arm.z, R(0));
     }
     // wave reductions: touching count and the FP64 penetrator partial sums of
     // this wave's 64 entries (a fixed tree over the lanes)
@@ -1909,25 +1919,21 @@ __global__ __launch_bounds__(256) void k_pene_finalize(DevParams P, PeneState* p
 // K4: clump force/torque as the sum of its spheres in ascending sorted order
 // (FP64 accumulation), then the second half-kick.
 // A wall-list sphere's record holds its raw sphere-sphere sums (.w = 1 on the
-// torque): add the wall and penetrator terms of k_walls and the clump torque
-// term, in the order of the other spheres' totals.
+// torque): add the wall and penetrator terms the wall blocks stored at the
+// same member slot and the clump torque term, in the order of the other
+// spheres' totals.
 template <class R>
 __device__ __forceinline__ void finish_listed(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
-                                              const WallBuf<R>& WL, const RecordBuf& rec, int record, int m, int c,
-                                              typename Vec4T<R>::type& f, typename Vec4T<R>::type& t) {
-    const typename Vec4T<R>::type* w4 = WL.res + 4 * (size_t)WL.pos_of[m];
+                                              const WallBuf<R>& WL, const RecordBuf& rec, int record, int k,
+                                              const int* mem, int c, typename Vec4T<R>::type& f,
+                                              typename Vec4T<R>::type& t) {
+    const typename Vec4T<R>::type* w4 = WL.res + 5 * (size_t)k;  // member slot k
     const V3<R> Fss = Ld4<R>::xyz(f), Tss = Ld4<R>::xyz(t);
     const V3<R> Fw = Ld4<R>::xyz(w4[0]), Tw = Ld4<R>::xyz(w4[1]), Fp = Ld4<R>::xyz(w4[2]), Tp = Ld4<R>::xyz(w4[3]);
     const V3<R> tF = Fss + Fw + Fp;
     const V3<R> tT = Tss + Tw + Tp;
-    V3<R> r;
-    if (sizeof(R) == sizeof(double)) {
-        const D4 pi4 = S.pos[m];
-        const D4 pc = C.pq[2 * c];
-        r = cvt3<R>(mk<double>(pi4.x - pc.x, pi4.y - pc.y, pi4.z - pc.z));
-    } else {
-        r = Ld4<R>::xyz(S.kin[m]);  // x_s - x_c = R(q) * local up to FP64 rounding
-    }
+    const V3<R> r = Ld4<R>::xyz(w4[4]);
+    const int m = record ? mem[k] : 0;
     const V3<R> Tq = mk<R>(tT.x + (r.y * tF.z - r.z * tF.y), tT.y + (r.z * tF.x - r.x * tF.z),
                            tT.z + (r.x * tF.y - r.y * tF.x));
     f = mk4<R>(tF.x, tF.y, tF.z, R(0));
@@ -1938,9 +1944,9 @@ __device__ __forceinline__ void finish_listed(const DevParams& P, const SphereBu
         rec.tot[2 * (size_t)m] = a;
         rec.tot[2 * (size_t)m + 1] = b;
         const V3<R> parts[6] = {Fss, Tss, Fw, Tw, Fp, Tp};
-        for (int k = 0; k < 6; ++k) {
-            D4 q; q.x = parts[k].x; q.y = parts[k].y; q.z = parts[k].z; q.w = 0.0;
-            rec.parts[6 * (size_t)m + k] = q;
+        for (int u = 0; u < 6; ++u) {
+            D4 q; q.x = parts[u].x; q.y = parts[u].y; q.z = parts[u].z; q.w = 0.0;
+            rec.parts[6 * (size_t)m + u] = q;
         }
     }
 }
@@ -1982,7 +1988,7 @@ __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (k0 + u >= e) continue;
-                if (t[u].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, mem[k0 + u], c, f[u], t[u]);
+                if (t[u].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, k0 + u, mem, c, f[u], t[u]);
                 const V3<double> fu = mk<double>(f[u].x, f[u].y, f[u].z), tu = mk<double>(t[u].x, t[u].y, t[u].z);
                 if (k0 + u == b) { F = fu; T = tu; }
                 else { F = F + fu; T = T + tu; }

@@ -41,7 +41,7 @@ def test_graph_replays_match_eager_steps(prec):
         ca, cb = a.clumps(), b.clumps()
         assert np.array_equal(ca["vel"], cb["vel"]) and np.array_equal(ca["angv"], cb["angv"]), call
     sa, sb = ctxs[0].stats(), ctxs[1].stats()
-    assert sa.graph_replays == 0 and sb.graph_replays > 20
+    assert sa.graph_replays == 0 and sb.graph_replays > 5
     assert sb.rebuilds >= 2 and seen > 0
     for c in ctxs:
         c.close()
@@ -61,6 +61,6 @@ def test_graph_replays_in_a_batch():
         for c in ctxs:
             c.step(41)
         assert np.array_equal(ctxs[0].spheres()["pos"], ctxs[1].spheres()["pos"]), call
-    assert ctxs[1].stats().graph_replays > 10 and ctxs[1].stats().rebuilds >= 2
+    assert ctxs[1].stats().graph_replays > 3 and ctxs[1].stats().rebuilds >= 2
     for c in ctxs:
         c.close()
