@@ -192,6 +192,60 @@ def run_c0(a):
                    "n_spheres": bed.n}, "forms": out}))
 
 
+def run_c3(a):
+    """Penetrator case (not the headline line): a C1-style bed of 200,000
+    clumps deposited and settled in a cylinder of radius 80 mm, then a
+    triangulated 60-degree cone of base radius 10 mm driven down at 0.1 m/s
+    from first contact, penetrator friction 0.4, FP32-mixed; reports the
+    throughput and the FP64 reaction force on the cone."""
+    import numpy as np
+
+    import nbc
+
+    if a.bed_cache and os.path.exists(a.bed_cache):
+        bed = nbc.Bed.load(a.bed_cache)
+    else:
+        bed = nbc.generate_bed(1, a.nclumps, seed=3, container=(0.08, 0, 0, 0))
+        bed.params.BranchLmtLen = 128
+        if a.settle_steps > 0:
+            bed = settle(bed, a.settle_steps, a.settle_dt)
+        if a.bed_cache:
+            bed.save(a.bed_cache)
+    bed.params.isIntruder = 1
+    bed.params.OutputInterval = a.steps
+    Rc = 0.010
+    alpha = np.radians(30.0)
+    Hc = Rc / np.tan(alpha)
+    pos = bed.pos.reshape(-1, 3)
+    rho = np.hypot(pos[:, 0], pos[:, 1])
+    m = rho < Rc + bed.radius
+    apex = (pos[m, 2] - (rho[m] * np.cos(alpha) - bed.radius[m]) / np.sin(alpha)).max() + 20e-6
+    pene = nbc.cone_penetrator(Rc, Hc, (0.0, 0.0, apex + Hc), (0.0, 0.0, -0.1), nseg=64)
+    opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER, pene_frcoe=0.4)
+    ctx = nbc.Context(bed.finalized_params(opt), opt)
+    ctx.upload(bed, bed.wall, pene)
+    ctx.step(a.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(a.steps)
+    t1 = time.perf_counter()
+    st = ctx.stats()
+    fz = ctx.penetrator()["cache_f"][:, 2]  # FP64 sum of the sphere forces on the cone, per timed step
+    ctx.close()
+    res = {"metric": "sphere-steps/s, C3 cone penetrator in a settled 1M-sphere bed (FP32-mixed)",
+           "value": bed.n * a.steps / (t1 - t0), "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": (t1 - t0) / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator)",
+           "config": {"workload": "C3: 200,000 clumps (C1 templates) settled in a cylinder R 80 mm, 60-degree cone "
+                                  "R 10 mm with 128 triangles at 0.1 m/s from first contact, friction 0.4, dt 5e-7 s",
+                      "n_spheres": bed.n, "triangles": int(pene.ntri)},
+           "penetrator": {"depth_m": float(0.1 * (a.warmup + a.steps) * bed.params.StepSize - 20e-6),
+                          "mean_reaction_force_z_N": float(-fz.mean()), "max_reaction_force_z_N": float(-fz.min()),
+                          "steps_in_contact": int(np.count_nonzero(fz)), "touching_wall_and_cone": st.touching_wall,
+                          "rebuilds": st.rebuilds, "near_refreshes": st.near_refreshes}}
+    print(json.dumps(res))
+
+
 def c4_beds(nbeds, seed0=1000, threads=16):
     """nbeds C1-style beds, clump counts uniform in [2000, 60000], cylinder
     radius scaled with count (60 mm at 200,000 clumps), seed seed0 + bed;
@@ -343,9 +397,9 @@ def main():
     ap.add_argument("--settle-dt", type=float, default=5e-6)
     ap.add_argument("--bed-cache", default=None,
                     help="load the settled bed from this .npz if it exists, else build it and save it there")
-    ap.add_argument("--workload", choices=("c0", "c1", "c2", "c4"), default="c1",
+    ap.add_argument("--workload", choices=("c0", "c1", "c2", "c3", "c4"), default="c1",
                     help="c1: the headline bed (default); c0: the small cube bed; c2: polydisperse bed on the "
-                         "hierarchical grid; c4: batched beds")
+                         "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
     ap.add_argument("--prof-steps", type=int, default=300,
                     help="steps of the second timed region, with HIP events around every kernel")
@@ -372,6 +426,9 @@ def main():
         return
     if a.workload == "c0":
         run_c0(a)
+        return
+    if a.workload == "c3":
+        run_c3(a)
         return
     if a.bed_cache and os.path.exists(a.bed_cache):
         bed = nbc.Bed.load(a.bed_cache)
