@@ -68,6 +68,27 @@ enum { NBC_FP64 = 0, NBC_MIXED = 1 };
 enum { NBC_PENE_CYLINDER = 0, NBC_PENE_SPHERE = 1, NBC_PENE_CONE = 2 };
 /* PenetratorMotionMode (src/dem_parameters.h:101). */
 enum { NBC_PENE_CONSTANT_VELOCITY = 0, NBC_PENE_GRAVITY = 1 };
+/* Which of the reference's run loops a context reproduces (nbc_options.driver).
+ * CLUMP: DEMClumpMain::RunSimulationClump (src/dem_simulation_clump.cu:3-261).
+ * The others are DEMmain's single-sphere loops, run as one-sphere clumps with
+ * the DEMmain force and integration laws (nbc_upload_particles):
+ * CUBE     RunSimulation2 (src/dem_simulation_cube.cu:3-226), cube wall of
+ *          src/dem_force_cube_wall.cu (no wall history, no wall rolling or
+ *          twisting torque);
+ * CYLINDER RunSimulation (src/dem_simulation_cylind.cu:3-211), cylinder wall
+ *          of src/dem_force_cylind_wall.cu; the history exchange that follows
+ *          every rebuild (:141) leaves new list slot k with the history last
+ *          written at old slot k;
+ * SPLIT    RunSimulation_Split (src/dem_simulation_split.cu:3-211): CYLINDER
+ *          with the glue kinematic types of src/dem_integrate_withglue.cu
+ *          (type 0 free, rebuild when |cum| > 0.1 r; type 1 driven at
+ *          v = (-0.2 y, -0.2 x, 0); type 2 fixed);
+ * HGRID    RunSimulation_HGrid (src/dem_simulation_hgrid.cu:3-201): cylinder
+ *          wall, inherited histories, the hierarchical grid (hgrid_levels,
+ *          default 3; the level table is generated, not read from grid.bt).
+ * The DEMmain loops take no penetrator here (nbc_upload_particles rejects one). */
+enum { NBC_DRIVER_CLUMP = 0, NBC_DRIVER_CUBE = 1, NBC_DRIVER_CYLINDER = 2, NBC_DRIVER_SPLIT = 3,
+       NBC_DRIVER_HGRID = 4 };
 
 /* ---- run parameters ----------------------------------------------------
  * The 38 world.par keys (src/world.par:18-64, parsed by
@@ -127,7 +148,9 @@ typedef struct nbc_options {
     double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
     double near_skin;        /* near-list skin as a fraction of Thresh (0: 0.25; >= 1: gap 2*Thresh,
                                 refreshed at rebuilds only) */
-    double reserved[3];
+    int32_t driver;          /* NBC_DRIVER_* (0: the clump loop)                */
+    int32_t reserved_i;
+    double reserved[2];
 } nbc_options;
 
 /* ---- boundaries -------------------------------------------------------- */
@@ -182,6 +205,21 @@ typedef struct nbc_clumps_in {       /* ch.00000000.bt columns */
     const double* prop;      /* 4n (Ix Iy Iz mass) */
 } nbc_clumps_in;
 
+/* DEMmain single-sphere state (ph.00000000.bt of src/dem_io.cu:156-201:
+ * rows "id sub type x y z vx vy vz qx qy qz qw wx wy wz Ix Iy Iz mass r").
+ * Property = (Ix, Iy, Iz, mass) as in the reference's double4. */
+typedef struct nbc_particles_in {
+    int32_t n;
+    int32_t reserved;
+    const int32_t* type;     /* n, glue kinematic type (SPLIT driver); may be NULL (all 0) */
+    const double* pos;       /* 3n */
+    const double* vel;       /* 3n */
+    const double* quat;      /* 4n (x y z w) */
+    const double* angv;      /* 3n */
+    const double* prop;      /* 4n (Ix Iy Iz mass) */
+    const double* radius;    /* n  */
+} nbc_particles_in;
+
 /* Per-step statistics of the last step (FP64 global sums). */
 typedef struct nbc_stats {
     int64_t step;            /* completed steps since upload                  */
@@ -220,6 +258,10 @@ int nbc_load_options(const char* path, nbc_options* out);
 /* ph text: n rows "id sub clumpId x y z lx ly lz mass radius". Buffers caller-owned. */
 int nbc_load_phase(const char* path, int32_t n, int32_t* clump_id, double* pos, double* local,
                    double* mass, double* radius);
+/* DEMmain ph text (src/dem_io.cu:156-201): n rows of 21 columns, see
+ * nbc_particles_in. Buffers caller-owned. */
+int nbc_load_particles(const char* path, int32_t n, int32_t* type, double* pos, double* vel, double* quat,
+                       double* angv, double* prop, double* radius);
 /* ch text: n rows "id x y z vx vy vz qx qy qz qw wx wy wz Ix Iy Iz mass". */
 int nbc_load_clumps(const char* path, int32_t n, double* pos, double* vel, double* quat,
                     double* angv, double* prop);
@@ -247,7 +289,10 @@ int nbc_make_cone_mesh(double radius, double height, int32_t nseg, double* tri_o
  *          templates, each clump scaled so that radii are log-uniform over
  *          [0.2, 2.0] mm, sequential deposition into a cube wall of section
  *          150 x 150 mm (container[0] overrides) whose height fits the bed;
- *          friction 0.6, meant for the hierarchical grid (3 levels).
+ *          friction 0.6, meant for the hierarchical grid (3 levels);
+ * kind 3 = single spheres for the DEMmain drivers: nclumps (default 100 000)
+ *          one-sphere clumps, radii as C1, solid-sphere mass and inertia,
+ *          deposited like C1 into a cylinder of radius 60 mm (container[0]).
  * The generator allocates the output arrays; release them with nbc_free_bed. */
 typedef struct nbc_bed_spec {
     int32_t kind;
@@ -305,6 +350,19 @@ int nbc_get_batch_ranges(nbc_ctx* ctx, int32_t* sphere_off, int32_t* clump_off);
  * isWallForce == 0; pene may be NULL when isIntruder == 0. */
 int nbc_upload(nbc_ctx* ctx, const nbc_spheres_in* s, const nbc_clumps_in* c,
                const nbc_wall* wall, const nbc_penetrator* pene);
+/* DEMmain drivers (opt->driver > 0): copy single-sphere state in, file order
+ * (DEMmain::CopyDataFromHostToDevice, src/dem.cu:62-72). Particle i becomes
+ * clump i with one sphere at its centre; params->ClumpNum must equal
+ * ParticleNum. The clump getters then return the particles in file order,
+ * nbc_get_particles in the current sorted order. */
+int nbc_upload_particles(nbc_ctx* ctx, const nbc_particles_in* p, const nbc_wall* wall);
+/* DEMmain::CopyDataToIo (src/dem.cu:113-127): the device arrays in the current
+ * sorted order; sorted_id is the reference's part_idx (identity), file_id its
+ * part_fix. type follows the sort in the CYLINDER, SPLIT and HGRID loops and
+ * stays in file order in the CUBE loop, which does not sort it
+ * (src/dem_simulation_cube.cu:102-108). Any pointer may be NULL. */
+int nbc_get_particles(nbc_ctx* ctx, int32_t* sorted_id, int32_t* file_id, int32_t* type, double* pos,
+                      double* vel, double* quat, double* angv, double* prop, double* radius);
 /* Advance nsteps full timesteps (src/dem_simulation_clump.cu:110-226). */
 int nbc_step(nbc_ctx* ctx, int32_t nsteps);
 /* Force a neighbour-list rebuild on the current positions. */
@@ -376,6 +434,11 @@ int nbc_autotune(const nbc_params* params, const nbc_options* base, const nbc_sp
 int nbc_write_particles_bin(const char* dir, int32_t step, int32_t n, const int32_t* sorted_id,
                             const int32_t* file_id, const int32_t* clump_id, const double* pos,
                             const double* local, const double* mass, const double* radius);
+/* DEMSphereIo::SaveParticles_Bin (src/dem_io.cu:290-305): ph.<step> of the
+ * DEMmain loops. */
+int nbc_write_spheres_bin(const char* dir, int32_t step, int32_t n, const int32_t* sorted_id,
+                          const int32_t* file_id, const int32_t* type, const double* pos, const double* vel,
+                          const double* quat, const double* angv, const double* prop, const double* radius);
 int nbc_write_clumps_bin(const char* dir, int32_t step, int32_t nc, const double* pos,
                          const double* vel, const double* quat, const double* angv,
                          const double* prop, const double* pene_pos, const double* pene_vel,
@@ -392,6 +455,12 @@ int nbc_write_contacts_bin(const char* dir, int32_t step, int32_t n, int32_t bra
  * files in workdir (world.par, IniConFile, ch.00000000.bt, Rocks/, optional
  * nbc.par), writing OutPart/OutClump/OutForce/OutCnt there. */
 int nbc_run_simulation_clump(const char* workdir, int32_t device);
+/* Runs one of DEMmain's single-sphere loops (driver = NBC_DRIVER_CUBE:
+ * RunSimulation2, CYLINDER: RunSimulation, SPLIT: RunSimulation_Split, HGRID:
+ * RunSimulation_HGrid) from world.par, IniConFile (21-column rows),
+ * Rocks/CubeWall.bt or Rocks/CylindWall.bt and the optional nbc.par, writing
+ * OutPart/ph.<step> every OutputInterval steps. */
+int nbc_run_simulation(const char* workdir, int32_t device, int32_t driver);
 
 #ifdef __cplusplus
 }

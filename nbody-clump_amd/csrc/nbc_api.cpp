@@ -118,6 +118,15 @@ int nbc_upload_batch(nbc_ctx* ctx, const nbc_spheres_in* spheres, const nbc_clum
     ENG(ctx);
     return ctx_err(ctx, ctx->eng->upload_batch(spheres, clumps, walls));
 }
+int nbc_upload_particles(nbc_ctx* ctx, const nbc_particles_in* p, const nbc_wall* wall) {
+    ENG(ctx);
+    return ctx_err(ctx, ctx->eng->upload_particles(p, wall));
+}
+int nbc_get_particles(nbc_ctx* ctx, int32_t* sorted_id, int32_t* file_id, int32_t* type, double* pos, double* vel,
+                      double* quat, double* angv, double* prop, double* radius) {
+    ENG(ctx);
+    return ctx_err(ctx, ctx->eng->get_particles(sorted_id, file_id, type, pos, vel, quat, angv, prop, radius));
+}
 int nbc_get_batch_ranges(nbc_ctx* ctx, int32_t* sphere_off, int32_t* clump_off) {
     ENG(ctx);
     return ctx_err(ctx, ctx->eng->get_batch_ranges(sphere_off, clump_off));

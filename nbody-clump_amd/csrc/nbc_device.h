@@ -126,6 +126,7 @@ struct DevParams {
     double wl_slack;      // wall list: distance a sphere may move after the list is built (2 * Thresh)
     double pene_move;     // wall list: penetrator travel allowed before the list is rebuilt
     int wall_type;        // NBC_WALL_*
+    int driver;           // NBC_DRIVER_*: 0 the clump loop, > 0 a DEMmain single-sphere loop
     int nfaces;           // wall faces in use (6 cube, 3 cylinder, 0 none)
     int pene_on;          // isIntruder
     int pene_face;        // history slot of the penetrator (= nfaces)

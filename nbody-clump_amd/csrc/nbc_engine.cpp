@@ -108,6 +108,19 @@ class Engine : public EngineBase {
         rbits = O.radix_bits > 0 ? O.radix_bits : 8;
         if (rbits > 8) rbits = 8;
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
+        driver = O.driver;
+        if (driver < NBC_DRIVER_CLUMP || driver > NBC_DRIVER_HGRID)
+            return fail(NBC_E_BADINPUT, "nbc_create: unknown driver");
+        if (driver != NBC_DRIVER_CLUMP) {
+            // DEMmain loops: one sphere per particle, their wall force is applied
+            // unconditionally (src/dem_simulation_cube.cu:175, src/dem_simulation_cylind.cu:157)
+            if (Nc != N) return fail(NBC_E_BADINPUT, "nbc_create: DEMmain drivers need ClumpNum == ParticleNum");
+            if (P.isIntruder) return fail(NBC_E_BADINPUT, "nbc_create: DEMmain drivers take no penetrator");
+            if ((driver == NBC_DRIVER_HGRID) != (P.Levels > 1))
+                return fail(NBC_E_BADINPUT, "nbc_create: the HGRID driver, and only it, uses the hierarchical grid "
+                                            "(nbc_finalize_params with hgrid_levels)");
+            wall_type = driver == NBC_DRIVER_CUBE ? NBC_WALL_CUBE : NBC_WALL_CYLINDER;
+        }
         nfaces = wall_type == NBC_WALL_CUBE ? 6 : (wall_type == NBC_WALL_CYLINDER ? 3 : 0);
         pair_cap_per = O.pair_capacity > 0 ? O.pair_capacity : 16;
         // small beds are launch-bound: their steady chunks replay a captured
@@ -157,7 +170,14 @@ class Engine : public EngineBase {
         // penetrator may move pene_move before the list is rebuilt
         dp.wl_slack = thresh_ok ? 2.0 * P.Thresh * (1.0 + 1e-9) + 1e-12 : inf;
         dp.pene_move = thresh_ok ? 8.0 * P.Thresh : inf;
+        if (driver == NBC_DRIVER_SPLIT) {
+            // glue loop: rebuilds at 0.1 r, and driven spheres move without
+            // asking for one: every list pair is near, every sphere wall-listed
+            dp.near_delta = inf;
+            dp.wl_slack = inf;
+        }
         dp.wall_type = wall_type; dp.nfaces = nfaces;
+        dp.driver = driver;
         dp.pene_on = P.isIntruder ? 1 : 0;
         dp.pene_face = nfaces;
         nwh = nfaces + (dp.pene_on ? 1 : 0);
@@ -173,6 +193,9 @@ class Engine : public EngineBase {
         }
         HIPC(dalloc(&C.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C.prop, Nc));
         HIPC(dalloc(&C.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C.snap, 2 * (size_t)Nc));
+        C.dist = nullptr;
+        C.type = nullptr;
+        if (driver) { HIPC(dalloc(&C.dist, (size_t)Nc)); HIPC(dalloc(&C.type, (size_t)Nc)); }
         // lists
         for (int b = 0; b < 2; ++b) {
             HIPC(dalloc(&Lf[b].foff, (size_t)N + 1));
@@ -291,7 +314,7 @@ class Engine : public EngineBase {
             if (b == 0) { F(s.sft); F(s.p32); F(s.slot); }
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
-        F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap);
+        F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap); F(C.dist); F(C.type);
         F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
@@ -331,6 +354,7 @@ class Engine : public EngineBase {
     // rebuilds every bed.
     int init_batch(const nbc_params* beds, int nb, const nbc_options& o, int dev) override {
         if (!beds || nb < 1) return fail(NBC_E_BADINPUT, "nbc_create_batch: no beds");
+        if (o.driver != NBC_DRIVER_CLUMP) return fail(NBC_E_BADINPUT, "nbc_create_batch: beds run the clump loop");
         nbc_params all = beds[0];
         long long ns = 0, nc = 0, cells = 0;
         soff.assign(1, 0);
@@ -470,6 +494,8 @@ class Engine : public EngineBase {
     int upload(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w, const nbc_penetrator* pe) override {
         HIPC(hipSetDevice(device));
         if (nbeds > 1 && !batch_upload) return fail(NBC_E_BADINPUT, "nbc_upload: a batch context takes nbc_upload_batch");
+        if (driver && !particle_upload)
+            return fail(NBC_E_BADINPUT, "nbc_upload: a DEMmain driver context takes nbc_upload_particles");
         if (!s || !c || s->n != N || c->n != Nc)
             return fail(NBC_E_BADINPUT, "nbc_upload: sphere/clump counts differ from the context parameters");
         for (int i = 0; i < N; ++i)
@@ -532,6 +558,10 @@ class Engine : public EngineBase {
         HIPC(hipMemcpy(C.prop, prop.data(), sizeof(D4) * prop.size(), hipMemcpyHostToDevice));
         HIPC(hipMemcpy(C.ft, ft.data(), sizeof(D4) * ft.size(), hipMemcpyHostToDevice));
         HIPC(hipMemset(C.snap, 0, sizeof(R4) * 2 * (size_t)Nc));
+        if (driver) {
+            HIPC(hipMemset(C.dist, 0, sizeof(D4) * (size_t)Nc));
+            HIPC(hipMemcpy(C.type, htype.data(), sizeof(int) * (size_t)Nc, hipMemcpyHostToDevice));
+        }
         // empty lists (src/dem_grid.cu:228-247: Nsum = 0, Lpb_pre = -1)
         lcur = 0;
         HIPC(hipMemset(Lf[0].foff, 0, sizeof(int) * ((size_t)N + 1)));
@@ -690,7 +720,11 @@ class Engine : public EngineBase {
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
         // history-valid bits of the old near list back into the old pair words
         KC(launch_near_sync(st, N, nfoff, nfw, nfs, Lo.fword, block));
-        KC(launch_hist_remap<R>(st, N, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword, Lo.hist[p], block));
+        if (driver == NBC_DRIVER_CYLINDER || driver == NBC_DRIVER_SPLIT)
+            KC(launch_hist_carry<R>(st, tot[0], nfront, p, Ln.fword, Ln.hist[p], Lo.fword, Lo.hist[p], block));
+        else
+            KC(launch_hist_remap<R>(st, N, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword, Lo.hist[p],
+                                    block));
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));
@@ -1167,6 +1201,62 @@ class Engine : public EngineBase {
         return 0;
     }
 
+    // DEMmain loops: particle i is clump i with one sphere at its centre
+    // (DEMmain::CopyDataFromHostToDevice, src/dem.cu:62-72)
+    int upload_particles(const nbc_particles_in* q, const nbc_wall* w) override {
+        if (!driver) return fail(NBC_E_BADINPUT, "nbc_upload_particles: the context runs the clump loop "
+                                                 "(set nbc_options.driver)");
+        if (!q || q->n != N || !q->pos || !q->vel || !q->quat || !q->angv || !q->prop || !q->radius)
+            return fail(NBC_E_BADINPUT, "nbc_upload_particles: particle count or arrays missing");
+        if (!w) return fail(NBC_E_BADINPUT, "nbc_upload_particles: the DEMmain loops need their wall");
+        std::vector<int> cid(N);
+        std::vector<double> local(3 * (size_t)N, 0.0), mass(N);
+        htype.assign(N, 0);
+        for (int i = 0; i < N; ++i) {
+            cid[i] = i;
+            mass[i] = q->prop[4 * (size_t)i + 3];
+            if (q->type) {
+                htype[i] = q->type[i];
+                if (htype[i] < 0 || htype[i] > 2) return fail(NBC_E_BADINPUT, "nbc_upload_particles: type not 0, 1 or 2");
+            }
+        }
+        hprop.assign(q->prop, q->prop + 4 * (size_t)N);
+        const nbc_spheres_in si{N, 0, cid.data(), q->pos, local.data(), mass.data(), q->radius};
+        const nbc_clumps_in ci{N, 0, q->pos, q->vel, q->quat, q->angv, q->prop};
+        particle_upload = true;
+        const int e = upload(&si, &ci, w, nullptr);
+        particle_upload = false;
+        return e;
+    }
+
+    // DEMmain::CopyDataToIo (src/dem.cu:113-127), current sorted order
+    int get_particles(int32_t* sid, int32_t* fid, int32_t* type, double* pos, double* vel, double* quat,
+                      double* angv, double* prop, double* radius) override {
+        if (!driver) return fail(NBC_E_BADINPUT, "nbc_get_particles: the context runs the clump loop");
+        if (!uploaded) return fail(NBC_E_BADINPUT, "nbc_get_particles: no state uploaded");
+        std::vector<int32_t> f(N), c(N);
+        std::vector<double> p3(3 * (size_t)N), r(N);
+        int e = get_spheres(f.data(), c.data(), p3.data(), nullptr, nullptr, nullptr, r.data());
+        if (e) return e;
+        std::vector<double> cv(3 * (size_t)N), cq(4 * (size_t)N), cw(3 * (size_t)N);
+        e = get_clumps(nullptr, cv.data(), cq.data(), cw.data(), nullptr, nullptr);
+        if (e) return e;
+        for (int i = 0; i < N; ++i) {
+            const int k = c[i];
+            if (sid) sid[i] = i;
+            if (fid) fid[i] = f[i];
+            // the cube loop does not sort Type (src/dem_simulation_cube.cu:102-108)
+            if (type) type[i] = driver == NBC_DRIVER_CUBE ? htype[i] : htype[k];
+            if (pos) for (int a = 0; a < 3; ++a) pos[3 * (size_t)i + a] = p3[3 * (size_t)i + a];
+            if (vel) for (int a = 0; a < 3; ++a) vel[3 * (size_t)i + a] = cv[3 * (size_t)k + a];
+            if (quat) for (int a = 0; a < 4; ++a) quat[4 * (size_t)i + a] = cq[4 * (size_t)k + a];
+            if (angv) for (int a = 0; a < 3; ++a) angv[3 * (size_t)i + a] = cw[3 * (size_t)k + a];
+            if (prop) for (int a = 0; a < 4; ++a) prop[4 * (size_t)i + a] = hprop[4 * (size_t)k + a];
+            if (radius) radius[i] = r[i];
+        }
+        return 0;
+    }
+
     int get_clumps(double* pos, double* vel, double* quat, double* angv, double* F, double* T) override {
         HIPC(hipSetDevice(device));
         HIPC(hipStreamSynchronize(st));
@@ -1310,6 +1400,11 @@ class Engine : public EngineBase {
     // and wall table (host copy and device copy), bed of each sphere
     int nbeds = 1;
     bool batch_upload = false;
+    // DEMmain loops
+    int driver = NBC_DRIVER_CLUMP;
+    bool particle_upload = false;
+    std::vector<int> htype;     // glue types, file order
+    std::vector<double> hprop;  // particle Property (Ix Iy Iz m), file order
     std::vector<int> soff, coff;
     std::vector<BedGeo> hbeds;
     BedGeo* dbeds = nullptr;

@@ -18,6 +18,10 @@ class EngineBase {
     virtual int init_batch(const nbc_params* beds, int nbeds, const nbc_options& o, int device) = 0;
     virtual int upload_batch(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w) = 0;
     virtual int get_batch_ranges(int32_t* sphere_off, int32_t* clump_off) = 0;
+    // DEMmain single-sphere loops
+    virtual int upload_particles(const nbc_particles_in* p, const nbc_wall* w) = 0;
+    virtual int get_particles(int32_t* sid, int32_t* fid, int32_t* type, double* pos, double* vel, double* quat,
+                              double* angv, double* prop, double* radius) = 0;
     virtual int step(int nsteps) = 0;
     virtual int force_rebuild() = 0;
     virtual int set_record(int on) = 0;

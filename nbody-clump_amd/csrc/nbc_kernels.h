@@ -34,6 +34,8 @@ struct ClumpBuf {
     D4* prop;                         // Ix Iy Iz mass
     D4* ft;                           // [2Nc] force, torque (FP64)
     typename Vec4T<R>::type* snap;    // [2Nc] velocity (w = state finite), angular velocity
+    D4* dist;                         // [Nc] DEMmain loops: the step's drift (cum += dist), else null
+    int* type;                        // [Nc] DEMmain loops: glue kinematic type, else null
 };
 
 // Verlet list: CSR front rows (owner i, partner j > i) with ping-pong history,
@@ -103,6 +105,10 @@ int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm
 template <class R>
 int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
                       const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
+// DEMmain cylinder/glue loops: new slot s takes old slot s's last history
+template <class R>
+int launch_hist_carry(hipStream_t st, long long nnew, long long nold, int p, uint32_t* nfword, R* nhist,
+                      const uint32_t* ofword, const R* ohist, int block);
 // Hierarchical grid search, phase 0 = counts, 1 = cross entries, 2 = rows.
 int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* pos, const uint32_t* cell,
                       const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, int* xcnt,

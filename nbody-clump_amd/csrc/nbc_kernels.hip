@@ -156,11 +156,45 @@ __device__ __forceinline__ void clump_drift(const DevParams& P, const ClumpBuf<R
                                             const D4& qc, V3<double> vel, V3<double> w, const V3<double>& F,
                                             const V3<double>& T, int* halt) {
     const double Dt = P.dt;
+    const int ty = P.driver == NBC_DRIVER_SPLIT ? C.type[i] : 0;
+    if (ty != 0) {
+        // glue kinematic types (src/dem_integrate_withglue.cu:3-63): type 1 is
+        // driven at v = (-0.2 y, -0.2 x, 0) from its position before the
+        // drift, type 2 stays put; neither is kicked or turned
+        V3<double> dist = mk<double>(0.0, 0.0, 0.0);
+        D4 po = pc;
+        if (ty == 1) {
+            const double Omega = 0.2;
+            vel.x = -Omega * pc.y;
+            vel.y = -Omega * pc.x;
+            vel.z = 0.0;
+            dist = Dt * vel;
+            po.x = pc.x + dist.x; po.y = pc.y + dist.y; po.z = pc.z + dist.z;
+        }
+        D4 vo; vo.x = vel.x; vo.y = vel.y; vo.z = vel.z; vo.w = 0.0;
+        D4 wo; wo.x = w.x; wo.y = w.y; wo.z = w.z; wo.w = 0.0;
+        D4 dd; dd.x = dist.x; dd.y = dist.y; dd.z = dist.z; dd.w = 0.0;
+        C.pq[2 * i] = po;
+        C.vw[2 * i] = vo;
+        C.vw[2 * i + 1] = wo;
+        C.dist[i] = dd;
+        const bool ok = finite3(vel) && finite3(w);
+        if (!ok) atomicOr(&halt[3], ST_CLUMP_BAD);
+        C.snap[2 * i] = mk4<R>((R)vel.x, (R)vel.y, (R)vel.z, ok ? R(1) : R(0));
+        C.snap[2 * i + 1] = mk4<R>((R)w.x, (R)w.y, (R)w.z, R(0));
+        return;
+    }
     V3<double> pos = xyz(pc);
     V3<double> dist = Dt * vel + 0.5 * Dt * Dt * F;
     pos = pos + dist;
     vel = vel + 0.5 * Dt * F;
     w = w + 0.5 * Dt * T;
+    if (P.driver) {
+        // DEMmain keeps each sphere's displacement as the sum of its drifts
+        // (src/dem_integrate.cu:36-38); k_sphere_pose adds this one
+        D4 dd; dd.x = dist.x; dd.y = dist.y; dd.z = dist.z; dd.w = 0.0;
+        C.dist[i] = dd;
+    }
     // attitude, src/dem_integrate_clumpsphere.cu:45-90 (T is the same scaled torque)
     double M[3][3];
     QuatDCM<double>(qc.x, qc.y, qc.z, qc.w, M);
@@ -236,8 +270,17 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         V3<double> up = xyz(pc) + arm;
         const D4 po = S.pos[i];
         const D4 cu = S.cum[i];
-        V3<double> cum = xyz(cu) + up - xyz(po);
-        over = Length<double>(cum) > P.Thresh;
+        V3<double> cum;
+        if (P.driver) {
+            const D4 dd = C.dist[c];
+            cum = xyz(cu) + xyz(dd);
+        } else {
+            cum = xyz(cu) + up - xyz(po);
+        }
+        // glue loop: type 0 rebuilds past 0.1 r, the kinematic types never ask
+        // (src/dem_integrate_withglue.cu:40-43)
+        if (P.driver == NBC_DRIVER_SPLIT) over = C.type[c] == 0 && Length<double>(cum) > 0.10 * po.w;
+        else over = Length<double>(cum) > P.Thresh;
         // the sphere left its near-list allowance (threshold kept in cum.w)
         near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > cu.w;
         D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
@@ -722,6 +765,26 @@ __global__ void k_hist_remap(int N, int p, const int* perm, const int* nfoff, ui
     }
 }
 
+// History after a rebuild in DEMmain's cylinder and glue loops
+// (src/dem_simulation_cylind.cu:130-141): CreateContList inherits the
+// histories into the array that the unconditional ForceExchange then swaps
+// out and clears, so new slot s reads the history written at old slot s in
+// the last step (zero where that pair was not touching or s is past the old list).
+template <class R>
+__global__ void k_hist_carry(long long nnew, long long nold, int p, uint32_t* nfword, R* nhist,
+                             const uint32_t* ofword, const R* ohist) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nnew) return;
+    uint32_t w = nfword[s] & ~(VALID(0) | VALID(1));
+    if (s < nold && (ofword[s] & VALID(p)) && !(w & INTRA)) {
+        const R* h = ohist + (size_t)s * HSTRIDE;
+        R* d = nhist + (size_t)s * HSTRIDE;
+        for (int k = 0; k < HSTRIDE; ++k) d[k] = h[k];
+        w |= VALID(p);
+    }
+    nfword[s] = w;
+}
+
 // ---------------------------------------------------------------------------
 // Near lists: the contact kernel's view of the Verlet list. Each sphere gets a
 // slack at the near-list build: with a = |cum| (its displacement since the
@@ -1040,7 +1103,10 @@ __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& 
     const V3<R> ARMi = in.armI + Ap * DistU;
     const V3<R> ARMj = in.armJ - An * DistU;
     const V3<R> Vrel = in.vJ - in.vI;
-    const V3<R> Vrot = Cross<R>(in.wJ, ARMj) - Cross<R>(in.wI, ARMi);
+    // DEMmain loops: the sphere's own spin about its centre
+    // (src/dem_force_particle.cu:87-88); the clump form otherwise
+    const V3<R> Vrot = P.driver ? An * Cross<R>(DistU, in.wJ) + Ap * Cross<R>(DistU, in.wI)
+                                : Cross<R>(in.wJ, ARMj) - Cross<R>(in.wI, ARMi);
     const V3<R> Vtot = Vrel + Vrot;
     const R ERR = Dot<R>(DistU, Vtot);
     const V3<R> Vnor = ERR * DistU;
@@ -1117,7 +1183,9 @@ __device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFact
     const V3<R> Arm = R(-1.0) * arm0 + Ap * DistU;
     const V3<R> Vwall = Cross<R>(ww, RVo);
     const V3<R> Vrel = cv - wv;
-    const V3<R> Vrot = Cross<R>(Arm, cw) - Vwall;
+    // DEMmain walls spin the sphere about its centre (src/dem_force_cube_wall.cu:54,
+    // src/dem_force_cylind_wall.cu:55)
+    const V3<R> Vrot = P.driver ? Ap * Cross<R>(DistU, cw) - Vwall : Cross<R>(Arm, cw) - Vwall;
     const V3<R> Vtot = Vrel + Vrot;
     const R ERR = Dot<R>(DistU, Vtot);
     const V3<R> Vnor = ERR * DistU;
@@ -1595,6 +1663,25 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                             const V3<double> RVb = RV - E * ea, RVt = E * ea;
                             const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
                             const V3<double> RVn = RV - RVo;
+                            if (P.driver == NBC_DRIVER_CUBE) {
+                                // DEMmain cube wall (src/dem_force_cube_wall.cu:3-148, :170-183):
+                                // the normal and Ap come from RVn, Dn = r - |RVn| unchecked;
+                                // its histories are never written (zero), and its rolling
+                                // and twisting moments are not applied
+                                const double DistL = Length<double>(RVn);
+                                const double Dn = ri - DistL;
+                                const double DistS = RVn.x * RVn.x + RVn.y * RVn.y + RVn.z * RVn.z;
+                                const R DistLr = (R)sqrt(DistS);
+                                const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
+                                const V3<R> Z = mk<R>(R(0), R(0), R(0));
+                                WallOut<R> o;
+                                wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi,
+                                              cvt3<R>(RVo), vi, wi, wv, ww, Z, Z, Z, o);
+                                Fw = Fw + o.nf + o.tf;
+                                Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
+                                n_w++;
+                                continue;
+                            }
                             const double sgn = up ? -1.0 : 1.0;
                             const V3<double> nin = mk<double>(a == 0 ? sgn : 0.0, a == 1 ? sgn : 0.0, a == 2 ? sgn : 0.0);
                             const double Ap = Dot<double>(RVn, nin);
@@ -2004,8 +2091,10 @@ __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S
     }
     V3<double> F = xyz(Fr), T = xyz(Tr);
     scale_ft(P, pr, F, T);
-    const V3<double> v = xyz(vr) + 0.5 * P.dt * F;
-    const V3<double> w = xyz(wr) + 0.5 * P.dt * T;
+    // glue kinematic types keep their velocities (src/dem_integrate_withglue.cu:139)
+    const bool kick = P.driver != NBC_DRIVER_SPLIT || C.type[c] == 0;
+    const V3<double> v = kick ? xyz(vr) + 0.5 * P.dt * F : xyz(vr);
+    const V3<double> w = kick ? xyz(wr) + 0.5 * P.dt * T : xyz(wr);
     if (NEXT) {
         // K1 of step n + 1 reads exactly this force, torque and velocity
         clump_drift<R>(P, C, c, pc, qc, v, w, F, T, halt);
@@ -2271,6 +2360,14 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
     LAUNCH_CHECK();
     return 0;
 }
+template <class R>
+int launch_hist_carry(hipStream_t st, long long nnew, long long nold, int p, uint32_t* nfword, R* nhist,
+                      const uint32_t* ofword, const R* ohist, int block) {
+    if (nnew <= 0) return 0;
+    k_hist_carry<R><<<nblk(nnew, block), block, 0, st>>>(nnew, nold, p, nfword, nhist, ofword, ohist);
+    LAUNCH_CHECK();
+    return 0;
+}
 int contacts_blocks(int N, int block) { return (int)nblk(N, block); }
 template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
@@ -2336,6 +2433,8 @@ int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
     template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
+    template int launch_hist_carry<R>(hipStream_t, long long, long long, int, uint32_t*, R*, const uint32_t*,       \
+                                      const R*, int);                                                                \
     template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
                                     WallBuf<R>, int, PeneState*, double*, unsigned long long*, int*, RecordBuf,      \
                                     bool, int, const int*, int);                                                     \

@@ -1,7 +1,8 @@
 // nbody_clump: drop-in executable for the reference's main()
 // (src/main.cpp:4-17). Runs the simulation from the current directory (or the
 // directory given as the first argument) and exits with EXIT_FAILURE and the
-// reference's message when the library reports a fatal condition.
+// reference's message when the library reports a fatal condition. nbc.par's
+// Driver key selects DEMmain's single-sphere loops instead of the clump loop.
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
@@ -11,7 +12,7 @@
 int main(int argc, char** argv) {
     const char* dir = argc > 1 ? argv[1] : ".";
     const int device = argc > 2 ? std::atoi(argv[2]) : 0;
-    const int e = nbc_run_simulation_clump(dir, device);
+    const int e = nbc_run_simulation(dir, device, 0);
     if (e) {
         std::fprintf(stderr, "%s\n", nbc_last_error(nullptr));
         return e == 7 ? 1 : EXIT_FAILURE;

@@ -167,6 +167,7 @@ struct Placed { double z, x, y, r; };
 
 // drop points tried per clump in the deposition (the lowest landing wins)
 constexpr int kDropTries = 8;
+constexpr double kPi = 3.14159265358979323846;
 
 }  // namespace
 
@@ -283,6 +284,24 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
         P.Frcoe = 0.6;
         P.GridLevels = 3;
         P.StepSize = 2e-7; P.EndStep = 5000; P.OutputInterval = 1000; P.BranchLmtLen = 256; P.Thresh = 0.08e-3;
+    } else if (spec->kind == 3) {
+        // DEMmain sphere bed: single spheres, lognormal radii as C1, solid
+        // sphere mass and inertia; deposited like C1 into a cylinder
+        if (nc <= 0) nc = 100000;
+        std::lognormal_distribution<double> LN(std::log(1.0e-3), 0.15);
+        ctpl.resize(nc);
+        for (int k = 0; k < nc; ++k) {
+            const double r = std::min(1.6e-3, std::max(0.6e-3, LN(g)));
+            Template t;
+            t.mass = f32(rho * 4.0 / 3.0 * kPi * r * r * r);
+            for (int a = 0; a < 3; ++a) t.I[a] = 0.4 * t.mass * r * r;
+            t.off = {{0, 0, 0}};
+            t.r = {f32(r)};
+            t.rbound = r;
+            tpl.push_back(t);
+            ctpl[k] = k;
+        }
+        P.StepSize = 5e-7; P.EndStep = 10000; P.OutputInterval = 1000; P.BranchLmtLen = 64; P.Thresh = 0.25e-3;
     } else {
         nbc::set_global_error("nbc_generate_bed: unknown kind");
         return NBC_E_BADINPUT;
@@ -346,7 +365,7 @@ extern "C" int nbc_generate_bed(const nbc_bed_spec* spec, nbc_bed_out* out) {
         // Every clump drops at a random (x, y) and stops at its first contact
         // with the floor or an earlier sphere, so the bed is contact-connected
         // without initial overlaps.
-        const bool cyl = spec->kind == 1;
+        const bool cyl = spec->kind == 1 || spec->kind == 3;
         const double Rc = spec->container[0] > 0 ? spec->container[0] : (cyl ? 0.06 : 0.15);
         const double ext = cyl ? 2.0 * Rc : Rc;  // extent of the section along x and y
         const double org = cyl ? -Rc : 0.0;      // its lower x / y

@@ -211,7 +211,14 @@ extern "C" int nbc_load_options(const char* path, nbc_options* out) {
         else if (k == "NearSkin") out->near_skin = std::atof(v.c_str());
         else if (k == "Graphs") out->graphs = std::atoi(v.c_str());
         else if (k == "BatchSteps") out->batch_steps = std::atoi(v.c_str());
-        else return err(NBC_E_PARAM, "nbc.par: unknown key " + k);
+        else if (k == "Driver") {
+            if (lv == "clump") out->driver = NBC_DRIVER_CLUMP;
+            else if (lv == "cube") out->driver = NBC_DRIVER_CUBE;
+            else if (lv == "cylinder") out->driver = NBC_DRIVER_CYLINDER;
+            else if (lv == "split") out->driver = NBC_DRIVER_SPLIT;
+            else if (lv == "hgrid") out->driver = NBC_DRIVER_HGRID;
+            else return err(NBC_E_PARAM, "nbc.par: Driver must be clump, cube, cylinder, split or hgrid");
+        } else return err(NBC_E_PARAM, "nbc.par: unknown key " + k);
     }
     return 0;
 }
@@ -246,6 +253,29 @@ extern "C" int nbc_load_clumps(const char* path, int32_t n, double* pos, double*
         in >> tmp >> pos[3 * i] >> pos[3 * i + 1] >> pos[3 * i + 2] >> vel[3 * i] >> vel[3 * i + 1] >> vel[3 * i + 2] >>
             quat[4 * i] >> quat[4 * i + 1] >> quat[4 * i + 2] >> quat[4 * i + 3] >> angv[3 * i] >> angv[3 * i + 1] >>
             angv[3 * i + 2] >> prop[4 * i] >> prop[4 * i + 1] >> prop[4 * i + 2] >> prop[4 * i + 3];
+        if (!in) {
+            std::ostringstream m;
+            m << "Error! " << path << " ends or is malformed at row " << i;
+            return err(NBC_E_IO, m.str());
+        }
+    }
+    return 0;
+}
+
+// DEMmain ph text, DEMSphereIo::LoadPhase (src/dem_io.cu:156-201):
+// "id sub type x y z vx vy vz qx qy qz qw wx wy wz Ix Iy Iz mass r".
+extern "C" int nbc_load_particles(const char* path, int32_t n, int32_t* type, double* pos, double* vel,
+                                  double* quat, double* angv, double* prop, double* radius) {
+    std::ifstream in(path ? path : "");
+    if (!in) return err(NBC_E_IO, std::string("Error! Cannot open ") + (path ? path : "(null)") + "!");
+    if (n < 0 || !type || !pos || !vel || !quat || !angv || !prop || !radius)
+        return err(NBC_E_BADINPUT, "nbc_load_particles: bad arguments");
+    double tmp;
+    for (int i = 0; i < n; ++i) {
+        in >> tmp >> tmp >> type[i] >> pos[3 * i] >> pos[3 * i + 1] >> pos[3 * i + 2] >> vel[3 * i] >>
+            vel[3 * i + 1] >> vel[3 * i + 2] >> quat[4 * i] >> quat[4 * i + 1] >> quat[4 * i + 2] >> quat[4 * i + 3] >>
+            angv[3 * i] >> angv[3 * i + 1] >> angv[3 * i + 2] >> prop[4 * i] >> prop[4 * i + 1] >> prop[4 * i + 2] >>
+            prop[4 * i + 3] >> radius[i];
         if (!in) {
             std::ostringstream m;
             m << "Error! " << path << " ends or is malformed at row " << i;
@@ -444,6 +474,26 @@ extern "C" int nbc_write_particles_bin(const char* dir, int32_t step, int32_t n,
     f->s(n);
     f->w(sorted_id, n); f->w(file_id, n); f->w(clump_id, n);
     f->w(pos, 3 * (size_t)n); f->w(local, 3 * (size_t)n); f->w(mass, n); f->w(radius, n);
+    e = f->close();
+    delete f;
+    return e ? err(NBC_E_IO, "Cannot write binary output") : 0;
+}
+
+// OutPart/ph.%08d.bin of the DEMmain loops, DEMSphereIo::SaveParticles_Bin
+// (src/dem_io.cu:290-305): count, sortedParticleID, ParticleID, Type, Pos,
+// Vel, Quat, AngSpd, Property, Radius.
+extern "C" int nbc_write_spheres_bin(const char* dir, int32_t step, int32_t n, const int32_t* sorted_id,
+                                     const int32_t* file_id, const int32_t* type, const double* pos,
+                                     const double* vel, const double* quat, const double* angv, const double* prop,
+                                     const double* radius) {
+    if (n < 0) return err(NBC_E_BADINPUT, "Negative binary output count.");
+    BinFile* f;
+    int e = open_out(dir, step, f, "ph");
+    if (e) return e;
+    f->s(n);
+    f->w(sorted_id, n); f->w(file_id, n); f->w(type, n);
+    f->w(pos, 3 * (size_t)n); f->w(vel, 3 * (size_t)n); f->w(quat, 4 * (size_t)n); f->w(angv, 3 * (size_t)n);
+    f->w(prop, 4 * (size_t)n); f->w(radius, n);
     e = f->close();
     delete f;
     return e ? err(NBC_E_IO, "Cannot write binary output") : 0;

@@ -218,3 +218,90 @@ extern "C" int nbc_run_simulation_clump(const char* workdir, int32_t device) {
     }
     return 0;
 }
+
+// nbc_run_simulation: DEMmain's single-sphere loops (src/dem_simulation_cube.cu,
+// src/dem_simulation_cylind.cu, src/dem_simulation_split.cu,
+// src/dem_simulation_hgrid.cu). Same inputs (world.par, IniConFile with the
+// 21-column rows of src/dem_io.cu:156-201, Rocks/CubeWall.bt or
+// Rocks/CylindWall.bt) and the OutPart/ph.<step> files of
+// DEMSphereIo::SaveParticles_Bin every OutputInterval steps.
+extern "C" int nbc_run_simulation(const char* workdir, int32_t device, int32_t driver) {
+    Run r;
+    r.dir = workdir ? workdir : "";
+    int e = nbc_load_options(r.path("nbc.par").c_str(), &r.O);
+    if (e) return e;
+    if (driver > 0) r.O.driver = driver;
+    if (r.O.driver == NBC_DRIVER_CLUMP) return nbc_run_simulation_clump(workdir, device);
+    if (r.O.driver < 0 || r.O.driver > NBC_DRIVER_HGRID) {
+        nbc::set_global_error("nbc_run_simulation: unknown driver");
+        return NBC_E_BADINPUT;
+    }
+    char ini[4096];
+    std::cout << "Load Parameters ..." << std::endl;
+    e = nbc_load_world_par(r.path("world.par").c_str(), &r.P, ini, sizeof ini);
+    if (e) return e;
+    std::cout << "========================= Finish loading parameters =========================" << std::endl;
+    const int N = r.P.ParticleNum;
+    if (N <= 0 || r.P.OutputInterval <= 0) {
+        nbc::set_global_error("world.par: NumberBody and OutputInterval must be positive");
+        return NBC_E_PARAM;
+    }
+    if (r.P.isIntruder) {
+        nbc::set_global_error("nbc_run_simulation: the DEMmain loops run here without a penetrator (isIntruder = 0)");
+        return NBC_E_BADINPUT;
+    }
+    // one sphere per particle
+    r.P.ClumpNum = N;
+    std::cout << "========================= Load Phase =========================" << std::endl;
+    std::vector<int32_t> type(N);
+    std::vector<double> pos(3 * (size_t)N), vel(3 * (size_t)N), quat(4 * (size_t)N), angv(3 * (size_t)N),
+        prop(4 * (size_t)N), radius(N);
+    e = nbc_load_particles(r.path(ini).c_str(), N, type.data(), pos.data(), vel.data(), quat.data(), angv.data(),
+                           prop.data(), radius.data());
+    if (e) return e;
+    std::cout << "========================= Finish loading particle data =========================" << std::endl;
+    if (r.O.driver == NBC_DRIVER_HGRID && r.O.hgrid_levels <= 1)
+        r.O.hgrid_levels = r.P.GridLevels > 1 ? r.P.GridLevels : 3;
+    if (r.O.driver != NBC_DRIVER_HGRID) r.O.hgrid_levels = 0;
+    e = nbc_finalize_params(&r.P, radius.data(), N, &r.O);
+    if (e) return e;
+    std::cout << "Configuration of Grid ..." << std::endl;
+    std::cout << "CellSize: " << r.P.CellSize[0] << " " << r.P.CellSize[1] << " " << r.P.CellSize[2] << std::endl;
+    std::cout << "WorldOrigin: " << r.P.WorldOrigin[0] << " " << r.P.WorldOrigin[1] << " " << r.P.WorldOrigin[2] << std::endl;
+    std::cout << "WorldSize: " << r.P.WorldSize[0] << " " << r.P.WorldSize[1] << " " << r.P.WorldSize[2] << std::endl;
+    std::cout << "GridDim: " << r.P.GridDim[0] << " " << r.P.GridDim[1] << " " << r.P.GridDim[2] << std::endl;
+    if (r.O.driver == NBC_DRIVER_CUBE) e = nbc_load_cube_wall(r.path("Rocks/CubeWall.bt").c_str(), &r.wall);
+    else e = nbc_load_cylinder_wall(r.path("Rocks/CylindWall.bt").c_str(), &r.wall);
+    if (e) return e;
+    const std::string out = r.path("OutPart");
+    if (!exists(out)) {
+        std::cout << "Create results folder: ./OutPart/" << std::endl;
+        ::mkdir(out.c_str(), 0755);
+    }
+    e = nbc_create(&r.ctx, &r.P, &r.O, device);
+    if (e) return e;
+    const nbc_particles_in pin{N, 0, type.data(), pos.data(), vel.data(), quat.data(), angv.data(), prop.data(),
+                               radius.data()};
+    e = nbc_upload_particles(r.ctx, &pin, &r.wall);
+    if (e) return e;
+    const int oi = r.P.OutputInterval;
+    std::vector<int32_t> sid(N), fid(N);
+    int ii = r.P.StartStep;
+    while (ii < r.P.EndStep) {
+        const int chunk = std::min(oi, r.P.EndStep - ii);
+        e = nbc_step(r.ctx, chunk);
+        if (e) return e;
+        ii += chunk;
+        if (chunk == oi) {
+            std::printf("StepSize = %d\n", ii);
+            std::fflush(stdout);
+            e = nbc_get_particles(r.ctx, sid.data(), fid.data(), type.data(), pos.data(), vel.data(), quat.data(),
+                                  angv.data(), prop.data(), radius.data());
+            if (e) return e;
+            e = nbc_write_spheres_bin(out.c_str(), ii, N, sid.data(), fid.data(), type.data(), pos.data(), vel.data(),
+                                      quat.data(), angv.data(), prop.data(), radius.data());
+            if (e) return e;
+        }
+    }
+    return 0;
+}

@@ -18,6 +18,7 @@ NBC_MAX_LEVELS = 8
 NBC_FP64, NBC_MIXED = 0, 1
 NBC_PENE_CYLINDER, NBC_PENE_SPHERE, NBC_PENE_CONE = 0, 1, 2
 NBC_PENE_CONSTANT_VELOCITY, NBC_PENE_GRAVITY = 0, 1
+NBC_DRIVER_CLUMP, NBC_DRIVER_CUBE, NBC_DRIVER_CYLINDER, NBC_DRIVER_SPLIT, NBC_DRIVER_HGRID = range(5)
 
 D3 = C.c_double * 3
 D4 = C.c_double * 4
@@ -49,7 +50,7 @@ class Options(C.Structure):
         ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
         ("hgrid_levels", C.c_int32), ("graphs", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
-        ("reserved", C.c_double * 3),
+        ("driver", C.c_int32), ("reserved_i", C.c_int32), ("reserved", C.c_double * 2),
     ]
 
 
@@ -93,6 +94,14 @@ class ClumpsIn(C.Structure):
     ]
 
 
+class ParticlesIn(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32), ("reserved", C.c_int32), ("type", C.POINTER(C.c_int32)),
+        ("pos", C.POINTER(C.c_double)), ("vel", C.POINTER(C.c_double)), ("quat", C.POINTER(C.c_double)),
+        ("angv", C.POINTER(C.c_double)), ("prop", C.POINTER(C.c_double)), ("radius", C.POINTER(C.c_double)),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("step", C.c_int64), ("rebuilds", C.c_int64), ("front_pairs", C.c_int64),
@@ -130,6 +139,7 @@ _SIGS = {
     "nbc_load_options": ([C.c_char_p, C.POINTER(Options)], C.c_int),
     "nbc_load_phase": ([C.c_char_p, I32, P, P, P, P, P], C.c_int),
     "nbc_load_clumps": ([C.c_char_p, I32, P, P, P, P, P], C.c_int),
+    "nbc_load_particles": ([C.c_char_p, I32, P, P, P, P, P, P, P], C.c_int),
     "nbc_load_cube_wall": ([C.c_char_p, C.POINTER(Wall)], C.c_int),
     "nbc_load_cylinder_wall": ([C.c_char_p, C.POINTER(Wall)], C.c_int),
     "nbc_load_penetrator": ([C.c_char_p, C.POINTER(Penetrator)], C.c_int),
@@ -144,6 +154,8 @@ _SIGS = {
     "nbc_get_batch_ranges": ([P, P, P], C.c_int),
     "nbc_destroy": ([P], C.c_int),
     "nbc_upload": ([P, C.POINTER(SpheresIn), C.POINTER(ClumpsIn), C.POINTER(Wall), C.POINTER(Penetrator)], C.c_int),
+    "nbc_upload_particles": ([P, C.POINTER(ParticlesIn), C.POINTER(Wall)], C.c_int),
+    "nbc_get_particles": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
     "nbc_step": ([P, I32], C.c_int),
     "nbc_rebuild": ([P], C.c_int),
     "nbc_set_record": ([P, I32], C.c_int),
@@ -162,11 +174,13 @@ _SIGS = {
     "nbc_profile_enable": ([P, I32], C.c_int),
     "nbc_profile_get": ([P, I32, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
     "nbc_write_particles_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P], C.c_int),
+    "nbc_write_spheres_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P, P, P], C.c_int),
     "nbc_write_clumps_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P, P, P], C.c_int),
     "nbc_write_force_history_bin": ([C.c_char_p, I32, I32, P, P], C.c_int),
     "nbc_write_yi_bin": ([C.c_char_p, I32, I32, P, P, P, P, P], C.c_int),
     "nbc_write_contacts_bin": ([C.c_char_p, I32, I32, I32, P, P, P], C.c_int),
     "nbc_run_simulation_clump": ([C.c_char_p, I32], C.c_int),
+    "nbc_run_simulation": ([C.c_char_p, I32, I32], C.c_int),
     "nbc_autotune": ([C.POINTER(Params), C.POINTER(Options), C.POINTER(SpheresIn), C.POINTER(ClumpsIn),
                       C.POINTER(Wall), C.POINTER(Penetrator), I32, I32, I32, C.POINTER(Options),
                       C.POINTER(TuneEntry), I32, C.POINTER(I32)], C.c_int),
@@ -276,6 +290,65 @@ class Bed:
             return cls(p, *[z[k] for k in cls._ARRAYS], w)
 
 
+class Particles:
+    """Host state of a DEMmain single-sphere bed in file order (numpy copies):
+    the columns of the 21-column ph file (type, pos, vel, quat, angv, prop =
+    Ix Iy Iz mass, radius)."""
+
+    _ARRAYS = ("type", "pos", "vel", "quat", "angv", "prop", "radius")
+
+    def __init__(self, params, type_, pos, vel, quat, angv, prop, radius, wall):
+        self.params = params
+        self.type = np.ascontiguousarray(type_, dtype=np.int32)
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        self.quat = np.ascontiguousarray(quat, dtype=np.float64).reshape(-1, 4)
+        self.angv = np.ascontiguousarray(angv, dtype=np.float64).reshape(-1, 3)
+        self.prop = np.ascontiguousarray(prop, dtype=np.float64).reshape(-1, 4)
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.wall = wall
+        self.params.ClumpNum = self.params.ParticleNum = len(self.radius)
+
+    @property
+    def n(self):
+        return len(self.radius)
+
+    @classmethod
+    def from_bed(cls, bed, wall=None):
+        """A bed of one-sphere clumps (generator kind 3) as DEMmain particles."""
+        if bed.n != bed.nc or np.any(bed.local != 0.0) or np.any(bed.clump_id != np.arange(bed.n)):
+            raise ValueError("from_bed: every clump must be one sphere at its centre")
+        p = Params()
+        C.memmove(C.byref(p), C.byref(bed.params), C.sizeof(Params))
+        return cls(p, np.zeros(bed.n, np.int32), bed.cpos.copy(), bed.cvel.copy(), bed.cquat.copy(),
+                   bed.cangv.copy(), bed.cprop.copy(), bed.radius.copy(), wall if wall is not None else bed.wall)
+
+    def particles_in(self):
+        def dp(a):
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+        return ParticlesIn(self.n, 0, self.type.ctypes.data_as(C.POINTER(C.c_int32)), dp(self.pos), dp(self.vel),
+                           dp(self.quat), dp(self.angv), dp(self.prop), dp(self.radius))
+
+    def finalized_params(self, opt=None):
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self.params), C.sizeof(Params))
+        check(lib().nbc_finalize_params(C.byref(p), ptr(self.radius), self.n, C.byref(opt) if opt else None))
+        return p
+
+    def save(self, path):
+        np.savez(path, params=np.frombuffer(bytes(self.params), np.uint8),
+                 wall=np.frombuffer(bytes(self.wall), np.uint8), **{k: getattr(self, k) for k in self._ARRAYS})
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path) as z:
+            if z["params"].size != C.sizeof(Params) or z["wall"].size != C.sizeof(Wall):
+                raise ValueError(f"{path}: parameter block layout does not match this library")
+            p = Params.from_buffer_copy(z["params"].tobytes())
+            w = Wall.from_buffer_copy(z["wall"].tobytes())
+            return cls(p, *[z[k] for k in cls._ARRAYS], w)
+
+
 def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
     spec = BedSpec(kind, nclumps, seed, D4(*container))
     out = BedOut()
@@ -299,8 +372,9 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0):
+                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0):
     o = Options()
+    o.driver = driver
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
     o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
     o.hgrid_levels = hgrid_levels
@@ -367,6 +441,21 @@ class Context:
         self._pene = pene
         self._c(lib().nbc_upload(self.h, C.byref(s), C.byref(c), C.byref(wall) if wall else None,
                                  C.byref(pene) if pene else None))
+
+    def upload_particles(self, part, wall):
+        """DEMmain drivers: one sphere per particle (nbc_upload_particles)."""
+        q = part.particles_in()
+        self._part = part
+        self._c(lib().nbc_upload_particles(self.h, C.byref(q), C.byref(wall)))
+
+    def particles(self):
+        N = self.N
+        d = dict(sid=np.zeros(N, np.int32), fid=np.zeros(N, np.int32), type=np.zeros(N, np.int32),
+                 pos=np.zeros((N, 3)), vel=np.zeros((N, 3)), quat=np.zeros((N, 4)), angv=np.zeros((N, 3)),
+                 prop=np.zeros((N, 4)), radius=np.zeros(N))
+        self._c(lib().nbc_get_particles(self.h, *[ptr(d[k]) for k in ("sid", "fid", "type", "pos", "vel", "quat",
+                                                                      "angv", "prop", "radius")]))
+        return d
 
     def step(self, n=1):
         self._c(lib().nbc_step(self.h, n))

@@ -29,6 +29,14 @@
 //    with the clump kinematics of src/dem_force_cube_wall_clump.cu:16-171)
 //    has no reference counterpart and is the oracle's own composition.
 //
+// DEMmain's single-sphere loops (src/dem_simulation_{cube,cylind,split,hgrid}.cu)
+// run as one-sphere clumps (orc_upload_particles) with their own laws, chosen
+// by nbc_options.driver: the sphere-spin Vrot (src/dem_force_particle.cu:87-88),
+// the per-sphere displacement as a sum of drifts (src/dem_integrate.cu:36-38),
+// the DEMmain cube wall (src/dem_force_cube_wall.cu: no history, no rolling or
+// twisting torque), the glue types (src/dem_integrate_withglue.cu) and the
+// cylinder loops' history exchange after a rebuild.
+//
 // Parity pins: tests/test_oracle_*.py check this file against closed forms
 // (Hertz normal force, Coulomb/rolling/twisting caps, free fall, torque-free
 // rotation, restitution band), brute-force O(N^2) neighbour sets,
@@ -117,6 +125,10 @@ struct orc_ctx {
     double pene_bound = 0.0;
     std::vector<d3> PFc, PTc, PpeneTc, PFcn, PFct, PTcn, PTct, PRVo, PRVn;
     std::vector<int> PLcFlg;
+    // DEMmain loops: glue types and the step's drift, per particle (= clump)
+    int driver = NBC_DRIVER_CLUMP;
+    std::vector<int> Type;
+    std::vector<d3> CDist;
     Hist3 PH, PHpre;
     d3 pene_pos{}, pene_vel{}, pene_angv{};
     d4 pene_quat{};
@@ -196,6 +208,10 @@ extern "C" orc_ctx* orc_create(const nbc_params* p, const nbc_options* o) {
     c->S = (long long)c->N * (c->B / 2);
     c->wall_type = p->isWallForce ? (c->O.wall_type ? c->O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
     c->pene_frcoe = (c->O.pene_frcoe > 0.0) ? c->O.pene_frcoe : p->Frcoe;
+    c->driver = c->O.driver;
+    // the DEMmain loops apply their wall unconditionally
+    if (c->driver == NBC_DRIVER_CUBE) c->wall_type = NBC_WALL_CUBE;
+    else if (c->driver != NBC_DRIVER_CLUMP) c->wall_type = NBC_WALL_CYLINDER;
     return c;
 }
 
@@ -294,7 +310,25 @@ extern "C" int orc_upload(orc_ctx* c, const nbc_spheres_in* s, const nbc_clumps_
     c->CacheFi.assign(oi, d3{0, 0, 0}); c->CacheTi.assign(oi, d3{0, 0, 0});
     c->Update = (unsigned)c->P.Update;
     c->step = 0; c->rebuilds = 0; c->count = 0; c->first = true;
+    c->CDist.assign(Nc, d3{0, 0, 0});
+    if (c->Type.size() != (size_t)Nc) c->Type.assign(Nc, 0);
     return 0;
+}
+
+// DEMmain::CopyDataFromHostToDevice (src/dem.cu:62-72): particle i is clump i
+// with one sphere at its centre.
+extern "C" int orc_upload_particles(orc_ctx* c, const nbc_particles_in* q, const nbc_wall* w) {
+    if (c->driver == NBC_DRIVER_CLUMP) return fail(c, NBC_E_BADINPUT, "orc_upload_particles: clump driver");
+    if (!q || q->n != c->N || c->Nc != c->N) return fail(c, NBC_E_BADINPUT, "orc_upload_particles: size mismatch");
+    const int N = c->N;
+    std::vector<int> cid(N);
+    std::vector<double> local(3 * (size_t)N, 0.0), mass(N);
+    for (int i = 0; i < N; ++i) { cid[i] = i; mass[i] = q->prop[4 * (size_t)i + 3]; }
+    c->Type.assign(N, 0);
+    if (q->type) c->Type.assign(q->type, q->type + N);
+    nbc_spheres_in si{N, 0, cid.data(), q->pos, local.data(), mass.data(), q->radius};
+    nbc_clumps_in ci{N, 0, q->pos, q->vel, q->quat, q->angv, q->prop};
+    return orc_upload(c, &si, &ci, w, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,6 +338,22 @@ static void integrate_before(orc_ctx* c) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < c->Nc; ++i) {
         d3 pos = c->CPos[i], vel = c->CVel[i], w = c->CAngSpd[i];
+        const int ty = c->driver == NBC_DRIVER_SPLIT ? c->Type[i] : 0;
+        if (ty == 2) {  // src/dem_integrate_withglue.cu:7
+            c->CDist[i] = d3{0, 0, 0};
+            continue;
+        }
+        if (ty == 1) {  // src/dem_integrate_withglue.cu:46-57
+            double Omega = 0.2;
+            vel.x = -Omega * pos.y;
+            vel.y = -Omega * pos.x;
+            vel.z = 0.0;
+            d3 dist = Dt * vel;
+            pos = pos + dist;
+            c->CPos[i] = pos; c->CVel[i] = vel;
+            c->CDist[i] = dist;
+            continue;
+        }
         d3 F = c->CFc[i], T = c->CTc[i];
         const d4 pr = c->CProp[i];
         F = (1.0 / pr.w) * F;
@@ -316,9 +366,11 @@ static void integrate_before(orc_ctx* c) {
         vel = vel + 0.5 * Dt * F;
         w = w + 0.5 * Dt * T;
         c->CPos[i] = pos; c->CVel[i] = vel; c->CAngSpd[i] = w;
+        c->CDist[i] = dist;
     }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < c->Nc; ++i) {
+        if (c->driver == NBC_DRIVER_SPLIT && c->Type[i] != 0) continue;  // src/dem_integrate_withglue.cu:68
         d3 W = c->CAngSpd[i], T = c->CTc[i];
         const d4 pr = c->CProp[i];
         T.x = (1.0 / pr.x) * T.x;
@@ -355,8 +407,20 @@ static void update_pos(orc_ctx* c) {
         Quat2IvDCM(c->CQuat[cl], DCM);
         d3 arm = Map(DCM, c->LocalPos[i]);
         d3 up = c->CPos[cl] + arm;
-        d3 cum = c->CumPos[i] + up - c->Pos[i];
-        if (Length(cum) > th) flag = 1u;
+        d3 cum;
+        if (c->driver) {
+            // DEMmain: cumpos = cumpos + dist (src/dem_integrate.cu:37); the glue
+            // loop rebuilds past 0.1 r for type 0 only (src/dem_integrate_withglue.cu:40)
+            cum = c->CumPos[i] + c->CDist[cl];
+            if (c->driver == NBC_DRIVER_SPLIT) {
+                if (c->Type[cl] == 0 && Length(cum) > 0.10 * c->Radius[i]) flag = 1u;
+            } else if (Length(cum) > th) {
+                flag = 1u;
+            }
+        } else {
+            cum = c->CumPos[i] + up - c->Pos[i];
+            if (Length(cum) > th) flag = 1u;
+        }
         c->CumPos[i] = cum;
         c->Pos[i] = up;
     }
@@ -662,7 +726,9 @@ static int rebuild(orc_ctx* c) {
                 }
         }
     }
-    c->D.zero();
+    // the cylinder loops exchange right after this and read D (the last
+    // written histories, old slots) as their Dpre; the others start D clean
+    if (c->driver != NBC_DRIVER_CYLINDER && c->driver != NBC_DRIVER_SPLIT) c->D.zero();
     c->Update = 0;
     std::fill(c->CumPos.begin(), c->CumPos.end(), d3{0, 0, 0});
     c->rebuilds++;
@@ -706,7 +772,8 @@ static bool ss_force(orc_ctx* c, long long k) {
     ARMi = ARMi + Ap * DistU;
     ARMj = ARMj - An * DistU;
     d3 Vrel = c->CVel[cJ] - c->CVel[cI];
-    d3 Vrot = Cross(c->CAngSpd[cJ], ARMj) - Cross(c->CAngSpd[cI], ARMi);
+    d3 Vrot = c->driver ? An * Cross(DistU, c->CAngSpd[cJ]) + Ap * Cross(DistU, c->CAngSpd[cI])  // particle.cu:88
+                        : Cross(c->CAngSpd[cJ], ARMj) - Cross(c->CAngSpd[cI], ARMi);
     d3 Vtot = Vrel + Vrot;
     double ERR = Dot(DistU, Vtot);
     d3 Vnor = ERR * DistU;
@@ -834,7 +901,8 @@ static bool wall_face(orc_ctx* c, int i, const WallKin& wk, const d3& RVo, const
     Arm = -1.0 * Arm + Ap * DistU;
     d3 Vwall = Cross(wk.angv, RVo);
     d3 Vrel = c->CVel[clumpI] - wk.vel;
-    d3 Vrot = Cross(Arm, c->CAngSpd[clumpI]) - Vwall;
+    d3 Vrot = c->driver ? Ap * Cross(DistU, c->CAngSpd[clumpI]) - Vwall  // src/dem_force_cylind_wall.cu:55
+                        : Cross(Arm, c->CAngSpd[clumpI]) - Vwall;
     d3 Vtot = Vrel + Vrot;
     double ERR = Dot(DistU, Vtot);
     d3 Vnor = ERR * DistU;
@@ -884,6 +952,43 @@ static bool wall_face(orc_ctx* c, int i, const WallKin& wk, const d3& RVo, const
     return true;
 }
 
+// DEMmain cube wall for one face: src/dem_force_cube_wall.cu:3-148 as it
+// runs: normal and Ap from RVn, Dn = r - |RVn| from the caller (:170-183),
+// histories read from arrays that are never written (zero), rolling and
+// twisting moments computed but not applied.
+static void wall_face_dm_cube(orc_ctx* c, int i, const WallKin& wk, const d3& RVo, const d3& RVn, double Dn,
+                              d3& Fc, d3& Tc) {
+    const nbc_params& P = c->P;
+    const int clumpI = c->ClumpId[i];
+    d3 Dist = RVn;
+    double DistS = Dist.x * Dist.x + Dist.y * Dist.y + Dist.z * Dist.z;
+    double DistL = std::sqrt(DistS);
+    d3 H = d3{0, 0, 0};
+    double DistR = 1.0 / DistL;
+    d3 DistU = DistR * Dist;
+    double Ap = DistL;
+    double Rij = c->Radius[i], Mij = c->Mass[i];
+    double Kn = 2.0 * P.Ymod * std::sqrt(Rij * Dn) / (3.0 * (1.0 - P.Prat * P.Prat));
+    double Cn = P.Damping * std::sqrt(Kn * Mij);
+    double Ks = 2.0 * P.Ymod / (1.0 + P.Prat) / (2.0 - P.Prat) * std::sqrt(Rij * Dn);
+    double Cs = 2.0 * P.Damping * std::sqrt(Ks * Mij);
+    d3 Vwall = Cross(wk.angv, RVo);
+    d3 Vrel = c->CVel[clumpI] - wk.vel;
+    d3 Vrot = Ap * Cross(DistU, c->CAngSpd[clumpI]) - Vwall;
+    d3 Vtot = Vrel + Vrot;
+    double ERR = Dot(DistU, Vtot);
+    d3 Vnor = ERR * DistU;
+    d3 Vtan = Vtot - Vnor;
+    d3 nf = Kn * Dn * DistU + Cn * Vnor;
+    double nfL = Length(nf);
+    d3 Ds = P.StepSize * Vtan;
+    d3 tf = -Ks * Ds + Cs * Vtan + H;
+    double tfL = Length(tf);
+    if (tfL > nfL * P.Frcoe) tf = P.Frcoe * nfL * (1.0 / tfL) * tf;
+    Fc = Fc + nf + tf;
+    Tc = Tc - Ap * Cross(DistU, tf);
+}
+
 static bool sphere_valid(orc_ctx* c, int i) {
     const int cid = c->ClumpId[i];
     if (cid < 0 || cid >= c->Nc) return false;
@@ -927,6 +1032,26 @@ static int wall_forces(orc_ctx* c) {
             const d3 nUx = mk3(-1.0, 0.0, 0.0), nLx = mk3(1.0, 0.0, 0.0);
             const d3 nUy = mk3(0.0, -1.0, 0.0), nLy = mk3(0.0, 1.0, 0.0);
             const d3 nUz = mk3(0.0, 0.0, -1.0), nLz = mk3(0.0, 0.0, 1.0);
+            if (c->driver == NBC_DRIVER_CUBE) {
+                // src/dem_force_cube_wall.cu:170-268, faces UpX DownX UpY DownY UpZ DownZ
+                const double E[3] = {ERRx, ERRy, ERRz};
+                const d3 RVt[3] = {RVtx, RVty, RVtz}, RVb[3] = {RVbx, RVby, RVbz};
+                for (int a = 0; a < 3; ++a) {
+                    for (int up = 1; up >= 0; --up) {
+                        const bool cand = up ? (E[a] > 0 && E[a] + r - S[a] > 0) : (E[a] < 0 && E[a] - r + S[a] < 0);
+                        if (!cand) continue;
+                        RVo = (S[a] / std::fabs(E[a])) * RVt[a] + RVb[a];
+                        RVn = RV - RVo;
+                        const double DistL = Length(RVn);
+                        wall_face_dm_cube(c, i, wk, RVo, RVn, r - DistL, F, T);
+                        touch++;
+                    }
+                }
+                c->WFc[i] = F;
+                c->WTc[i] = T;
+                if (!finite3(F) || !finite3(T)) bad = std::min(bad, i);
+                continue;
+            }
             // face order and history sets: 0 UpX, 1 LowX, 2 UpY, 3 LowY, 4 UpZ, 5 LowZ
             if (ERRx > 0 && ERRx + r - S[0] > 0) {
                 RVo = (S[0] / std::fabs(ERRx)) * RVtx + RVbx; RVn = RV - RVo;
@@ -1272,6 +1397,7 @@ static void clump_reduce_and_kick(orc_ctx* c) {
     const double Dt = c->P.StepSize, g = c->P.Gravity;
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < Nc; ++i) {
+        if (c->driver == NBC_DRIVER_SPLIT && c->Type[i] != 0) continue;  // src/dem_integrate_withglue.cu:139
         d3 Fi = c->CFc[i], Ti = c->CTc[i];
         const d4 pr = c->CProp[i];
         Fi = (1.0 / pr.w) * Fi;
@@ -1296,6 +1422,12 @@ static int step_once(orc_ctx* c) {
         c->rebuild_next = false;
         int e = rebuild(c);
         if (e) return e;
+        if (c->driver == NBC_DRIVER_CYLINDER || c->driver == NBC_DRIVER_SPLIT) {
+            // src/dem_simulation_cylind.cu:141: ForceExchange follows the rebuild
+            // too, swapping the inherited histories out (and clearing them)
+            c->D.swap(c->Dpre);
+            c->D.zero();
+        }
     } else {
         c->D.swap(c->Dpre);  // DEMForce::ForceExchange
         c->D.zero();

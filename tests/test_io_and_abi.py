@@ -248,13 +248,19 @@ def test_nbc_par_keys(tmp_path):
     f = tmp_path / "nbc.par"
     f.write_text("Precision = mixed\nWallType = cylinder\nBlockSize = 128\nRadixBits = 6\nPairCapacity = 20\n"
                  "PeneFrcoe = 0.4\nCellSize = 0.006\nHGridLevels = 3\nNearSkin = 0.5\nGraphs = -1\n"
-                 "BatchSteps = 8\n")
+                 "BatchSteps = 8\nDriver = split\n")
     o = nbc.Options()
     assert nbc.lib().nbc_load_options(str(f).encode(), C.byref(o)) == 0
     assert (o.precision, o.wall_type, o.block_size, o.radix_bits, o.pair_capacity) == \
         (nbc.NBC_MIXED, nbc.NBC_WALL_CYLINDER, 128, 6, 20)
     assert (o.pene_frcoe, o.cell_size, o.hgrid_levels, o.near_skin, o.graphs, o.batch_steps) == \
         (0.4, 0.006, 3, 0.5, -1, 8)
+    assert o.driver == nbc.NBC_DRIVER_SPLIT
+    for name, want in (("clump", 0), ("cube", 1), ("cylinder", 2), ("split", 3), ("hgrid", 4)):
+        f.write_text(f"Driver = {name}\n")
+        assert nbc.lib().nbc_load_options(str(f).encode(), C.byref(o)) == 0 and o.driver == want
+    f.write_text("Driver = spheres\n")
+    assert nbc.lib().nbc_load_options(str(f).encode(), C.byref(o)) == nbc.NBC_E_PARAM
     o2 = nbc.Options()
     assert nbc.lib().nbc_load_options(str(tmp_path / "absent.par").encode(), C.byref(o2)) == 0
     assert o2.precision == nbc.NBC_FP64 and o2.hgrid_levels == 0
@@ -273,3 +279,49 @@ def test_c2_bed_text_asks_for_the_hierarchical_grid(tmp_path):
     assert o.hgrid_levels == 3 and o.wall_type == nbc.NBC_WALL_CUBE
     rc, p, _ = load_params(str(tmp_path / "world.par"))
     assert rc == 0 and p.GridLevels == 3 and p.Frcoe == 0.6
+
+
+def test_particle_text_and_binary_layout(tmp_path):
+    # DEMmain ph text (21 columns) in, DEMSphereIo::SaveParticles_Bin layout out
+    rng = np.random.default_rng(4)
+    n = 7
+    t = rng.integers(0, 3, n).astype(np.int32)
+    cols = [rng.normal(size=(n, k)) for k in (3, 3, 4, 3, 4)] + [rng.uniform(1e-3, 2e-3, n)]
+    rows = np.column_stack([np.arange(n), np.zeros(n), t] + cols)
+    np.savetxt(tmp_path / "ph.txt", rows, fmt=["%d", "%d", "%d"] + ["%.17g"] * 18)
+    t2 = np.zeros(n, np.int32)
+    got = [np.zeros_like(c) for c in cols]
+    assert nbc.lib().nbc_load_particles(str(tmp_path / "ph.txt").encode(), n, nbc.ptr(t2),
+                                        *[nbc.ptr(g) for g in got]) == 0
+    assert np.array_equal(t2, t) and all(np.array_equal(a, b) for a, b in zip(got, cols))
+    # a short file is an I/O error naming the row
+    assert nbc.lib().nbc_load_particles(str(tmp_path / "ph.txt").encode(), n + 1, nbc.ptr(np.zeros(n + 1, np.int32)),
+                                        *[nbc.ptr(np.zeros((n + 1,) + g.shape[1:])) for g in got]) == nbc.NBC_E_IO
+    assert b"row 7" in nbc.lib().nbc_last_error(None)
+    sid, fid = np.arange(n, dtype=np.int32), rng.permutation(n).astype(np.int32)
+    assert nbc.lib().nbc_write_spheres_bin(str(tmp_path).encode(), 5, n, nbc.ptr(sid), nbc.ptr(fid), nbc.ptr(t),
+                                           *[nbc.ptr(c) for c in cols]) == 0
+    raw = open(tmp_path / "ph.00000005.bin", "rb").read()
+    assert len(raw) == 4 + 3 * 4 * n + 8 * n * 18
+    assert np.frombuffer(raw[:4], np.int32)[0] == n
+    ints = np.frombuffer(raw[4:4 + 12 * n], np.int32).reshape(3, n)
+    assert np.array_equal(ints[0], sid) and np.array_equal(ints[1], fid) and np.array_equal(ints[2], t)
+    dbl = np.frombuffer(raw[4 + 12 * n:], np.float64)
+    assert np.array_equal(dbl, np.concatenate([c.ravel() for c in cols]))
+
+
+def test_demmain_driver_argument_checks():
+    bed = nbc.generate_bed(3, 50, seed=2, container=(0.01, 0, 0, 0))
+    assert bed.n == bed.nc == 50 and np.all(bed.local == 0.0)
+    part = nbc.Particles.from_bed(bed)
+    assert np.array_equal(part.pos, bed.cpos) and np.all(part.type == 0)
+    # solid spheres: I = 0.4 m r^2
+    assert np.allclose(part.prop[:, 0], 0.4 * part.prop[:, 3] * part.radius ** 2, rtol=1e-6)
+    with pytest.raises(ValueError):
+        nbc.Particles.from_bed(nbc.generate_bed(0, 10, seed=1))
+    if nbc.device_count() == 0:
+        return
+    opt = nbc.make_options(driver=nbc.NBC_DRIVER_HGRID)
+    h = C.c_void_p()
+    assert nbc.lib().nbc_create(C.byref(h), C.byref(part.finalized_params(opt)), C.byref(opt), 0) == \
+        nbc.NBC_E_BADINPUT
