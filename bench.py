@@ -146,6 +146,82 @@ def run_c2(a):
     }))
 
 
+def run_c5(a):
+    """DEMmain's single-sphere cylinder loop (RunSimulation, not the headline
+    line): 1,000,000 single spheres (radii as C1) deposited in a cylinder of
+    radius 60 mm, settled, then stepped at dt 5e-7 s in the FP32-mixed and
+    FP64 forms; plus the glue loop (RunSimulation_Split) with every tenth
+    sphere fixed and every tenth driven."""
+    import numpy as np
+
+    import nbc
+
+    if a.bed_cache and os.path.exists(a.bed_cache):
+        part = nbc.Particles.load(a.bed_cache)
+    else:
+        t0 = time.perf_counter()
+        part = nbc.Particles.from_bed(nbc.generate_bed(3, a.nclumps if a.nclumps != 200000 else 1000000, seed=1))
+        print(f"[bench] C5 bed: {part.n} spheres in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if a.settle_steps > 0:
+            opt = nbc.make_options(precision=nbc.NBC_MIXED, driver=nbc.NBC_DRIVER_CYLINDER)
+            saved = part.params.StepSize
+            part.params.StepSize = a.settle_dt
+            ctx = nbc.Context(part.finalized_params(opt), opt)
+            ctx.upload_particles(part, part.wall)
+            for done in range(0, a.settle_steps, 5000):
+                ctx.step(min(5000, a.settle_steps - done))
+                st = ctx.stats()
+                print(f"[bench] settle step {st.step}: touching {st.touching_ss / part.n:.3f}/sphere, "
+                      f"KE {st.kinetic_energy:.4e} J", file=sys.stderr, flush=True)
+            cl = ctx.clumps()
+            for k in ("pos", "vel", "quat", "angv"):
+                getattr(part, k)[:] = cl[k]
+            ctx.close()
+            part.params.StepSize = saved
+        if a.bed_cache:
+            part.save(a.bed_cache)
+    forms = {}
+    for name, driver, prec in (("cylinder_mixed", nbc.NBC_DRIVER_CYLINDER, nbc.NBC_MIXED),
+                               ("cylinder_fp64", nbc.NBC_DRIVER_CYLINDER, nbc.NBC_FP64),
+                               ("split_mixed", nbc.NBC_DRIVER_SPLIT, nbc.NBC_MIXED)):
+        if prec == nbc.NBC_FP64 and a.no_fp64:
+            continue
+        p2 = nbc.Particles(part.params, part.type.copy(), part.pos, part.vel, part.quat, part.angv, part.prop,
+                           part.radius, part.wall)
+        if driver == nbc.NBC_DRIVER_SPLIT:
+            p2.type[:] = np.where(np.arange(part.n) % 10 == 3, 2, np.where(np.arange(part.n) % 10 == 7, 1, 0))
+        opt = nbc.make_options(precision=prec, driver=driver)
+        ctx = nbc.Context(p2.finalized_params(opt), opt)
+        ctx.upload_particles(p2, p2.wall)
+        ctx.step(a.warmup)
+        s0 = ctx.stats()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.steps)
+        t1 = time.perf_counter()
+        s1 = ctx.stats()
+        ctx.profile(True)
+        ctx.step(max(a.prof_steps, 1))
+        ctx.sync()
+        k3_ms, k3_n = ctx.profile_get(1)
+        forms[name] = dict(sphere_steps_per_s=part.n * a.steps / (t1 - t0), ms_per_step=(t1 - t0) / a.steps * 1e3,
+                           k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, rebuilds=s1.rebuilds - s0.rebuilds,
+                           touching_ss_per_sphere=s1.touching_ss / part.n, touching_wall=s1.touching_wall)
+        print(f"[bench] C5 {name}: {forms[name]}", file=sys.stderr, flush=True)
+        ctx.close()
+    m = forms["cylinder_mixed"]
+    print(json.dumps({
+        "metric": "sphere-steps/s, DEMmain single-sphere cylinder loop (FP32-mixed)",
+        "value": m["sphere_steps_per_s"], "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator, seed 1)",
+        "config": {"workload": "C5 bed: 1,000,000 single spheres, lognormal radii (median 1 mm), cylinder R 60 mm, "
+                               "dt 5e-7 s, settled; DEMmain cylinder loop",
+                   "n_spheres": part.n},
+        "forms": forms,
+    }))
+
+
 def run_c0(a):
     """The reference's small CPU-runnable case (not the headline line): 2,000
     clumps of 3 collinear spheres (6,000 spheres) in a 40 x 40 x 80 mm cube,
@@ -397,9 +473,10 @@ def main():
     ap.add_argument("--settle-dt", type=float, default=5e-6)
     ap.add_argument("--bed-cache", default=None,
                     help="load the settled bed from this .npz if it exists, else build it and save it there")
-    ap.add_argument("--workload", choices=("c0", "c1", "c2", "c3", "c4"), default="c1",
+    ap.add_argument("--workload", choices=("c0", "c1", "c2", "c3", "c4", "c5"), default="c1",
                     help="c1: the headline bed (default); c0: the small cube bed; c2: polydisperse bed on the "
-                         "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds")
+                         "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds; c5: DEMmain "
+                         "single-sphere loops")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
     ap.add_argument("--prof-steps", type=int, default=300,
                     help="steps of the second timed region, with HIP events around every kernel")
@@ -429,6 +506,9 @@ def main():
         return
     if a.workload == "c3":
         run_c3(a)
+        return
+    if a.workload == "c5":
+        run_c5(a)
         return
     if a.bed_cache and os.path.exists(a.bed_cache):
         bed = nbc.Bed.load(a.bed_cache)
