@@ -196,6 +196,13 @@ class Engine : public EngineBase {
         C.dist = nullptr;
         C.type = nullptr;
         if (driver) { HIPC(dalloc(&C.dist, (size_t)Nc)); HIPC(dalloc(&C.type, (size_t)Nc)); }
+        // second set for the clump renumbering at rebuilds
+        HIPC(dalloc(&C2.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C2.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C2.prop, Nc));
+        HIPC(dalloc(&C2.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C2.snap, 2 * (size_t)Nc));
+        C2.dist = nullptr;
+        C2.type = nullptr;
+        if (driver) { HIPC(dalloc(&C2.dist, (size_t)Nc)); HIPC(dalloc(&C2.type, (size_t)Nc)); }
+        HIPC(dalloc(&cfile, Nc)); HIPC(dalloc(&cfile2, Nc)); HIPC(dalloc(&cfirst, Nc)); HIPC(dalloc(&cnew, Nc));
         // lists
         for (int b = 0; b < 2; ++b) {
             HIPC(dalloc(&Lf[b].foff, (size_t)N + 1));
@@ -315,6 +322,8 @@ class Engine : public EngineBase {
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap); F(C.dist); F(C.type);
+        F(C2.pq); F(C2.vw); F(C2.prop); F(C2.ft); F(C2.snap); F(C2.dist); F(C2.type);
+        F(cfile); F(cfile2); F(cfirst); F(cnew);
         F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
@@ -506,6 +515,17 @@ class Engine : public EngineBase {
                               "Simulation stopped.", i);
                 return fail(NBC_E_NONFINITE, b);
             }
+        {
+            // every clump holds a sphere (clumps are renumbered by their first sphere)
+            std::vector<char> has(Nc, 0);
+            for (int i = 0; i < N; ++i) has[s->clump_id[i]] = 1;
+            for (int k = 0; k < Nc; ++k)
+                if (!has[k]) {
+                    char b[120];
+                    std::snprintf(b, sizeof b, "nbc_upload: clump %d has no sphere", k);
+                    return fail(NBC_E_BADINPUT, b);
+                }
+        }
         if (nfaces > 0) {
             if (!w) return fail(NBC_E_BADINPUT, "nbc_upload: isWallForce is set but no wall was given");
             bool okw = true;
@@ -558,6 +578,11 @@ class Engine : public EngineBase {
         HIPC(hipMemcpy(C.prop, prop.data(), sizeof(D4) * prop.size(), hipMemcpyHostToDevice));
         HIPC(hipMemcpy(C.ft, ft.data(), sizeof(D4) * ft.size(), hipMemcpyHostToDevice));
         HIPC(hipMemset(C.snap, 0, sizeof(R4) * 2 * (size_t)Nc));
+        {
+            std::vector<int> iota(Nc);
+            for (int k = 0; k < Nc; ++k) iota[k] = k;
+            HIPC(hipMemcpy(cfile, iota.data(), sizeof(int) * (size_t)Nc, hipMemcpyHostToDevice));
+        }
         if (driver) {
             HIPC(hipMemset(C.dist, 0, sizeof(D4) * (size_t)Nc));
             HIPC(hipMemcpy(C.type, htype.data(), sizeof(int) * (size_t)Nc, hipMemcpyHostToDevice));
@@ -653,6 +678,18 @@ class Engine : public EngineBase {
         KC(launch_gather<R>(st, dp, nwh, perm, so, sn));
         scur ^= 1;
         SphereBuf<R>& s = S[scur];
+        // 3b. clumps renumbered by their first sphere (no counterpart in the
+        //     reference): neighbouring spheres' clump data in neighbouring lines
+        {
+            KC(launch_fill_i32(st, cfirst, Nc, N));
+            KC(launch_clump_renumber_first(st, N, s.cid, cfirst, block));
+            KC(launch_keys_from_int(st, cfirst, key0, val0, Nc));
+            int sec = 0;
+            KC(launch_radix_sort(st, key0, val0, key1, val1, Nc, bits_for(N), rbits, hist, hscan, stmp, stotal, &sec));
+            KC(launch_clump_gather<R>(st, Nc, sec ? val1 : val0, C, C2, cfile, cfile2, cnew, s.cid, N, block));
+            std::swap(C, C2);
+            std::swap(cfile, cfile2);
+        }
         // 4. neighbour counts, overflow check (src/dem_grid.cu:466-552, 709-719)
         FrontList& Lo = Lf[lcur];
         FrontList& Ln = Lf[lcur ^ 1];
@@ -1089,7 +1126,13 @@ class Engine : public EngineBase {
         HIPC(hipStreamSynchronize(st));
         SphereBuf<R>& s = S[scur];
         if (fid) HIPC(hipMemcpy(fid, s.fid, sizeof(int) * N, hipMemcpyDeviceToHost));
-        if (cid) HIPC(hipMemcpy(cid, s.cid, sizeof(int) * N, hipMemcpyDeviceToHost));
+        if (cid) {
+            // device clump indices are renumbered at rebuilds: report file indices
+            HIPC(hipMemcpy(cid, s.cid, sizeof(int) * N, hipMemcpyDeviceToHost));
+            std::vector<int> cf(Nc);
+            HIPC(hipMemcpy(cf.data(), cfile, sizeof(int) * (size_t)Nc, hipMemcpyDeviceToHost));
+            for (int i = 0; i < N; ++i) cid[i] = cf[cid[i]];
+        }
         if (pos || radius) {
             std::vector<D4> v(N);
             HIPC(hipMemcpy(v.data(), s.pos, sizeof(D4) * N, hipMemcpyDeviceToHost));
@@ -1260,9 +1303,17 @@ class Engine : public EngineBase {
     int get_clumps(double* pos, double* vel, double* quat, double* angv, double* F, double* T) override {
         HIPC(hipSetDevice(device));
         HIPC(hipStreamSynchronize(st));
-        std::vector<D4> a(2 * (size_t)Nc);
+        // file order: device clump c is file clump cf[c]
+        std::vector<int> cf(Nc);
+        HIPC(hipMemcpy(cf.data(), cfile, sizeof(int) * (size_t)Nc, hipMemcpyDeviceToHost));
+        std::vector<D4> a(2 * (size_t)Nc), d(2 * (size_t)Nc);
+        auto fetch = [&](const D4* src) -> hipError_t {
+            hipError_t e = hipMemcpy(d.data(), src, sizeof(D4) * d.size(), hipMemcpyDeviceToHost);
+            for (int c = 0; c < Nc; ++c) { a[2 * (size_t)cf[c]] = d[2 * (size_t)c]; a[2 * (size_t)cf[c] + 1] = d[2 * (size_t)c + 1]; }
+            return e;
+        };
         if (pos || quat) {
-            HIPC(hipMemcpy(a.data(), C.pq, sizeof(D4) * a.size(), hipMemcpyDeviceToHost));
+            HIPC(fetch(C.pq));
             for (int i = 0; i < Nc; ++i) {
                 if (pos) { pos[3 * i] = a[2 * i].x; pos[3 * i + 1] = a[2 * i].y; pos[3 * i + 2] = a[2 * i].z; }
                 if (quat) {
@@ -1272,14 +1323,14 @@ class Engine : public EngineBase {
             }
         }
         if (vel || angv) {
-            HIPC(hipMemcpy(a.data(), C.vw, sizeof(D4) * a.size(), hipMemcpyDeviceToHost));
+            HIPC(fetch(C.vw));
             for (int i = 0; i < Nc; ++i) {
                 if (vel) { vel[3 * i] = a[2 * i].x; vel[3 * i + 1] = a[2 * i].y; vel[3 * i + 2] = a[2 * i].z; }
                 if (angv) { angv[3 * i] = a[2 * i + 1].x; angv[3 * i + 1] = a[2 * i + 1].y; angv[3 * i + 2] = a[2 * i + 1].z; }
             }
         }
         if (F || T) {
-            HIPC(hipMemcpy(a.data(), C.ft, sizeof(D4) * a.size(), hipMemcpyDeviceToHost));
+            HIPC(fetch(C.ft));
             for (int i = 0; i < Nc; ++i) {
                 if (F) { F[3 * i] = a[2 * i].x; F[3 * i + 1] = a[2 * i].y; F[3 * i + 2] = a[2 * i].z; }
                 if (T) { T[3 * i] = a[2 * i + 1].x; T[3 * i + 1] = a[2 * i + 1].y; T[3 * i + 2] = a[2 * i + 1].z; }
@@ -1404,6 +1455,10 @@ class Engine : public EngineBase {
     int driver = NBC_DRIVER_CLUMP;
     bool particle_upload = false;
     std::vector<int> htype;     // glue types, file order
+    // clump renumbering: second clump buffer set, file index of each device
+    // clump (cfile), first sphere per clump, old -> new index
+    ClumpBuf<R> C2{};
+    int *cfile = nullptr, *cfile2 = nullptr, *cfirst = nullptr, *cnew = nullptr;
     std::vector<double> hprop;  // particle Property (Ix Iy Iz m), file order
     std::vector<int> soff, coff;
     std::vector<BedGeo> hbeds;

@@ -105,6 +105,13 @@ int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm
 template <class R>
 int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
                       const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
+// Clump renumbering at a rebuild: first[c] = first sphere of clump c (first
+// filled with N before), then the clump arrays move to the order of first
+// (order[new] = old) and every sphere's clump index follows.
+int launch_clump_renumber_first(hipStream_t st, int N, const int* cid, int* first, int block);
+template <class R>
+int launch_clump_gather(hipStream_t st, int Nc, const int* order, ClumpBuf<R> src, ClumpBuf<R> dst,
+                        const int* cfile, int* cfile_out, int* newid, int* cid, int N, int block);
 // DEMmain cylinder/glue loops: new slot s takes old slot s's last history
 template <class R>
 int launch_hist_carry(hipStream_t st, long long nnew, long long nold, int p, uint32_t* nfword, R* nhist,

@@ -563,6 +563,43 @@ __global__ void k_gather(DevParams P, int nface, const int* perm, SphereBuf<R> s
     }
 }
 
+// Clump renumbering at a rebuild (no counterpart in the reference, whose
+// clump arrays stay in file order): clumps take the order of their first
+// sphere in the new sphere order, so the clump data of neighbouring spheres
+// sits in neighbouring lines. first[c] = smallest sorted index of c's spheres.
+__global__ void k_clump_first(int N, const int* cid, int* first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) atomicMin(&first[cid[i]], i);
+}
+
+// New clump c takes old clump order[c]: every clump array moves, newid maps
+// old to new, cfile follows (file index of each clump).
+template <class R>
+__global__ void k_clump_gather(int Nc, const int* order, ClumpBuf<R> src, ClumpBuf<R> dst, const int* cfile,
+                               int* cfile_out, int* newid) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nc) return;
+    const int o = order[c];
+    dst.pq[2 * c] = src.pq[2 * o];
+    dst.pq[2 * c + 1] = src.pq[2 * o + 1];
+    dst.vw[2 * c] = src.vw[2 * o];
+    dst.vw[2 * c + 1] = src.vw[2 * o + 1];
+    dst.prop[c] = src.prop[o];
+    dst.ft[2 * c] = src.ft[2 * o];
+    dst.ft[2 * c + 1] = src.ft[2 * o + 1];
+    dst.snap[2 * c] = src.snap[2 * o];
+    dst.snap[2 * c + 1] = src.snap[2 * o + 1];
+    if (src.dist) dst.dist[c] = src.dist[o];
+    if (src.type) dst.type[c] = src.type[o];
+    cfile_out[c] = cfile[o];
+    newid[o] = c;
+}
+
+__global__ void k_remap_cid(int N, int* cid, const int* newid) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) cid[i] = newid[cid[i]];
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour search, src/dem_grid.cu:466-552: 27 cells (z outer, x inner),
 // ascending j inside a cell, cut-off 1.5 (ri + rj). The search cell uses the
@@ -2360,6 +2397,20 @@ int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* 
     LAUNCH_CHECK();
     return 0;
 }
+int launch_clump_renumber_first(hipStream_t st, int N, const int* cid, int* first, int block) {
+    k_clump_first<<<nblk(N, block), block, 0, st>>>(N, cid, first);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_clump_gather(hipStream_t st, int Nc, const int* order, ClumpBuf<R> src, ClumpBuf<R> dst,
+                        const int* cfile, int* cfile_out, int* newid, int* cid, int N, int block) {
+    k_clump_gather<R><<<nblk(Nc, block), block, 0, st>>>(Nc, order, src, dst, cfile, cfile_out, newid);
+    LAUNCH_CHECK();
+    k_remap_cid<<<nblk(N, block), block, 0, st>>>(N, cid, newid);
+    LAUNCH_CHECK();
+    return 0;
+}
 template <class R>
 int launch_hist_carry(hipStream_t st, long long nnew, long long nold, int p, uint32_t* nfword, R* nhist,
                       const uint32_t* ofword, const R* ohist, int block) {
@@ -2433,6 +2484,8 @@ int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
     template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
+    template int launch_clump_gather<R>(hipStream_t, int, const int*, ClumpBuf<R>, ClumpBuf<R>, const int*, int*,   \
+                                        int*, int*, int, int);                                                         \
     template int launch_hist_carry<R>(hipStream_t, long long, long long, int, uint32_t*, R*, const uint32_t*,       \
                                       const R*, int);                                                                \
     template int launch_contacts<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, ListBuf<R>, int,       \
