@@ -1513,6 +1513,13 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
         if (qn > Q - 64) flush();
     };
 
+    // the back list's first chunk is fetched before the front scan
+    const int B0 = s_bo[wv][0], B1 = s_bo[wv][64];
+    int onext = 0, enext = 0;
+    if (B0 + lane < B1) {
+        onext = __builtin_nontemporal_load(L.nbo + B0 + lane);
+        enext = __builtin_nontemporal_load(L.nbe + B0 + lane);
+    }
     // front entries: pairs the row's sphere owns (it writes their history)
     {
         const int F0 = s_fo[wv][0], F1 = s_fo[wv][64];
@@ -1537,12 +1544,6 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     // back entries: the owner holds the history; the same pair is evaluated
     // again from the same inputs and the row's sphere takes the reaction
     {
-        const int B0 = s_bo[wv][0], B1 = s_bo[wv][64];
-        int onext = 0, enext = 0;
-        if (B0 + lane < B1) {
-            onext = __builtin_nontemporal_load(L.nbo + B0 + lane);
-            enext = __builtin_nontemporal_load(L.nbe + B0 + lane);
-        }
         for (int c = B0; c < B1; c += 64) {
             const int e = c + lane;
             const int o = onext, ne = enext;
