@@ -1124,7 +1124,9 @@ __device__ __forceinline__ void hm_coef(const DevParams& P, R Rij, R Mij, R Dn, 
     }
 }
 
-template <class R>
+// DM: a DEMmain single-sphere loop (spin about the sphere centre), fixed at
+// compile time so that the clump loop's kernels carry no extra registers
+template <class R, bool DM>
 __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& in, PairOut<R>& o) {
     const R Beta = (R)P.Beta, Frcoe = (R)P.Frcoe;
     const R dt = (R)P.dt, Coh = (R)P.Cohesion;
@@ -1142,8 +1144,8 @@ __device__ __forceinline__ void pair_force(const DevParams& P, const PairIn<R>& 
     const V3<R> Vrel = in.vJ - in.vI;
     // DEMmain loops: the sphere's own spin about its centre
     // (src/dem_force_particle.cu:87-88); the clump form otherwise
-    const V3<R> Vrot = P.driver ? An * Cross<R>(DistU, in.wJ) + Ap * Cross<R>(DistU, in.wI)
-                                : Cross<R>(in.wJ, ARMj) - Cross<R>(in.wI, ARMi);
+    const V3<R> Vrot = DM ? An * Cross<R>(DistU, in.wJ) + Ap * Cross<R>(DistU, in.wI)
+                          : Cross<R>(in.wJ, ARMj) - Cross<R>(in.wI, ARMi);
     const V3<R> Vtot = Vrel + Vrot;
     const R ERR = Dot<R>(DistU, Vtot);
     const V3<R> Vnor = ERR * DistU;
@@ -1208,7 +1210,7 @@ template <class R>
 struct WallOut {
     V3<R> nf, tf, rm, tm, ht, htw, hr, arm;
 };
-template <class R>
+template <class R, bool DM>
 __device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFactor, const V3<R>& DistU, R Ap, R Dn,
                                            R Rij, R Mij, const V3<R>& arm0, const V3<R>& RVo, const V3<R>& cv,
                                            const V3<R>& cw, const V3<R>& wv, const V3<R>& ww, const V3<R>& H,
@@ -1222,7 +1224,7 @@ __device__ __forceinline__ void wall_force(const DevParams& P, R Frcoe, R CsFact
     const V3<R> Vrel = cv - wv;
     // DEMmain walls spin the sphere about its centre (src/dem_force_cube_wall.cu:54,
     // src/dem_force_cylind_wall.cu:55)
-    const V3<R> Vrot = P.driver ? Ap * Cross<R>(DistU, cw) - Vwall : Cross<R>(Arm, cw) - Vwall;
+    const V3<R> Vrot = DM ? Ap * Cross<R>(DistU, cw) - Vwall : Cross<R>(Arm, cw) - Vwall;
     const V3<R> Vtot = Vrel + Vrot;
     const R ERR = Dot<R>(DistU, Vtot);
     const V3<R> Vnor = ERR * DistU;
@@ -1371,7 +1373,7 @@ __device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneSta
 // history-valid bits. Each lane adds its sphere's results in queue order,
 // which is its slot order: front entries ascending, then back entries in
 // back-row order.
-template <class R, bool RECORD>
+template <class R, bool RECORD, bool DM>
 __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
                                               const ListBuf<R>& L, int p, const WallBuf<R>& WL,
                                               unsigned long long* counters, int* bad, const RecordBuf& rec, int remap,
@@ -1450,7 +1452,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                     if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
                     else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
                     PairOut<R> o;
-                    pair_force<R>(P, in, o);
+                    pair_force<R, DM>(P, in, o);
                     V3<R> f, t;
                     if (side == 0) {
                         store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
@@ -1641,7 +1643,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
 // sphere-sphere kernel and leaves (Fw, Tw, Fp, Tp) per list entry for it. One
 // lane per list entry; the list is in sphere order, so the per-wave FP64
 // penetrator partial sums keep the order of the sphere index.
-template <class R, bool RECORD>
+template <class R, bool RECORD, bool DM>
 __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
                                            const WallBuf<R>& WL, int len, PeneState* pst, double* pene_partial,
                                            unsigned long long* counters, int* bad, const RecordBuf& rec, int vblock) {
@@ -1701,7 +1703,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                             const V3<double> RVb = RV - E * ea, RVt = E * ea;
                             const V3<double> RVo = (Sd / fabs(E)) * RVt + RVb;
                             const V3<double> RVn = RV - RVo;
-                            if (P.driver == NBC_DRIVER_CUBE) {
+                            if (DM && P.driver == NBC_DRIVER_CUBE) {
                                 // DEMmain cube wall (src/dem_force_cube_wall.cu:3-148, :170-183):
                                 // the normal and Ap come from RVn, Dn = r - |RVn| unchecked;
                                 // its histories are never written (zero), and its rolling
@@ -1713,7 +1715,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                                 const V3<R> DistU = (R(1.0) / DistLr) * cvt3<R>(RVn);
                                 const V3<R> Z = mk<R>(R(0), R(0), R(0));
                                 WallOut<R> o;
-                                wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi,
+                                wall_force<R, DM>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi,
                                               cvt3<R>(RVo), vi, wi, wv, ww, Z, Z, Z, o);
                                 Fw = Fw + o.nf + o.tf;
                                 Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
@@ -1731,7 +1733,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                                 if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
                                 else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
                                 WallOut<R> o;
-                                wall_force<R>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
+                                wall_force<R, DM>(P, (R)P.Frcoe, R(2.0), cvt3<R>(nin), (R)Ap, (R)Dn, Rr, mi, armi,
                                               cvt3<R>(RVo), vi, wi, wv, ww, H, Mt, Mr, o);
                                 Fw = Fw + o.nf + o.tf;
                                 Tw = Tw - (R)Ap * Cross<R>(cvt3<R>(nin), o.tf);
@@ -1771,7 +1773,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                                 if ((wf >> f) & 1u) load_hist<R>(hp, H, Mt, Mr);
                                 else { H = mk<R>(R(0), R(0), R(0)); Mt = H; Mr = H; }
                                 WallOut<R> o;
-                                wall_force<R>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
+                                wall_force<R, DM>(P, (R)P.Frcoe, R(2.0), DistU, DistLr, (R)Dn, Rr, mi, armi, cvt3<R>(RVo),
                                               vi, wi, wv, ww, H, Mt, Mr, o);
                                 Fw = Fw + o.nf + o.tf;
                                 Tw = Tw - DistLr * Cross<R>(DistU, o.tf);
@@ -1875,7 +1877,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                     const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
                     const V3<R> RVor = cvt3<R>(RVo);
                     WallOut<R> o;
-                    wall_force<R>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
+                    wall_force<R, false>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
                                   pw, H, Mt, Mr, o);
                     const V3<R> cf = mk<R>(R(0), R(0), R(0));
                     Fp = o.nf + o.tf;
@@ -1939,35 +1941,35 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
 // the other writes after k_clump_after). The FP32 form asks for five waves per
 // SIMD, which the contact blocks need and the rarer wall blocks pay for with
 // a few spills.
-template <class R, bool RECORD>
+template <class R, bool RECORD, bool DM>
 __device__ __forceinline__ void step3_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
                                            const ListBuf<R>& L, int p, const WallBuf<R>& WL, int len, int nwb,
                                            PeneState* pst, double* pene_partial, unsigned long long* counters,
                                            int* bad, const RecordBuf& rec, int remap, const int* halt, int n_arg) {
     if (*halt <= step_of(P, n_arg)) return;  // a rebuild of step n comes first
     if ((int)blockIdx.x < nwb)
-        walls_body<R, RECORD>(P, S, C, WL, len, pst, pene_partial, counters, bad, rec, blockIdx.x);
+        walls_body<R, RECORD, DM>(P, S, C, WL, len, pst, pene_partial, counters, bad, rec, blockIdx.x);
     else
-        contacts_body<R, RECORD>(P, S, C, L, p, WL, counters, bad, rec, remap, halt, (int)blockIdx.x - nwb,
+        contacts_body<R, RECORD, DM>(P, S, C, L, p, WL, counters, bad, rec, remap, halt, (int)blockIdx.x - nwb,
                                  (int)gridDim.x - nwb);
 }
 
-template <bool RECORD>
+template <bool RECORD, bool DM>
 __global__ __launch_bounds__(256, 5) void k_contacts_f32(DevParams P, SphereBuf<float> S, ClumpBuf<float> C,
                                                          ListBuf<float> L, int p, WallBuf<float> WL, int len, int nwb,
                                                          PeneState* pst, double* pene_partial,
                                                          unsigned long long* counters, int* bad, RecordBuf rec,
                                                          int remap, const int* halt, int n) {
-    step3_body<float, RECORD>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
+    step3_body<float, RECORD, DM>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
 }
 
-template <bool RECORD>
-__global__ __launch_bounds__(256) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
+template <bool RECORD, bool DM>
+__global__ __launch_bounds__(256, 3) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
                                                       ListBuf<double> L, int p, WallBuf<double> WL, int len, int nwb,
                                                       PeneState* pst, double* pene_partial,
                                                       unsigned long long* counters, int* bad, RecordBuf rec,
                                                       int remap, const int* halt, int n) {
-    step3_body<double, RECORD>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
+    step3_body<double, RECORD, DM>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -2427,21 +2429,17 @@ int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf
                     RecordBuf rec, bool record, int remap, const int* halt, int step) {
     const int nwb = wlen > 0 ? (int)nblk(wlen, 256) : 0;
     const unsigned nb = nblk(P.N, 256) + (unsigned)nwb;
+#define NBC_K3(K, REC, DM) K<REC, DM><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, \
+                                                          remap, halt, step)
+    const bool dm = P.driver != NBC_DRIVER_CLUMP;
     if constexpr (sizeof(R) == sizeof(float)) {
-        if (record)
-            k_contacts_f32<true><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
-                                                     halt, step);
-        else
-            k_contacts_f32<false><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
-                                                      halt, step);
+        if (dm) { if (record) NBC_K3(k_contacts_f32, true, true); else NBC_K3(k_contacts_f32, false, true); }
+        else { if (record) NBC_K3(k_contacts_f32, true, false); else NBC_K3(k_contacts_f32, false, false); }
     } else {
-        if (record)
-            k_contacts_f64<true><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
-                                                     halt, step);
-        else
-            k_contacts_f64<false><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, remap,
-                                                      halt, step);
+        if (dm) { if (record) NBC_K3(k_contacts_f64, true, true); else NBC_K3(k_contacts_f64, false, true); }
+        else { if (record) NBC_K3(k_contacts_f64, true, false); else NBC_K3(k_contacts_f64, false, false); }
     }
+#undef NBC_K3
     LAUNCH_CHECK();
     return 0;
 }
