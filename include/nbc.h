@@ -86,7 +86,11 @@ enum { NBC_PENE_CONSTANT_VELOCITY = 0, NBC_PENE_GRAVITY = 1 };
  * HGRID    RunSimulation_HGrid (src/dem_simulation_hgrid.cu:3-201): cylinder
  *          wall, inherited histories, the hierarchical grid (hgrid_levels,
  *          default 3; the level table is generated, not read from grid.bt).
- * The DEMmain loops take no penetrator here (nbc_upload_particles rejects one). */
+ * With a penetrator (isIntruder) the DEMmain loops use its sphere law
+ * (src/dem_force_penetrator.cu:8-168: the sphere's own spin, cohesion) and
+ * leave its force and torque out of the drift's half-kick and the attitude
+ * (src/dem_integrate.cu:23-25, 74-75); nbc_get_clumps' F and T then hold the
+ * contact + wall totals that the next drift uses. */
 enum { NBC_DRIVER_CLUMP = 0, NBC_DRIVER_CUBE = 1, NBC_DRIVER_CYLINDER = 2, NBC_DRIVER_SPLIT = 3,
        NBC_DRIVER_HGRID = 4 };
 
@@ -351,11 +355,13 @@ int nbc_get_batch_ranges(nbc_ctx* ctx, int32_t* sphere_off, int32_t* clump_off);
 int nbc_upload(nbc_ctx* ctx, const nbc_spheres_in* s, const nbc_clumps_in* c,
                const nbc_wall* wall, const nbc_penetrator* pene);
 /* DEMmain drivers (opt->driver > 0): copy single-sphere state in, file order
- * (DEMmain::CopyDataFromHostToDevice, src/dem.cu:62-72). Particle i becomes
+ * (DEMmain::CopyDataFromHostToDevice, src/dem.cu:62-72); pene may be NULL when
+ * isIntruder == 0. Particle i becomes
  * clump i with one sphere at its centre; params->ClumpNum must equal
  * ParticleNum. The clump getters then return the particles in file order,
  * nbc_get_particles in the current sorted order. */
-int nbc_upload_particles(nbc_ctx* ctx, const nbc_particles_in* p, const nbc_wall* wall);
+int nbc_upload_particles(nbc_ctx* ctx, const nbc_particles_in* p, const nbc_wall* wall,
+                         const nbc_penetrator* pene);
 /* DEMmain::CopyDataToIo (src/dem.cu:113-127): the device arrays in the current
  * sorted order; sorted_id is the reference's part_idx (identity), file_id its
  * part_fix. type follows the sort in the CYLINDER, SPLIT and HGRID loops and
@@ -443,6 +449,10 @@ int nbc_write_clumps_bin(const char* dir, int32_t step, int32_t nc, const double
                          const double* vel, const double* quat, const double* angv,
                          const double* prop, const double* pene_pos, const double* pene_vel,
                          const double* pene_quat, const double* pene_angv);
+/* DEMSphereIo::SaveParticles_Bin's intruder file (src/dem_io.cu:307-317):
+ * intruder.<step>.bin in the clump layout, no clumps, the intruder as id -1. */
+int nbc_write_intruder_bin(const char* dir, int32_t step, const double* pos, const double* vel,
+                           const double* quat, const double* angv);
 int nbc_write_force_history_bin(const char* dir, int32_t step, int32_t interval,
                                 const double* force, const double* torque);
 int nbc_write_yi_bin(const char* dir, int32_t step, int32_t n, const int32_t* clump_id,
@@ -458,8 +468,11 @@ int nbc_run_simulation_clump(const char* workdir, int32_t device);
 /* Runs one of DEMmain's single-sphere loops (driver = NBC_DRIVER_CUBE:
  * RunSimulation2, CYLINDER: RunSimulation, SPLIT: RunSimulation_Split, HGRID:
  * RunSimulation_HGrid) from world.par, IniConFile (21-column rows),
- * Rocks/CubeWall.bt or Rocks/CylindWall.bt and the optional nbc.par, writing
- * OutPart/ph.<step> every OutputInterval steps. */
+ * Rocks/CubeWall.bt or Rocks/CylindWall.bt, Rocks/Penetrator.bt and the
+ * optional nbc.par, writing OutPart/ph.<step> every OutputInterval steps and,
+ * with isIntruder in the cube loop, OutPart/intruder.<step>, OutForce/fc.<step>
+ * and, every 2 OutputInterval steps, OutCnt/{Nei,Fcn,Cpt}.<step>
+ * (src/dem_simulation_cube.cu:182-222). driver 0 takes nbc.par's Driver. */
 int nbc_run_simulation(const char* workdir, int32_t device, int32_t driver);
 
 #ifdef __cplusplus

@@ -118,9 +118,10 @@ int nbc_upload_batch(nbc_ctx* ctx, const nbc_spheres_in* spheres, const nbc_clum
     ENG(ctx);
     return ctx_err(ctx, ctx->eng->upload_batch(spheres, clumps, walls));
 }
-int nbc_upload_particles(nbc_ctx* ctx, const nbc_particles_in* p, const nbc_wall* wall) {
+int nbc_upload_particles(nbc_ctx* ctx, const nbc_particles_in* p, const nbc_wall* wall,
+                         const nbc_penetrator* pene) {
     ENG(ctx);
-    return ctx_err(ctx, ctx->eng->upload_particles(p, wall));
+    return ctx_err(ctx, ctx->eng->upload_particles(p, wall, pene));
 }
 int nbc_get_particles(nbc_ctx* ctx, int32_t* sorted_id, int32_t* file_id, int32_t* type, double* pos, double* vel,
                       double* quat, double* angv, double* prop, double* radius) {

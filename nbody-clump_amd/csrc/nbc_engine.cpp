@@ -115,7 +115,6 @@ class Engine : public EngineBase {
             // DEMmain loops: one sphere per particle, their wall force is applied
             // unconditionally (src/dem_simulation_cube.cu:175, src/dem_simulation_cylind.cu:157)
             if (Nc != N) return fail(NBC_E_BADINPUT, "nbc_create: DEMmain drivers need ClumpNum == ParticleNum");
-            if (P.isIntruder) return fail(NBC_E_BADINPUT, "nbc_create: DEMmain drivers take no penetrator");
             if ((driver == NBC_DRIVER_HGRID) != (P.Levels > 1))
                 return fail(NBC_E_BADINPUT, "nbc_create: the HGRID driver, and only it, uses the hierarchical grid "
                                             "(nbc_finalize_params with hgrid_levels)");
@@ -1246,7 +1245,7 @@ class Engine : public EngineBase {
 
     // DEMmain loops: particle i is clump i with one sphere at its centre
     // (DEMmain::CopyDataFromHostToDevice, src/dem.cu:62-72)
-    int upload_particles(const nbc_particles_in* q, const nbc_wall* w) override {
+    int upload_particles(const nbc_particles_in* q, const nbc_wall* w, const nbc_penetrator* pe) override {
         if (!driver) return fail(NBC_E_BADINPUT, "nbc_upload_particles: the context runs the clump loop "
                                                  "(set nbc_options.driver)");
         if (!q || q->n != N || !q->pos || !q->vel || !q->quat || !q->angv || !q->prop || !q->radius)
@@ -1267,7 +1266,7 @@ class Engine : public EngineBase {
         const nbc_spheres_in si{N, 0, cid.data(), q->pos, local.data(), mass.data(), q->radius};
         const nbc_clumps_in ci{N, 0, q->pos, q->vel, q->quat, q->angv, q->prop};
         particle_upload = true;
-        const int e = upload(&si, &ci, w, nullptr);
+        const int e = upload(&si, &ci, w, pe);
         particle_upload = false;
         return e;
     }

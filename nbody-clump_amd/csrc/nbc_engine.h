@@ -19,7 +19,7 @@ class EngineBase {
     virtual int upload_batch(const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* w) = 0;
     virtual int get_batch_ranges(int32_t* sphere_off, int32_t* clump_off) = 0;
     // DEMmain single-sphere loops
-    virtual int upload_particles(const nbc_particles_in* p, const nbc_wall* w) = 0;
+    virtual int upload_particles(const nbc_particles_in* p, const nbc_wall* w, const nbc_penetrator* pe) = 0;
     virtual int get_particles(int32_t* sid, int32_t* fid, int32_t* type, double* pos, double* vel, double* quat,
                               double* angv, double* prop, double* radius) = 0;
     virtual int step(int nsteps) = 0;

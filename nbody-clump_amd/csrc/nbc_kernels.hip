@@ -1877,10 +1877,12 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                     const V3<R> pw = mk<R>((R)pst->angv[0], (R)pst->angv[1], (R)pst->angv[2]);
                     const V3<R> RVor = cvt3<R>(RVo);
                     WallOut<R> o;
-                    wall_force<R, false>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
-                                  pw, H, Mt, Mr, o);
-                    const V3<R> cf = mk<R>(R(0), R(0), R(0));
-                    Fp = o.nf + o.tf;
+                    wall_force<R, DM>(P, (R)P.pene_frcoe, R(1.0), DistU, DistLr, (R)Dn, Rij, Mij, armi, RVor, vi, wi, pv,
+                                      pw, H, Mt, Mr, o);
+                    // DEMmain's penetrator keeps the cohesion term (src/dem_force_penetrator.cu:119-123)
+                    const V3<R> cf = DM ? (-(R)P.Cohesion * ((R)P.Beta * Rij) * ((R)P.Beta * Rij)) * DistU
+                                        : mk<R>(R(0), R(0), R(0));
+                    Fp = DM ? o.nf + o.tf + cf : o.nf + o.tf;
                     Tp = -DistLr * Cross<R>(DistU, o.tf);
                     V3<R> pTc = R(-1.0) * Cross<R>(RVor, (o.nf + o.tf + cf));
                     Tp = Tp - o.rm - o.tm;
@@ -2096,6 +2098,7 @@ __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S
     D4 pc, qc;
     if (NEXT) { pc = C.pq[2 * c]; qc = C.pq[2 * c + 1]; }
     D4 Fr, Tr;
+    D4 F1r, T1r;  // what the next K1 reads (C.ft)
     if (b >= 0 && e > b) {
         // members in ascending sorted order; k_contacts wrote each sphere's
         // totals at its member slot, so the clump's records are contiguous;
@@ -2123,11 +2126,31 @@ __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S
         }
         Fr.x = F.x; Fr.y = F.y; Fr.z = F.z; Fr.w = 0.0;
         Tr.x = T.x; Tr.y = T.y; Tr.z = T.z; Tr.w = 0.0;
-        C.ft[2 * c] = Fr;
-        C.ft[2 * c + 1] = Tr;
+        if (P.driver && P.pene_on) {
+            // DEMmain loops: the next drift's half-kick and the attitude leave
+            // the penetrator term out (src/dem_integrate.cu:23-25, 74-75); the
+            // clump is its one sphere, whose contact (+ wall) totals these are
+            typedef typename Vec4T<R>::type R4;
+            const R4 f0 = S.sft[2 * (size_t)b], t0 = S.sft[2 * (size_t)b + 1];
+            V3<R> a = Ld4<R>::xyz(f0), q = Ld4<R>::xyz(t0);
+            if (t0.w != R(0)) {
+                const R4* w4 = WL.res + 5 * (size_t)b;
+                a = a + Ld4<R>::xyz(w4[0]);
+                q = q + Ld4<R>::xyz(w4[1]);
+            }
+            F1r.x = a.x; F1r.y = a.y; F1r.z = a.z; F1r.w = 0.0;
+            T1r.x = q.x; T1r.y = q.y; T1r.z = q.z; T1r.w = 0.0;
+        } else {
+            F1r = Fr;
+            T1r = Tr;
+        }
+        C.ft[2 * c] = F1r;
+        C.ft[2 * c + 1] = T1r;
     } else {
         Fr = C.ft[2 * c];
         Tr = C.ft[2 * c + 1];
+        F1r = Fr;
+        T1r = Tr;
     }
     V3<double> F = xyz(Fr), T = xyz(Tr);
     scale_ft(P, pr, F, T);
@@ -2136,7 +2159,14 @@ __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S
     const V3<double> v = kick ? xyz(vr) + 0.5 * P.dt * F : xyz(vr);
     const V3<double> w = kick ? xyz(wr) + 0.5 * P.dt * T : xyz(wr);
     if (NEXT) {
-        // K1 of step n + 1 reads exactly this force, torque and velocity
+        // K1 of step n + 1 reads this force, torque and velocity (what it
+        // stored in C.ft, for the DEMmain loops with a penetrator)
+        if (P.driver && P.pene_on) {
+            V3<double> F1 = xyz(F1r), T1 = xyz(T1r);
+            scale_ft(P, pr, F1, T1);
+            clump_drift<R>(P, C, c, pc, qc, v, w, F1, T1, halt);
+            return;
+        }
         clump_drift<R>(P, C, c, pc, qc, v, w, F, T, halt);
         return;
     }

@@ -500,14 +500,13 @@ extern "C" int nbc_write_spheres_bin(const char* dir, int32_t step, int32_t n, c
 }
 
 // OutClump/ch.%08d.bin, src/dem_binary_io.h:148-180 (penetrator first as id -1)
-extern "C" int nbc_write_clumps_bin(const char* dir, int32_t step, int32_t nc, const double* pos, const double* vel,
-                                    const double* quat, const double* angv, const double* prop,
-                                    const double* ppos, const double* pvel, const double* pquat,
-                                    const double* pangv) {
+static int write_clumps(const char* dir, const char* prefix, int32_t step, int32_t nc, const double* pos,
+                        const double* vel, const double* quat, const double* angv, const double* prop,
+                        const double* ppos, const double* pvel, const double* pquat, const double* pangv) {
     const bool hp = ppos != nullptr;
     if (hp && (!pvel || !pquat || !pangv)) return err(NBC_E_BADINPUT, "Incomplete intruder output data.");
     BinFile* f;
-    int e = open_out(dir, step, f, "ch");
+    int e = open_out(dir, step, f, prefix);
     if (e) return e;
     const int32_t count = nc + (hp ? 1 : 0);
     std::vector<int32_t> ids(count);
@@ -529,6 +528,21 @@ extern "C" int nbc_write_clumps_bin(const char* dir, int32_t step, int32_t nc, c
     e = f->close();
     delete f;
     return e ? err(NBC_E_IO, "Cannot write binary output") : 0;
+}
+
+extern "C" int nbc_write_clumps_bin(const char* dir, int32_t step, int32_t nc, const double* pos, const double* vel,
+                                    const double* quat, const double* angv, const double* prop,
+                                    const double* ppos, const double* pvel, const double* pquat,
+                                    const double* pangv) {
+    return write_clumps(dir, "ch", step, nc, pos, vel, quat, angv, prop, ppos, pvel, pquat, pangv);
+}
+
+// OutPart/intruder.%08d.bin of the DEMmain loops (src/dem_io.cu:307-317): the
+// clump layout with no clumps and the intruder as id -1
+extern "C" int nbc_write_intruder_bin(const char* dir, int32_t step, const double* pos, const double* vel,
+                                      const double* quat, const double* angv) {
+    if (!pos) return err(NBC_E_BADINPUT, "Incomplete intruder output data.");
+    return write_clumps(dir, "intruder", step, 0, nullptr, nullptr, nullptr, nullptr, nullptr, pos, vel, quat, angv);
 }
 
 // OutForce/fc.%08d.bin, src/dem_binary_io.h:123-144 (every 100th entry)

@@ -246,10 +246,6 @@ extern "C" int nbc_run_simulation(const char* workdir, int32_t device, int32_t d
         nbc::set_global_error("world.par: NumberBody and OutputInterval must be positive");
         return NBC_E_PARAM;
     }
-    if (r.P.isIntruder) {
-        nbc::set_global_error("nbc_run_simulation: the DEMmain loops run here without a penetrator (isIntruder = 0)");
-        return NBC_E_BADINPUT;
-    }
     // one sphere per particle
     r.P.ClumpNum = N;
     std::cout << "========================= Load Phase =========================" << std::endl;
@@ -261,7 +257,7 @@ extern "C" int nbc_run_simulation(const char* workdir, int32_t device, int32_t d
     if (e) return e;
     std::cout << "========================= Finish loading particle data =========================" << std::endl;
     if (r.O.driver == NBC_DRIVER_HGRID && r.O.hgrid_levels <= 1)
-        r.O.hgrid_levels = r.P.GridLevels > 1 ? r.P.GridLevels : 3;
+        r.O.hgrid_levels = r.P.GridLevels > 1 ? (int)r.P.GridLevels : 3;
     if (r.O.driver != NBC_DRIVER_HGRID) r.O.hgrid_levels = 0;
     e = nbc_finalize_params(&r.P, radius.data(), N, &r.O);
     if (e) return e;
@@ -270,38 +266,79 @@ extern "C" int nbc_run_simulation(const char* workdir, int32_t device, int32_t d
     std::cout << "WorldOrigin: " << r.P.WorldOrigin[0] << " " << r.P.WorldOrigin[1] << " " << r.P.WorldOrigin[2] << std::endl;
     std::cout << "WorldSize: " << r.P.WorldSize[0] << " " << r.P.WorldSize[1] << " " << r.P.WorldSize[2] << std::endl;
     std::cout << "GridDim: " << r.P.GridDim[0] << " " << r.P.GridDim[1] << " " << r.P.GridDim[2] << std::endl;
-    if (r.O.driver == NBC_DRIVER_CUBE) e = nbc_load_cube_wall(r.path("Rocks/CubeWall.bt").c_str(), &r.wall);
+    const bool cube = r.O.driver == NBC_DRIVER_CUBE;
+    if (cube) e = nbc_load_cube_wall(r.path("Rocks/CubeWall.bt").c_str(), &r.wall);
     else e = nbc_load_cylinder_wall(r.path("Rocks/CylindWall.bt").c_str(), &r.wall);
     if (e) return e;
-    const std::string out = r.path("OutPart");
-    if (!exists(out)) {
-        std::cout << "Create results folder: ./OutPart/" << std::endl;
-        ::mkdir(out.c_str(), 0755);
+    // the loops load the penetrator file whether or not it is used (src/dem_simulation_cube.cu:56-62)
+    e = nbc_load_penetrator(r.path("Rocks/Penetrator.bt").c_str(), &r.pene);
+    if (e) return e;
+    r.have_pene = true;
+    const bool intruder = r.P.isIntruder != 0;
+    // OutPart, and OutCnt for the cube loop's contact files; OutForce (written
+    // with the intruder, src/dem_binary_io.h:138) is created here as well
+    std::vector<std::string> folders = {"OutPart"};
+    if (intruder) {
+        folders.push_back("OutForce");
+        if (cube) folders.push_back("OutCnt");
     }
+    for (const std::string& fo : folders)
+        if (!exists(r.path(fo))) {
+            std::cout << "Create results folder: ./" << fo << "/" << std::endl;
+            ::mkdir(r.path(fo).c_str(), 0755);
+        }
     e = nbc_create(&r.ctx, &r.P, &r.O, device);
     if (e) return e;
     const nbc_particles_in pin{N, 0, type.data(), pos.data(), vel.data(), quat.data(), angv.data(), prop.data(),
                                radius.data()};
-    e = nbc_upload_particles(r.ctx, &pin, &r.wall);
+    e = nbc_upload_particles(r.ctx, &pin, &r.wall, intruder ? &r.pene : nullptr);
     if (e) return e;
-    const int oi = r.P.OutputInterval;
+    const int oi = r.P.OutputInterval, nth = 2 * oi;
+    const bool contacts = intruder && cube;  // src/dem_simulation_cube.cu:210-221
     std::vector<int32_t> sid(N), fid(N);
+    int count = 0, count2 = 0;
     int ii = r.P.StartStep;
     while (ii < r.P.EndStep) {
-        const int chunk = std::min(oi, r.P.EndStep - ii);
-        e = nbc_step(r.ctx, chunk);
+        const int to_out = oi - count, to_cnt = contacts ? nth - count2 : oi;
+        const int chunk = std::min(std::min(to_out, to_cnt), r.P.EndStep - ii);
+        if (chunk > 1) {
+            e = nbc_step(r.ctx, chunk - 1);
+            if (e) return e;
+            ii += chunk - 1; count += chunk - 1; count2 += chunk - 1;
+        }
+        if (contacts && count2 + 1 == nth) nbc_set_record(r.ctx, 1);
+        e = nbc_step(r.ctx, 1);
         if (e) return e;
-        ii += chunk;
-        if (chunk == oi) {
-            std::printf("StepSize = %d\n", ii);
+        count++; count2++;
+        if (count == oi) {
+            std::printf("StepSize = %d\n", ii + 1);
             std::fflush(stdout);
+            const std::string out = r.path("OutPart");
             e = nbc_get_particles(r.ctx, sid.data(), fid.data(), type.data(), pos.data(), vel.data(), quat.data(),
                                   angv.data(), prop.data(), radius.data());
             if (e) return e;
-            e = nbc_write_spheres_bin(out.c_str(), ii, N, sid.data(), fid.data(), type.data(), pos.data(), vel.data(),
-                                      quat.data(), angv.data(), prop.data(), radius.data());
+            e = nbc_write_spheres_bin(out.c_str(), ii + 1, N, sid.data(), fid.data(), type.data(), pos.data(),
+                                      vel.data(), quat.data(), angv.data(), prop.data(), radius.data());
             if (e) return e;
+            if (intruder) {
+                std::vector<double> cf(3 * (size_t)oi), ct(3 * (size_t)oi);
+                double pp[3], pv[3], pq[4], pw[3];
+                e = nbc_get_penetrator(r.ctx, pp, pv, pq, pw, cf.data(), ct.data(), nullptr);
+                if (e) return e;
+                e = nbc_write_intruder_bin(out.c_str(), ii + 1, pp, pv, pq, pw);
+                if (e) return e;
+                e = nbc_write_force_history_bin(r.path("OutForce").c_str(), ii + 1, oi, cf.data(), ct.data());
+                if (e) return e;
+            }
+            count = 0;
         }
+        if (contacts && count2 == nth) {
+            e = save_contacts(r, ii + 1);
+            if (e) return e;
+            count2 = 0;
+        }
+        if (!contacts) count2 = 0;
+        ++ii;
     }
     return 0;
 }

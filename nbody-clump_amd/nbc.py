@@ -154,7 +154,7 @@ _SIGS = {
     "nbc_get_batch_ranges": ([P, P, P], C.c_int),
     "nbc_destroy": ([P], C.c_int),
     "nbc_upload": ([P, C.POINTER(SpheresIn), C.POINTER(ClumpsIn), C.POINTER(Wall), C.POINTER(Penetrator)], C.c_int),
-    "nbc_upload_particles": ([P, C.POINTER(ParticlesIn), C.POINTER(Wall)], C.c_int),
+    "nbc_upload_particles": ([P, C.POINTER(ParticlesIn), C.POINTER(Wall), C.POINTER(Penetrator)], C.c_int),
     "nbc_get_particles": ([P, P, P, P, P, P, P, P, P, P], C.c_int),
     "nbc_step": ([P, I32], C.c_int),
     "nbc_rebuild": ([P], C.c_int),
@@ -176,6 +176,7 @@ _SIGS = {
     "nbc_write_particles_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P], C.c_int),
     "nbc_write_spheres_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P, P, P], C.c_int),
     "nbc_write_clumps_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P, P, P], C.c_int),
+    "nbc_write_intruder_bin": ([C.c_char_p, I32, P, P, P, P], C.c_int),
     "nbc_write_force_history_bin": ([C.c_char_p, I32, I32, P, P], C.c_int),
     "nbc_write_yi_bin": ([C.c_char_p, I32, I32, P, P, P, P, P], C.c_int),
     "nbc_write_contacts_bin": ([C.c_char_p, I32, I32, I32, P, P, P], C.c_int),
@@ -442,11 +443,11 @@ class Context:
         self._c(lib().nbc_upload(self.h, C.byref(s), C.byref(c), C.byref(wall) if wall else None,
                                  C.byref(pene) if pene else None))
 
-    def upload_particles(self, part, wall):
+    def upload_particles(self, part, wall, pene=None):
         """DEMmain drivers: one sphere per particle (nbc_upload_particles)."""
         q = part.particles_in()
-        self._part = part
-        self._c(lib().nbc_upload_particles(self.h, C.byref(q), C.byref(wall)))
+        self._part, self._pene = part, pene
+        self._c(lib().nbc_upload_particles(self.h, C.byref(q), C.byref(wall), C.byref(pene) if pene else None))
 
     def particles(self):
         N = self.N

@@ -317,7 +317,8 @@ extern "C" int orc_upload(orc_ctx* c, const nbc_spheres_in* s, const nbc_clumps_
 
 // DEMmain::CopyDataFromHostToDevice (src/dem.cu:62-72): particle i is clump i
 // with one sphere at its centre.
-extern "C" int orc_upload_particles(orc_ctx* c, const nbc_particles_in* q, const nbc_wall* w) {
+extern "C" int orc_upload_particles(orc_ctx* c, const nbc_particles_in* q, const nbc_wall* w,
+                                    const nbc_penetrator* pe) {
     if (c->driver == NBC_DRIVER_CLUMP) return fail(c, NBC_E_BADINPUT, "orc_upload_particles: clump driver");
     if (!q || q->n != c->N || c->Nc != c->N) return fail(c, NBC_E_BADINPUT, "orc_upload_particles: size mismatch");
     const int N = c->N;
@@ -328,7 +329,7 @@ extern "C" int orc_upload_particles(orc_ctx* c, const nbc_particles_in* q, const
     if (q->type) c->Type.assign(q->type, q->type + N);
     nbc_spheres_in si{N, 0, cid.data(), q->pos, local.data(), mass.data(), q->radius};
     nbc_clumps_in ci{N, 0, q->pos, q->vel, q->quat, q->angv, q->prop};
-    return orc_upload(c, &si, &ci, w, nullptr);
+    return orc_upload(c, &si, &ci, w, pe);
 }
 
 // ---------------------------------------------------------------------------
@@ -1281,7 +1282,9 @@ static void pene_forces(orc_ctx* c) {
         Arm = -1.0 * Arm + Ap * DistU;
         d3 Vwall = Cross(pw, c->PRVo[i]);
         d3 Vrel = c->CVel[clumpI] - pv;
-        d3 Vrot = Cross(Arm, c->CAngSpd[clumpI]) - Vwall;
+        // DEMmain loops: the sphere's own spin and the cohesion term
+        // (src/dem_force_penetrator.cu:91, 119-121)
+        d3 Vrot = c->driver ? Ap * Cross(DistU, c->CAngSpd[clumpI]) - Vwall : Cross(Arm, c->CAngSpd[clumpI]) - Vwall;
         d3 Vtot = Vrel + Vrot;
         double ERR = Dot(DistU, Vtot);
         d3 Vnor = ERR * DistU;
@@ -1298,8 +1301,8 @@ static void pene_forces(orc_ctx* c) {
         } else {
             th = tf - Cs * Vtan;
         }
-        d3 cf = mk3(0.0, 0.0, 0.0);
-        d3 Fc = nf + tf;
+        d3 cf = c->driver ? -P.Cohesion * (P.Beta * Rij) * (P.Beta * Rij) * DistU : mk3(0.0, 0.0, 0.0);
+        d3 Fc = c->driver ? nf + tf + cf : nf + tf;
         d3 Tc = -Ap * Cross(DistU, tf);
         d3 pTc = -1.0 * Cross(c->PRVo[i], (nf + tf + cf));
         c->PFcn[i] = nf;
@@ -1407,6 +1410,16 @@ static void clump_reduce_and_kick(orc_ctx* c) {
         Ti.z = (1.0 / pr.z) * Ti.z;
         c->CVel[i] = c->CVel[i] + 0.5 * Dt * Fi;
         c->CAngSpd[i] = c->CAngSpd[i] + 0.5 * Dt * Ti;
+    }
+    if (c->driver && c->P.isIntruder) {
+        // DEMmain loops: the next drift's half-kick and the attitude take
+        // Fc + Fw and Tc + Tw only (src/dem_integrate.cu:23-25, 74-75); the
+        // clump is its one sphere at its centre
+        for (int i = 0; i < N; ++i) {
+            const int cid = c->ClumpId[i];
+            c->CFc[cid] = c->Fc[i] + c->WFc[i];
+            c->CTc[cid] = c->Tc[i] + c->WTc[i];
+        }
     }
 }
 

@@ -27,7 +27,8 @@ _SIGS = {
     "orc_get_threads": ([], C.c_int),
     "orc_upload": ([P, C.POINTER(nbc.SpheresIn), C.POINTER(nbc.ClumpsIn), C.POINTER(nbc.Wall),
                     C.POINTER(nbc.Penetrator)], C.c_int),
-    "orc_upload_particles": ([P, C.POINTER(nbc.ParticlesIn), C.POINTER(nbc.Wall)], C.c_int),
+    "orc_upload_particles": ([P, C.POINTER(nbc.ParticlesIn), C.POINTER(nbc.Wall), C.POINTER(nbc.Penetrator)],
+                             C.c_int),
     "orc_step": ([P, I32], C.c_int),
     "orc_rebuild": ([P], C.c_int),
     "orc_get_spheres": ([P, P, P, P, P, P, P, P], C.c_int),
@@ -111,10 +112,10 @@ class Oracle:
         self._c(lib().orc_upload(self.h, C.byref(s), C.byref(c), C.byref(wall) if wall else None,
                                  C.byref(pene) if pene else None))
 
-    def upload_particles(self, part, wall):
+    def upload_particles(self, part, wall, pene=None):
         q = part.particles_in()
-        self._part = part
-        self._c(lib().orc_upload_particles(self.h, C.byref(q), C.byref(wall)))
+        self._part, self._pene = part, pene
+        self._c(lib().orc_upload_particles(self.h, C.byref(q), C.byref(wall), C.byref(pene) if pene else None))
 
     def step(self, n=1):
         self._c(lib().orc_step(self.h, n))

@@ -192,3 +192,83 @@ def test_run_simulation_cylinder_loop(tmp_path):
         radius = dbl[-n:]
         assert np.array_equal(radius, s["radius"])
     ref.close()
+
+
+@pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
+@pytest.mark.parametrize("driver", [nbc.NBC_DRIVER_CUBE, nbc.NBC_DRIVER_CYLINDER])
+def test_penetrator(driver, prec):
+    """A sphere penetrator pressed into the top of the bed at 1 m/s: the
+    DEMmain penetrator law and the drift without its term, 300 steps."""
+    part, wall = make_part(driver)
+    top = int(np.argmax(part.pos[:, 2]))
+    R = 3e-3
+    pp = part.pos[top] + np.array([0.0, 0.0, part.radius[top] + R - 1e-5])
+    pene = nbc.sphere_penetrator(R, tuple(pp), (0.0, 0.0, -1.0))
+    part.params.isIntruder = 1
+    banner(f"DEMmain driver {driver} with a sphere penetrator, precision {prec}")
+    opt = nbc.make_options(precision=prec, driver=driver)
+    ctx = nbc.Context(part.finalized_params(opt), opt)
+    ctx.upload_particles(part, wall, pene)
+    ref = orc.Oracle(orc.finalize_params(part), opt)
+    ref.upload_particles(part, wall, pene)
+    seen = 0
+    for k in range(15):
+        ctx.step(20)
+        ref.step(20)
+        pa, pb = ctx.penetrator(), ref.penetrator()
+        seen = max(seen, np.abs(pb["cache_f"]).max())
+        if prec == nbc.NBC_FP64:
+            assert ctx.stats().touching_wall == ref.stats().touching_wall, k
+            assert relerr(pa["cache_f"], pb["cache_f"]) <= 1e-12, k
+    assert seen > 0
+    ca, cb = ctx.clumps(), ref.clumps()
+    if prec == nbc.NBC_FP64:
+        for k in ("pos", "vel", "angv", "F", "T"):
+            assert relerr(ca[k], cb[k]) <= 1e-12, k
+    else:
+        assert np.abs(ca["pos"] - cb["pos"]).max() < 1e-7
+        assert relerr(ctx.penetrator()["cache_f"], ref.penetrator()["cache_f"]) <= 1e-3
+    ctx.close()
+    ref.close()
+
+
+def test_run_simulation_cube_loop_with_intruder(tmp_path):
+    """The cube loop with isIntruder: ph, intruder and fc files every
+    OutputInterval, Nei/Fcn/Cpt every two."""
+    part, wall = make_part(nbc.NBC_DRIVER_CUBE, n=600)
+    banner(f"DEMmain cube run loop with intruder, N={part.n}")
+    top = int(np.argmax(part.pos[:, 2]))
+    spec = nbc.BedSpec(3, 600, 7, nbc.D4(0.02, 0, 0, 0))
+    out = nbc.BedOut()
+    assert nbc.lib().nbc_generate_bed(C.byref(spec), C.byref(out)) == 0
+    R = 3e-3
+    pene = nbc.sphere_penetrator(R, tuple(part.pos[top] + np.array([0.0, 0.0, part.radius[top] + R - 1e-5])),
+                                 (0.0, 0.0, -1.0))
+    try:
+        out.params.EndStep, out.params.OutputInterval, out.params.isIntruder = 40, 10, 1
+        out.wall = wall
+        assert nbc.lib().nbc_save_bed_text(str(tmp_path).encode(), C.byref(out), C.byref(pene)) == 0
+    finally:
+        nbc.lib().nbc_free_bed(C.byref(out))
+    rows = np.column_stack([np.arange(part.n), np.zeros(part.n), part.type, part.pos, part.vel, part.quat,
+                            part.angv, part.prop, part.radius])
+    np.savetxt(tmp_path / "ph.00000000.bt", rows, fmt=["%d", "%d", "%d"] + ["%.17g"] * 18)
+    (tmp_path / "nbc.par").write_text("Driver = cube\n")
+    assert nbc.lib().nbc_run_simulation(str(tmp_path).encode(), 0, 0) == 0, nbc.lib().nbc_last_error(None)
+    for step in (10, 20, 30, 40):
+        assert os.path.getsize(tmp_path / "OutPart" / f"ph.{step:08d}.bin") == 4 + 12 * part.n + 8 * 18 * part.n
+        assert os.path.getsize(tmp_path / "OutPart" / f"intruder.{step:08d}.bin") == 4 + 4 + 8 * 17
+        assert (tmp_path / "OutForce" / f"fc.{step:08d}.bin").exists()
+    for step in (20, 40):
+        for k in ("Nei", "Fcn", "Cpt"):
+            assert (tmp_path / "OutCnt" / f"{k}.{step:08d}.bin").exists()
+    assert not (tmp_path / "OutCnt" / "Nei.00000010.bin").exists()
+    # the intruder file holds the penetrator state of the oracle run
+    opt = nbc.make_options(driver=nbc.NBC_DRIVER_CUBE)
+    part.params.isIntruder = 1
+    ref = orc.Oracle(orc.finalize_params(part), opt)
+    ref.upload_particles(part, wall, pene)
+    ref.step(40)
+    raw = np.frombuffer(open(tmp_path / "OutPart" / "intruder.00000040.bin", "rb").read()[8:], np.float64)
+    assert np.abs(raw[:3] - ref.penetrator()["pos"]).max() <= 1e-15
+    ref.close()

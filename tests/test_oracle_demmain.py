@@ -205,3 +205,59 @@ def test_cylinder_loop_swaps_histories_at_a_rebuild():
     assert np.abs(fc[2:, [0, 2]]).max() <= 1e-12 * np.abs(fc[2:, 1]).max()
     # with the exchange, B read A's slipping history from the slot it now holds
     assert np.abs(fy[2:, 0]).max() > 0.1 * np.abs(fy[2:, 1]).max()
+
+
+def pene_case(driver, cohesion=0.0):
+    # one particle under a sphere penetrator pressing down at constant velocity,
+    # the particle spinning and sliding: the penetrator law and the drift
+    r = 1e-3
+    bed = single_sphere_clumps([[0.0, 0.0, 0.0]], r, vel=[[0.02, 0.0, 0.0]], angv=[[5.0, -3.0, 20.0]],
+                               Gravity=0.0, Cohesion=cohesion, isIntruder=1, isWallForce=1)
+    pene = nbc.sphere_penetrator(2e-3, (0.0, 0.0, r + 2e-3 - 2e-6), (0.0, 0.0, -0.01))
+    if driver == nbc.NBC_DRIVER_CLUMP:
+        opt = nbc.make_options(wall_type=nbc.NBC_WALL_CYLINDER)
+        o = orc.Oracle(orc.finalize_params(bed), opt)
+        o.upload(bed, big_cylinder(), pene)
+    else:
+        opt = nbc.make_options(driver=driver)
+        o = orc.Oracle(orc.finalize_params(part_of(bed)), opt)
+        o.upload_particles(part_of(bed), big_cylinder(), pene)
+    return o, bed
+
+
+@pytest.mark.parametrize("cohesion", [0.0, 2e4])
+def test_penetrator_law_matches_the_clump_loop(cohesion):
+    # the DEMmain penetrator differs from the clump one by the association of
+    # the spin term and by the cohesion term -Coh (Beta Rij)^2 n along the
+    # contact normal (src/dem_force_penetrator.cu:91, 119)
+    out = {}
+    for driver in (nbc.NBC_DRIVER_CLUMP, nbc.NBC_DRIVER_CYLINDER):
+        o, bed = pene_case(driver, cohesion)
+        o.step(1)
+        f, t = o.sphere_forces(3)
+        out[driver] = (f[0], t[0], o.clumps()["pos"][0] - o.penetrator()["pos"])
+        assert o.stats().touching_wall == 1
+    (fc, tc, pc), (fd, td, pd) = out[nbc.NBC_DRIVER_CLUMP], out[nbc.NBC_DRIVER_CYLINDER]
+    assert np.abs(tc - td).max() <= 1e-13 * np.abs(tc).max()
+    r, R, P = 1e-3, 2e-3, bed.params
+    rij = r * R / (r + R)
+    n = pd / np.linalg.norm(pd)  # from the penetrator's centre to the particle's, at the force evaluation
+    want = -cohesion * (P.Beta * rij) ** 2 * n
+    assert np.abs((fd - fc) - want).max() <= 1e-13 * np.abs(fc).max() + 1e-15
+    if cohesion:
+        assert np.abs(fd - fc).max() > 1e-6 * np.abs(fc).max()
+
+
+def test_demmain_drift_leaves_the_penetrator_out():
+    # the clump force the next drift uses is Fc + Fw in the DEMmain loops
+    # (src/dem_integrate.cu:23-25) and Fc + Fw + Fi in the clump loop
+    for driver in (nbc.NBC_DRIVER_CLUMP, nbc.NBC_DRIVER_CYLINDER):
+        o, _ = pene_case(driver)
+        o.step(2)
+        F = o.clumps()["F"][0]
+        f1, t1 = o.sphere_forces(1)
+        f2, t2 = o.sphere_forces(2)
+        f3, t3 = o.sphere_forces(3)
+        assert np.abs(f3).max() > 0
+        want = f1[0] + f2[0] if driver else f1[0] + f2[0] + f3[0]
+        assert np.array_equal(F, want), driver
