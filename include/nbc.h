@@ -84,8 +84,9 @@ enum { NBC_PENE_CONSTANT_VELOCITY = 0, NBC_PENE_GRAVITY = 1 };
  *          (type 0 free, rebuild when |cum| > 0.1 r; type 1 driven at
  *          v = (-0.2 y, -0.2 x, 0); type 2 fixed);
  * HGRID    RunSimulation_HGrid (src/dem_simulation_hgrid.cu:3-201): cylinder
- *          wall, inherited histories, the hierarchical grid (hgrid_levels,
- *          default 3; the level table is generated, not read from grid.bt).
+ *          wall, inherited histories, the hierarchical grid: the level table
+ *          of grid.bt (nbc_load_grid_levels) with each particle's type as its
+ *          level (level_from_type), or one generated by radius (hgrid_levels).
  * With a penetrator (isIntruder) the DEMmain loops use its sphere law
  * (src/dem_force_penetrator.cu:8-168: the sphere's own spin, cohesion) and
  * leave its force and torque out of the drift's half-kick and the attitude
@@ -153,7 +154,9 @@ typedef struct nbc_options {
     double near_skin;        /* near-list skin as a fraction of Thresh (0: 0.25; >= 1: gap 2*Thresh,
                                 refreshed at rebuilds only) */
     int32_t driver;          /* NBC_DRIVER_* (0: the clump loop)                */
-    int32_t reserved_i;
+    int32_t level_from_type; /* HGRID driver: a particle's grid level is its type
+                                (src/dem_hgrid.cu:213), with the level table of
+                                nbc_load_grid_levels; 0: levels by radius      */
     double reserved[2];
 } nbc_options;
 
@@ -257,6 +260,13 @@ int nbc_load_world_par(const char* path, nbc_params* out, char* iniconfile, size
  * opt->hgrid_levels > 1 it also fills the level table (see nbc_params); the
  * margin and the uniform fields use the coarsest level's cell. */
 int nbc_finalize_params(nbc_params* p, const double* radius, int32_t n, const nbc_options* opt);
+/* grid.bt of the DEMmain hgrid loop (DEMSphereIo::LoadGridLevels,
+ * src/dem_io.cu:204-248): p->GridLevels rows "level cellSize nx ny nz offset
+ * n"; fills the level table of p (level 0 the coarsest, offsets contiguous)
+ * in place of nbc_finalize_params, without its origin margin. Use with
+ * nbc_options.level_from_type = 1; every particle's radius must fit its level
+ * (3 r <= cellSize). */
+int nbc_load_grid_levels(const char* path, nbc_params* p);
 /* Optional extension file nbc.par ("key = value"); missing file leaves defaults. */
 int nbc_load_options(const char* path, nbc_options* out);
 /* ph text: n rows "id sub clumpId x y z lx ly lz mass radius". Buffers caller-owned. */
@@ -467,7 +477,8 @@ int nbc_write_contacts_bin(const char* dir, int32_t step, int32_t n, int32_t bra
 int nbc_run_simulation_clump(const char* workdir, int32_t device);
 /* Runs one of DEMmain's single-sphere loops (driver = NBC_DRIVER_CUBE:
  * RunSimulation2, CYLINDER: RunSimulation, SPLIT: RunSimulation_Split, HGRID:
- * RunSimulation_HGrid) from world.par, IniConFile (21-column rows),
+ * RunSimulation_HGrid, which reads grid.bt when present and then takes each
+ * particle's type as its level) from world.par, IniConFile (21-column rows),
  * Rocks/CubeWall.bt or Rocks/CylindWall.bt, Rocks/Penetrator.bt and the
  * optional nbc.par, writing OutPart/ph.<step> every OutputInterval steps and,
  * with isIntruder in the cube loop, OutPart/intruder.<step>, OutForce/fc.<step>

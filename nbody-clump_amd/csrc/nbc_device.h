@@ -116,6 +116,7 @@ struct DevParams {
     int ldim[MAX_LEVELS][3];
     double lcell[MAX_LEVELS];
     double lrmin[MAX_LEVELS];  // a sphere's level is the first t with r > lrmin[t]
+    const int* clevel;         // or, when set, clevel[clump index] (DEMmain hgrid loop: the particle's type)
     double dt, Ymod, Prat, Frcoe, Beta, Cohesion, Gravity, Damping, Thresh;
     double pene_frcoe;
     double cKn, cKs, idt; // 2E/(3(1-nu^2)), 2E/(1+nu)/(2-nu), 1/dt (FP32 force path)

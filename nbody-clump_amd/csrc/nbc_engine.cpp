@@ -120,6 +120,8 @@ class Engine : public EngineBase {
                                             "(nbc_finalize_params with hgrid_levels)");
             wall_type = driver == NBC_DRIVER_CUBE ? NBC_WALL_CUBE : NBC_WALL_CYLINDER;
         }
+        if (O.level_from_type && driver != NBC_DRIVER_HGRID)
+            return fail(NBC_E_BADINPUT, "nbc_create: level_from_type belongs to the HGRID driver");
         nfaces = wall_type == NBC_WALL_CUBE ? 6 : (wall_type == NBC_WALL_CYLINDER ? 3 : 0);
         pair_cap_per = O.pair_capacity > 0 ? O.pair_capacity : 16;
         // small beds are launch-bound: their steady chunks replay a captured
@@ -583,6 +585,22 @@ class Engine : public EngineBase {
             HIPC(hipMemcpy(cfile, iota.data(), sizeof(int) * (size_t)Nc, hipMemcpyHostToDevice));
         }
         if (driver) {
+            if (O.level_from_type) {
+                // the particle's type is its grid level (src/dem_hgrid.cu:213); the
+                // 27-cell search of its level is complete when 3 r fits the cell
+                for (int k = 0; k < Nc; ++k) {
+                    const int t = htype[k];
+                    if (t < 0 || t >= P.Levels) return fail(NBC_E_BADINPUT, "nbc_upload_particles: type is not a grid level");
+                    if (!(3.0 * s->radius[k] <= P.LevelCell[t] * (1.0 + 1e-12))) {
+                        char b[160];
+                        std::snprintf(b, sizeof b, "nbc_upload_particles: particle %d is too large for grid level %d", k, t);
+                        return fail(NBC_E_BADINPUT, b);
+                    }
+                }
+                dp.clevel = C.type;
+            } else {
+                dp.clevel = nullptr;
+            }
             HIPC(hipMemset(C.dist, 0, sizeof(D4) * (size_t)Nc));
             HIPC(hipMemcpy(C.type, htype.data(), sizeof(int) * (size_t)Nc, hipMemcpyHostToDevice));
         }
@@ -663,7 +681,7 @@ class Engine : public EngineBase {
         (void)t0;
         hipEvent_t e0 = prof_begin();
         // 1. hash + stable sort by cell (src/dem_grid.cu:155-305)
-        KC(launch_hash(st, dp, so.pos, key0, val0));
+        KC(launch_hash(st, dp, so.pos, so.cid, key0, val0));
         int second = 0;
         KC(launch_radix_sort(st, key0, val0, key1, val1, N, bits_for(P.CellNum - 1), rbits, hist, hscan, stmp,
                              stotal, &second));
@@ -688,6 +706,7 @@ class Engine : public EngineBase {
             KC(launch_clump_gather<R>(st, Nc, sec ? val1 : val0, C, C2, cfile, cfile2, cnew, s.cid, N, block));
             std::swap(C, C2);
             std::swap(cfile, cfile2);
+            if (dp.clevel) dp.clevel = C.type;
         }
         // 4. neighbour counts, overflow check (src/dem_grid.cu:466-552, 709-719)
         FrontList& Lo = Lf[lcur];
@@ -1259,7 +1278,8 @@ class Engine : public EngineBase {
             mass[i] = q->prop[4 * (size_t)i + 3];
             if (q->type) {
                 htype[i] = q->type[i];
-                if (htype[i] < 0 || htype[i] > 2) return fail(NBC_E_BADINPUT, "nbc_upload_particles: type not 0, 1 or 2");
+                if (driver == NBC_DRIVER_SPLIT && (htype[i] < 0 || htype[i] > 2))
+                    return fail(NBC_E_BADINPUT, "nbc_upload_particles: glue type not 0, 1 or 2");
             }
         }
         hprop.assign(q->prop, q->prop + 4 * (size_t)N);

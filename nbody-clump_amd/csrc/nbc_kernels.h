@@ -79,7 +79,7 @@ int radix_tmp_ints(int n, int rbits);
 int launch_radix_sort(hipStream_t st, uint32_t* k0, int* v0, uint32_t* k1, int* v1, int n, int bits, int rbits,
                       int* hist, int* hscan, int* stmp, int* stotal, int* in_second);
 int launch_keys_from_int(hipStream_t st, const int* a, uint32_t* k, int* v, int n);
-int launch_hash(hipStream_t st, const DevParams& P, const D4* pos, uint32_t* key, int* val);
+int launch_hash(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, uint32_t* key, int* val);
 int launch_seg_bounds(hipStream_t st, const uint32_t* key, int n, int nkeys, int* beg, int* end);
 int launch_neighbors(hipStream_t st, const DevParams& P, bool fill, const D4* pos, const uint32_t* cell,
                      const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, const int* foff,

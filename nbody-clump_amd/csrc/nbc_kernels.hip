@@ -341,7 +341,7 @@ __device__ __forceinline__ int level_of(const DevParams& P, double r) {
 // Cell coordinate of x on a level of cell size s (src/dem_hgrid.cu:225-227).
 __device__ __forceinline__ int hcell(double x, double o, double s) { return (int)floor((x - o) / s); }
 
-__global__ void k_hash(DevParams P, const D4* pos, uint32_t* key, int* val) {
+__global__ void k_hash(DevParams P, const D4* pos, const int* cid, uint32_t* key, int* val) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     const D4 p = pos[i];
@@ -355,7 +355,7 @@ __global__ void k_hash(DevParams P, const D4* pos, uint32_t* key, int* val) {
             p.x < P.boxlo[0]) {
             idx = P.CellNum - 1;
         } else {
-            const int t = level_of(P, p.w);
+            const int t = P.clevel ? P.clevel[cid[i]] : level_of(P, p.w);
             const double s = P.lcell[t];
             const int* nd = P.ldim[t];
             int cx = hcell(p.x, P.origin[0], s), cy = hcell(p.y, P.origin[1], s), cz = hcell(p.z, P.origin[2], s);
@@ -695,7 +695,7 @@ __global__ void k_hneighbors(DevParams P, const D4* pos, const uint32_t* cell, c
     }
     const D4 pi = pos[i];
     const double ri = pi.w;
-    const int my = level_of(P, ri);
+    const int my = P.clevel ? P.clevel[cid[i]] : level_of(P, ri);
     int ng = 0, ni = 0, nx = 0;
     int fb = 0, nf = 0, bb = 0, nbi = 0, xb = 0, xn = 0, ci = 0;
     if (PH == 1) { xb = xoff[i]; xn = xcnt[i]; }
@@ -2297,8 +2297,8 @@ int launch_radix_sort(hipStream_t st, uint32_t* k0, int* v0, uint32_t* k1, int* 
     return 0;
 }
 
-int launch_hash(hipStream_t st, const DevParams& P, const D4* pos, uint32_t* key, int* val) {
-    k_hash<<<nblk(P.N, 256), 256, 0, st>>>(P, pos, key, val);
+int launch_hash(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, uint32_t* key, int* val) {
+    k_hash<<<nblk(P.N, 256), 256, 0, st>>>(P, pos, cid, key, val);
     LAUNCH_CHECK();
     return 0;
 }

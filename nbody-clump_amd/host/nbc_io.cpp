@@ -223,6 +223,48 @@ extern "C" int nbc_load_options(const char* path, nbc_options* out) {
     return 0;
 }
 
+// grid.bt, DEMSphereIo::LoadGridLevels (src/dem_io.cu:204-248): GridLevels rows
+// "level cellSize nx ny nz offset n". The level table replaces the margin and
+// mesh of nbc_finalize_params (the reference's hgrid loop calls LoadGridLevels
+// instead of SetMeshSize); the last cell is the out-of-box bucket.
+extern "C" int nbc_load_grid_levels(const char* path, nbc_params* p) {
+    if (!p) return err(NBC_E_BADINPUT, "nbc_load_grid_levels: null parameters");
+    std::ifstream in(path ? path : "");
+    if (!in) return err(NBC_E_IO, std::string("Error! Cannot open ") + (path ? path : "(null)") + "!");
+    const int L = (int)p->GridLevels;
+    if (L < 2 || L > NBC_MAX_LEVELS) return err(NBC_E_BADINPUT, "nbc_load_grid_levels: GridLevels must be 2..8");
+    long long total = 0;
+    for (int t = 0; t < L; ++t) {
+        int id = 0, nx = 0, ny = 0, nz = 0, off = 0, tmp = 0;
+        double cs = 0.0;
+        if (!(in >> id >> cs >> nx >> ny >> nz >> off >> tmp)) {
+            std::ostringstream m;
+            m << "Error! " << path << " ends or is malformed at row " << t;
+            return err(NBC_E_IO, m.str());
+        }
+        if (!(cs > 0.0) || nx <= 0 || ny <= 0 || nz <= 0 || off != total || (t > 0 && !(cs < p->LevelCell[t - 1])))
+            return err(NBC_E_BADINPUT, "nbc_load_grid_levels: levels must have positive sizes, contiguous offsets "
+                                       "and shrinking cells (level 0 the coarsest)");
+        p->LevelCell[t] = cs;
+        p->LevelDim[t][0] = nx; p->LevelDim[t][1] = ny; p->LevelDim[t][2] = nz;
+        p->LevelOff[t] = off;
+        p->LevelRmin[t] = t + 1 < L ? cs / 3.0 : -1.0;
+        total += (long long)nx * ny * nz;
+        if (total > (1ll << 30)) return err(NBC_E_BADINPUT, "nbc_load_grid_levels: too many cells");
+    }
+    const double lnCOR = std::log(p->NormRes);
+    p->Damping = 2.0 * std::sqrt(5.0 / 6.0) * lnCOR / std::sqrt(lnCOR * lnCOR + 3.14159265359 * 3.14159265359);
+    for (int a = 0; a < 3; ++a) {
+        p->CellSize[a] = p->LevelCell[0];
+        p->i_CellSize[a] = 1.0 / p->LevelCell[0];
+        p->GridDim[a] = (uint32_t)p->LevelDim[0][a];
+    }
+    p->Levels = L;
+    p->CellNum = (int32_t)total + 1;
+    p->finalized = 1;
+    return 0;
+}
+
 // ph text, src/dem_subsphere_io.cu:166-189: "id sub clumpId x y z lx ly lz mass radius".
 extern "C" int nbc_load_phase(const char* path, int32_t n, int32_t* clump_id, double* pos, double* local,
                               double* mass, double* radius) {

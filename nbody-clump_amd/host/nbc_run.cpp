@@ -256,11 +256,21 @@ extern "C" int nbc_run_simulation(const char* workdir, int32_t device, int32_t d
                            prop.data(), radius.data());
     if (e) return e;
     std::cout << "========================= Finish loading particle data =========================" << std::endl;
-    if (r.O.driver == NBC_DRIVER_HGRID && r.O.hgrid_levels <= 1)
-        r.O.hgrid_levels = r.P.GridLevels > 1 ? (int)r.P.GridLevels : 3;
-    if (r.O.driver != NBC_DRIVER_HGRID) r.O.hgrid_levels = 0;
-    e = nbc_finalize_params(&r.P, radius.data(), N, &r.O);
-    if (e) return e;
+    if (r.O.driver == NBC_DRIVER_HGRID && exists(r.path("grid.bt"))) {
+        // the reference's hgrid loop: levels from grid.bt, a particle's level
+        // is its type (src/dem_simulation_hgrid.cu:15, src/dem_hgrid.cu:213)
+        std::cout << "========================= Load HGrid =========================" << std::endl;
+        e = nbc_load_grid_levels(r.path("grid.bt").c_str(), &r.P);
+        if (e) return e;
+        r.O.level_from_type = 1;
+        std::cout << "========================= Finish loading hgrid data =========================" << std::endl;
+    } else {
+        if (r.O.driver == NBC_DRIVER_HGRID && r.O.hgrid_levels <= 1)
+            r.O.hgrid_levels = r.P.GridLevels > 1 ? (int)r.P.GridLevels : 3;
+        if (r.O.driver != NBC_DRIVER_HGRID) r.O.hgrid_levels = 0;
+        e = nbc_finalize_params(&r.P, radius.data(), N, &r.O);
+        if (e) return e;
+    }
     std::cout << "Configuration of Grid ..." << std::endl;
     std::cout << "CellSize: " << r.P.CellSize[0] << " " << r.P.CellSize[1] << " " << r.P.CellSize[2] << std::endl;
     std::cout << "WorldOrigin: " << r.P.WorldOrigin[0] << " " << r.P.WorldOrigin[1] << " " << r.P.WorldOrigin[2] << std::endl;

@@ -50,7 +50,7 @@ class Options(C.Structure):
         ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
         ("hgrid_levels", C.c_int32), ("graphs", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
-        ("driver", C.c_int32), ("reserved_i", C.c_int32), ("reserved", C.c_double * 2),
+        ("driver", C.c_int32), ("level_from_type", C.c_int32), ("reserved", C.c_double * 2),
     ]
 
 
@@ -137,6 +137,7 @@ _SIGS = {
     "nbc_load_world_par": ([C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_size_t], C.c_int),
     "nbc_finalize_params": ([C.POINTER(Params), P, I32, C.POINTER(Options)], C.c_int),
     "nbc_load_options": ([C.c_char_p, C.POINTER(Options)], C.c_int),
+    "nbc_load_grid_levels": ([C.c_char_p, C.POINTER(Params)], C.c_int),
     "nbc_load_phase": ([C.c_char_p, I32, P, P, P, P, P], C.c_int),
     "nbc_load_clumps": ([C.c_char_p, I32, P, P, P, P, P], C.c_int),
     "nbc_load_particles": ([C.c_char_p, I32, P, P, P, P, P, P, P], C.c_int),
@@ -373,9 +374,10 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0):
+                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0, level_from_type=0):
     o = Options()
     o.driver = driver
+    o.level_from_type = level_from_type
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits
     o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
     o.hgrid_levels = hgrid_levels

@@ -478,11 +478,16 @@ static int sphere_level(const nbc_params& P, double r) {
     return t;
 }
 
-static int cell_hash_h(const nbc_params& P, const d3& p, double r) {
+// Level of sorted sphere i: by radius, or, in the DEMmain hgrid loop with
+// grid.bt levels, the particle's type (src/dem_hgrid.cu:213).
+static int level_of(const orc_ctx* c, int i) {
+    return c->O.level_from_type ? c->Type[c->ClumpId[i]] : sphere_level(c->P, c->Radius[i]);
+}
+
+static int cell_hash_h(const nbc_params& P, const d3& p, int t) {
     if (p.z > P.BoxUpLmt[2] || p.y > P.BoxUpLmt[1] || p.x > P.BoxUpLmt[0] || p.z < P.BoxLowLmt[2] ||
         p.y < P.BoxLowLmt[1] || p.x < P.BoxLowLmt[0])
         return P.CellNum - 1;
-    const int t = sphere_level(P, r);
     const double sz = P.LevelCell[t];
     const double x[3] = {p.x, p.y, p.z};
     int ci[3];
@@ -498,7 +503,7 @@ static void neighbor_search(orc_ctx* c) {
     std::vector<int> cel(N);
     const bool hg = c->P.Levels > 1;
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < N; ++i) cel[i] = hg ? cell_hash_h(c->P, c->Pos[i], c->Radius[i]) : cell_hash(c->P, c->Pos[i]);
+    for (int i = 0; i < N; ++i) cel[i] = hg ? cell_hash_h(c->P, c->Pos[i], level_of(c, i)) : cell_hash(c->P, c->Pos[i]);
     // Stable sort of (cell, index): CUB SortPairs is a stable LSD radix sort
     // over the identity values written by GetParticleCellIndex_cuk.
     std::vector<int> order(N);
@@ -599,7 +604,7 @@ static int create_anei_h(orc_ctx* c) {
             if (c->part_cel[index] == P.CellNum - 1) continue;
             const d3 pi = c->Pos[index];
             const double ri = c->Radius[index];
-            const int my = sphere_level(P, ri);
+            const int my = level_of(c, index);
             int* row = &c->Anei[(size_t)index * B];
             int ng = 0, ni = 0;
             for (int T = my; T >= 0; --T) {

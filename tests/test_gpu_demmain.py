@@ -272,3 +272,54 @@ def test_run_simulation_cube_loop_with_intruder(tmp_path):
     raw = np.frombuffer(open(tmp_path / "OutPart" / "intruder.00000040.bin", "rb").read()[8:], np.float64)
     assert np.abs(raw[:3] - ref.penetrator()["pos"]).max() <= 1e-15
     ref.close()
+
+
+def grid_bt_part(seed=11):
+    """A DEMmain bed with three radius classes, its generated 3-level table
+    written as grid.bt, and each particle's type a random level it fits."""
+    part, wall = make_part(nbc.NBC_DRIVER_HGRID, speed=0.3, seed=seed)
+    rng = np.random.default_rng(seed)
+    part.radius[:] = part.radius * rng.choice([0.4, 0.7, 1.0], part.n)
+    p = orc.finalize_params(part, 0.0, 3)
+    ok = [[t for t in range(3) if 3.0 * r <= p.LevelCell[t]] for r in part.radius]
+    part.type[:] = [rng.choice(o) for o in ok]
+    return part, wall, p
+
+
+@pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
+def test_hgrid_levels_from_grid_bt(prec):
+    part, wall, p = grid_bt_part()
+    assert set(part.type) == {0, 1, 2}
+    banner(f"DEMmain hgrid loop, levels by type, precision {prec}, N={part.n}")
+    opt = nbc.make_options(precision=prec, driver=nbc.NBC_DRIVER_HGRID, level_from_type=1)
+    ctx = nbc.Context(p, opt)
+    ctx.upload_particles(part, wall)
+    ref = orc.Oracle(p, opt)
+    ref.upload_particles(part, wall)
+    ctx.step(1)
+    ref.step(1)
+    assert_lists_equal(ctx, ref)
+    for k in range(15):
+        ctx.step(20)
+        ref.step(20)
+        if prec == nbc.NBC_FP64:
+            sa, sb = ctx.stats(), ref.stats()
+            assert (sa.touching_ss, sa.rebuilds) == (sb.touching_ss, sb.rebuilds), k
+    assert ref.stats().rebuilds >= 3
+    ca, cb = ctx.clumps(), ref.clumps()
+    if prec == nbc.NBC_FP64:
+        assert_lists_equal(ctx, ref)
+        assert relerr(ca["pos"], cb["pos"]) <= 1e-12 and relerr(ca["vel"], cb["vel"]) <= 1e-12
+    else:
+        assert np.abs(ca["pos"] - cb["pos"]).max() < 1e-7
+    # a type that is no level, or a particle too large for its level, is refused
+    bad = nbc.Particles(part.params, part.type.copy(), part.pos, part.vel, part.quat, part.angv, part.prop,
+                        part.radius, part.wall)
+    bad.type[0] = 3
+    with pytest.raises(nbc.NbcError):
+        ctx.upload_particles(bad, wall)
+    bad.type[:] = 2
+    with pytest.raises(nbc.NbcError):
+        ctx.upload_particles(bad, wall)
+    ctx.close()
+    ref.close()

@@ -261,3 +261,85 @@ def test_demmain_drift_leaves_the_penetrator_out():
         assert np.abs(f3).max() > 0
         want = f1[0] + f2[0] if driver else f1[0] + f2[0] + f3[0]
         assert np.array_equal(F, want), driver
+
+
+def write_grid_bt(path, p):
+    with open(path, "w") as f:
+        for t in range(p.Levels):
+            d = p.LevelDim[t]
+            f.write(f"{t} {p.LevelCell[t]!r} {d[0]} {d[1]} {d[2]} {p.LevelOff[t]} {d[0] * d[1] * d[2]}\n")
+
+
+def test_grid_bt_levels_by_type(tmp_path):
+    # a grid.bt holding the generated 3-level table, and every particle's type
+    # set to its radius level: the same lists and forces as the radius rule;
+    # every particle on level 0 instead: the KD-tree pair set (complete search)
+    import ctypes as C
+    from scipy.spatial import cKDTree
+    bed = nbc.generate_bed(3, 1500, seed=9, container=(0.012, 0, 0, 0))
+    # widen the radii so that three levels are populated
+    rng = np.random.default_rng(9)
+    part = nbc.Particles.from_bed(bed)
+    part.radius[:] = part.radius * rng.choice([0.4, 0.7, 1.0], part.n)
+    opt_r = nbc.make_options(driver=nbc.NBC_DRIVER_HGRID, hgrid_levels=3)
+    pr = orc.finalize_params(part, 0.0, 3)
+    write_grid_bt(tmp_path / "grid.bt", pr)
+    pg = nbc.Params()
+    C.memmove(C.byref(pg), C.byref(pr), C.sizeof(nbc.Params))
+    for t in range(3):
+        pg.LevelCell[t] = pg.LevelDim[t][0] = pg.LevelOff[t] = 0
+    pg.GridLevels = 3
+    assert nbc.lib().nbc_load_grid_levels(str(tmp_path / "grid.bt").encode(), C.byref(pg)) == 0
+    for t in range(3):
+        assert pg.LevelCell[t] == pr.LevelCell[t] and list(pg.LevelDim[t]) == list(pr.LevelDim[t])
+        assert pg.LevelOff[t] == pr.LevelOff[t]
+    assert pg.CellNum == pr.CellNum and pg.Levels == 3
+    lvl = np.array([next(t for t in range(3) if t == 2 or r > pr.LevelRmin[t]) for r in part.radius])
+    assert set(lvl) == {0, 1, 2}
+    res = []
+    for by_type in (0, 1):
+        q = nbc.Particles(part.params, lvl if by_type else part.type, part.pos, part.vel, part.quat, part.angv,
+                          part.prop, part.radius, part.wall)
+        opt = nbc.make_options(driver=nbc.NBC_DRIVER_HGRID, hgrid_levels=0 if by_type else 3,
+                               level_from_type=by_type)
+        o = orc.Oracle(pg if by_type else pr, opt)
+        o.upload_particles(q, part.wall)
+        o.step(3)
+        res.append((o.pairs(), o.sphere_forces(1)[0], o.spheres()["fid"]))
+    (pa, fa, ia), (pb, fb, ib) = res
+    assert np.array_equal(ia, ib)
+    for k in ("foff", "fj", "boff", "bi", "bslot"):
+        assert np.array_equal(pa[k], pb[k]), k
+    assert np.array_equal(fa, fb)
+    # all on level 0: a uniform search at the coarsest cell
+    q = nbc.Particles(part.params, np.zeros(part.n, np.int32), part.pos, part.vel, part.quat, part.angv, part.prop,
+                      part.radius, part.wall)
+    o = orc.Oracle(pg, nbc.make_options(driver=nbc.NBC_DRIVER_HGRID, level_from_type=1))
+    o.upload_particles(q, part.wall)
+    o.rebuild()
+    s, p = o.spheres(), o.pairs()
+    own = np.repeat(np.arange(part.n), np.diff(p["foff"]))
+    rows = {tuple(sorted(x)) for x in zip(s["fid"][own].tolist(), s["fid"][p["fj"]].tolist())}
+    tree = cKDTree(part.pos)
+    cand = tree.query_pairs(3.0 * part.radius.max() * (1 + 1e-9), output_type="ndarray")
+    d2 = ((part.pos[cand[:, 0]] - part.pos[cand[:, 1]]) ** 2).sum(1)
+    lim = 1.5 * (part.radius[cand[:, 0]] + part.radius[cand[:, 1]])
+    want = {tuple(sorted(x)) for x in cand[d2 < lim * lim].tolist()}
+    assert rows == want
+
+
+def test_grid_bt_rejects_bad_tables(tmp_path):
+    import ctypes as C
+    p = nbc.Params()
+    p.GridLevels = 2
+    p.NormRes = 0.5
+    f = tmp_path / "grid.bt"
+    f.write_text("0 0.003 10 10 10 0 1000\n1 0.001 30 30 30 999 27000\n")  # offset not contiguous
+    assert nbc.lib().nbc_load_grid_levels(str(f).encode(), C.byref(p)) == nbc.NBC_E_BADINPUT
+    f.write_text("0 0.001 30 30 30 0 27000\n1 0.003 10 10 10 27000 1000\n")  # finer level first
+    assert nbc.lib().nbc_load_grid_levels(str(f).encode(), C.byref(p)) == nbc.NBC_E_BADINPUT
+    f.write_text("0 0.003 10 10 10 0 1000\n")  # a row short
+    assert nbc.lib().nbc_load_grid_levels(str(f).encode(), C.byref(p)) == nbc.NBC_E_IO
+    f.write_text("0 0.003 10 10 10 0 1000\n1 0.001 30 30 30 1000 27000\n")
+    assert nbc.lib().nbc_load_grid_levels(str(f).encode(), C.byref(p)) == 0
+    assert p.Levels == 2 and p.CellNum == 28001 and p.LevelOff[1] == 1000 and p.finalized == 1
