@@ -189,8 +189,8 @@ class Engine : public EngineBase {
             HIPC(dalloc(&s.pos, N)); HIPC(dalloc(&s.cum, N)); HIPC(dalloc(&s.loc, N)); HIPC(dalloc(&s.kin, N));
             HIPC(dalloc(&s.cid, N)); HIPC(dalloc(&s.fid, N)); HIPC(dalloc(&s.wflag, N));
             HIPC(dalloc(&s.wh, (size_t)std::max(nwh, 1) * N * HSTRIDE));
-            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.p32, N)); HIPC(dalloc(&s.slot, N)); }
-            else { s.sft = S[0].sft; s.p32 = S[0].p32; s.slot = S[0].slot; }
+            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.slot, N)); }
+            else { s.sft = S[0].sft; s.slot = S[0].slot; }
         }
         HIPC(dalloc(&C.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C.prop, Nc));
         HIPC(dalloc(&C.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C.snap, 2 * (size_t)Nc));
@@ -319,7 +319,7 @@ class Engine : public EngineBase {
         for (int b = 0; b < 2; ++b) {
             SphereBuf<R>& s = S[b];
             F(s.pos); F(s.cum); F(s.loc); F(s.kin); F(s.cid); F(s.fid); F(s.wflag); F(s.wh);
-            if (b == 0) { F(s.sft); F(s.p32); F(s.slot); }
+            if (b == 0) { F(s.sft); F(s.slot); }
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap); F(C.dist); F(C.type);
@@ -563,7 +563,6 @@ class Engine : public EngineBase {
         HIPC(hipMemset(s0.wh, 0, sizeof(R) * (size_t)std::max(nwh, 1) * N * HSTRIDE));
         HIPC(hipMemset(S[1].wflag, 0, sizeof(uint32_t) * N));
         HIPC(hipMemset(s0.sft, 0, sizeof(R4) * 2 * (size_t)N));
-        HIPC(hipMemset(s0.p32, 0xff, sizeof(F4) * (size_t)N));  // NaN: pre-filter passes everything
         std::vector<D4> pq(2 * (size_t)Nc), vw(2 * (size_t)Nc), prop(Nc), ft(2 * (size_t)Nc);
         for (int i = 0; i < Nc; ++i) {
             pq[2 * i] = D4{c->pos[3 * i], c->pos[3 * i + 1], c->pos[3 * i + 2], 0.0};

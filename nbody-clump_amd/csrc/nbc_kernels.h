@@ -23,7 +23,6 @@ struct SphereBuf {
     R* wh;                            // [MAX_FACES][N][HSTRIDE] wall/penetrator histories
     typename Vec4T<R>::type* sft;     // [2N] force, torque about the clump centre, in member-slot order
     int* slot;                        // member slot of the sphere (its index in the clump member order)
-    F4* p32;                          // FP32 (x, y, z) - box centre, radius; w < 0: use FP64 only
 };
 
 // Per-clump arrays (FP64 state; snap is the contact kernel's copy in R).

@@ -288,7 +288,6 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         S.pos[i] = pn;
         S.cum[i] = cn;
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
-        S.p32[i] = pos32(P, up, po.w);
     }
     // the rebuild request (src/dem_clump.cu:132-134) records the step index:
     // later kernels of this step and all kernels of later steps stand down
@@ -546,7 +545,6 @@ __global__ void k_gather(DevParams P, int nface, const int* perm, SphereBuf<R> s
     const int o = perm[i];
     const D4 po = src.pos[o];
     dst.pos[i] = po;
-    dst.p32[i] = pos32(P, xyz(po), po.w);
     D4 z; z.x = 0.0; z.y = 0.0; z.z = 0.0; z.w = 0.0;
     dst.cum[i] = z;
     dst.loc[i] = src.loc[o];
@@ -1406,7 +1404,8 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             s_fo[wv][64] = L.nfoff[ie];
             s_bo[wv][64] = L.nboff[ie];
         }
-        F4 a = S.p32[i < P.N ? i : P.N - 1];  // clamped load, then a value select
+        const D4 pa = S.pos[i < P.N ? i : P.N - 1];  // clamped load, then a value select
+        F4 a = pos32(P, xyz(pa), pa.w);
         if (!active) a.w = -1.0f;
         s_p32[wv][lane] = a;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -1537,7 +1536,10 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_fo[wv][r + st] <= e) r += st;
                 j = (int)(w & JMASK);
-                cand = maybe_touch(P, s_p32[wv][r], S.p32[j]);
+                // the partner's FP32 copy is made here from its FP64 position: a
+                // gather fetches the same DRAM sector for 16 or 32 bytes
+                const D4 pj = S.pos[j];
+                cand = maybe_touch(P, s_p32[wv][r], pos32(P, xyz(pj), pj.w));
                 if (!cand && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
             }
             push(cand, r, e, j);
@@ -1559,7 +1561,8 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_bo[wv][r + st] <= e) r += st;
-                cand = maybe_touch(P, s_p32[wv][r], S.p32[o]);
+                const D4 po = S.pos[o];
+                cand = maybe_touch(P, s_p32[wv][r], pos32(P, xyz(po), po.w));
             }
             push(cand, r + 64, ne, o);
         }
