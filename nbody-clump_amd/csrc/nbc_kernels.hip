@@ -2147,8 +2147,12 @@ __global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S
             F1r = Fr;
             T1r = Tr;
         }
-        C.ft[2 * c] = F1r;
-        C.ft[2 * c + 1] = T1r;
+        // a K1 launched on its own reads C.ft; the fused one below takes the
+        // values from registers, and the last step of a call runs without it
+        if (!NEXT) {
+            C.ft[2 * c] = F1r;
+            C.ft[2 * c + 1] = T1r;
+        }
     } else {
         Fr = C.ft[2 * c];
         Tr = C.ft[2 * c + 1];
@@ -2492,13 +2496,18 @@ template <class R>
 int launch_clump_after(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL,
                        RecordBuf rec, bool record, const int* mbeg, const int* mend, const int* mem, int block,
                        int* halt, int step, bool next, PeneState* pst, unsigned long long* counters) {
-    const unsigned g = nblk(P.Nc, block);
+    // one wave per block: there are few clumps (C1: 200,000 = 3,125 waves), and
+    // 64-thread blocks spread them over the CUs with a tail of one wave instead
+    // of one 256-thread block (K4 34 vs 39 us on C1)
+    (void)block;
+    const int kb = 64;
+    const unsigned g = nblk(P.Nc, kb);
     if (next)
-        k_clump_after<R, true><<<g, block, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt, step, pst,
-                                                    counters);
+        k_clump_after<R, true><<<g, kb, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt, step, pst,
+                                                 counters);
     else
-        k_clump_after<R, false><<<g, block, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt, step,
-                                                     pst, counters);
+        k_clump_after<R, false><<<g, kb, 0, st>>>(P, S, C, WL, rec, record ? 1 : 0, mbeg, mend, mem, halt, step,
+                                                  pst, counters);
     LAUNCH_CHECK();
     return 0;
 }
