@@ -189,8 +189,8 @@ class Engine : public EngineBase {
             HIPC(dalloc(&s.pos, N)); HIPC(dalloc(&s.cum, N)); HIPC(dalloc(&s.loc, N)); HIPC(dalloc(&s.kin, N));
             HIPC(dalloc(&s.cid, N)); HIPC(dalloc(&s.fid, N)); HIPC(dalloc(&s.wflag, N));
             HIPC(dalloc(&s.wh, (size_t)std::max(nwh, 1) * N * HSTRIDE));
-            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.slot, N)); }
-            else { s.sft = S[0].sft; s.slot = S[0].slot; }
+            if (b == 0) { HIPC(dalloc(&s.sft, 2 * (size_t)N)); HIPC(dalloc(&s.slot, N)); HIPC(dalloc(&s.nthr, N)); }
+            else { s.sft = S[0].sft; s.slot = S[0].slot; s.nthr = S[0].nthr; }
         }
         HIPC(dalloc(&C.pq, 2 * (size_t)Nc)); HIPC(dalloc(&C.vw, 2 * (size_t)Nc)); HIPC(dalloc(&C.prop, Nc));
         HIPC(dalloc(&C.ft, 2 * (size_t)Nc)); HIPC(dalloc(&C.snap, 2 * (size_t)Nc));
@@ -319,7 +319,7 @@ class Engine : public EngineBase {
         for (int b = 0; b < 2; ++b) {
             SphereBuf<R>& s = S[b];
             F(s.pos); F(s.cum); F(s.loc); F(s.kin); F(s.cid); F(s.fid); F(s.wflag); F(s.wh);
-            if (b == 0) { F(s.sft); F(s.slot); }
+            if (b == 0) { F(s.sft); F(s.slot); F(s.nthr); }
             F(Lf[b].foff); F(Lf[b].fword); F(Lf[b].hist[0]); F(Lf[b].hist[1]);
         }
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap); F(C.dist); F(C.type);
@@ -547,6 +547,7 @@ class Engine : public EngineBase {
         for (int i = 0; i < N; ++i) {
             pos[i].x = s->pos[3 * i]; pos[i].y = s->pos[3 * i + 1]; pos[i].z = s->pos[3 * i + 2]; pos[i].w = s->radius[i];
             cum[i].x = cum[i].y = cum[i].z = cum[i].w = 0.0;
+            if (ref_disp<R>(driver)) cum[i] = pos[i];
             loc[i].x = (R)s->local[3 * i]; loc[i].y = (R)s->local[3 * i + 1]; loc[i].z = (R)s->local[3 * i + 2];
             loc[i].w = (R)s->mass[i];
             kin[i].x = kin[i].y = kin[i].z = R(0); kin[i].w = (R)s->mass[i];
@@ -782,9 +783,9 @@ class Engine : public EngineBase {
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));
-        KC(launch_near_build(st, dp, s.pos, s.cum, nslack, Ln.foff, Ln.fword, boff, bent, fcnt, nfoff, nfw, nfs,
-                             slot2near, bcnt,
-                             nboff, nbo, nbe, stmp, stotal + 2, block));
+        KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, Ln.foff, Ln.fword,
+                             boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2,
+                             block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         nfront = tot[0];
@@ -900,8 +901,9 @@ class Engine : public EngineBase {
         FrontList& L = Lf[lcur];
         KC(launch_near_sync(st, N, nfoff, nfw, nfs, L.fword, block));
         SphereBuf<R>& s = S[scur];
-        KC(launch_near_build(st, dp, s.pos, s.cum, nslack, L.foff, L.fword, boff, bent, fcnt, nfoff, nfw, nfs,
-                             slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2, block));
+        KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, L.foff, L.fword,
+                             boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2,
+                             block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         nnear++;
@@ -1161,6 +1163,14 @@ class Engine : public EngineBase {
         if (cum) {
             std::vector<D4> v(N);
             HIPC(hipMemcpy(v.data(), s.cum, sizeof(D4) * N, hipMemcpyDeviceToHost));
+            if (ref_disp<R>(driver)) {
+                // v holds the positions at the last rebuild
+                std::vector<D4> p(N);
+                HIPC(hipMemcpy(p.data(), s.pos, sizeof(D4) * N, hipMemcpyDeviceToHost));
+                for (int i = 0; i < N; ++i) {
+                    v[i].x = p[i].x - v[i].x; v[i].y = p[i].y - v[i].y; v[i].z = p[i].z - v[i].z;
+                }
+            }
             for (int i = 0; i < N; ++i) { cum[3 * i] = v[i].x; cum[3 * i + 1] = v[i].y; cum[3 * i + 2] = v[i].z; }
         }
         if (local || mass) {

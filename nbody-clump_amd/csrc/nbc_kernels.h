@@ -14,7 +14,8 @@ namespace nbc {
 template <class R>
 struct SphereBuf {
     D4* pos;                          // x y z radius (FP64)
-    D4* cum;                          // displacement since the last rebuild (FP64)
+    D4* cum;                          // displacement since the last rebuild (FP64); with ref_disp the
+                                      // position at the last rebuild, w = radius
     typename Vec4T<R>::type* loc;     // local offset (body frame), mass
     typename Vec4T<R>::type* kin;     // R(q) * local (world frame), mass
     int* cid;                         // clump index
@@ -23,7 +24,17 @@ struct SphereBuf {
     R* wh;                            // [MAX_FACES][N][HSTRIDE] wall/penetrator histories
     typename Vec4T<R>::type* sft;     // [2N] force, torque about the clump centre, in member-slot order
     int* slot;                        // member slot of the sphere (its index in the clump member order)
+    float* nthr;                      // with ref_disp: near-list request threshold (|disp|^2), rounded down
 };
+
+// FP32-mixed clump loops keep the position at the last rebuild instead of the
+// accumulated displacement: the displacement is x - x_rebuild, so the pose
+// kernel neither reads the previous position nor writes the displacement
+// back (68 instead of 128 bytes of pose traffic per sphere). The FP64 form and
+// the DEMmain loops accumulate it as the reference does
+// (src/dem_clump.cu:139-141), so their rebuild steps stay bit-exact.
+template <class R>
+__host__ __device__ inline bool ref_disp(int driver) { return sizeof(R) == 4 && driver == 0; }
 
 // Per-clump arrays (FP64 state; snap is the contact kernel's copy in R).
 template <class R>
@@ -124,10 +135,10 @@ int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* b
                         const int* xend, unsigned* overflow, int block);
 // Near lists of the current list at the current positions; also sets each
 // sphere's slack and its near-list request threshold (cum.w).
-int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, double* slack, const int* foff,
-                      const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw,
-                      int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal,
-                      int block);
+int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, float* nthr, double* slack,
+                      const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
+                      uint32_t* nfw, int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp,
+                      int* stotal, int block);
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
                      int block);
 // Sphere-sphere contacts of every sphere and, in the same launch, the wall and

@@ -268,25 +268,34 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         QuatIvDCM<double>(q.x, q.y, q.z, q.w, M);
         V3<double> arm = Map<double>(M, mk<double>((double)lo.x, (double)lo.y, (double)lo.z));
         V3<double> up = xyz(pc) + arm;
-        const D4 po = S.pos[i];
         const D4 cu = S.cum[i];
-        V3<double> cum;
-        if (P.driver) {
-            const D4 dd = C.dist[c];
-            cum = xyz(cu) + xyz(dd);
+        if (ref_disp<R>(P.driver)) {
+            // displacement = x - x_rebuild; cum holds x_rebuild and the radius
+            const V3<double> cum = up - xyz(cu);
+            over = Length<double>(cum) > P.Thresh;
+            near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > (double)S.nthr[i];
+            D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = cu.w;
+            S.pos[i] = pn;
         } else {
-            cum = xyz(cu) + up - xyz(po);
+            const D4 po = S.pos[i];
+            V3<double> cum;
+            if (P.driver) {
+                const D4 dd = C.dist[c];
+                cum = xyz(cu) + xyz(dd);
+            } else {
+                cum = xyz(cu) + up - xyz(po);
+            }
+            // glue loop: type 0 rebuilds past 0.1 r, the kinematic types never ask
+            // (src/dem_integrate_withglue.cu:40-43)
+            if (P.driver == NBC_DRIVER_SPLIT) over = C.type[c] == 0 && Length<double>(cum) > 0.10 * po.w;
+            else over = Length<double>(cum) > P.Thresh;
+            // the sphere left its near-list allowance (threshold kept in cum.w)
+            near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > cu.w;
+            D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
+            D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = cu.w;
+            S.pos[i] = pn;
+            S.cum[i] = cn;
         }
-        // glue loop: type 0 rebuilds past 0.1 r, the kinematic types never ask
-        // (src/dem_integrate_withglue.cu:40-43)
-        if (P.driver == NBC_DRIVER_SPLIT) over = C.type[c] == 0 && Length<double>(cum) > 0.10 * po.w;
-        else over = Length<double>(cum) > P.Thresh;
-        // the sphere left its near-list allowance (threshold kept in cum.w)
-        near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > cu.w;
-        D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
-        D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = cu.w;
-        S.pos[i] = pn;
-        S.cum[i] = cn;
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
     }
     // the rebuild request (src/dem_clump.cu:132-134) records the step index:
@@ -546,7 +555,7 @@ __global__ void k_gather(DevParams P, int nface, const int* perm, SphereBuf<R> s
     const D4 po = src.pos[o];
     dst.pos[i] = po;
     D4 z; z.x = 0.0; z.y = 0.0; z.z = 0.0; z.w = 0.0;
-    dst.cum[i] = z;
+    dst.cum[i] = ref_disp<R>(P.driver) ? po : z;
     dst.loc[i] = src.loc[o];
     dst.kin[i] = src.kin[o];
     dst.cid[i] = src.cid[o];
@@ -844,11 +853,17 @@ __device__ __forceinline__ bool near_pair(const D4& a, const D4& b, uint32_t w, 
 }
 
 // Slack of every sphere and its near-list request threshold (|cum|^2 above
-// cum.w asks for a new near list), written into cum.w.
-__global__ void k_near_slack(DevParams P, D4* cum, double* slack) {
+// it asks for a new near list), written into cum.w, or with nthr given
+// (cum = position at the last rebuild) into nthr rounded down, so that the
+// request comes no later than with the exact threshold.
+__global__ void k_near_slack(DevParams P, const D4* pos, D4* cum, float* nthr, double* slack) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     D4 c = cum[i];
+    if (nthr) {
+        const D4 p = pos[i];
+        c.x = p.x - c.x; c.y = p.y - c.y; c.z = p.z - c.z;
+    }
     const double a = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
     const double d = P.near_delta;
     double sl, trig;
@@ -864,8 +879,12 @@ __global__ void k_near_slack(DevParams P, D4* cum, double* slack) {
         trig = __longlong_as_double(0x7ff0000000000000ll);
     }
     slack[i] = sl;
-    c.w = trig;
-    cum[i] = c;
+    if (nthr) {
+        nthr[i] = __double2float_rd(trig);
+    } else {
+        c.w = trig;
+        cum[i] = c;
+    }
 }
 
 // The near-list passes work like the contact kernel: a wave takes 64
@@ -2358,16 +2377,16 @@ int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* b
     return 0;
 }
 
-int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, double* slack, const int* foff,
-                      const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff, uint32_t* nfw,
-                      int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp, int* stotal,
-                      int block) {
+int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, float* nthr, double* slack,
+                      const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
+                      uint32_t* nfw, int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp,
+                      int* stotal, int block) {
     const int N = P.N;
     const unsigned g = nblk(N, block);
     int e;
     if ((e = launch_fill_i32(st, ncnt + N, 1, 0))) return e;
     if ((e = launch_fill_i32(st, nbcnt + N, 1, 0))) return e;
-    k_near_slack<<<g, block, 0, st>>>(P, cum, slack);
+    k_near_slack<<<g, block, 0, st>>>(P, pos, cum, nthr, slack);
     LAUNCH_CHECK();
     const unsigned gw = nblk(N, 256);  // 4 waves of 64 spheres per block
     k_near_scan<<<gw, 256, 0, st>>>(P, pos, slack, foff, fword, ncnt, slot2near);
