@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "nbody-clump_amd"))
 
 HBM_PEAK = 8.0e12
+AMORT_STEPS = 1000
 
 
 def run_form(bed, precision, steps, warmup, prof_steps):
@@ -47,6 +48,13 @@ def run_form(bed, precision, steps, warmup, prof_steps):
     ctx.step(steps)  # returns after a stream synchronize
     t1 = time.perf_counter()
     s1 = ctx.stats()
+    # amortised window: 1,000 further steps whatever --steps is, so that the
+    # rebuilds and near-list refreshes a longer run meets are in the figure
+    ctx.sync()
+    ta = time.perf_counter()
+    ctx.step(AMORT_STEPS)
+    tb = time.perf_counter()
+    sa = ctx.stats()
     # kernel durations: HIP events around every kernel class, recorded on the
     # stream each kernel runs on, over a second timed region of the same run
     # (the events add gaps between kernels, so the throughput region runs
@@ -57,7 +65,9 @@ def run_form(bed, precision, steps, warmup, prof_steps):
     s2 = ctx.stats()
     k3_ms, k3_n = ctx.profile_get(1)
     out = dict(seconds=t1 - t0, k3_ms=k3_ms, k3_n=k3_n, rebuilds=s1.rebuilds - s0.rebuilds,
-               prof_rebuilds=s2.rebuilds - s1.rebuilds,
+               prof_rebuilds=s2.rebuilds - sa.rebuilds, kinetic_energy=s1.kinetic_energy,
+               amortised=dict(steps=AMORT_STEPS, ms_per_step=(tb - ta) / AMORT_STEPS * 1e3,
+                              rebuilds=sa.rebuilds - s1.rebuilds, near_refreshes=sa.near_refreshes - s1.near_refreshes),
                touching_ss=0.5 * (s1.touching_ss + s2.touching_ss), touching_wall=0.5 * (s1.touching_wall + s2.touching_wall),
                front_pairs=s2.front_pairs, near_pairs=s2.near_pairs,
                near_refreshes=s1.near_refreshes - s0.near_refreshes)
@@ -438,7 +448,62 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 5}))
 
 
-def cpu_baseline(bed, sample_steps):
+def _relerr(a, b):
+    import numpy as np
+
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def oracle_error(bed, ref, prec, steps):
+    """Per-output error of one GPU form against the FP64 oracle `ref`, both
+    advanced `steps` steps from the same bed (the last one recorded): the
+    outputs of src/dem_force_clump.cu:165-191 (per-sphere totals),
+    src/dem_clumpsphere.cu:110-151 (clump F, T), the clump state, the pair
+    histories and the contact set."""
+    import numpy as np
+
+    import nbc
+
+    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
+    ctx = nbc.Context(bed.finalized_params(opt), opt)
+    ctx.upload(bed, bed.wall)
+    ctx.step(steps - 1)
+    ctx.set_record(True)
+    ctx.step(1)
+    sa, sb = ctx.spheres(), ref.spheres()
+    ia, ib = np.argsort(sa["fid"]), np.argsort(sb["fid"])  # file order
+    fa, ta = ctx.sphere_forces(0)
+    fb, tb = ref.sphere_forces(0)
+    ca, cb = ctx.clumps(), ref.clumps()
+    pa, pb = ctx.pairs(), ref.pairs()
+    lists_equal = all(np.array_equal(pa[k], pb[k]) for k in ("foff", "fj", "boff", "bi", "bslot")) and \
+        np.array_equal(sa["fid"], sb["fid"])
+    st_a, st_b = ctx.stats(), ref.stats()
+    out = dict(sphere_F=_relerr(fa[ia], fb[ib]), sphere_T=_relerr(ta[ia], tb[ib]),
+               clump_F=_relerr(ca["F"], cb["F"]), clump_T=_relerr(ca["T"], cb["T"]),
+               clump_vel=_relerr(ca["vel"], cb["vel"]), clump_angv=_relerr(ca["angv"], cb["angv"]),
+               clump_pos_abs_m=float(np.abs(ca["pos"] - cb["pos"]).max()),
+               clump_quat_abs=float(np.abs(ca["quat"] - cb["quat"]).max()),
+               lists_bitexact=bool(lists_equal), touching_ss=[int(st_a.touching_ss), int(st_b.touching_ss)],
+               touching_wall=[int(st_a.touching_wall), int(st_b.touching_wall)],
+               rebuilds=[int(st_a.rebuilds), int(st_b.rebuilds)])
+    if lists_equal:
+        ha, hb = ctx.pair_history(), ref.pair_history()
+        xa, _ = ctx.pair_exports()
+        xb, _ = ref.pair_exports()
+        out["pair_history"] = _relerr(ha, hb)
+        out["contact_set_equal"] = bool(np.array_equal(np.linalg.norm(xa, axis=1) > 0, np.linalg.norm(xb, axis=1) > 0))
+    ctx.close()
+    return out
+
+
+def cpu_baseline(bed, sample_steps, with_error=True):
+    """The FP64 oracle on the host cores: the reference's slot-padded dataflow
+    ("faithful") over `sample_steps` steps after the list-building step, with
+    one forced rebuild halfway (src/dem_simulation_clump.cu:124-175), then the
+    compact slot mode (CSR pair range) over the same number of steps without
+    it. The compact run is the FP64 reference for the per-output error of both
+    GPU forms on the same steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
@@ -446,17 +511,41 @@ def cpu_baseline(bed, sample_steps):
 
     orc.set_threads(os.cpu_count() or 1)
     p = orc.finalize_params(bed)
-    o = orc.Oracle(p, nbc.make_options(wall_type=nbc.NBC_WALL_CYLINDER))
+    opt = nbc.make_options(wall_type=nbc.NBC_WALL_CYLINDER)
+    o = orc.Oracle(p, opt)
     o.upload(bed, bed.wall)
     o.step(1)  # first step builds the lists
+    half = sample_steps // 2
     t0 = time.perf_counter()
-    o.step(sample_steps)
+    o.step(half)
+    o.request_rebuild()
+    o.step(sample_steps - half)
     dt = time.perf_counter() - t0
+    forced = o.stats().rebuilds
     o.close()
-    return dict(value=bed.n * sample_steps / dt, unit="sphere-steps/s", cores=orc.threads(), kind="port",
-                sample=f"FP64 oracle (slot-padded reference dataflow, OpenMP), same 1.12M-sphere bed, "
-                       f"{sample_steps} steps after the first (list-building) step, BranchLmtLen "
-                       f"{bed.params.BranchLmtLen}, {dt:.1f} s")
+    oc = orc.Oracle(p, opt, compact=True)
+    oc.upload(bed, bed.wall)
+    oc.step(1)
+    t0 = time.perf_counter()
+    oc.step(sample_steps)
+    dtc = time.perf_counter() - t0
+    res = dict(value=bed.n * sample_steps / dt, unit="sphere-steps/s", cores=orc.threads(), kind="port",
+               sample=f"FP64 oracle, faithful mode (the reference's slot-padded dataflow, OpenMP), same "
+                      f"{bed.n}-sphere bed, {sample_steps} steps after the first (list-building) step with one "
+                      f"forced rebuild halfway ({forced} rebuilds in all), BranchLmtLen "
+                      f"{bed.params.BranchLmtLen}, {dt:.1f} s",
+               compact=dict(value=bed.n * sample_steps / dtc, unit="sphere-steps/s",
+                            sample=f"compact slot mode (CSR pair range), same steps without the forced rebuild, "
+                                   f"{dtc:.1f} s"))
+    err = None
+    if with_error:
+        err = {"steps": 1 + sample_steps,
+               "reference": "FP64 oracle (compact mode) from the same timed bed"}
+        for prec, name in ((nbc.NBC_FP64, "fp64"), (nbc.NBC_MIXED, "mixed")):
+            err[name] = oracle_error(bed, oc, prec, 1 + sample_steps)
+            print(f"[bench] oracle error {name}: {err[name]}", file=sys.stderr, flush=True)
+    oc.close()
+    return res, err
 
 
 def main():
@@ -561,7 +650,13 @@ def main():
             "n_spheres": N, "n_clumps": bed.nc, "touching_ss_per_sphere": mixed["touching_ss"] / N,
             "list_pairs_per_sphere": mixed["front_pairs"] / N, "rebuilds_in_timed_steps": mixed["rebuilds"],
             "near_entries_per_sphere": mixed["near_pairs"] / N, "near_refreshes_in_timed_steps": mixed["near_refreshes"],
+            "kinetic_energy_J": mixed["kinetic_energy"],
         },
+        # the same run continued for 1,000 steps: rebuilds and near-list
+        # refreshes included, so short timed windows do not flatter the rate
+        "amortised": {"mixed": dict(mixed["amortised"], sphere_steps_per_s=N / mixed["amortised"]["ms_per_step"] * 1e3),
+                      "fp64": (dict(fp64["amortised"], sphere_steps_per_s=N / fp64["amortised"]["ms_per_step"] * 1e3)
+                               if fp64 else None)},
         "roofline": {
             "kernel": "k_contacts_f32",
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -581,7 +676,7 @@ def main():
         res["speedup_vs_fp64"] = value / v64
     if not a.no_cpu:
         try:
-            res["cpu_baseline"] = cpu_baseline(bed, a.cpu_steps)
+            res["cpu_baseline"], res["oracle_error"] = cpu_baseline(bed, a.cpu_steps)
         except Exception as e:  # report, do not hide
             res["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(res))

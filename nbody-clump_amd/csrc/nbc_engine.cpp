@@ -517,6 +517,42 @@ class Engine : public EngineBase {
                 return fail(NBC_E_NONFINITE, b);
             }
         {
+            // Invalid sphere or clump state is refused here, before anything is
+            // copied to the device. The reference finds it one step later in the
+            // cube-wall kernel (src/dem_force_cube_wall_clump.cu:191-203) and
+            // reports the sorted index there; the first bad index in file order
+            // is what the host can name.
+            auto fin = [](const double* v, int k) {
+                for (int a = 0; a < k; ++a)
+                    if (!std::isfinite(v[a])) return false;
+                return true;
+            };
+            for (int i = 0; i < N; ++i) {
+                const int q = s->clump_id[i];
+                const char* what = nullptr;
+                if (!fin(s->pos + 3 * (size_t)i, 3)) what = "sphere position";
+                else if (!fin(s->local + 3 * (size_t)i, 3)) what = "sphere body-frame offset";
+                else if (!(std::isfinite(s->radius[i]) && s->radius[i] > 0.0)) what = "sphere radius";
+                else if (!(std::isfinite(s->mass[i]) && s->mass[i] > 0.0)) what = "sphere mass";
+                else if (!fin(c->pos + 3 * (size_t)q, 3)) what = "clump position";
+                else if (!fin(c->vel + 3 * (size_t)q, 3)) what = "clump velocity";
+                else if (!fin(c->angv + 3 * (size_t)q, 3)) what = "clump angular velocity";
+                else if (!fin(c->quat + 4 * (size_t)q, 4)) what = "clump quaternion";
+                else {
+                    const double* pr = c->prop + 4 * (size_t)q;
+                    if (!(fin(pr, 4) && pr[0] > 0.0 && pr[1] > 0.0 && pr[2] > 0.0 && pr[3] > 0.0))
+                        what = "clump inertia or mass";
+                }
+                if (what) {
+                    char b[200];
+                    std::snprintf(b, sizeof b,
+                                  "FATAL: Invalid input (%s) at particle index %d in file order. Simulation "
+                                  "stopped.", what, i);
+                    return fail(NBC_E_NONFINITE, b);
+                }
+            }
+        }
+        {
             // every clump holds a sphere (clumps are renumbered by their first sphere)
             std::vector<char> has(Nc, 0);
             for (int i = 0; i < N; ++i) has[s->clump_id[i]] = 1;
@@ -564,6 +600,9 @@ class Engine : public EngineBase {
         HIPC(hipMemset(s0.wh, 0, sizeof(R) * (size_t)std::max(nwh, 1) * N * HSTRIDE));
         HIPC(hipMemset(S[1].wflag, 0, sizeof(uint32_t) * N));
         HIPC(hipMemset(s0.sft, 0, sizeof(R4) * 2 * (size_t)N));
+        // near-list request thresholds: +inf until the first list build, so the
+        // first step's k_sphere_pose (before any rebuild) reads defined values
+        KC(launch_fill_u32(st, reinterpret_cast<uint32_t*>(s0.nthr), N, 0x7f800000u));
         std::vector<D4> pq(2 * (size_t)Nc), vw(2 * (size_t)Nc), prop(Nc), ft(2 * (size_t)Nc);
         for (int i = 0; i < Nc; ++i) {
             pq[2 * i] = D4{c->pos[3 * i], c->pos[3 * i + 1], c->pos[3 * i + 2], 0.0};
@@ -1279,24 +1318,27 @@ class Engine : public EngineBase {
         if (!q || q->n != N || !q->pos || !q->vel || !q->quat || !q->angv || !q->prop || !q->radius)
             return fail(NBC_E_BADINPUT, "nbc_upload_particles: particle count or arrays missing");
         if (!w) return fail(NBC_E_BADINPUT, "nbc_upload_particles: the DEMmain loops need their wall");
-        std::vector<int> cid(N);
+        std::vector<int> cid(N), ty(N, 0);
         std::vector<double> local(3 * (size_t)N, 0.0), mass(N);
-        htype.assign(N, 0);
         for (int i = 0; i < N; ++i) {
             cid[i] = i;
             mass[i] = q->prop[4 * (size_t)i + 3];
             if (q->type) {
-                htype[i] = q->type[i];
-                if (driver == NBC_DRIVER_SPLIT && (htype[i] < 0 || htype[i] > 2))
+                ty[i] = q->type[i];
+                if (driver == NBC_DRIVER_SPLIT && (ty[i] < 0 || ty[i] > 2))
                     return fail(NBC_E_BADINPUT, "nbc_upload_particles: glue type not 0, 1 or 2");
             }
         }
-        hprop.assign(q->prop, q->prop + 4 * (size_t)N);
+        std::vector<int> ty_old;
+        ty_old.swap(htype);
+        htype = ty;  // upload() reads the types
         const nbc_spheres_in si{N, 0, cid.data(), q->pos, local.data(), mass.data(), q->radius};
         const nbc_clumps_in ci{N, 0, q->pos, q->vel, q->quat, q->angv, q->prop};
         particle_upload = true;
         const int e = upload(&si, &ci, w, pe);
         particle_upload = false;
+        if (e) htype.swap(ty_old);  // a refused upload leaves the context as it was
+        else hprop.assign(q->prop, q->prop + 4 * (size_t)N);
         return e;
     }
 

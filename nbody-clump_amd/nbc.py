@@ -397,6 +397,20 @@ def sphere_penetrator(radius, pos, vel, mass=1.0, motion=NBC_PENE_CONSTANT_VELOC
     return p
 
 
+def cylinder_penetrator(radius, half_height, pos, vel, axis=(0.0, 0.0, 1.0), mass=1.0,
+                        motion=NBC_PENE_CONSTANT_VELOCITY):
+    """The reference's cylinder penetrator (PenetratorShape 0): radius, half
+    height along the unit axis dir (src/dem_contact_penetrator.cu:42-83)."""
+    p = Penetrator()
+    p.shape, p.motion, p.radius, p.height = NBC_PENE_CYLINDER, motion, radius, half_height
+    p.dir[:] = axis
+    p.prop[:] = (1.0, 1.0, 1.0, mass)
+    p.pos[:] = pos
+    p.vel[:] = vel
+    p.quat[:] = (0.0, 0.0, 0.0, 1.0)
+    return p
+
+
 def cone_penetrator(radius, height, pos, vel, nseg=64, mass=1.0):
     p = Penetrator()
     p.shape, p.motion, p.radius, p.height = NBC_PENE_CONE, NBC_PENE_CONSTANT_VELOCITY, radius, height

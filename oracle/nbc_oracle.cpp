@@ -80,6 +80,14 @@ struct Hist3 {  // one (Dv, Dt, Dr) history set: tangential, twisting, rolling
     }
 };
 
+void zero_range(Hist3& h, long long n) {
+    std::fill(h.v.begin(), h.v.begin() + n, d3{0, 0, 0});
+    std::fill(h.t.begin(), h.t.begin() + n, d3{0, 0, 0});
+    std::fill(h.r.begin(), h.r.begin() + n, d3{0, 0, 0});
+}
+
+void zero_range(std::vector<d3>& v, long long n) { std::fill(v.begin(), v.begin() + n, d3{0, 0, 0}); }
+
 template <class T>
 void gather(std::vector<T>& a, const std::vector<int>& perm) {
     std::vector<T> c(a);
@@ -141,6 +149,11 @@ struct orc_ctx {
     long long touching_ss = 0, touching_wall = 0;
     bool first = true;
     bool rebuild_next = false;
+    // compact mode: per-step work and clears cover the used slot range
+    // [0, S_used) and the used row entries only (a CSR pair list), instead of
+    // all N*BranchLmtLen/2 slots and N*BranchLmtLen row entries
+    bool compact = false;
+    long long S_used = 0, S_high = 0;
 };
 
 static int fail(orc_ctx* c, int code, const std::string& m) {
@@ -237,6 +250,16 @@ extern "C" int orc_upload(orc_ctx* c, const nbc_spheres_in* s, const nbc_clumps_
     const int N = c->N, Nc = c->Nc, B = c->B;
     if (!s || !cl || s->n != N || cl->n != Nc || N <= 0 || Nc <= 0 || B < 2)
         return fail(c, NBC_E_BADINPUT, "orc_upload: size mismatch");
+    // An out-of-range clump id would be read out of bounds by UpdatePos
+    // (src/dem_clump.cu:120) before the wall kernel's check
+    // (src/dem_force_cube_wall_clump.cu:191-194) sees it; refuse it here.
+    for (int i = 0; i < N; ++i)
+        if (s->clump_id[i] < 0 || s->clump_id[i] >= Nc) {
+            char m[160];
+            std::snprintf(m, sizeof m, "FATAL: Invalid input (clump id) at particle index %d in file order. "
+                                       "Simulation stopped.", i);
+            return fail(c, NBC_E_NONFINITE, m);
+        }
     c->ClumpId.assign(s->clump_id, s->clump_id + N);
     c->Pos.resize(N);
     c->LocalPos.resize(N);
@@ -310,6 +333,7 @@ extern "C" int orc_upload(orc_ctx* c, const nbc_spheres_in* s, const nbc_clumps_
     c->CacheFi.assign(oi, d3{0, 0, 0}); c->CacheTi.assign(oi, d3{0, 0, 0});
     c->Update = (unsigned)c->P.Update;
     c->step = 0; c->rebuilds = 0; c->count = 0; c->first = true;
+    c->S_used = 0; c->S_high = 0;
     c->CDist.assign(Nc, d3{0, 0, 0});
     if (c->Type.size() != (size_t)Nc) c->Type.assign(Nc, 0);
     return 0;
@@ -677,12 +701,25 @@ static int rebuild(orc_ctx* c) {
     // last written history (caller's Dv) and writes the caller's Dv_pre.
     c->Lpb.swap(c->Lpb_pre);
     c->Nsum.swap(c->Nsum_pre);
+    if (c->compact) {
+        // only the slots and row entries any list has used can hold values
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < N; ++i) {
+            int* rowA = &c->Anei[(size_t)i * B];
+            int* rowR = &c->Aref[(size_t)i * B];
+            for (int g = 0; g < c->Ng[i]; ++g) rowA[g] = rowR[g] = -1;
+            for (int g = B - c->Ni[i]; g < B; ++g) rowA[g] = rowR[g] = -1;
+        }
+        std::fill(c->Lpa.begin(), c->Lpa.begin() + c->S_high, -1);
+        std::fill(c->Lpb.begin(), c->Lpb.begin() + c->S_high, -1);
+    } else {
+        std::fill(c->Lpa.begin(), c->Lpa.end(), -1);
+        std::fill(c->Lpb.begin(), c->Lpb.end(), -1);
+        std::fill(c->Anei.begin(), c->Anei.end(), -1);
+        std::fill(c->Aref.begin(), c->Aref.end(), -1);
+    }
     std::fill(c->Ng.begin(), c->Ng.end(), 0);
     std::fill(c->Ni.begin(), c->Ni.end(), 0);
-    std::fill(c->Lpa.begin(), c->Lpa.end(), -1);
-    std::fill(c->Lpb.begin(), c->Lpb.end(), -1);
-    std::fill(c->Anei.begin(), c->Anei.end(), -1);
-    std::fill(c->Aref.begin(), c->Aref.end(), -1);
     if (c->P.Levels > 1 ? create_anei_h(c) : create_anei(c)) {
         char m[256];
         std::snprintf(m, sizeof m,
@@ -714,8 +751,11 @@ static int rebuild(orc_ctx* c) {
     // (last written), write the caller's Dv_pre.
     Hist3& src = c->D;
     Hist3& dst = c->Dpre;
-    dst.zero();
     const long long np = acc;
+    c->S_used = np;
+    c->S_high = std::max(c->S_high, np);
+    if (c->compact) zero_range(dst, c->S_high);
+    else dst.zero();
 #pragma omp parallel for schedule(static)
     for (long long k = 0; k < np; ++k) {
         const int i = c->Lpa[k], j = c->Lpb[k];
@@ -734,7 +774,10 @@ static int rebuild(orc_ctx* c) {
     }
     // the cylinder loops exchange right after this and read D (the last
     // written histories, old slots) as their Dpre; the others start D clean
-    if (c->driver != NBC_DRIVER_CYLINDER && c->driver != NBC_DRIVER_SPLIT) c->D.zero();
+    if (c->driver != NBC_DRIVER_CYLINDER && c->driver != NBC_DRIVER_SPLIT) {
+        if (c->compact) zero_range(c->D, c->S_high);
+        else c->D.zero();
+    }
     c->Update = 0;
     std::fill(c->CumPos.begin(), c->CumPos.end(), d3{0, 0, 0});
     c->rebuilds++;
@@ -840,8 +883,9 @@ static bool ss_force(orc_ctx* c, long long k) {
 static void sphere_forces(orc_ctx* c) {
     const int N = c->N, B = c->B;
     long long touch = 0;
+    const long long S = c->compact ? c->S_used : c->S;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : touch)
-    for (long long k = 0; k < c->S; ++k)
+    for (long long k = 0; k < S; ++k)
         if (ss_force(c, k)) touch++;
     c->touching_ss = touch;
     // SumContForce_clump (src/dem_force_clump.cu:165-191)
@@ -1372,6 +1416,8 @@ static void pene_integrate_after(orc_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
+static void clump_kick(orc_ctx* c);
+
 // src/dem_clump.cu:89-101 and src/dem_clumpsphere.cu:86-151, then
 // src/dem_integrate_clumpsphere.cu:110-141.
 static void clump_reduce_and_kick(orc_ctx* c) {
@@ -1402,6 +1448,12 @@ static void clump_reduce_and_kick(orc_ctx* c) {
     }
     for (int k = 0; k < Nc; ++k)
         if (seen[k]) { c->CFc[k] = F[k]; c->CTc[k] = T[k]; }
+    clump_kick(c);
+}
+
+// src/dem_integrate_clumpsphere.cu:110-141: second half-kick from CFc/CTc.
+static void clump_kick(orc_ctx* c) {
+    const int N = c->N, Nc = c->Nc;
     const double Dt = c->P.StepSize, g = c->P.Gravity;
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < Nc; ++i) {
@@ -1444,18 +1496,24 @@ static int step_once(orc_ctx* c) {
             // src/dem_simulation_cylind.cu:141: ForceExchange follows the rebuild
             // too, swapping the inherited histories out (and clearing them)
             c->D.swap(c->Dpre);
-            c->D.zero();
+            if (c->compact) zero_range(c->D, c->S_high);
+            else c->D.zero();
         }
     } else {
         c->D.swap(c->Dpre);  // DEMForce::ForceExchange
-        c->D.zero();
+        if (c->compact) zero_range(c->D, c->S_used);
+        else c->D.zero();
     }
     c->first = false;
     for (int f = 0; f < nf; ++f) { c->WH[f].swap(c->WHpre[f]); c->WH[f].zero(); }
     if (c->P.isIntruder) { c->PH.swap(c->PHpre); c->PH.zero(); }
     // ForceZero (src/dem_force.cu:75-86, src/dem_cube_wall.cu:208-211)
     zero3(c->Fc); zero3(c->Tc);
-    zero3(c->CacheFc); zero3(c->CacheTc); zero3(c->CacheTc2); zero3(c->CacheFcn); zero3(c->CacheCtp);
+    if (c->compact) {
+        for (auto* v : {&c->CacheFc, &c->CacheTc, &c->CacheTc2, &c->CacheFcn, &c->CacheCtp}) zero_range(*v, c->S_used);
+    } else {
+        zero3(c->CacheFc); zero3(c->CacheTc); zero3(c->CacheTc2); zero3(c->CacheFcn); zero3(c->CacheCtp);
+    }
     zero3(c->WFc); zero3(c->WTc);
     c->touching_wall = 0;
     sphere_forces(c);
@@ -1487,6 +1545,27 @@ extern "C" int orc_step(orc_ctx* c, int n) {
     for (int k = 0; k < n; ++k) {
         int e = step_once(c);
         if (e) return e;
+    }
+    return 0;
+}
+
+// Compact mode (CSR pair range instead of the slot-padded layout); the
+// results are identical, only the per-step clears and slot loops shrink.
+extern "C" void orc_set_compact(orc_ctx* c, int on) { c->compact = on != 0; }
+
+// Test hook for the integrator alone (no contacts): nsteps of the clump
+// half-kick + drift + attitude (src/dem_integrate_clumpsphere.cu:3-90) and the
+// second half-kick (:110-141) with each clump's force and torque held at the
+// given values (3*Nc doubles each, world frame). Sphere poses are not updated.
+extern "C" int orc_integrate_only(orc_ctx* c, int nsteps, const double* F, const double* T) {
+    for (int k = 0; k < nsteps; ++k) {
+        for (int i = 0; i < c->Nc; ++i) {
+            c->CFc[i] = mk3(F[3 * i], F[3 * i + 1], F[3 * i + 2]);
+            c->CTc[i] = mk3(T[3 * i], T[3 * i + 1], T[3 * i + 2]);
+        }
+        integrate_before(c);
+        clump_kick(c);
+        c->step++;
     }
     return 0;
 }

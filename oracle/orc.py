@@ -43,6 +43,8 @@ _SIGS = {
     "orc_get_pair_exports": ([P, P, P], C.c_int),
     "orc_get_stats": ([P, C.POINTER(nbc.Stats)], C.c_int),
     "orc_bruteforce_pairs": ([P, P, P, C.c_longlong], C.c_longlong),
+    "orc_set_compact": ([P, I32], None),
+    "orc_integrate_only": ([P, I32, P, P], C.c_int),
 }
 _lib = None
 
@@ -84,11 +86,13 @@ def finalize_params(bed, cell_size=0.0, levels=0):
 class Oracle:
     """Same getter surface as nbc.Context, computed by the FP64 restatement."""
 
-    def __init__(self, params, opt=None):
+    def __init__(self, params, opt=None, compact=False):
         self.params = params
         self.h = lib().orc_create(C.byref(params), C.byref(opt) if opt else None)
         self.N = params.ParticleNum
         self.Nc = params.ClumpNum
+        if compact:
+            lib().orc_set_compact(self.h, 1)
 
     def close(self):
         if self.h:
@@ -122,6 +126,13 @@ class Oracle:
 
     def rebuild(self):
         self._c(lib().orc_rebuild(self.h))
+
+    def integrate_only(self, n, F, T):
+        """n integrator steps with every clump's force and torque held at F, T
+        (Nc x 3 each); no contacts, sphere poses not updated."""
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(self.Nc, 3)
+        T = np.ascontiguousarray(T, dtype=np.float64).reshape(self.Nc, 3)
+        self._c(lib().orc_integrate_only(self.h, n, ptr(F), ptr(T)))
 
     def request_rebuild(self):
         lib().orc_request_rebuild(self.h)

@@ -89,6 +89,27 @@ def test_run_simulation_clump_on_reference_inputs(tmp_path):
         if nb:
             assert np.array_equal(nei[i, B - nb:], pr["bi"][pr["boff"][i]:pr["boff"][i + 1]])
         assert np.all(nei[i, ng:B - nb] == -1)
+    # Fcn / Cpt rows (src/dem_force_particle.cu:238-268): front entries +CacheFcn,
+    # back entries -CacheFcn of the owner's slot, contact points as stored
+    fcn, cpt = o.pair_exports()
+    rows = {}
+    for name in ("Fcn", "Cpt"):
+        raw = open(d / f"OutCnt/{name}.00000300.bin", "rb").read()
+        assert tuple(np.frombuffer(raw[:8], np.int32)) == (N, B)
+        rows[name] = np.frombuffer(raw[8:], np.float64).reshape(N, B, 3)
+    ef, ec = np.zeros((N, B, 3)), np.zeros((N, B, 3))
+    for i in range(N):
+        a, b = pr["foff"][i], pr["foff"][i + 1]
+        ef[i, :b - a], ec[i, :b - a] = fcn[a:b], cpt[a:b]
+        a, b = pr["boff"][i], pr["boff"][i + 1]
+        if b > a:
+            sl = pr["bslot"][a:b]
+            assert np.all(sl >= 0)
+            ef[i, B - (b - a):], ec[i, B - (b - a):] = -fcn[sl], cpt[sl]
+    scale = np.abs(fcn).max()
+    assert scale > 0, "no touching pair at the output step"
+    assert np.abs(rows["Fcn"] - ef).max() <= 1e-12 * scale
+    assert np.abs(rows["Cpt"] - ec).max() <= 1e-12
 
 
 def test_mixed_precision_run_and_overflow_error(tmp_path):

@@ -90,3 +90,29 @@ def rodrigues(axis, angle):
     k = np.asarray(axis, float) / np.linalg.norm(axis)
     K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
     return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+def check_mixed_history(ctx, ref, mu):
+    """FP32-mixed pair histories against the oracle (needs a recorded step).
+
+    The tangential history is discontinuous where the Coulomb cap switches
+    (src/dem_force_clump.cu:100-110): a slipping pair stores the capped force
+    tf, a sticking one tf - Cs vt. A pair whose |tf| lies within FP32 rounding
+    of mu |Fn| can take the other branch, so pairs at the cap on either side
+    (|h_t| = mu |Fn| to 1e-5) are counted apart; every other pair holds 1e-4
+    of the largest history. The rolling and twisting histories are continuous
+    at their caps."""
+    ha, hb = ctx.pair_history(), ref.pair_history()
+    fa, _ = ctx.pair_exports()
+    fb, _ = ref.pair_exports()
+    na, nb = np.linalg.norm(fa, axis=1), np.linalg.norm(fb, axis=1)
+    ta, tb = np.linalg.norm(ha[:, :3], axis=1), np.linalg.norm(hb[:, :3], axis=1)
+    at_cap = ((nb > 0) & (np.abs(tb - mu * nb) <= 1e-5 * mu * nb)) | ((na > 0) & (np.abs(ta - mu * na) <= 1e-5 * mu * na))
+    scale = max(np.abs(hb).max(), 1e-300)
+    err = np.abs(ha - hb).max(axis=1) / scale
+    flips = int(np.count_nonzero(at_cap & (err > 1e-4)))
+    e_rest = err[~at_cap].max() if np.any(~at_cap) else 0.0
+    print(f"  mixed history: {int(at_cap.sum())} pairs at the Coulomb cap, {flips} took the other branch; "
+          f"other pairs max rel err {e_rest:.2e}", flush=True)
+    assert e_rest <= 1e-4
+    assert flips <= max(2, at_cap.sum() // 100)
