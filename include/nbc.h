@@ -415,6 +415,21 @@ int nbc_get_penetrator(nbc_ctx* ctx, double* pos, double* vel, double* quat, dou
 int nbc_get_pene_exports(nbc_ctx* ctx, double* fn, double* ft, double* tn, double* tt);
 int nbc_get_pair_exports(nbc_ctx* ctx, double* fcn, double* cpt);
 
+/* ---- standalone grid build on device buffers ------------------------- */
+/* DEMGrid::NeighborSearch (src/dem_grid.cu:649-678) without a context: the
+ * cell hash of GetParticleCellIndex_cuk (:155-202; the hierarchical grid of
+ * src/dem_hgrid.cu:205-242 when p->Levels > 1, levels by radius), the stable
+ * sort of SortParticleByCellIndex (:278-305) and FindBegEndInCell_cuk
+ * (:105-153). p is finalized; every buffer is device memory: d_pos4 holds n
+ * rows (x, y, z, radius); d_cell[n] receives the sorted cell keys (part_cel),
+ * d_perm[n] the new -> old index (part_idx_pre), d_beg / d_end[CellNum] the
+ * first and one-past-last sorted index of each cell (-1 when empty).
+ * stream is a hipStream_t (NULL: the default stream). The call allocates the
+ * sort's temporaries (about 16 n + 8 n + small bytes), runs on the stream and
+ * synchronizes it before freeing them. */
+int nbc_grid_build(const nbc_params* p, const double* d_pos4, int32_t n, uint32_t* d_cell, int32_t* d_perm,
+                   int32_t* d_beg, int32_t* d_end, void* stream);
+
 /* ---- timing of library kernels on the library stream ----------------- */
 /* Kernel classes: 0 sphere kinematics (K2; K1 of the first step of a call),
  * 1 sphere-sphere contacts, 2 clump reduction and second half-kick (with K1
