@@ -429,6 +429,17 @@ int nbc_get_pair_exports(nbc_ctx* ctx, double* fcn, double* cpt);
  * synchronizes it before freeing them. */
 int nbc_grid_build(const nbc_params* p, const double* d_pos4, int32_t n, uint32_t* d_cell, int32_t* d_perm,
                    int32_t* d_beg, int32_t* d_end, void* stream);
+/* nbc_grid_build without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results; the sort's temporaries are a fixed device array of
+ * the library (about 32 MiB per GPU, held until the process ends), so n is at
+ * most NBC_GRID_NOALLOC_MAX_SPHERES (larger n: NBC_E_BADINPUT). The call is
+ * asynchronous on `stream` and does not synchronize. Every call on one GPU
+ * shares that array: calls that overlap in time on one GPU race. */
+#define NBC_GRID_NOALLOC_MAX_SPHERES 2097152
+int nbc_grid_build_noalloc(const nbc_params* p, const double* d_pos4, int32_t n, uint32_t* d_cell,
+                           int32_t* d_perm, int32_t* d_beg, int32_t* d_end, void* stream);
 
 /* ---- timing of library kernels on the library stream ----------------- */
 /* Kernel classes: 0 sphere kinematics (K2; K1 of the first step of a call),

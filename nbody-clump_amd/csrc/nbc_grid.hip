@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/nbc.h"
@@ -145,5 +146,72 @@ extern "C" int nbc_grid_build(const nbc_params* p, const double* d_pos4, int32_t
     (void)hipFree(buf);
     if (e) return hip_fail("nbc_grid_build: launch", e);
     if (h != hipSuccess) return hip_fail("nbc_grid_build: hipStreamSynchronize", h);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant for callers that cannot call hipMalloc (code under
+// HIP graph capture, tight serving loops): the sort's temporaries live in a
+// file-scope device array of fixed size, one copy per GPU for the life of
+// the process. Every scratch word a kernel reads was written earlier in the
+// same call (digit counts, their scan, block sums, the second key/value
+// buffer), so nothing carries over from a previous call and nothing needs
+// clearing. All calls on one GPU share the array: calls that overlap in time
+// on one GPU (concurrent streams, threads) race and must be serialized by
+// the caller.
+// 4 buffers of NBC_GRID_NOALLOC_MAX_SPHERES ints (second keys, second
+// values, digit counts and their scan at 8-bit digits) plus the scan block
+// sums and totals: 32 MiB + 64 KiB
+static __device__ __align__(256) unsigned char scratch[4 * sizeof(int) * (size_t)NBC_GRID_NOALLOC_MAX_SPHERES +
+                                                        64 * 1024];
+
+namespace {
+
+constexpr int32_t NOALLOC_MAX = NBC_GRID_NOALLOC_MAX_SPHERES;
+constexpr size_t SCRATCH_BYTES = sizeof(scratch);
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy).
+int scratch_on_current_device(void** out) {
+    int dev = -1;
+    hipError_t h = hipGetDevice(&dev);
+    if (h != hipSuccess) return hip_fail("nbc_grid_build_noalloc: hipGetDevice", h);
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(NBC_E_NODEVICE, "nbc_grid_build_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        h = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (h != hipSuccess || !p) return hip_fail("nbc_grid_build_noalloc: hipGetSymbolAddress", h);
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int nbc_grid_build_noalloc(const nbc_params* p, const double* d_pos4, int32_t n, uint32_t* d_cell,
+                                      int32_t* d_perm, int32_t* d_beg, int32_t* d_end, void* stream) {
+    DevParams dp;
+    int e = check_args("nbc_grid_build_noalloc", p, d_pos4, n, d_cell, d_perm, d_beg, d_end, &dp);
+    if (e) return e;
+    if (n > NOALLOC_MAX) {
+        char b[160];
+        std::snprintf(b, sizeof b, "nbc_grid_build_noalloc: %d spheres exceed the scratch limit of %d", n, NOALLOC_MAX);
+        return fail(NBC_E_BADINPUT, b);
+    }
+    if (grid_scratch_layout(n, nullptr, nullptr) > SCRATCH_BYTES)
+        return fail(NBC_E_BADINPUT, "nbc_grid_build_noalloc: scratch layout exceeds the static array");
+    void* base = nullptr;
+    e = scratch_on_current_device(&base);
+    if (e) return e;
+    GridScratch g;
+    grid_scratch_layout(n, (char*)base, &g);
+    e = grid_build(dp, d_pos4, d_cell, d_perm, d_beg, d_end, g, (hipStream_t)stream);
+    if (e) return hip_fail("nbc_grid_build_noalloc: launch", e);
     return 0;
 }

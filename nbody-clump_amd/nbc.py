@@ -173,6 +173,7 @@ _SIGS = {
     "nbc_get_pene_exports": ([P, P, P, P, P], C.c_int),
     "nbc_get_pair_exports": ([P, P, P], C.c_int),
     "nbc_grid_build": ([C.POINTER(Params), P, I32, P, P, P, P, P], C.c_int),
+    "nbc_grid_build_noalloc": ([C.POINTER(Params), P, I32, P, P, P, P, P], C.c_int),
     "nbc_profile_enable": ([P, I32], C.c_int),
     "nbc_profile_get": ([P, I32, C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
     "nbc_write_particles_bin": ([C.c_char_p, I32, I32, P, P, P, P, P, P, P], C.c_int),
