@@ -534,6 +534,10 @@ class Engine : public EngineBase {
                 else if (!fin(s->local + 3 * (size_t)i, 3)) what = "sphere body-frame offset";
                 else if (!(std::isfinite(s->radius[i]) && s->radius[i] > 0.0)) what = "sphere radius";
                 else if (!(std::isfinite(s->mass[i]) && s->mass[i] > 0.0)) what = "sphere mass";
+                else if (sizeof(R) == sizeof(float) &&
+                         !(std::isfinite((float)s->local[3 * (size_t)i]) && std::isfinite((float)s->local[3 * (size_t)i + 1]) &&
+                           std::isfinite((float)s->local[3 * (size_t)i + 2]) && (float)s->mass[i] > 0.0f))
+                    what = "sphere offset or mass outside the FP32 range";
                 else if (!fin(c->pos + 3 * (size_t)q, 3)) what = "clump position";
                 else if (!fin(c->vel + 3 * (size_t)q, 3)) what = "clump velocity";
                 else if (!fin(c->angv + 3 * (size_t)q, 3)) what = "clump angular velocity";
