@@ -325,7 +325,7 @@ class Engine : public EngineBase {
         F(C.pq); F(C.vw); F(C.prop); F(C.ft); F(C.snap); F(C.dist); F(C.type);
         F(C2.pq); F(C2.vw); F(C2.prop); F(C2.ft); F(C2.snap); F(C2.dist); F(C2.type);
         F(cfile); F(cfile2); F(cfirst); F(cnew);
-        F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near);
+        F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near); F(hn[0]); F(hn[1]);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
@@ -817,7 +817,7 @@ class Engine : public EngineBase {
                                 bent, dstatus + 2, block));
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
         // history-valid bits of the old near list back into the old pair words
-        KC(launch_near_sync(st, N, nfoff, nfw, nfs, Lo.fword, block));
+        KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, Lo.fword, hn[p], Lo.hist[p], p, block));
         if (driver == NBC_DRIVER_CYLINDER || driver == NBC_DRIVER_SPLIT)
             KC(launch_hist_carry<R>(st, tot[0], nfront, p, Ln.fword, Ln.hist[p], Lo.fword, Lo.hist[p], block));
         else
@@ -831,6 +831,7 @@ class Engine : public EngineBase {
                              block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
+        { int e = near_hist_in(Ln.hist[p], p); if (e) return e; }
         nfront = tot[0];
         nback = tot[1];
         // 6. clump member lists in ascending sorted order (replaces the per-step
@@ -910,7 +911,7 @@ class Engine : public EngineBase {
         }
         ListBuf<R> L;
         L.nfoff = nfoff; L.nfw = nfw; L.nfs = nfs; L.nboff = nboff; L.nbo = nbo; L.nbe = nbe;
-        L.hist[0] = Lf[lcur].hist[0]; L.hist[1] = Lf[lcur].hist[1];
+        L.hist[0] = hn[0]; L.hist[1] = hn[1];
         const int* halt = (const int*)dstatus;
         int* bad = (int*)(dstatus + 1);
         hipEvent_t e0 = prof_begin();
@@ -938,19 +939,38 @@ class Engine : public EngineBase {
 
     // Near lists of the current Verlet list at the current positions (a
     // sphere left its near-list allowance between rebuilds).
-    int near_refresh() {
+    int near_refresh(int q) {
         gen++;
         hipEvent_t e0 = prof_begin();
         FrontList& L = Lf[lcur];
-        KC(launch_near_sync(st, N, nfoff, nfw, nfs, L.fword, block));
+        KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, L.fword, hn[q], L.hist[q], q, block));
         SphereBuf<R>& s = S[scur];
         KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, L.foff, L.fword,
                              boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2,
                              block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
+        { int e = near_hist_in(L.hist[q], q); if (e) return e; }
         nnear++;
         prof_end(6, e0, -1);
+        return 0;
+    }
+
+    // The contact kernel keeps the pair histories in near order (dense: the
+    // touching pairs of a wave sit in its contiguous near segment); after a
+    // near build the latest buffer q moves in from front-slot order, and
+    // k_near_sync moves it back before the lists change or are read.
+    int near_hist_in(const R* hs_q, int q) {
+        if ((long long)near_total > hn_cap) {
+            dfree(hn[0]); dfree(hn[1]);
+            hn[0] = hn[1] = nullptr;
+            hn_cap = 0;
+            const long long cap = (long long)(near_total * 1.25) + 1024;
+            HIPC(dalloc(&hn[0], (size_t)cap * HSTRIDE));
+            HIPC(dalloc(&hn[1], (size_t)cap * HSTRIDE));
+            hn_cap = cap;
+        }
+        KC(launch_near_hist_gather<R>(st, near_total, nfw, nfs, hs_q, hn[q], q));
         return 0;
     }
 
@@ -1026,7 +1046,7 @@ class Engine : public EngineBase {
         if ((why & ST_REBUILD) || !(why & (ST_WLIST | ST_NEAR))) {
             if ((e = rebuild((int)(h & 1)))) return e;
         } else {
-            if ((why & ST_NEAR) && (e = near_refresh())) return e;
+            if ((why & ST_NEAR) && (e = near_refresh((int)(h & 1)))) return e;
             if ((why & ST_WLIST) && (e = build_wall_list())) return e;
         }
         if ((e = clear_halt())) return e;
@@ -1273,9 +1293,9 @@ class Engine : public EngineBase {
         HIPC(hipStreamSynchronize(st));
         if (!nfront) return 0;
         const FrontList& L = Lf[lcur];
-        KC(launch_near_sync(st, N, nfoff, nfw, nfs, L.fword, block));
-        HIPC(hipStreamSynchronize(st));
         const int q = (int)(nstep & 1);  // step nstep-1 (parity q^1) wrote buffer q
+        KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, L.fword, hn[q], L.hist[q], q, block));
+        HIPC(hipStreamSynchronize(st));
         std::vector<uint32_t> w(nfront);
         std::vector<R> h((size_t)nfront * HSTRIDE);
         HIPC(hipMemcpy(w.data(), L.fword, sizeof(uint32_t) * nfront, hipMemcpyDeviceToHost));
@@ -1573,6 +1593,8 @@ class Engine : public EngineBase {
     uint32_t* nfw = nullptr;
     int *nfs = nullptr, *nbo = nullptr, *nbe = nullptr;
     long long ncap = 0;
+    R* hn[2] = {nullptr, nullptr};  // [hn_cap][HSTRIDE] pair histories in near order
+    long long hn_cap = 0;
     int near_total = 0;  // near pairs (front and back entries alike)
     RecordBuf rec;
     bool uploaded = false, first = true, init_update = false, record_next = false, record_last = false, prof = false;

@@ -58,7 +58,8 @@ struct ListBuf {
     int* nboff;                       // [N + 1] near back rows
     int* nbo;                         // [near] owner of the back entry's pair
     int* nbe;                         // [near] the pair's index in the owner's near row
-    R* hist[2];                       // [front slots][HSTRIDE] ping-pong histories
+    R* hist[2];                       // [near front entries][HSTRIDE] ping-pong histories in near order
+                                      // (in front-slot order outside the step kernels)
 };
 
 // Wall list: spheres that may touch a wall or the penetrator until the list
@@ -139,8 +140,12 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum
                       const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
                       uint32_t* nfw, int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp,
                       int* stotal, int block);
+template <class R>
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
-                     int block);
+                     const R* hn, R* hs, int q, int block);
+// Latest history (buffer q) of the near pairs from front-slot order into near order.
+template <class R>
+int launch_near_hist_gather(hipStream_t st, int nnear, const uint32_t* nfw, const int* nfs, const R* hs, R* hn, int q);
 // Sphere-sphere contacts of every sphere and, in the same launch, the wall and
 // penetrator contacts of the wlen spheres on the wall list.
 template <class R>

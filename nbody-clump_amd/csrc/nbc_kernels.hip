@@ -1077,15 +1077,33 @@ __global__ __launch_bounds__(256) void k_nback_put(int N, const int* boff, const
     }
 }
 
-// history-valid bits of the near pairs back into the pair words
-__global__ void k_near_sync(int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword) {
+// history-valid bits of the near pairs back into the pair words, and the
+// latest history (buffer q, kept in near order between near builds) back to
+// the pairs' front slots
+template <class R>
+__global__ void k_near_sync(int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
+                            const R* hn, R* hs, int q) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const uint32_t V = VALID(0) | VALID(1);
     for (int o = nfoff[i]; o < nfoff[i + 1]; ++o) {
         const int s = nfs[o];
-        fword[s] = (fword[s] & ~V) | (nfw[o] & V);
+        const uint32_t w = nfw[o];
+        fword[s] = (fword[s] & ~V) | (w & V);
+        if (w & VALID(q))
+            for (int k = 0; k < HSTRIDE; ++k) hs[(size_t)s * HSTRIDE + k] = hn[(size_t)o * HSTRIDE + k];
     }
+}
+
+// The latest history (buffer q) of the new near pairs, from their front slots
+// into near order.
+template <class R>
+__global__ void k_near_hist_gather(int nnear, const uint32_t* nfw, const int* nfs, const R* hs, R* hn, int q) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nnear) return;
+    if (!(nfw[o] & VALID(q))) return;
+    const int s = nfs[o];
+    for (int k = 0; k < HSTRIDE; ++k) hn[(size_t)o * HSTRIDE + k] = hs[(size_t)s * HSTRIDE + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1458,7 +1476,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                     const int cI = S.cid[I], cJ = S.cid[J];
                     const R4 svI = C.snap[2 * cI], swI = C.snap[2 * cI + 1];
                     const R4 svJ = C.snap[2 * cJ], swJ = C.snap[2 * cJ + 1];
-                    const int s = L.nfs[e];  // front slot: history and record index
+                    const int s = RECORD ? L.nfs[e] : 0;  // front slot: record index
                     const double DistL = sqrt(DistS);
                     const double Dn = pI.w + pJ.w - DistL;
                     PairIn<R> in;
@@ -1467,13 +1485,13 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                     in.armI = Ld4<R>::xyz(kI); in.armJ = Ld4<R>::xyz(kJ);
                     in.vI = Ld4<R>::xyz(svI); in.wI = Ld4<R>::xyz(swI); in.vJ = Ld4<R>::xyz(svJ);
                     in.wJ = Ld4<R>::xyz(swJ);
-                    if (w & VALID(p)) load_hist<R>(hcur + (size_t)s * HSTRIDE, in.H, in.Mt, in.Mr);
+                    if (w & VALID(p)) load_hist<R>(hcur + (size_t)e * HSTRIDE, in.H, in.Mt, in.Mr);
                     else { in.H = mk<R>(R(0), R(0), R(0)); in.Mt = in.H; in.Mr = in.H; }
                     PairOut<R> o;
                     pair_force<R, DM>(P, in, o);
                     V3<R> f, t;
                     if (side == 0) {
-                        store_hist<R>(hnew + (size_t)s * HSTRIDE, o.ht, o.htw, o.hr);
+                        store_hist<R>(hnew + (size_t)e * HSTRIDE, o.ht, o.htw, o.hr);
                         if (!(w & VALID(p ^ 1))) L.nfw[e] = w | VALID(p ^ 1);
                         f = o.cfc;
                         t = o.ctc;
@@ -2401,9 +2419,17 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum
     LAUNCH_CHECK();
     return 0;
 }
+template <class R>
 int launch_near_sync(hipStream_t st, int N, const int* nfoff, const uint32_t* nfw, const int* nfs, uint32_t* fword,
-                     int block) {
-    k_near_sync<<<nblk(N, block), block, 0, st>>>(N, nfoff, nfw, nfs, fword);
+                     const R* hn, R* hs, int q, int block) {
+    k_near_sync<R><<<nblk(N, block), block, 0, st>>>(N, nfoff, nfw, nfs, fword, hn, hs, q);
+    LAUNCH_CHECK();
+    return 0;
+}
+template <class R>
+int launch_near_hist_gather(hipStream_t st, int nnear, const uint32_t* nfw, const int* nfs, const R* hs, R* hn, int q) {
+    if (nnear <= 0) return 0;
+    k_near_hist_gather<R><<<nblk(nnear, 256), 256, 0, st>>>(nnear, nfw, nfs, hs, hn, q);
     LAUNCH_CHECK();
     return 0;
 }
@@ -2542,6 +2568,9 @@ int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
                                         int, int*, int);                                                             \
     template int launch_sphere_pose<R>(hipStream_t, const DevParams&, SphereBuf<R>, ClumpBuf<R>, int*, int, int);    \
     template int launch_gather<R>(hipStream_t, const DevParams&, int, const int*, SphereBuf<R>, SphereBuf<R>);                    \
+    template int launch_near_sync<R>(hipStream_t, int, const int*, const uint32_t*, const int*, uint32_t*,          \
+                                     const R*, R*, int, int);                                                       \
+    template int launch_near_hist_gather<R>(hipStream_t, int, const uint32_t*, const int*, const R*, R*, int);      \
     template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
                                       const uint32_t*, const R*, int);                                               \
     template int launch_clump_gather<R>(hipStream_t, int, const int*, ClumpBuf<R>, ClumpBuf<R>, const int*, int*,   \
