@@ -77,12 +77,13 @@ def run_form(bed, precision, steps, warmup, prof_steps):
     return out
 
 
-def settle(bed, steps, dt):
+def settle(bed, steps, dt, wall_type=None):
     """Let the deposited bed collapse under gravity (FP32-mixed form, larger
     step) so the benchmark runs on a contact-rich packing."""
     import nbc
 
-    opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER)
+    opt = nbc.make_options(precision=nbc.NBC_MIXED,
+                           wall_type=nbc.NBC_WALL_CYLINDER if wall_type is None else wall_type)
     saved = bed.params.StepSize
     bed.params.StepSize = dt
     ctx = nbc.Context(bed.finalized_params(opt), opt)
@@ -235,24 +236,31 @@ def run_c5(a):
 def run_c0(a):
     """The reference's small CPU-runnable case (not the headline line): 2,000
     clumps of 3 collinear spheres (6,000 spheres) in a 40 x 40 x 80 mm cube,
-    dt 1e-6 s, FP64 and FP32-mixed on the GPU next to the FP64 oracle on the
-    host cores."""
+    settled under gravity until the spheres rest on each other, then --steps
+    steps (20,000 in the configuration) at dt 1e-6 s: FP64 and FP32-mixed on
+    the GPU next to the FP64 oracle on the host cores over the same steps,
+    in its faithful (slot-padded) and compact modes."""
     import nbc
 
     bed = nbc.generate_bed(0, 2000, seed=1)
+    t0 = time.perf_counter()
+    bed = settle(bed, 100000, 5e-6, wall_type=nbc.NBC_WALL_CUBE)
+    print(f"[bench] C0 settled 100000 steps of 5e-6 s in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     out = {}
     for prec, name in ((nbc.NBC_FP64, "fp64"), (nbc.NBC_MIXED, "mixed")):
         opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CUBE)
         ctx = nbc.Context(bed.finalized_params(opt), opt)
         ctx.upload(bed, bed.wall)
         ctx.step(a.warmup)
+        s0 = ctx.stats()
         ctx.sync()
         t0 = time.perf_counter()
         ctx.step(a.steps)
         t1 = time.perf_counter()
         st = ctx.stats()
         out[name] = dict(sphere_steps_per_s=bed.n * a.steps / (t1 - t0), us_per_step=(t1 - t0) / a.steps * 1e6,
-                         rebuilds=st.rebuilds, touching_ss=st.touching_ss)
+                         rebuilds=st.rebuilds - s0.rebuilds, touching_ss=st.touching_ss,
+                         touching_wall=st.touching_wall, kinetic_energy_J=st.kinetic_energy)
         print(f"[bench] C0 {name}: {out[name]}", file=sys.stderr, flush=True)
         ctx.close()
     if not a.no_cpu:
@@ -260,21 +268,25 @@ def run_c0(a):
         import orc
 
         orc.set_threads(os.cpu_count() or 1)
-        o = orc.Oracle(orc.finalize_params(bed), nbc.make_options(wall_type=nbc.NBC_WALL_CUBE))
-        o.upload(bed, bed.wall)
-        o.step(a.warmup)
-        n = min(a.steps, 2000)
-        t0 = time.perf_counter()
-        o.step(n)
-        out["cpu_oracle_fp64"] = dict(sphere_steps_per_s=bed.n * n / (time.perf_counter() - t0), cores=orc.threads())
-        o.close()
+        for compact, name in ((False, "cpu_oracle_fp64_faithful"), (True, "cpu_oracle_fp64_compact")):
+            o = orc.Oracle(orc.finalize_params(bed), nbc.make_options(wall_type=nbc.NBC_WALL_CUBE), compact=compact)
+            o.upload(bed, bed.wall)
+            o.step(a.warmup)
+            t0 = time.perf_counter()
+            o.step(a.steps)
+            dt_cpu = time.perf_counter() - t0
+            st = o.stats()
+            out[name] = dict(sphere_steps_per_s=bed.n * a.steps / dt_cpu, seconds=dt_cpu, cores=orc.threads(),
+                             steps=a.steps, touching_ss=st.touching_ss)
+            print(f"[bench] C0 {name}: {out[name]}", file=sys.stderr, flush=True)
+            o.close()
     m = out["mixed"]
     print(json.dumps({
         "metric": "sphere-steps/s, C0 small bed (FP32-mixed)", "value": m["sphere_steps_per_s"],
         "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": m["us_per_step"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator, seed 1)",
-        "config": {"workload": "C0: 2,000 clumps of 3 collinear spheres, cube 40x40x80 mm, dt 1e-6 s",
+        "config": {"workload": "C0: 2,000 clumps of 3 collinear spheres, cube 40x40x80 mm, settled, dt 1e-6 s",
                    "n_spheres": bed.n}, "forms": out}))
 
 
@@ -282,8 +294,11 @@ def run_c3(a):
     """Penetrator case (not the headline line): a C1-style bed of 200,000
     clumps deposited and settled in a cylinder of radius 80 mm, then a
     triangulated 60-degree cone of base radius 10 mm driven down at 0.1 m/s
-    from first contact, penetrator friction 0.4, FP32-mixed; reports the
-    throughput and the FP64 reaction force on the cone."""
+    from first contact for --pene-depth (40 mm: 800,000 steps of 5e-7 s),
+    penetrator friction 0.4, FP32-mixed. Reports the throughput of the first
+    --steps steps and the resistance force versus depth: the FP64 reaction on
+    the cone recorded every step (src/dem_integrate_penetrator.cu:97-111),
+    averaged in 0.5 mm depth bins."""
     import numpy as np
 
     import nbc
@@ -297,38 +312,74 @@ def run_c3(a):
             bed = settle(bed, a.settle_steps, a.settle_dt)
         if a.bed_cache:
             bed.save(a.bed_cache)
+    chunk = 20000
     bed.params.isIntruder = 1
-    bed.params.OutputInterval = a.steps
+    bed.params.OutputInterval = chunk
+    dt = bed.params.StepSize
+    v = 0.1
     Rc = 0.010
     alpha = np.radians(30.0)
     Hc = Rc / np.tan(alpha)
     pos = bed.pos.reshape(-1, 3)
     rho = np.hypot(pos[:, 0], pos[:, 1])
     m = rho < Rc + bed.radius
-    apex = (pos[m, 2] - (rho[m] * np.cos(alpha) - bed.radius[m]) / np.sin(alpha)).max() + 20e-6
-    pene = nbc.cone_penetrator(Rc, Hc, (0.0, 0.0, apex + Hc), (0.0, 0.0, -0.1), nseg=64)
+    gap0 = 20e-6
+    apex = (pos[m, 2] - (rho[m] * np.cos(alpha) - bed.radius[m]) / np.sin(alpha)).max() + gap0
+    pene = nbc.cone_penetrator(Rc, Hc, (0.0, 0.0, apex + Hc), (0.0, 0.0, -v), nseg=64)
     opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER, pene_frcoe=0.4)
     ctx = nbc.Context(bed.finalized_params(opt), opt)
     ctx.upload(bed, bed.wall, pene)
-    ctx.step(a.warmup)
+    total = int(round((a.pene_depth + gap0) / (v * dt)))
+    fz = np.zeros(total)
+    done = 0
+    t_steps = None
+
+    def run_chunk(k):
+        nonlocal done
+        ctx.step(k)
+        cf = ctx.penetrator()["cache_f"][:, 2]
+        steps = np.arange(done, done + k)
+        fz[done:done + k] = cf[steps % chunk]
+        done += k
+
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.step(a.steps)
+    run_chunk(a.steps)  # the timed window starts at first contact
     t1 = time.perf_counter()
     st = ctx.stats()
-    fz = ctx.penetrator()["cache_f"][:, 2]  # FP64 reaction on the cone per timed step (+z resists the descent)
+    t_steps = t1 - t0
+    print(f"[bench] C3 timed {a.steps} steps: {t_steps / a.steps * 1e3:.4f} ms/step", file=sys.stderr, flush=True)
+    tc = time.perf_counter()
+    while done < total:
+        run_chunk(min(chunk - done % chunk, total - done))
+        if done % (10 * chunk) == 0 or done == total:
+            s2 = ctx.stats()
+            print(f"[bench] C3 drive: depth {(done * v * dt - gap0) * 1e3:.2f} mm, last-chunk mean Fz "
+                  f"{fz[max(done - chunk, 0):done].mean():.3f} N, rebuilds {s2.rebuilds}, near refreshes "
+                  f"{s2.near_refreshes}, {time.perf_counter() - tc:.0f} s", file=sys.stderr, flush=True)
+    drive_s = time.perf_counter() - tc
+    s3 = ctx.stats()
     ctx.close()
+    depth = np.arange(1, total + 1) * v * dt - gap0  # apex depth below first contact after each step
+    edges = np.arange(0.0, a.pene_depth + 1e-12, 0.5e-3)
+    curve = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = (depth > lo) & (depth <= hi)
+        if np.any(sel):
+            curve.append([round(hi * 1e3, 3), float(fz[sel].mean()), float(fz[sel].max())])
     res = {"metric": "sphere-steps/s, C3 cone penetrator in a settled 1M-sphere bed (FP32-mixed)",
-           "value": bed.n * a.steps / (t1 - t0), "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": (t1 - t0) / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "value": bed.n * a.steps / t_steps, "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": 0, "ms_per_step": t_steps / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator)",
            "config": {"workload": "C3: 200,000 clumps (C1 templates) settled in a cylinder R 80 mm, 60-degree cone "
                                   "R 10 mm with 128 triangles at 0.1 m/s from first contact, friction 0.4, dt 5e-7 s",
                       "n_spheres": bed.n, "triangles": int(pene.ntri)},
-           "penetrator": {"depth_m": float(0.1 * (a.warmup + a.steps) * bed.params.StepSize - 20e-6),
-                          "mean_reaction_force_z_N": float(fz.mean()), "max_reaction_force_z_N": float(fz.max()),
-                          "steps_in_contact": int(np.count_nonzero(fz)), "touching_wall_and_cone": st.touching_wall,
-                          "rebuilds": st.rebuilds, "near_refreshes": st.near_refreshes}}
+           "penetrator": {"depth_m": float(depth[-1]), "steps": total, "drive_seconds": drive_s + t_steps,
+                          "ms_per_step_whole_drive": (drive_s + t_steps) / total * 1e3,
+                          "rebuilds": s3.rebuilds, "near_refreshes": s3.near_refreshes,
+                          "contacts_at_end": s3.touching_wall,
+                          "force_vs_depth": {"columns": ["depth_mm (bin upper edge)", "mean Fz N", "max Fz N"],
+                                             "rows": curve}}}
     print(json.dumps(res))
 
 
@@ -567,6 +618,7 @@ def main():
                          "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds; c5: DEMmain "
                          "single-sphere loops")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
+    ap.add_argument("--pene-depth", type=float, default=0.04, help="c3: cone travel from first contact (m)")
     ap.add_argument("--prof-steps", type=int, default=300,
                     help="steps of the second timed region, with HIP events around every kernel")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
