@@ -404,10 +404,74 @@ def c4_beds(nbeds, seed0=1000, threads=16):
         return list(ex.map(one, range(nbeds)))
 
 
+def c4_bed_error(beds, picks, steps=10):
+    """Per-bed error of the FP32-mixed batch against the FP64 oracle of each
+    picked bed: the whole batch and the picked beds' oracles advance `steps`
+    steps together (an oracle rebuilds wherever the batch did), the last step
+    recorded; per-sphere totals, clump force and torque and positions are
+    compared in file order."""
+    import numpy as np
+
+    import nbc
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+
+    orc.set_threads(os.cpu_count() or 1)
+    opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER)
+    ctx = nbc.BatchContext([b.finalized_params(opt) for b in beds], opt)
+    ctx.upload(beds)
+    refs = {}
+    for b in picks:
+        o = orc.Oracle(orc.finalize_params(beds[b]), nbc.make_options(wall_type=nbc.NBC_WALL_CYLINDER), compact=True)
+        o.upload(beds[b], beds[b].wall)
+        refs[b] = o
+    for k in range(steps):
+        r0 = ctx.stats().rebuilds
+        if k == steps - 1:
+            ctx.set_record(True)
+        ctx.step(1)
+        rebuilt = ctx.stats().rebuilds > r0
+        for o in refs.values():
+            if rebuilt and k > 0:
+                o.request_rebuild()
+            o.step(1)
+    so, co = ctx.ranges()
+    sp = ctx.spheres()
+    fa, ta = ctx.sphere_forces(0)
+    cl = ctx.clumps()
+    out = {}
+    for b, o in refs.items():
+        s0, s1 = so[b], so[b + 1]
+        n = s1 - s0
+        loc = sp["fid"][s0:s1] - s0
+        F, T, X = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        F[loc], T[loc], X[loc] = fa[s0:s1], ta[s0:s1], sp["pos"][s0:s1]
+        sb = o.spheres()
+        fb, tb = o.sphere_forces(0)
+        Fb, Tb, Xb = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        Fb[sb["fid"]], Tb[sb["fid"]], Xb[sb["fid"]] = fb, tb, sb["pos"]
+        cb = o.clumps()
+        c0, c1 = co[b], co[b + 1]
+        rel = lambda x, y: float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300))
+        out[f"bed{b}"] = dict(n_spheres=int(n), sphere_F=rel(F, Fb), sphere_T=rel(T, Tb),
+                              clump_F=rel(cl["F"][c0:c1], cb["F"]), clump_T=rel(cl["T"][c0:c1], cb["T"]),
+                              pos_abs_m=float(np.abs(X - Xb).max()), touching_ss_oracle=int(o.stats().touching_ss))
+        o.close()
+    ctx.close()
+    return out
+
+
 def run_c4(a):
     """Batched beds (not the headline line): nbeds independent C1-style beds
-    advanced by one context (one launch set per step), FP32-mixed and FP64;
-    next to it the first 16 beds stepped as separate single-bed contexts."""
+    advanced by one context (one launch set per step), FP32-mixed and FP64.
+    The options come from the batch autotuner (nbc_autotune_batch) run on every
+    eighth bed; the full batch is also timed with its steady chunks as HIP
+    graph replays and without; the per-bed error of the mixed batch against
+    the FP64 oracle is reported for the smallest, the median and the largest
+    bed; next to it the first 16 beds stepped as separate single-bed contexts."""
+    import numpy as np
+
     import nbc
 
     t0 = time.perf_counter()
@@ -415,32 +479,55 @@ def run_c4(a):
     N = sum(b.n for b in beds)
     print(f"[bench] C4: {a.nbeds} beds, {N} spheres, {sum(b.nc for b in beds)} clumps in "
           f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    out = {}
-    for prec, name in ((nbc.NBC_MIXED, "batch_mixed"), (nbc.NBC_FP64, "batch_fp64")):
-        opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
+    base = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER)
+    t0 = time.perf_counter()
+    sample = beds[::8]
+    tuned, table = nbc.autotune_batch(sample, base, steps=100, flags=nbc.NBC_TUNE_CELL)
+    tune = dict(sample_beds=len(sample), sample_spheres=sum(b.n for b in sample), seconds=time.perf_counter() - t0,
+                best=dict(near_skin=tuned.near_skin, batch_steps=tuned.batch_steps, radix_bits=tuned.radix_bits,
+                          block_size=tuned.block_size, graphs=tuned.graphs, cell_size=tuned.cell_size),
+                candidates=[dict(near_skin=o.near_skin, batch_steps=o.batch_steps, radix_bits=o.radix_bits,
+                                 block_size=o.block_size, graphs=o.graphs, cell_size=o.cell_size, ms_per_step=ms)
+                            for o, ms in table])
+    print(f"[bench] C4 autotune: {tune['best']} in {tune['seconds']:.1f} s", file=sys.stderr, flush=True)
+
+    def timed(opt, name):
         ctx = nbc.BatchContext([b.finalized_params(opt) for b in beds], opt)
         ctx.upload(beds)
         ctx.step(a.warmup)
         s0 = ctx.stats()
-        ctx.profile(True)
         ctx.sync()
         t0 = time.perf_counter()
         ctx.step(a.steps)
         t1 = time.perf_counter()
         s1 = ctx.stats()
+        ctx.profile(True)
+        ctx.step(max(a.prof_steps // 3, 1))
+        ctx.sync()
         k3_ms, k3_n = ctx.profile_get(1)
-        out[name] = dict(sphere_steps_per_s=N * a.steps / (t1 - t0), ms_per_step=(t1 - t0) / a.steps * 1e3,
-                         k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
-                         touching_ss_per_sphere=s1.touching_ss / N)
-        print(f"[bench] C4 {name}: {out[name]}", file=sys.stderr, flush=True)
+        rb_ms, rb_n = ctx.profile_get(3)
+        r = dict(sphere_steps_per_s=N * a.steps / (t1 - t0), ms_per_step=(t1 - t0) / a.steps * 1e3,
+                 k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
+                 rebuild_ms=rb_ms / max(rb_n, 1), graph_replays=s1.graph_replays - s0.graph_replays,
+                 touching_ss_per_sphere=s1.touching_ss / N, kinetic_energy_J=s1.kinetic_energy)
+        print(f"[bench] C4 {name}: {r}", file=sys.stderr, flush=True)
         ctx.close()
+        return r
+
+    out = {}
+    out["batch_mixed_tuned"] = timed(tuned, "batch_mixed_tuned")
+    for g, name in ((1, "batch_mixed_graphs_on"), (-1, "batch_mixed_graphs_off")):
+        o = nbc.Options.from_buffer_copy(tuned)
+        o.graphs = g
+        out[name] = timed(o, name)
+    out["batch_mixed_default"] = timed(base, "batch_mixed_default")
+    out["batch_fp64"] = timed(nbc.make_options(precision=nbc.NBC_FP64, wall_type=nbc.NBC_WALL_CYLINDER), "batch_fp64")
     # the same kind of beds one context each, stepped one after another
     sub = beds[:16]
     ns = sum(b.n for b in sub)
-    opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CYLINDER)
     ctxs = []
     for b in sub:
-        c = nbc.Context(b.finalized_params(opt), opt)
+        c = nbc.Context(b.finalized_params(base), base)
         c.upload(b, b.wall)
         c.step(a.warmup)
         ctxs.append(c)
@@ -453,19 +540,25 @@ def run_c4(a):
         c.sync()
     t1 = time.perf_counter()
     out["separate_contexts_mixed_16_beds"] = dict(sphere_steps_per_s=ns * a.steps / (t1 - t0), n_spheres=ns)
-    print(f"[bench] C4 separate: {out['separate_contexts_mixed_16_beds']}", file=sys.stderr, flush=True)
     for c in ctxs:
         c.close()
-    m = out["batch_mixed"]
+    err = None
+    if not a.no_cpu:
+        sizes = np.array([b.n for b in beds])
+        order = np.argsort(sizes)
+        picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+        err = c4_bed_error(beds, picks)
+        print(f"[bench] C4 per-bed oracle error: {err}", file=sys.stderr, flush=True)
+    m = out["batch_mixed_tuned"]
     print(json.dumps({
-        "metric": "sphere-steps/s, C4 batched beds (FP32-mixed, one context)",
+        "metric": "sphere-steps/s, C4 batched beds (FP32-mixed, one context, autotuned)",
         "value": m["sphere_steps_per_s"], "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator)",
         "config": {"workload": f"C4: {a.nbeds} C1-style beds, clump counts uniform in [2000, 60000], cylinder "
                                "radius scaled with count, as deposited, dt 5e-7 s",
                    "n_spheres": N, "n_beds": a.nbeds},
-        "forms": out,
+        "autotune": tune, "forms": out, "oracle_error_per_bed": err,
     }))
 
 

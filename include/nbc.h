@@ -451,8 +451,8 @@ int nbc_profile_get(nbc_ctx* ctx, int32_t kernel_class, double* total_ms, int64_
 
 /* ---- autotuner --------------------------------------------------------- */
 /* Picks options for one bed by timing candidates on the device: coordinate
- * descent over near_skin, batch_steps, radix_bits and block_size (none of them
- * changes any result), then, when asked, the cell size (1, 1.25 or 1.5 times
+ * descent over near_skin, batch_steps, radix_bits, block_size and graphs (none
+ * of them changes any result), then, when asked, the cell size (1, 1.25 or 1.5 times
  * 3*max radius) and the hierarchical grid (off, 2 or 3 levels), which change
  * the sphere order and so the FP64 sums by rounding. Each candidate gets its
  * own context: upload, warm-up steps (the first rebuild included), then
@@ -471,6 +471,16 @@ int nbc_autotune(const nbc_params* params, const nbc_options* base, const nbc_sp
                  const nbc_clumps_in* c, const nbc_wall* wall, const nbc_penetrator* pene, int32_t device,
                  int32_t steps, int32_t flags, nbc_options* best, nbc_tune_entry* entries, int32_t cap,
                  int32_t* n_entries);
+/* The same coordinate descent for a batch context of nbeds beds (arrays of
+ * nbeds parameter blocks, not finalized, spheres, clumps and walls, as for
+ * nbc_create_batch / nbc_upload_batch): near_skin, batch_steps, radix_bits,
+ * block_size, graphs and, with NBC_TUNE_CELL, the cell size; every candidate
+ * is one batch context stepped as a whole. No counterpart in the reference,
+ * which runs one bed per process. */
+int nbc_autotune_batch(int32_t nbeds, const nbc_params* params, const nbc_options* base,
+                       const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* walls, int32_t device,
+                       int32_t steps, int32_t flags, nbc_options* best, nbc_tune_entry* entries, int32_t cap,
+                       int32_t* n_entries);
 
 /* ---- binary writers (src/dem_binary_io.h layout, native little-endian) */
 int nbc_write_particles_bin(const char* dir, int32_t step, int32_t n, const int32_t* sorted_id,

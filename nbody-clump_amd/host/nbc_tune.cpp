@@ -19,25 +19,31 @@ void set_global_error(const std::string& m);
 
 namespace {
 
+// One bed, or nbeds beds advanced by one batch context.
 struct Bed {
-    const nbc_params* params;
-    const nbc_spheres_in* s;
-    const nbc_clumps_in* c;
-    const nbc_wall* wall;
-    const nbc_penetrator* pene;
+    int nbeds;
+    const nbc_params* params;     // nbeds blocks, not finalized
+    const nbc_spheres_in* s;      // nbeds
+    const nbc_clumps_in* c;       // nbeds
+    const nbc_wall* wall;         // nbeds (or one; may be NULL)
+    const nbc_penetrator* pene;   // single bed only
     int device;
     int steps;
 };
 
 // ms per step of one candidate, < 0 when it fails
 double run_candidate(const Bed& b, const nbc_options& o) {
-    nbc_params p = *b.params;
-    if (nbc_finalize_params(&p, b.s->radius, b.s->n, &o) != 0) return -1.0;
+    std::vector<nbc_params> p(b.params, b.params + b.nbeds);
+    for (int k = 0; k < b.nbeds; ++k)
+        if (nbc_finalize_params(&p[k], b.s[k].radius, b.s[k].n, &o) != 0) return -1.0;
     nbc_ctx* ctx = nullptr;
-    if (nbc_create(&ctx, &p, &o, b.device) != 0) return -1.0;
+    const int ec = b.nbeds > 1 ? nbc_create_batch(&ctx, b.nbeds, p.data(), &o, b.device)
+                               : nbc_create(&ctx, p.data(), &o, b.device);
+    if (ec != 0) return -1.0;
     double ms = -1.0;
     const int warm = b.steps / 10 > 20 ? b.steps / 10 : 20;
-    if (nbc_upload(ctx, b.s, b.c, b.wall, b.pene) == 0 && nbc_step(ctx, warm) == 0 && nbc_synchronize(ctx) == 0) {
+    const int eu = b.nbeds > 1 ? nbc_upload_batch(ctx, b.s, b.c, b.wall) : nbc_upload(ctx, b.s, b.c, b.wall, b.pene);
+    if (eu == 0 && nbc_step(ctx, warm) == 0 && nbc_synchronize(ctx) == 0) {
         const auto t0 = std::chrono::steady_clock::now();
         if (nbc_step(ctx, b.steps) == 0 && nbc_synchronize(ctx) == 0) {
             const auto t1 = std::chrono::steady_clock::now();
@@ -48,27 +54,15 @@ double run_candidate(const Bed& b, const nbc_options& o) {
     return ms;
 }
 
-}  // namespace
-
-extern "C" int nbc_autotune(const nbc_params* params, const nbc_options* base, const nbc_spheres_in* s,
-                            const nbc_clumps_in* c, const nbc_wall* wall, const nbc_penetrator* pene, int32_t device,
-                            int32_t steps, int32_t flags, nbc_options* best, nbc_tune_entry* entries, int32_t cap,
-                            int32_t* n_entries) {
-    if (!params || !s || !c || !best) {
-        nbc::set_global_error("nbc_autotune: null argument");
-        return NBC_E_BADINPUT;
-    }
-    if (params->finalized) {
-        nbc::set_global_error("nbc_autotune: pass world.par values before nbc_finalize_params");
-        return NBC_E_BADINPUT;
-    }
+int tune(const Bed& b, const nbc_options* base, int32_t flags, nbc_options* best, nbc_tune_entry* entries,
+         int32_t cap, int32_t* n_entries) {
     int32_t ndev = 0;
     if (nbc_device_count(&ndev) != 0 || ndev <= 0) return NBC_E_NODEVICE;
-    Bed b{params, s, c, wall, pene, device, steps > 0 ? steps : 300};
     nbc_options cur;
     if (base) cur = *base; else std::memset(&cur, 0, sizeof(cur));
     double rmax = 0.0;
-    for (int i = 0; i < s->n; ++i) rmax = std::max(rmax, s->radius[i]);
+    for (int k = 0; k < b.nbeds; ++k)
+        for (int i = 0; i < b.s[k].n; ++i) rmax = std::max(rmax, b.s[k].radius[i]);
     int count = 0;
     double best_ms = -1.0;
     auto record = [&](const nbc_options& o, double ms) {
@@ -92,10 +86,11 @@ extern "C" int nbc_autotune(const nbc_params* params, const nbc_options* base, c
     sweep([](nbc_options& o, double v) { o.batch_steps = (int32_t)v; }, {2, 4, 8, 16});
     sweep([](nbc_options& o, double v) { o.radix_bits = (int32_t)v; }, {4, 6, 8});
     sweep([](nbc_options& o, double v) { o.block_size = (int32_t)v; }, {128, 256});
+    sweep([](nbc_options& o, double v) { o.graphs = (int32_t)v; }, {-1, 1});
     if (flags & NBC_TUNE_CELL)
         sweep([rmax](nbc_options& o, double v) { o.cell_size = v * 3.0 * rmax; o.hgrid_levels = 0; },
               {1.0, 1.25, 1.5});
-    if (flags & NBC_TUNE_HGRID)
+    if ((flags & NBC_TUNE_HGRID) && b.nbeds == 1)
         sweep([](nbc_options& o, double v) { o.hgrid_levels = (int32_t)v; if (v > 1) o.cell_size = 0.0; },
               {0, 2, 3});
     if (n_entries) *n_entries = count;
@@ -105,4 +100,41 @@ extern "C" int nbc_autotune(const nbc_params* params, const nbc_options* base, c
     }
     *best = cur;
     return 0;
+}
+
+int check_raw(const nbc_params* p, int n, const char* who) {
+    for (int k = 0; k < n; ++k)
+        if (p[k].finalized) {
+            nbc::set_global_error(std::string(who) + ": pass world.par values before nbc_finalize_params");
+            return NBC_E_BADINPUT;
+        }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int nbc_autotune(const nbc_params* params, const nbc_options* base, const nbc_spheres_in* s,
+                            const nbc_clumps_in* c, const nbc_wall* wall, const nbc_penetrator* pene, int32_t device,
+                            int32_t steps, int32_t flags, nbc_options* best, nbc_tune_entry* entries, int32_t cap,
+                            int32_t* n_entries) {
+    if (!params || !s || !c || !best) {
+        nbc::set_global_error("nbc_autotune: null argument");
+        return NBC_E_BADINPUT;
+    }
+    if (int e = check_raw(params, 1, "nbc_autotune")) return e;
+    const Bed b{1, params, s, c, wall, pene, device, steps > 0 ? steps : 300};
+    return tune(b, base, flags, best, entries, cap, n_entries);
+}
+
+extern "C" int nbc_autotune_batch(int32_t nbeds, const nbc_params* params, const nbc_options* base,
+                                  const nbc_spheres_in* s, const nbc_clumps_in* c, const nbc_wall* walls,
+                                  int32_t device, int32_t steps, int32_t flags, nbc_options* best,
+                                  nbc_tune_entry* entries, int32_t cap, int32_t* n_entries) {
+    if (nbeds < 1 || !params || !s || !c || !best) {
+        nbc::set_global_error("nbc_autotune_batch: null argument or no beds");
+        return NBC_E_BADINPUT;
+    }
+    if (int e = check_raw(params, nbeds, "nbc_autotune_batch")) return e;
+    const Bed b{nbeds, params, s, c, walls, nullptr, device, steps > 0 ? steps : 300};
+    return tune(b, base, flags & ~NBC_TUNE_HGRID, best, entries, cap, n_entries);
 }

@@ -185,6 +185,9 @@ _SIGS = {
     "nbc_write_contacts_bin": ([C.c_char_p, I32, I32, I32, P, P, P], C.c_int),
     "nbc_run_simulation_clump": ([C.c_char_p, I32], C.c_int),
     "nbc_run_simulation": ([C.c_char_p, I32, I32], C.c_int),
+    "nbc_autotune_batch": ([I32, C.POINTER(Params), C.POINTER(Options), C.POINTER(SpheresIn), C.POINTER(ClumpsIn),
+                            C.POINTER(Wall), I32, I32, I32, C.POINTER(Options), C.POINTER(TuneEntry), I32,
+                            C.POINTER(I32)], C.c_int),
     "nbc_autotune": ([C.POINTER(Params), C.POINTER(Options), C.POINTER(SpheresIn), C.POINTER(ClumpsIn),
                       C.POINTER(Wall), C.POINTER(Penetrator), I32, I32, I32, C.POINTER(Options),
                       C.POINTER(TuneEntry), I32, C.POINTER(I32)], C.c_int),
@@ -624,6 +627,26 @@ def autotune(bed, base=None, wall=None, pene=None, steps=300, flags=0, device=0)
                              C.byref(wall) if wall else None, C.byref(pene) if pene else None, device, steps, flags,
                              C.byref(best), ent, cap, C.byref(n)))
     return best, [(ent[k].opt, ent[k].ms_per_step) for k in range(min(n.value, cap))]
+
+
+def autotune_batch(beds, base=None, steps=100, flags=0, device=0):
+    """Fastest options for a batch of beds (nbc_autotune_batch): returns
+    (options, [(options, ms_per_step), ...]). The beds' parameters must not be
+    finalized."""
+    n = len(beds)
+    p_arr, s_arr, c_arr, w_arr = (Params * n)(), (SpheresIn * n)(), (ClumpsIn * n)(), (Wall * n)()
+    for b, bed in enumerate(beds):
+        C.memmove(C.byref(p_arr[b]), C.byref(bed.params), C.sizeof(Params))
+        s_arr[b], c_arr[b] = bed.spheres_in(), bed.clumps_in()
+        if bed.wall is not None:
+            C.memmove(C.byref(w_arr[b]), C.byref(bed.wall), C.sizeof(Wall))
+    best = Options()
+    cap = 64
+    ent = (TuneEntry * cap)()
+    cnt = I32()
+    check(lib().nbc_autotune_batch(n, p_arr, C.byref(base) if base else None, s_arr, c_arr, w_arr, device, steps,
+                                   flags, C.byref(best), ent, cap, C.byref(cnt)))
+    return best, [(ent[k].opt, ent[k].ms_per_step) for k in range(min(cnt.value, cap))]
 
 
 def download_bed(ctx, template):

@@ -1,7 +1,7 @@
 """Autotuner on the GPU: it measures every candidate, and the options it picks
 without NBC_TUNE_CELL / NBC_TUNE_HGRID leave every FP64 result bit-identical
-to the default options (near skin, chunk size, radix bits and block size
-change no arithmetic)."""
+to the default options (near skin, chunk size, radix bits, block size and
+graph replays change no arithmetic); the batch autotuner likewise."""
 import numpy as np
 import pytest
 
@@ -18,7 +18,7 @@ def test_autotune_picks_result_preserving_options():
     bed.params.StepSize = 5e-6
     base = nbc.make_options(precision=nbc.NBC_FP64, wall_type=nbc.NBC_WALL_CYLINDER)
     best, entries = nbc.autotune(bed, base, wall=bed.wall, steps=60)
-    assert len(entries) == 4 + 4 + 3 + 2
+    assert len(entries) == 4 + 4 + 3 + 2 + 2
     assert all(ms > 0 for _, ms in entries)
     # a sweep keeps the incoming value (0: default) unless a candidate beats it
     assert best.near_skin in (0.125, 0.25, 0.5, 1.0) and best.batch_steps in (0, 2, 4, 8, 16)
@@ -41,7 +41,7 @@ def test_autotune_cell_and_hgrid_sweeps():
     bed = nbc.generate_bed(2, 800, seed=6, container=(0.02, 0, 0, 0))
     base = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CUBE)
     best, entries = nbc.autotune(bed, base, wall=bed.wall, steps=40, flags=nbc.NBC_TUNE_CELL | nbc.NBC_TUNE_HGRID)
-    assert len(entries) == 4 + 4 + 3 + 2 + 3 + 3
+    assert len(entries) == 4 + 4 + 3 + 2 + 2 + 3 + 3
     assert all(ms > 0 for _, ms in entries)
     assert best.hgrid_levels in (0, 2, 3)
     # the picked options run
@@ -50,3 +50,29 @@ def test_autotune_cell_and_hgrid_sweeps():
     ctx.step(20)
     assert ctx.stats().touching_ss > 0
     ctx.close()
+
+
+def test_autotune_batch_picks_result_preserving_options():
+    print("[case] batch autotune on three C1-style beds", flush=True)
+    beds = [nbc.generate_bed(1, nc, seed=sd, container=(R, 0, 0, 0))
+            for nc, sd, R in ((300, 31, 0.012), (200, 32, 0.01), (400, 33, 0.015))]
+    rng = np.random.default_rng(3)
+    for b in beds:
+        b.cvel[:] = rng.uniform(-0.3, 0.3, b.cvel.shape)
+        b.params.StepSize = 5e-6
+    base = nbc.make_options(precision=nbc.NBC_FP64, wall_type=nbc.NBC_WALL_CYLINDER)
+    best, entries = nbc.autotune_batch(beds, base, steps=40)
+    assert len(entries) == 4 + 4 + 3 + 2 + 2
+    assert all(ms > 0 for _, ms in entries)
+    assert best.cell_size == 0.0 and best.hgrid_levels == 0
+    out = []
+    for opt in (base, best):
+        ctx = nbc.BatchContext([b.finalized_params(opt) for b in beds], opt)
+        ctx.upload(beds)
+        ctx.step(150)
+        st = ctx.stats()
+        out.append((ctx.spheres()["pos"], ctx.clumps()["vel"], st.rebuilds, st.touching_ss))
+        ctx.close()
+    (pa, va, ra, ta), (pb, vb, rb, tb) = out
+    assert ra == rb and ta == tb and ra >= 2
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
