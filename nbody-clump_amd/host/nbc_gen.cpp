@@ -574,7 +574,9 @@ extern "C" int nbc_save_bed_text(const char* dir, const nbc_bed_out* b, const nb
         f << std::setprecision(17);
         if (pene) {
             if (pene->shape == NBC_PENE_SPHERE) f << "sphere " << pene->radius << "\n";
-            else if (pene->shape == NBC_PENE_CONE) f << "cone " << pene->radius << " " << pene->height << "\n";
+            else if (pene->shape == NBC_PENE_CONE)  // the mesh holds two triangles per rim segment
+                f << "cone " << pene->radius << " " << pene->height << " " << (pene->ntri > 0 ? pene->ntri / 2 : 64)
+                  << "\n";
             else f << "cylinder " << pene->radius << " " << pene->height << "\n" << pene->dir[0] << " " << pene->dir[1]
                    << " " << pene->dir[2] << "\n";
             f << pene->prop[0] << " " << pene->prop[1] << " " << pene->prop[2] << " " << pene->prop[3] << "\n"

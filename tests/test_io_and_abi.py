@@ -325,3 +325,38 @@ def test_demmain_driver_argument_checks():
     h = C.c_void_p()
     assert nbc.lib().nbc_create(C.byref(h), C.byref(part.finalized_params(opt)), C.byref(opt), 0) == \
         nbc.NBC_E_BADINPUT
+
+
+def test_generator_cli_writes_a_run_directory(tmp_path):
+    """nbc_generate (the command-line generator, the role of
+    球簇生成器/generate_dem_bed.m) writes the same bed as nbc_generate_bed for
+    the same kind, count and seed, a cone penetrator above the bed, and
+    refuses to overwrite ph/ch files."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(nbc.LIB_PATH), "nbc_generate")
+    d = tmp_path / "run"
+    r = subprocess.run([exe, "--kind", "1", "--clumps", "150", "--seed", "9", "--cone", "0.004", "0.0069", "16",
+                        "0.1", "--out", str(d)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    ref = nbc.generate_bed(1, 150, seed=9)
+    p = nbc.Params()
+    ini = C.create_string_buffer(256)
+    assert nbc.lib().nbc_load_world_par(str(d / "world.par").encode(), C.byref(p), ini, 256) == 0
+    assert (p.ParticleNum, p.ClumpNum, p.isIntruder) == (ref.n, ref.nc, 1)
+    n = p.ParticleNum
+    cid = np.zeros(n, np.int32)
+    pos, loc, m, rad = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    assert nbc.lib().nbc_load_phase(str(d / "ph.00000000.bt").encode(), n,
+                                    *[nbc.ptr(a) for a in (cid, pos, loc, m, rad)]) == 0
+    assert np.array_equal(cid, ref.clump_id) and np.array_equal(pos, ref.pos) and np.array_equal(rad, ref.radius)
+    pe = nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(str(d / "Rocks" / "Penetrator.bt").encode(), C.byref(pe)) == 0
+    assert pe.shape == nbc.NBC_PENE_CONE and pe.vel[2] == -0.1 and pe.ntri == 2 * 16
+    # the cone's apex (z = -H in its frame) sits 20 um above the highest sphere under it
+    rho = np.hypot(ref.pos[:, 0] - ref.wall.pos[0], ref.pos[:, 1] - ref.wall.pos[1])
+    top = (ref.pos[rho < 0.004 + ref.radius, 2] + ref.radius[rho < 0.004 + ref.radius]).max()
+    assert pe.pos[2] - pe.height == pytest.approx(top + 20e-6, abs=1e-12)
+    nbc.lib().nbc_free_mesh(C.byref(pe))
+    r2 = subprocess.run([exe, "--kind", "1", "--clumps", "150", "--out", str(d)], capture_output=True, text=True)
+    assert r2.returncode == 1 and "refusing to overwrite" in r2.stderr
