@@ -542,6 +542,7 @@ def run_c4(a):
     out["separate_contexts_mixed_16_beds"] = dict(sphere_steps_per_s=ns * a.steps / (t1 - t0), n_spheres=ns)
     for c in ctxs:
         c.close()
+    settling = c4_settling(beds, tuned, a.c4_settle_steps) if a.c4_settle_steps > 0 else None
     err = None
     if not a.no_cpu:
         sizes = np.array([b.n for b in beds])
@@ -558,8 +559,62 @@ def run_c4(a):
         "config": {"workload": f"C4: {a.nbeds} C1-style beds, clump counts uniform in [2000, 60000], cylinder "
                                "radius scaled with count, as deposited, dt 5e-7 s",
                    "n_spheres": N, "n_beds": a.nbeds},
-        "autotune": tune, "forms": out, "oracle_error_per_bed": err,
+        "autotune": tune, "forms": out, "settling_rebuilds": settling, "oracle_error_per_bed": err,
     }))
+
+
+def c4_settling(beds, opt, steps, dt=5e-6, sample_every=8):
+    """Cost of rebuilding every bed whenever one bed needs it, on beds that
+    are still settling: the batch is stepped `steps` times at dt (ten times the
+    timed step, so the beds fall and rebuild often) with the rebuilds timed
+    (profile class 3), and every `sample_every`-th bed is stepped alone over
+    the same window to count the rebuilds it needs by itself. The saving
+    bound of per-bed rebuilds is the share of the union's rebuild work (sphere
+    count x rebuilds) that the beds' own rebuilds would not need, estimated
+    on the sample."""
+    import nbc
+
+    old = [b.params.StepSize for b in beds]
+    for b in beds:
+        b.params.StepSize = dt
+    try:
+        N = sum(b.n for b in beds)
+        ctx = nbc.BatchContext([b.finalized_params(opt) for b in beds], opt)
+        ctx.upload(beds)
+        ctx.step(1)
+        ctx.profile(True)
+        s0 = ctx.stats()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+        s1 = ctx.stats()
+        rb_ms, rb_n = ctx.profile_get(3)
+        ctx.close()
+        union = s1.rebuilds - s0.rebuilds
+        own, work_own, work_union = [], 0.0, 0.0
+        for b in beds[::sample_every]:
+            c = nbc.Context(b.finalized_params(opt), opt)
+            c.upload(b, b.wall)
+            c.step(1)
+            r0 = c.stats().rebuilds
+            c.step(steps)
+            own.append(c.stats().rebuilds - r0)
+            c.close()
+            work_own += b.n * own[-1]
+            work_union += b.n * union
+        res = dict(dt=dt, steps=steps, ms_per_step=(t1 - t0) / steps * 1e3, union_rebuilds=union,
+                   rebuild_ms_each=rb_ms / max(rb_n, 1), rebuild_share_of_time=rb_ms / ((t1 - t0) * 1e3),
+                   sampled_beds=len(own), own_rebuilds_mean=sum(own) / len(own), own_rebuilds_max=max(own),
+                   per_bed_rebuild_work_fraction=work_own / max(work_union, 1.0),
+                   per_bed_saving_bound_ms_per_step=rb_ms * (1 - work_own / max(work_union, 1.0)) / steps,
+                   n_spheres=N)
+        print(f"[bench] C4 settling rebuilds: {res}", file=sys.stderr, flush=True)
+        return res
+    finally:
+        for b, s in zip(beds, old):
+            b.params.StepSize = s
 
 
 def profiled_traffic():
@@ -711,6 +766,8 @@ def main():
                          "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds; c5: DEMmain "
                          "single-sphere loops")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
+    ap.add_argument("--c4-settle-steps", type=int, default=200,
+                    help="c4: steps of the settling window that measures the union-rebuild cost (0: skip)")
     ap.add_argument("--pene-depth", type=float, default=0.04, help="c3: cone travel from first contact (m)")
     ap.add_argument("--prof-steps", type=int, default=300,
                     help="steps of the second timed region, with HIP events around every kernel")
