@@ -563,15 +563,16 @@ def run_c4(a):
     }))
 
 
-def c4_settling(beds, opt, steps, dt=5e-6, sample_every=8):
+def c4_settling(beds, opt, steps, pre=10000, dt=5e-6, sample_every=8):
     """Cost of rebuilding every bed whenever one bed needs it, on beds that
-    are still settling: the batch is stepped `steps` times at dt (ten times the
-    timed step, so the beds fall and rebuild often) with the rebuilds timed
-    (profile class 3), and every `sample_every`-th bed is stepped alone over
-    the same window to count the rebuilds it needs by itself. The saving
-    bound of per-bed rebuilds is the share of the union's rebuild work (sphere
-    count x rebuilds) that the beds' own rebuilds would not need, estimated
-    on the sample."""
+    are settling: the deposited batch falls for `pre` steps of dt (ten times
+    the timed step) until its beds move fast enough to rebuild often, then
+    `steps` further steps are timed with the rebuilds timed beside them
+    (profile class 3). Every `sample_every`-th bed is stepped alone through
+    the same pre-fall and window to count the rebuilds it needs by itself.
+    The saving bound of per-bed rebuilds is the share of the union's rebuild
+    work (sphere count x rebuilds) that the beds' own rebuilds would not
+    need, estimated on the sample."""
     import nbc
 
     old = [b.params.StepSize for b in beds]
@@ -581,9 +582,14 @@ def c4_settling(beds, opt, steps, dt=5e-6, sample_every=8):
         N = sum(b.n for b in beds)
         ctx = nbc.BatchContext([b.finalized_params(opt) for b in beds], opt)
         ctx.upload(beds)
-        ctx.step(1)
+        chunk = 2000
+        for done in range(0, pre, chunk):
+            ctx.step(min(chunk, pre - done))
+            s = ctx.stats()
+            print(f"[bench] C4 settling: step {s.step}, {s.rebuilds} union rebuilds", file=sys.stderr, flush=True)
         ctx.profile(True)
         s0 = ctx.stats()
+        r_ms0, r_n0 = ctx.profile_get(3)
         ctx.sync()
         t0 = time.perf_counter()
         ctx.step(steps)
@@ -591,24 +597,25 @@ def c4_settling(beds, opt, steps, dt=5e-6, sample_every=8):
         t1 = time.perf_counter()
         s1 = ctx.stats()
         rb_ms, rb_n = ctx.profile_get(3)
+        rb_ms, rb_n = rb_ms - r_ms0, rb_n - r_n0
         ctx.close()
         union = s1.rebuilds - s0.rebuilds
         own, work_own, work_union = [], 0.0, 0.0
         for b in beds[::sample_every]:
             c = nbc.Context(b.finalized_params(opt), opt)
             c.upload(b, b.wall)
-            c.step(1)
+            c.step(pre)
             r0 = c.stats().rebuilds
             c.step(steps)
             own.append(c.stats().rebuilds - r0)
             c.close()
             work_own += b.n * own[-1]
             work_union += b.n * union
-        res = dict(dt=dt, steps=steps, ms_per_step=(t1 - t0) / steps * 1e3, union_rebuilds=union,
+        frac = work_own / max(work_union, 1.0)
+        res = dict(dt=dt, pre_steps=pre, steps=steps, ms_per_step=(t1 - t0) / steps * 1e3, union_rebuilds=union,
                    rebuild_ms_each=rb_ms / max(rb_n, 1), rebuild_share_of_time=rb_ms / ((t1 - t0) * 1e3),
                    sampled_beds=len(own), own_rebuilds_mean=sum(own) / len(own), own_rebuilds_max=max(own),
-                   per_bed_rebuild_work_fraction=work_own / max(work_union, 1.0),
-                   per_bed_saving_bound_ms_per_step=rb_ms * (1 - work_own / max(work_union, 1.0)) / steps,
+                   per_bed_rebuild_work_fraction=frac, per_bed_saving_bound_ms_per_step=rb_ms * (1 - frac) / steps,
                    n_spheres=N)
         print(f"[bench] C4 settling rebuilds: {res}", file=sys.stderr, flush=True)
         return res
@@ -766,7 +773,7 @@ def main():
                          "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds; c5: DEMmain "
                          "single-sphere loops")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
-    ap.add_argument("--c4-settle-steps", type=int, default=200,
+    ap.add_argument("--c4-settle-steps", type=int, default=1000,
                     help="c4: steps of the settling window that measures the union-rebuild cost (0: skip)")
     ap.add_argument("--pene-depth", type=float, default=0.04, help="c3: cone travel from first contact (m)")
     ap.add_argument("--prof-steps", type=int, default=300,
