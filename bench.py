@@ -119,8 +119,10 @@ def run_c2(a):
         if a.bed_cache:
             bed.save(a.bed_cache)
     out = {}
-    for levels in (3, 0):
-        opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CUBE, hgrid_levels=levels)
+    grids = (("hgrid3_nested", 3, 1), ("hgrid3", 3, 0), ("uniform", 0, 0))
+    for key, levels, nested in [g for g in grids if g[0] in a.c2_grids.split(",")]:
+        opt = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CUBE, hgrid_levels=levels,
+                               hgrid_nested=nested)
         ctx = nbc.Context(bed.finalized_params(opt), opt)
         ctx.upload(bed, bed.wall)
         ctx.step(a.warmup)
@@ -132,26 +134,28 @@ def run_c2(a):
         t1 = time.perf_counter()
         s1 = ctx.stats()
         k3_ms, k3_n = ctx.profile_get(1)
+        k2_ms, k2_n = ctx.profile_get(0)
+        k4_ms, k4_n = ctx.profile_get(2)
         r_ms0, r_n0 = ctx.profile_get(3)
         for _ in range(3):
             ctx.rebuild()
         ctx.sync()
         r_ms, r_n = ctx.profile_get(3)
-        key = f"hgrid{levels}" if levels else "uniform"
         out[key] = dict(sphere_steps_per_s=bed.n * a.steps / (t1 - t0), ms_per_step=(t1 - t0) / a.steps * 1e3,
-                        k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
+                        k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, k_sphere_pose_us=k2_ms / max(k2_n, 1) * 1e3,
+                        k_clump_after_us=k4_ms / max(k4_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
                         rebuild_ms=(r_ms - r_ms0) / max(r_n - r_n0, 1), touching_ss_per_sphere=s1.touching_ss / bed.n,
                         list_pairs_per_sphere=s1.front_pairs / bed.n)
         print(f"[bench] C2 {key}: {out[key]}", file=sys.stderr, flush=True)
         ctx.close()
-    h = out["hgrid3"]
+    h = out.get("hgrid3_nested") or next(iter(out.values()))
     print(json.dumps({
         "metric": "sphere-steps/s, C2 polydisperse bed on the hierarchical grid (FP32-mixed)",
         "value": h["sphere_steps_per_s"], "unit": "sphere-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 forces / f64 integration", "data": "synthetic (deterministic generator, seed 1)",
         "config": {"workload": "C2 bed: 571,429 clumps of 1-6 spheres, radii log-uniform 0.2-2.0 mm, cube section "
-                               "150 mm, mu 0.6, dt 2e-7 s, 3-level hierarchical grid, as deposited",
+                               "150 mm, mu 0.6, dt 2e-7 s, 3-level nested hierarchical grid, as deposited",
                    "n_spheres": bed.n, "n_clumps": bed.nc},
         "forms": out,
     }))
@@ -773,6 +777,8 @@ def main():
                          "hierarchical grid; c3: cone penetrator in a settled bed; c4: batched beds; c5: DEMmain "
                          "single-sphere loops")
     ap.add_argument("--nbeds", type=int, default=256, help="beds of the c4 workload")
+    ap.add_argument("--c2-grids", default="hgrid3_nested,hgrid3,uniform",
+                    help="c2: grids to time (hgrid3_nested, hgrid3, uniform)")
     ap.add_argument("--c4-settle-steps", type=int, default=1000,
                     help="c4: steps of the settling window that measures the union-rebuild cost (0: skip)")
     ap.add_argument("--pene-depth", type=float, default=0.04, help="c3: cone travel from first contact (m)")

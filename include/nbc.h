@@ -135,6 +135,18 @@ typedef struct nbc_params {
     int32_t LevelDim[NBC_MAX_LEVELS][3];
     double LevelCell[NBC_MAX_LEVELS];
     double LevelRmin[NBC_MAX_LEVELS];
+    /* Nested levels (nbc_options.hgrid_nested; no counterpart in the
+       reference): LevelCell[t] = LevelCell[0] / 2^t with LevelCell[0] =
+       3 x max radius, a sphere on the finest level whose cell is at least
+       3 r (LevelRmin[t] = LevelCell[t+1] / 3), LevelDim[t] = GridDim * 2^t.
+       Each coarse (level-0) cell owns a block of LevelBlock = sum 8^t keys:
+       key = coarse * LevelBlock + LevelOff[t] + (sz * 2^t + sy) * 2^t + sx,
+       with coarse = (Z * ny0 + Y) * nx0 + X, (X, Y, Z) = the level-t cell
+       coordinates shifted right by t and (sx, sy, sz) their low t bits. The
+       spheres of a coarse cell are contiguous in sorted order; a pair on two
+       levels belongs to the front row of the coarser sphere. */
+    int32_t LevelNested;
+    int32_t LevelBlock;
 } nbc_params;
 
 /* Options that are not world.par keys (read from the optional nbc.par file by
@@ -157,7 +169,10 @@ typedef struct nbc_options {
     int32_t level_from_type; /* HGRID driver: a particle's grid level is its type
                                 (src/dem_hgrid.cu:213), with the level table of
                                 nbc_load_grid_levels; 0: levels by radius      */
-    double reserved[2];
+    int32_t hgrid_nested;    /* with hgrid_levels 2..4: nested power-of-two levels
+                                numbered coarse-cell-major (see nbc_params)    */
+    int32_t reserved_i;
+    double reserved[1];
 } nbc_options;
 
 /* ---- boundaries -------------------------------------------------------- */

@@ -110,13 +110,16 @@ struct DevParams {
     uint32_t G[3];
     double origin[3], cell[3], icell[3], boxlo[3], boxup[3];
     // hierarchical grid (L > 1): level 0 holds the largest spheres; cells are
-    // numbered level-major from loff[t]; the last cell is the out-of-box bucket
+    // numbered level-major from loff[t] (or nested, below); the last cell is
+    // the out-of-box bucket
     int L;
     int loff[MAX_LEVELS];
     int ldim[MAX_LEVELS][3];
     double lcell[MAX_LEVELS];
     double lrmin[MAX_LEVELS];  // a sphere's level is the first t with r > lrmin[t]
     const int* clevel;         // or, when set, clevel[clump index] (DEMmain hgrid loop: the particle's type)
+    int nested;                // 1: nested power-of-two levels numbered coarse-cell-major,
+    int lblock;                //    lblock keys per coarse cell (loff[t] is the offset inside it)
     double dt, Ymod, Prat, Frcoe, Beta, Cohesion, Gravity, Damping, Thresh;
     double pene_frcoe;
     double cKn, cKs, idt; // 2E/(3(1-nu^2)), 2E/(1+nu)/(2-nu), 1/dt (FP32 force path)

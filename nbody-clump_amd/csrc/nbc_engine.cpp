@@ -87,7 +87,22 @@ class Engine : public EngineBase {
                                         "(call nbc_finalize_params first)");
         if ((long long)N > (long long)JMASK)
             return fail(NBC_E_BADINPUT, "nbc_create: too many spheres for the 29-bit pair index");
-        if (P.Levels > 1) {
+        if (P.Levels > 1 && P.LevelNested) {
+            // nested levels: each coarse cell owns a block of sum 8^t keys
+            bool ok = P.Levels <= 4;
+            long long blk = 0;
+            for (int t = 0; ok && t < P.Levels; ++t) {
+                ok = P.LevelOff[t] == blk && P.LevelCell[t] > 0.0;
+                for (int a = 0; a < 3; ++a) ok = ok && P.LevelDim[t][a] == (int)(P.GridDim[a] << t);
+                blk += 1ll << (3 * t);
+            }
+            const long long coarse = (long long)P.GridDim[0] * P.GridDim[1] * P.GridDim[2];
+            if (!ok || blk != P.LevelBlock || coarse * blk + 1 != P.CellNum)
+                return fail(NBC_E_BADINPUT, "nbc_create: inconsistent nested hierarchical grid table (call "
+                                            "nbc_finalize_params with hgrid_levels and hgrid_nested)");
+            if (O.level_from_type)
+                return fail(NBC_E_BADINPUT, "nbc_create: grid.bt level tables are not nested");
+        } else if (P.Levels > 1) {
             // the kernels index cells through this table: it must tile [0, CellNum - 1)
             bool ok = P.Levels <= MAX_LEVELS;
             long long off = 0;
@@ -148,6 +163,8 @@ class Engine : public EngineBase {
             dp.lcell[t] = on ? P.LevelCell[t] : 0.0;
             dp.lrmin[t] = on ? P.LevelRmin[t] : 0.0;
         }
+        dp.nested = dp.L > 1 && P.LevelNested ? 1 : 0;
+        dp.lblock = dp.nested ? P.LevelBlock : 0;
         dp.dt = P.StepSize; dp.Ymod = P.Ymod; dp.Prat = P.Prat; dp.Frcoe = P.Frcoe; dp.Beta = P.Beta;
         dp.Cohesion = P.Cohesion; dp.Gravity = P.Gravity; dp.Damping = P.Damping; dp.Thresh = P.Thresh;
         dp.pene_frcoe = O.pene_frcoe > 0.0 ? O.pene_frcoe : P.Frcoe;
@@ -821,8 +838,8 @@ class Engine : public EngineBase {
         if (driver == NBC_DRIVER_CYLINDER || driver == NBC_DRIVER_SPLIT)
             KC(launch_hist_carry<R>(st, tot[0], nfront, p, Ln.fword, Ln.hist[p], Lo.fword, Lo.hist[p], block));
         else
-            KC(launch_hist_remap<R>(st, N, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword, Lo.hist[p],
-                                    block));
+            KC(launch_hist_remap<R>(st, dp, s.pos, s.cid, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword,
+                                    Lo.hist[p], block));
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));

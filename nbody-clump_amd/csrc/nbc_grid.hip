@@ -95,6 +95,9 @@ int check_args(const char* who, const nbc_params* p, const double* pos4, long lo
     }
     dp->L = p->Levels > 1 ? p->Levels : 1;
     if (dp->L > MAX_LEVELS) return fail(NBC_E_BADINPUT, w + ": too many grid levels");
+    dp->nested = dp->L > 1 && p->LevelNested ? 1 : 0;
+    dp->lblock = dp->nested ? p->LevelBlock : 0;
+    if (dp->nested && (dp->L > 4 || p->LevelBlock < 1)) return fail(NBC_E_BADINPUT, w + ": bad nested grid table");
     for (int t = 0; t < dp->L && dp->L > 1; ++t) {
         dp->loff[t] = p->LevelOff[t];
         for (int a = 0; a < 3; ++a) dp->ldim[t][a] = p->LevelDim[t][a];

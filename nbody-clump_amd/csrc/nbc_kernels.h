@@ -114,8 +114,9 @@ int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
 template <class R>
 int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst);
 template <class R>
-int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
-                      const int* ofoff, const uint32_t* ofword, const R* ohist, int block);
+int launch_hist_remap(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
+                      const int* nfoff, uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword,
+                      const R* ohist, int block);
 // Clump renumbering at a rebuild: first[c] = first sphere of clump c (first
 // filled with N before), then the clump arrays move to the order of first
 // (order[new] = old) and every sphere's clump index follows.

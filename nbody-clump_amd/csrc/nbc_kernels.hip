@@ -349,6 +349,18 @@ __device__ __forceinline__ int level_of(const DevParams& P, double r) {
 // Cell coordinate of x on a level of cell size s (src/dem_hgrid.cu:225-227).
 __device__ __forceinline__ int hcell(double x, double o, double s) { return (int)floor((x - o) / s); }
 
+// Key of level-t cell (cx, cy, cz), inside the level's box. Level-major
+// (src/dem_hgrid.cu:228): loff[t] + (cz ny + cy) nx + cx. Nested: the coarse
+// cell (the coordinates shifted right by t) picks a block of lblock keys, the
+// level picks loff[t] inside it and the low t bits of each coordinate the
+// sub-cell, so the spheres of one coarse cell are contiguous after the sort.
+__device__ __forceinline__ int hkey(const DevParams& P, int t, int cx, int cy, int cz) {
+    if (!P.nested) return P.loff[t] + (cz * P.ldim[t][1] + cy) * P.ldim[t][0] + cx;
+    const int m = (1 << t) - 1;
+    const int coarse = ((cz >> t) * P.ldim[0][1] + (cy >> t)) * P.ldim[0][0] + (cx >> t);
+    return coarse * P.lblock + P.loff[t] + ((((cz & m) << t) | (cy & m)) << t | (cx & m));
+}
+
 __global__ void k_hash(DevParams P, const D4* pos, const int* cid, uint32_t* key, int* val) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
@@ -370,7 +382,7 @@ __global__ void k_hash(DevParams P, const D4* pos, const int* cid, uint32_t* key
             cx = cx < 0 ? 0 : (cx > nd[0] - 1 ? nd[0] - 1 : cx);
             cy = cy < 0 ? 0 : (cy > nd[1] - 1 ? nd[1] - 1 : cy);
             cz = cz < 0 ? 0 : (cz > nd[2] - 1 ? nd[2] - 1 : cz);
-            idx = P.loff[t] + (cz * nd[1] + cy) * nd[0] + cx;
+            idx = hkey(P, t, cx, cy, cz);
         }
         key[i] = (uint32_t)idx;
         return;
@@ -682,9 +694,10 @@ __global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, co
 // Hierarchical search, src/dem_hgrid_contlist.cu:57-161 with deterministic
 // rows. A sphere of level t scans levels t..0 (27 cells each, z outer / x
 // inner, ascending j, cut-off 1.5 (ri + rj)). Same level: j > i front, j < i
-// back. Coarser level: j < i (cells are numbered level-major, so coarser
-// spheres sort first); i lists j in its back row and emits the cross entry
-// (j, i), which becomes a front entry of j. PH 0 counts (same-level fronts,
+// back. Coarser level: the coarser sphere j owns the pair (with level-major
+// numbering it sorts first; with nested numbering it may sort after i and
+// owns the pair all the same); i lists j in its back row and emits the cross
+// entry (j, i), which becomes a front entry of j. PH 0 counts (same-level fronts,
 // backs, cross entries), PH 1 writes the cross entries at xoff[i], PH 2 writes
 // the rows: same-level fronts in discovery order, then the cross partners
 // xval[xbeg[i] .. xend[i]) (ascending i after the stable sort by j); backs in
@@ -717,7 +730,7 @@ __global__ void k_hneighbors(DevParams P, const D4* pos, const uint32_t* cell, c
                 for (int cx = -1; cx <= 1; cx++) {
                     const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
                     if (jx < 0 || jx >= nxd || jy < 0 || jy >= nyd || jz < 0 || jz >= nzd) continue;
-                    const int cidx = P.loff[T] + (jz * nyd + jy) * nxd + jx;
+                    const int cidx = hkey(P, T, jx, jy, jz);
                     const int start = cbeg[cidx];
                     if (start == -1) continue;
                     const int end = cend[cidx];
@@ -782,19 +795,23 @@ __global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, co
 // History inheritance, src/dem_grid.cu:603-647: each new front pair looks up
 // its pair in the old list of the lower old index and copies the history of
 // buffer p, negated when the orientation flipped. Unmatched pairs start from
-// zero.
+// zero. With nested grid levels a pair on two levels stays in the coarser
+// sphere's row whatever the indices, so its orientation never flips.
 template <class R>
-__global__ void k_hist_remap(int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
-                             const int* ofoff, const uint32_t* ofword, const R* ohist) {
+__global__ void k_hist_remap(DevParams P, const D4* pos, const int* cid, int p, const int* perm, const int* nfoff,
+                             uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword, const R* ohist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= P.N) return;
     const int io = perm[i];
+    const int li = P.nested ? (P.clevel ? P.clevel[cid[i]] : level_of(P, pos[i].w)) : 0;
     for (int s = nfoff[i]; s < nfoff[i + 1]; ++s) {
         uint32_t w = nfword[s];
         const int j = (int)(w & JMASK);
         const int jo = perm[j];
-        const int a = io < jo ? io : jo, b = io < jo ? jo : io;
-        const bool neg = jo < io;
+        const int lj = P.nested ? (P.clevel ? P.clevel[cid[j]] : level_of(P, pos[j].w)) : 0;
+        const bool keep = li != lj || io < jo;  // the old row is i's own
+        const int a = keep ? io : jo, b = keep ? jo : io;
+        const bool neg = !keep;
         int q = -1;
         for (int t = ofoff[a]; t < ofoff[a + 1]; ++t)
             if ((int)(ofword[t] & JMASK) == b) q = t;
@@ -2476,9 +2493,11 @@ int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm
     return 0;
 }
 template <class R>
-int launch_hist_remap(hipStream_t st, int N, int p, const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
-                      const int* ofoff, const uint32_t* ofword, const R* ohist, int block) {
-    k_hist_remap<R><<<nblk(N, block), block, 0, st>>>(N, p, perm, nfoff, nfword, nhist, ofoff, ofword, ohist);
+int launch_hist_remap(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
+                      const int* nfoff, uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword,
+                      const R* ohist, int block) {
+    k_hist_remap<R><<<nblk(P.N, block), block, 0, st>>>(P, pos, cid, p, perm, nfoff, nfword, nhist, ofoff, ofword,
+                                                        ohist);
     LAUNCH_CHECK();
     return 0;
 }
@@ -2571,8 +2590,8 @@ int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
     template int launch_near_sync<R>(hipStream_t, int, const int*, const uint32_t*, const int*, uint32_t*,          \
                                      const R*, R*, int, int);                                                       \
     template int launch_near_hist_gather<R>(hipStream_t, int, const uint32_t*, const int*, const R*, R*, int);      \
-    template int launch_hist_remap<R>(hipStream_t, int, int, const int*, const int*, uint32_t*, R*, const int*,      \
-                                      const uint32_t*, const R*, int);                                               \
+    template int launch_hist_remap<R>(hipStream_t, const DevParams&, const D4*, const int*, int, const int*,         \
+                                      const int*, uint32_t*, R*, const int*, const uint32_t*, const R*, int);        \
     template int launch_clump_gather<R>(hipStream_t, int, const int*, ClumpBuf<R>, ClumpBuf<R>, const int*, int*,   \
                                         int*, int*, int, int);                                                         \
     template int launch_hist_carry<R>(hipStream_t, long long, long long, int, uint32_t*, R*, const uint32_t*,       \

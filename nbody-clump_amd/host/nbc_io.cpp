@@ -141,8 +141,33 @@ extern "C" int nbc_finalize_params(nbc_params* p, const double* radius, int32_t 
     if (cells > 2.0e9) return err(NBC_E_BADINPUT, "nbc_finalize_params: grid too large");
     p->CellNum = (int32_t)(p->GridDim[0] * p->GridDim[1] * p->GridDim[2]);
     p->Levels = 1;
+    p->LevelNested = 0;
+    p->LevelBlock = 0;
     const int L = opt ? opt->hgrid_levels : 0;
-    if (L > 1) {
+    if (opt && opt->hgrid_nested && L <= 1)
+        return err(NBC_E_BADINPUT, "nbc_finalize_params: hgrid_nested needs hgrid_levels 2..4");
+    if (L > 1 && opt->hgrid_nested) {
+        // nested power-of-two levels in coarse-cell-major order: level 0 is
+        // the uniform grid of 3 x max radius, level t halves it t times
+        if (L > 4) return err(NBC_E_BADINPUT, "nbc_finalize_params: a nested hierarchical grid has at most 4 levels");
+        if (opt->cell_size > 0.0) return err(NBC_E_BADINPUT, "nbc_finalize_params: CellSize and HGridLevels exclude each other");
+        int block = 0;
+        for (int t = 0; t < L; ++t) {
+            p->LevelCell[t] = mesh / (double)(1 << t);
+            p->LevelOff[t] = block;
+            block += 1 << (3 * t);
+            for (int a = 0; a < 3; ++a) p->LevelDim[t][a] = (int32_t)(p->GridDim[a] << t);
+        }
+        for (int t = 0; t < L; ++t) p->LevelRmin[t] = t + 1 < L ? p->LevelCell[t + 1] / 3.0 : -1.0;
+        if (cells * block + 1.0 > 2.0e9) return err(NBC_E_BADINPUT, "nbc_finalize_params: hierarchical grid too large");
+        for (int a = 0; a < 3; ++a)
+            if ((double)p->GridDim[a] * (1 << (L - 1)) > 1.0e9)
+                return err(NBC_E_BADINPUT, "nbc_finalize_params: hierarchical grid too large");
+        p->Levels = L;
+        p->LevelNested = 1;
+        p->LevelBlock = block;
+        p->CellNum = (int32_t)(cells * block) + 1;  // + the out-of-box bucket
+    } else if (L > 1) {
         // hierarchical grid: geometric radius boundaries between the smallest
         // and the largest radius, level 0 the largest spheres; every level
         // shares the origin set above (margin = the coarsest cell)
@@ -208,6 +233,7 @@ extern "C" int nbc_load_options(const char* path, nbc_options* out) {
         else if (k == "PeneFrcoe") out->pene_frcoe = std::atof(v.c_str());
         else if (k == "CellSize") out->cell_size = std::atof(v.c_str());
         else if (k == "HGridLevels") out->hgrid_levels = std::atoi(v.c_str());
+        else if (k == "HGridNested") out->hgrid_nested = std::atoi(v.c_str());
         else if (k == "NearSkin") out->near_skin = std::atof(v.c_str());
         else if (k == "Graphs") out->graphs = std::atoi(v.c_str());
         else if (k == "BatchSteps") out->batch_steps = std::atoi(v.c_str());
@@ -260,6 +286,8 @@ extern "C" int nbc_load_grid_levels(const char* path, nbc_params* p) {
         p->GridDim[a] = (uint32_t)p->LevelDim[0][a];
     }
     p->Levels = L;
+    p->LevelNested = 0;
+    p->LevelBlock = 0;
     p->CellNum = (int32_t)total + 1;
     p->finalized = 1;
     return 0;

@@ -41,6 +41,7 @@ class Params(C.Structure):
         ("Levels", C.c_int32), ("LevelOff", C.c_int32 * NBC_MAX_LEVELS),
         ("LevelDim", (C.c_int32 * 3) * NBC_MAX_LEVELS), ("LevelCell", C.c_double * NBC_MAX_LEVELS),
         ("LevelRmin", C.c_double * NBC_MAX_LEVELS),
+        ("LevelNested", C.c_int32), ("LevelBlock", C.c_int32),
     ]
 
 
@@ -50,7 +51,8 @@ class Options(C.Structure):
         ("radix_bits", C.c_int32), ("pair_capacity", C.c_int32), ("batch_steps", C.c_int32),
         ("hgrid_levels", C.c_int32), ("graphs", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
-        ("driver", C.c_int32), ("level_from_type", C.c_int32), ("reserved", C.c_double * 2),
+        ("driver", C.c_int32), ("level_from_type", C.c_int32), ("hgrid_nested", C.c_int32),
+        ("reserved_i", C.c_int32), ("reserved", C.c_double),
     ]
 
 
@@ -379,8 +381,10 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
-                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0, level_from_type=0):
+                 cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0, level_from_type=0,
+                 hgrid_nested=0):
     o = Options()
+    o.hgrid_nested = hgrid_nested
     o.driver = driver
     o.level_from_type = level_from_type
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits

@@ -184,6 +184,8 @@ extern "C" int orc_finalize_params(nbc_params* p, const double* radius, int n, d
     }
     p->CellNum = (int32_t)(p->GridDim[0] * p->GridDim[1] * p->GridDim[2]);
     p->Levels = 1;
+    p->LevelNested = 0;
+    p->LevelBlock = 0;
     if (levels > 1) {
         double rmin = radius[0], rmax = radius[0];
         for (int i = 1; i < n; ++i) {
@@ -208,6 +210,32 @@ extern "C" int orc_finalize_params(nbc_params* p, const double* radius, int n, d
         p->CellNum = off + 1;
     }
     p->finalized = 1;
+    return 0;
+}
+
+// Nested hierarchical grid (no counterpart in the reference): level t has
+// cells of 3 r_max / 2^t on the uniform grid's origin and margin, a sphere
+// sits on the finest level whose cell is at least 3 r, and every coarse cell
+// owns a block of sum_t 8^t keys (level t at offset sum_{u<t} 8^u, then its
+// 8^t sub-cells z/y/x-major), so the sort keeps a coarse cell's spheres
+// together.
+extern "C" int orc_finalize_params_nested(nbc_params* p, const double* radius, int n, int levels) {
+    if (levels < 2 || levels > 4) return NBC_E_BADINPUT;
+    const int e = orc_finalize_params(p, radius, n, 0.0, 0);
+    if (e) return e;
+    const double s0 = p->CellSize[0];
+    int block = 0;
+    for (int t = 0; t < levels; ++t) {
+        p->LevelCell[t] = s0 / (double)(1 << t);
+        p->LevelOff[t] = block;
+        block += 1 << (3 * t);
+        for (int a = 0; a < 3; ++a) p->LevelDim[t][a] = (int32_t)(p->GridDim[a] << t);
+    }
+    for (int t = 0; t < levels; ++t) p->LevelRmin[t] = t + 1 < levels ? p->LevelCell[t + 1] / 3.0 : -1.0;
+    p->Levels = levels;
+    p->LevelNested = 1;
+    p->LevelBlock = block;
+    p->CellNum = (int32_t)((long long)p->GridDim[0] * p->GridDim[1] * p->GridDim[2] * block) + 1;
     return 0;
 }
 
@@ -508,6 +536,14 @@ static int level_of(const orc_ctx* c, int i) {
     return c->O.level_from_type ? c->Type[c->ClumpId[i]] : sphere_level(c->P, c->Radius[i]);
 }
 
+// Key of level-t cell (x, y, z): level-major (src/dem_hgrid.cu:228) or nested.
+static int level_key(const nbc_params& P, int t, int x, int y, int z) {
+    if (!P.LevelNested) return P.LevelOff[t] + (z * P.LevelDim[t][1] + y) * P.LevelDim[t][0] + x;
+    const int n = 1 << t;
+    const int coarse = ((z / n) * P.LevelDim[0][1] + (y / n)) * P.LevelDim[0][0] + (x / n);
+    return coarse * P.LevelBlock + P.LevelOff[t] + ((z % n) * n + (y % n)) * n + (x % n);
+}
+
 static int cell_hash_h(const nbc_params& P, const d3& p, int t) {
     if (p.z > P.BoxUpLmt[2] || p.y > P.BoxUpLmt[1] || p.x > P.BoxUpLmt[0] || p.z < P.BoxLowLmt[2] ||
         p.y < P.BoxLowLmt[1] || p.x < P.BoxLowLmt[0])
@@ -519,7 +555,7 @@ static int cell_hash_h(const nbc_params& P, const d3& p, int t) {
         int v = (int)std::floor((x[a] - P.WorldOrigin[a]) / sz);
         ci[a] = v < 0 ? 0 : (v > P.LevelDim[t][a] - 1 ? P.LevelDim[t][a] - 1 : v);
     }
-    return P.LevelOff[t] + (ci[2] * P.LevelDim[t][1] + ci[1]) * P.LevelDim[t][0] + ci[0];
+    return level_key(P, t, ci[0], ci[1], ci[2]);
 }
 
 static void neighbor_search(orc_ctx* c) {
@@ -606,8 +642,9 @@ static int create_anei(orc_ctx* c) {
 // Hierarchical search, src/dem_hgrid_contlist.cu:57-161: a sphere of level t
 // searches levels t..0 (27 cells each, z outer / x inner, cut-off
 // 1.5 (ri + rj)). Same level: split by index as in the uniform grid. Coarser
-// level T < t: the partner j has the lower sorted index (cells are numbered
-// level-major), so i lists j in its back row and j lists i in its front row.
+// level T < t: the coarser partner j owns the pair (with level-major cells it
+// has the lower sorted index; with nested cells it owns it either way), so i
+// lists j in its back row and j lists i in its front row.
 // Corrected semantics (the reference appends with atomicAdd into a row its
 // owner is writing without one): j's own same-level fronts come first, then its
 // finer partners in ascending index; both halves are checked for overflow.
@@ -642,7 +679,7 @@ static int create_anei_h(orc_ctx* c) {
                         for (int cx = -1; cx <= 1; cx++) {
                             const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
                             if (jx < 0 || jx >= nd[0] || jy < 0 || jy >= nd[1] || jz < 0 || jz >= nd[2]) continue;
-                            const int cidx = P.LevelOff[T] + (jz * nd[1] + jy) * nd[0] + jx;
+                            const int cidx = level_key(P, T, jx, jy, jz);
                             const int start = c->cell_beg[cidx];
                             if (start == -1) continue;
                             const int end = c->cell_end[cidx];
@@ -760,7 +797,9 @@ static int rebuild(orc_ctx* c) {
     for (long long k = 0; k < np; ++k) {
         const int i = c->Lpa[k], j = c->Lpb[k];
         const int ip = perm[i], jp = perm[j];
-        if (jp > ip) {
+        // nested levels: a pair on two levels stays in the coarser sphere's row
+        const bool cross = c->P.Levels > 1 && c->P.LevelNested && level_of(c, i) != level_of(c, j);
+        if (cross || jp > ip) {
             const int b = ip == 0 ? 0 : c->Nsum_pre[ip - 1];
             for (int q = b; q < c->Nsum_pre[ip]; q++)
                 if (jp == c->Lpb_pre[q]) { dst.v[k] = src.v[q]; dst.t[k] = src.t[q]; dst.r[k] = src.r[q]; }

@@ -19,6 +19,7 @@ P = C.c_void_p
 I32 = C.c_int32
 _SIGS = {
     "orc_finalize_params": ([C.POINTER(nbc.Params), P, I32, C.c_double, I32], C.c_int),
+    "orc_finalize_params_nested": ([C.POINTER(nbc.Params), P, I32, I32], C.c_int),
     "orc_request_rebuild": ([P], None),
     "orc_create": ([C.POINTER(nbc.Params), C.POINTER(nbc.Options)], P),
     "orc_destroy": ([P], None),
@@ -75,10 +76,13 @@ def threads():
     return lib().orc_get_threads()
 
 
-def finalize_params(bed, cell_size=0.0, levels=0):
+def finalize_params(bed, cell_size=0.0, levels=0, nested=False):
     p = nbc.Params()
     C.memmove(C.byref(p), C.byref(bed.params), C.sizeof(nbc.Params))
-    rc = lib().orc_finalize_params(C.byref(p), ptr(bed.radius), bed.n, cell_size, levels)
+    if nested:
+        rc = lib().orc_finalize_params_nested(C.byref(p), ptr(bed.radius), bed.n, levels)
+    else:
+        rc = lib().orc_finalize_params(C.byref(p), ptr(bed.radius), bed.n, cell_size, levels)
     assert rc == 0
     return p
 

@@ -33,8 +33,8 @@ def grid_build(params, p4, fn="nbc_grid_build"):
     return res
 
 
-def oracle_grid(bed, params, levels=0):
-    o = orc.Oracle(orc.finalize_params(bed, 0.0, levels))
+def oracle_grid(bed, params, levels=0, nested=False):
+    o = orc.Oracle(orc.finalize_params(bed, 0.0, levels, nested=nested))
     o.upload(bed, None)
     o.rebuild()
     g = o.grid()
@@ -54,7 +54,7 @@ def check_sort_invariants(g, n, cellnum):
     assert np.all(g["beg"][~occ] == -1) and np.all(g["end"][~occ] == -1)
 
 
-@pytest.mark.parametrize("kind", ["uniform", "hgrid"])
+@pytest.mark.parametrize("kind", ["uniform", "hgrid", "hgrid-nested"])
 def test_grid_build_matches_oracle(kind):
     print(f"[case] nbc_grid_build vs oracle, {kind}", flush=True)
     if kind == "uniform":
@@ -62,10 +62,10 @@ def test_grid_build_matches_oracle(kind):
         opt, levels = nbc.make_options(), 0
     else:
         bed = nbc.generate_bed(2, 2000, seed=3)
-        opt, levels = nbc.make_options(hgrid_levels=3), 3
+        opt, levels = nbc.make_options(hgrid_levels=3, hgrid_nested=int(kind == "hgrid-nested")), 3
     params = bed.finalized_params(opt)
     g = grid_build(params, pos4(bed))
-    r = oracle_grid(bed, params, levels)
+    r = oracle_grid(bed, params, levels, nested=kind == "hgrid-nested")
     for k in ("cell", "perm", "beg", "end"):
         assert np.array_equal(g[k], r[k]), k
     check_sort_invariants(g, bed.n, params.CellNum)

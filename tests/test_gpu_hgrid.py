@@ -23,11 +23,11 @@ def banner(msg):
     print(f"[case] {msg}", flush=True)
 
 
-def setup(bed, prec, levels=3):
-    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CUBE, hgrid_levels=levels)
+def setup(bed, prec, levels=3, nested=False):
+    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CUBE, hgrid_levels=levels, hgrid_nested=int(nested))
     ctx = nbc.Context(bed.finalized_params(opt), opt)
     ctx.upload(bed, bed.wall)
-    ref = orc.Oracle(orc.finalize_params(bed, 0.0, levels), opt)
+    ref = orc.Oracle(orc.finalize_params(bed, 0.0, levels, nested=nested), opt)
     ref.upload(bed, bed.wall)
     return ctx, ref
 
@@ -52,10 +52,11 @@ def c2_small():
     return nbc.generate_bed(2, 1500, seed=5, container=(0.025, 0, 0, 0))
 
 
+@pytest.mark.parametrize("nested", [False, True])
 @pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
-def test_hgrid_first_step(c2_small, prec):
-    banner(f"hgrid first step, precision {prec}, N={c2_small.n}")
-    ctx, ref = setup(c2_small, prec)
+def test_hgrid_first_step(c2_small, prec, nested):
+    banner(f"hgrid first step, precision {prec}, nested {nested}, N={c2_small.n}")
+    ctx, ref = setup(c2_small, prec, nested=nested)
     ctx.set_record(True)
     ctx.step(1)
     ref.step(1)
@@ -74,17 +75,20 @@ def test_hgrid_first_step(c2_small, prec):
     ref.close()
 
 
+@pytest.mark.parametrize("nested", [False, True])
 @pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
-def test_hgrid_steps_with_rebuilds(c2_small, prec):
+def test_hgrid_steps_with_rebuilds(c2_small, prec, nested):
     """Random clump velocities up to 0.5 m/s at dt 2e-6 s: a rebuild every few
-    tens of steps, so cross-level histories are carried over many times."""
-    banner(f"hgrid 400 steps with rebuilds, precision {prec}")
+    tens of steps, so cross-level histories are carried over many times (with
+    nested levels, also those of coarse owners that sort after their
+    partner)."""
+    banner(f"hgrid 400 steps with rebuilds, precision {prec}, nested {nested}")
     bed = nbc.generate_bed(2, 1500, seed=5, container=(0.025, 0, 0, 0))
     rng = np.random.default_rng(5)
     bed.cvel[:] = rng.uniform(-0.5, 0.5, bed.cvel.shape)
     bed.cangv[:] = rng.uniform(-100.0, 100.0, bed.cangv.shape)
     bed.params.StepSize = 2e-6
-    ctx, ref = setup(bed, prec)
+    ctx, ref = setup(bed, prec, nested=nested)
     seen = 0
     for k in range(20):
         ctx.step(20)
