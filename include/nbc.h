@@ -171,7 +171,9 @@ typedef struct nbc_options {
                                 nbc_load_grid_levels; 0: levels by radius      */
     int32_t hgrid_nested;    /* with hgrid_levels 2..4: nested power-of-two levels
                                 numbered coarse-cell-major (see nbc_params)    */
-    int32_t reserved_i;
+    int32_t single_pass;     /* neighbour search: 0 a single pass into fixed-capacity
+                                rows when they fit (else the count and fill
+                                passes), -1 always the count and fill passes */
     double reserved[1];
 } nbc_options;
 
@@ -257,6 +259,7 @@ typedef struct nbc_stats {
     int64_t near_refreshes;  /* near-list refreshes between rebuilds since upload */
     int64_t near_pairs;      /* pairs on the contact kernel's near list         */
     int64_t graph_replays;   /* chunks of steps run as HIP graph replays since upload */
+    int64_t single_pass_rebuilds; /* rebuilds whose rows came from the single-pass search */
 } nbc_stats;
 
 typedef struct nbc_ctx nbc_ctx;

@@ -248,6 +248,8 @@ class Engine : public EngineBase {
         }
         HIPC(dalloc(&mbeg, Nc)); HIPC(dalloc(&mend, Nc)); HIPC(dalloc(&mem, N));
         HIPC(dalloc(&dstatus, 4));
+        HIPC(dalloc(&ellmax, 1));
+        ell_k = O.single_pass < 0 ? 0 : std::min(P.BranchLmtLen / 2, 32);
         HIPC(dalloc(&counters, 2));  // touching pairs, touching wall contacts
         HIPC(dalloc(&pst, 1));
         nbk = contacts_blocks(N, block);
@@ -311,6 +313,31 @@ class Engine : public EngineBase {
         return hipSuccess;
     }
 
+    // entry-major rows of the single-pass search for capacity K; K shrinks to
+    // what fits in a fixed budget of device memory (0: multi-pass search)
+    int size_ell(int K) {
+        const int tables = dp.L > 1 ? 3 : 2;
+        const long long budget = 16ll << 30;
+        const long long kmax = budget / ((long long)N * 4 * tables);
+        if (K > kmax) K = (int)kmax;
+        if (K < 8) return 0;
+        const long long need = (long long)K * N;
+        if (need > ell_cap) {
+            dfree(ell_f); dfree(ell_b); dfree(ell_x);
+            ell_f = nullptr; ell_b = ell_x = nullptr; ell_cap = 0;
+            const long long cap = need + need / 4;
+            if (dalloc(&ell_f, (size_t)cap) != hipSuccess || dalloc(&ell_b, (size_t)cap) != hipSuccess ||
+                (tables == 3 && dalloc(&ell_x, (size_t)cap) != hipSuccess)) {
+                (void)hipGetLastError();
+                dfree(ell_f); dfree(ell_b); dfree(ell_x);
+                ell_f = nullptr; ell_b = ell_x = nullptr;
+                return 0;
+            }
+            ell_cap = cap;
+        }
+        return K;
+    }
+
     // near pairs are a subset of the front list: every near array fits in the
     // front-list capacity (one entry per near pair on each side)
     hipError_t grow_near(long long fcap) {
@@ -348,6 +375,7 @@ class Engine : public EngineBase {
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
+        F(ell_f); F(ell_b); F(ell_x); F(ellmax);
         F(dbeds); F(dbid); F(nslack);
         if (hstatus) (void)hipHostFree(hstatus);
         if (hring) (void)hipHostFree(hring);
@@ -725,6 +753,7 @@ class Engine : public EngineBase {
         nrebuild = 0;
         nnear = 0;
         nreplay = 0;
+        ell_fills = 0;
         count = 0;
         uploaded = true;
         HIPC(hipDeviceSynchronize());
@@ -774,23 +803,47 @@ class Engine : public EngineBase {
         KC(launch_fill_i32(st, fcnt + N, 1, 0));
         KC(launch_fill_i32(st, bcnt + N, 1, 0));
         KC(launch_fill_u32(st, dstatus + 2, 1, 0u));
+        KC(launch_fill_u32(st, ellmax, 1, 0u));
         const uint32_t* xkey = nullptr;
         const int* xsrt = nullptr;
+        // single pass into entry-major rows when they fit (bit 2 of the
+        // overflow word: a row was longer than K, fill with the search passes)
+        const int K = size_ell(ell_k);
+        const int* sidx = nullptr;
+        uint32_t ovf_ell = 0;
         if (dp.L > 1) {
             // hierarchical grid: counts, cross entries (j, i) sorted by j, then
             // the front count of each owner
             KC(launch_fill_i32(st, xcnt + N, 1, 0));
-            KC(launch_hneighbors(st, dp, 0, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, nullptr, nullptr,
-                                 nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, dstatus + 2,
-                                 block));
+            if (K > 0) {
+                if (dp.nested) {
+                    // threads in level order: a wave's lanes scan the same levels
+                    KC(launch_level_keys(st, dp, s.pos, s.cid, key0, val0, block));
+                    int sec = 0;
+                    KC(launch_radix_sort(st, key0, val0, key1, val1, N, bits_for(dp.L - 1), rbits, hist, hscan, stmp,
+                                         stotal, &sec));
+                    sidx = sec ? val1 : val0;
+                }
+                KC(launch_hneighbors_ell(st, dp, s.pos, cellkey, cbeg, cend, s.cid, sidx, K, fcnt, bcnt, xcnt, ell_f,
+                                         ell_b, ell_x, dstatus + 2, ellmax, block));
+            } else {
+                KC(launch_hneighbors(st, dp, 0, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, nullptr, nullptr,
+                                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                     dstatus + 2, block));
+            }
             KC(launch_exclusive_scan(st, xcnt, xoff, N + 1, stmp, stotal + 3));
             int X = 0;
             HIPC(hipMemcpyAsync(&X, stotal + 3, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPC(hipMemcpyAsync(&ovf_ell, dstatus + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             HIPC(hipStreamSynchronize(st));
             if (X < 0) return fail(NBC_E_NEIGHBOR_OVERFLOW, "cross-level pair list exceeds 2^31 entries");
             HIPC(grow_cross(std::max<long long>(X, 1)));
-            KC(launch_hneighbors(st, dp, 1, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, xoff, xk0, xv0,
-                                 nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, dstatus + 2, block));
+            if (K > 0 && !(ovf_ell & 2u))
+                KC(launch_cross_from_ell(st, N, sidx, xcnt, xoff, ell_x, xk0, xv0, block));
+            else
+                KC(launch_hneighbors(st, dp, 1, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, xoff, xk0, xv0,
+                                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, dstatus + 2,
+                                     block));
             int sec = 0;
             if (X > 0)
                 KC(launch_radix_sort(st, xk0, xv0, xk1, xv1, X, bits_for(N - 1), rbits, xhist, xhscan, xstmp,
@@ -799,6 +852,9 @@ class Engine : public EngineBase {
             xsrt = sec ? xv1 : xv0;
             KC(launch_seg_bounds(st, xkey, X, N, xbeg, xend));
             KC(launch_hfront_total(st, N, P.BranchLmtLen / 2, fcnt, bcnt, xbeg, xend, dstatus + 2, block));
+        } else if (K > 0) {
+            KC(launch_neighbors_ell(st, dp, s.pos, cellkey, cbeg, cend, s.cid, K, fcnt, bcnt, ell_f, ell_b,
+                                    dstatus + 2, ellmax, block));
         } else {
             KC(launch_neighbors(st, dp, false, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, nullptr, nullptr,
                                 nullptr, nullptr, dstatus + 2, block));
@@ -809,8 +865,13 @@ class Engine : public EngineBase {
         uint32_t ovf = 0;
         HIPC(hipMemcpyAsync(tot, stotal, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(&ovf, dstatus + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        unsigned longest = 0;
+        HIPC(hipMemcpyAsync(&longest, ellmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
-        if (ovf) {
+        // next rebuild's capacity: the longest row with a quarter to spare
+        if (O.single_pass >= 0)
+            ell_k = std::min(P.BranchLmtLen / 2, std::max(8, (int)((longest + longest / 4 + 8) & ~7u)));
+        if (ovf & 1u) {
             char b[320];
             std::snprintf(b, sizeof b,
                           "FATAL: Neighbor list overflow (BranchLmtLen=%d). Simulation stopped before using the "
@@ -826,12 +887,19 @@ class Engine : public EngineBase {
         }
         if ((long long)tot[1] > bcap) HIPC(grow_back((long long)(tot[1] * 1.25) + 1024));
         // 5. fill rows, resolve back slots, remap histories (src/dem_grid.cu:561-647)
-        if (dp.L > 1)
+        const bool from_ell = K > 0 && !(ovf & 2u);
+        if (dp.L > 1 && from_ell)
+            KC(launch_hrows_from_ell(st, N, sidx, s.cid, Ln.foff, bcnt, boff, xbeg, xend, xsrt, ell_f, ell_b, Ln.fword,
+                                     bent, block));
+        else if (dp.L > 1)
             KC(launch_hneighbors(st, dp, 2, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, xcnt, xoff, nullptr,
                                  nullptr, Ln.foff, boff, xbeg, xend, xsrt, Ln.fword, bent, dstatus + 2, block));
+        else if (from_ell)
+            KC(launch_rows_from_ell(st, N, fcnt, bcnt, Ln.foff, boff, ell_f, ell_b, Ln.fword, bent, block));
         else
             KC(launch_neighbors(st, dp, true, s.pos, cellkey, cbeg, cend, s.cid, fcnt, bcnt, Ln.foff, boff, Ln.fword,
                                 bent, dstatus + 2, block));
+        ell_fills += from_ell ? 1 : 0;
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
         // history-valid bits of the old near list back into the old pair words
         KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, Lo.fword, hn[p], Lo.hist[p], p, block));
@@ -1520,6 +1588,7 @@ class Engine : public EngineBase {
         s->near_refreshes = nnear;
         s->near_pairs = near_total;
         s->graph_replays = nreplay;
+        s->single_pass_rebuilds = ell_fills;
         s->kinetic_energy = ke;
         for (int a = 0; a < 3; ++a) { s->pene_force[a] = -ps.totF[a]; s->pene_torque[a] = ps.totT[a]; }
         return 0;
@@ -1558,6 +1627,15 @@ class Engine : public EngineBase {
     uint32_t *xk0 = nullptr, *xk1 = nullptr;
     int *xv0 = nullptr, *xv1 = nullptr, *xhist = nullptr, *xhscan = nullptr, *xstmp = nullptr;
     long long xcap = 0;
+    // single-pass neighbour search: entry-major rows of capacity ell_k
+    // (fronts, backs, cross entries), sized from the longest row of the
+    // previous rebuild; ell_k = 0 falls back to the multi-pass search
+    uint32_t* ell_f = nullptr;
+    int *ell_b = nullptr, *ell_x = nullptr;
+    long long ell_cap = 0;  // entries per table
+    int ell_k = 0;
+    unsigned* ellmax = nullptr;
+    long long ell_fills = 0;
     // batch of beds: sphere / clump offsets per bed (nbeds + 1), per-bed grid
     // and wall table (host copy and device copy), bed of each sphere
     int nbeds = 1;

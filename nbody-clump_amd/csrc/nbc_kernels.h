@@ -133,6 +133,23 @@ int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* p
                       const int* cbeg, const int* cend, const int* cid, int* fcnt, int* bcnt, int* xcnt,
                       const int* xoff, uint32_t* xkey, int* xval, const int* foff, const int* boff, const int* xbeg,
                       const int* xend, const int* xsrt, uint32_t* fword, int2* bent, unsigned* overflow, int block);
+// single-pass neighbour search into entry-major rows of capacity K, and the
+// compactions into the CSR rows (see nbc_kernels.hip)
+int launch_neighbors_ell(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* cell, const int* cbeg,
+                         const int* cend, const int* cid, int K, int* fcnt, int* bcnt, uint32_t* ef, int* eb,
+                         unsigned* overflow, unsigned* ellmax, int block);
+int launch_rows_from_ell(hipStream_t st, int N, const int* fcnt, const int* bcnt, const int* foff, const int* boff,
+                         const uint32_t* ef, const int* eb, uint32_t* fword, int2* bent, int block);
+int launch_hneighbors_ell(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* cell, const int* cbeg,
+                          const int* cend, const int* cid, const int* sidx, int K, int* fcnt, int* bcnt, int* xcnt,
+                          uint32_t* ef, int* eb, int* ex, unsigned* overflow, unsigned* ellmax, int block);
+int launch_cross_from_ell(hipStream_t st, int N, const int* sidx, const int* xcnt, const int* xoff, const int* ex,
+                          uint32_t* xkey, int* xval, int block);
+int launch_hrows_from_ell(hipStream_t st, int N, const int* sidx, const int* cid, const int* foff, const int* bcnt,
+                          const int* boff, const int* xbeg, const int* xend, const int* xsrt, const uint32_t* ef,
+                          const int* eb, uint32_t* fword, int2* bent, int block);
+int launch_level_keys(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, uint32_t* key, int* val,
+                      int block);
 int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* bcnt, const int* xbeg,
                         const int* xend, unsigned* overflow, int block);
 // Near lists of the current list at the current positions; also sets each

@@ -784,6 +784,200 @@ __global__ void k_hfront_total(int N, int half, int* fcnt, const int* bcnt, cons
     if (f > half || bcnt[i] > half) atomicOr(overflow, 1u);
 }
 
+// ---------------------------------------------------------------------------
+// Single-pass neighbour search. The count pass of k_neighbors / k_hneighbors
+// also stores each row into a fixed-capacity table kept entry-major (entry g
+// of thread k at g * N + k, so a wave's lanes write neighbouring words); once
+// the counts are scanned, a compaction kernel moves the rows into the CSR
+// arrays in the order the fill pass would write them, so the second (and, on
+// the hierarchical grid, third) search pass is not needed. A row longer than
+// K sets bit 2 of *overflow: the counts stay exact, only the stores stop, and
+// the host then fills the rows with the multi-pass kernels. ellmax receives
+// the longest row (front or back), from which the host sizes K for the next
+// rebuild.
+__device__ __forceinline__ void wave_max_to(unsigned* dst, int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, (unsigned)v);
+}
+
+// Uniform grid (src/dem_grid.cu:466-552): the loop of k_neighbors; fronts as
+// pair words, backs in discovery order (reversed by the compaction).
+__global__ void k_neighbors_ell(DevParams P, const D4* pos, const uint32_t* cell, const int* cbeg, const int* cend,
+                                const int* cid, int K, int* fcnt, int* bcnt, uint32_t* ef, int* eb, unsigned* overflow,
+                                unsigned* ellmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int ng = 0, ni = 0;
+    if (i < P.N) {
+        const int half = P.B / 2;
+        const double* O = P.origin;
+        const double* ic = P.icell;
+        const uint32_t* G = P.G;
+        int coff = 0, last = P.CellNum - 1;
+        if (P.nbeds > 1) {
+            const BedGeo& g = P.beds[P.bid[i]];
+            O = g.origin; ic = g.icell; G = g.G;
+            coff = g.cell_off;
+            last = g.cell_off + g.cell_num - 1;
+        }
+        if ((int)cell[i] != last) {
+            const D4 pi = pos[i];
+            const double ri = pi.w;
+            const int ci = cid[i];
+            const int cix = (int)floor((pi.x - O[0]) * ic[0]);
+            const int ciy = (int)floor((pi.y - O[1]) * ic[1]);
+            const int ciz = (int)floor((pi.z - O[2]) * ic[2]);
+            for (int cz = -1; cz <= 1; cz++)
+                for (int cy = -1; cy <= 1; cy++)
+                    for (int cx = -1; cx <= 1; cx++) {
+                        const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
+                        if (jx < 0 || (uint32_t)jx >= G[0] || jy < 0 || (uint32_t)jy >= G[1] || jz < 0 ||
+                            (uint32_t)jz >= G[2])
+                            continue;
+                        const int cidx = coff + (int)(((uint32_t)jz * G[1] + (uint32_t)jy) * G[0] + (uint32_t)jx);
+                        const int start = cbeg[cidx];
+                        if (start == -1) continue;
+                        const int end = cend[cidx];
+                        for (int j = start; j < end; j++) {
+                            if (j == i) continue;
+                            const D4 pj = pos[j];
+                            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                            const double ds = dx * dx + dy * dy + dz * dz;
+                            const double lim = 1.50 * (ri + pj.w);
+                            if (ds < lim * lim) {
+                                if (i < j) {
+                                    if (ng < K) ef[(size_t)ng * P.N + i] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+                                    ng++;
+                                } else {
+                                    if (ni < K) eb[(size_t)ni * P.N + i] = j;
+                                    ni++;
+                                }
+                            }
+                        }
+                    }
+        }
+        fcnt[i] = ng;
+        bcnt[i] = ni;
+        if (ng > half || ni > half) atomicOr(overflow, 1u);
+        if (ng > K || ni > K) atomicOr(overflow, 2u);
+    }
+    wave_max_to(ellmax, max(ng, ni));
+}
+
+__global__ void k_rows_from_ell(int N, const int* fcnt, const int* bcnt, const int* foff, const int* boff,
+                                const uint32_t* ef, const int* eb, uint32_t* fword, int2* bent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int nf = fcnt[i], fo = foff[i];
+    for (int g = 0; g < nf; ++g) fword[fo + g] = ef[(size_t)g * N + i];
+    const int nb = bcnt[i], bo = boff[i];
+    for (int g = 0; g < nb; ++g) bent[bo + nb - 1 - g] = make_int2(eb[(size_t)g * N + i], -1);
+}
+
+// Hierarchical grid: the loop of k_hneighbors (phase 0) storing own fronts,
+// backs and cross entries; thread k takes sphere sidx[k] (the spheres of one
+// level together, so that a wave's lanes run the same number of levels), or
+// sphere k without sidx.
+__global__ void k_hneighbors_ell(DevParams P, const D4* pos, const uint32_t* cell, const int* cbeg, const int* cend,
+                                 const int* cid, const int* sidx, int K, int* fcnt, int* bcnt, int* xcnt, uint32_t* ef,
+                                 int* eb, int* ex, unsigned* overflow, unsigned* ellmax) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int ng = 0, ni = 0;
+    if (k < P.N) {
+        const int i = sidx ? sidx[k] : k;
+        int nx = 0;
+        if ((int)cell[i] != P.CellNum - 1) {
+            const D4 pi = pos[i];
+            const double ri = pi.w;
+            const int ci = cid[i];
+            const int my = P.clevel ? P.clevel[ci] : level_of(P, ri);
+            for (int T = my; T >= 0; --T) {
+                const double s = P.lcell[T];
+                const int nxd = P.ldim[T][0], nyd = P.ldim[T][1], nzd = P.ldim[T][2];
+                const int cix = hcell(pi.x, P.origin[0], s), ciy = hcell(pi.y, P.origin[1], s);
+                const int ciz = hcell(pi.z, P.origin[2], s);
+                for (int cz = -1; cz <= 1; cz++)
+                    for (int cy = -1; cy <= 1; cy++)
+                        for (int cx = -1; cx <= 1; cx++) {
+                            const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
+                            if (jx < 0 || jx >= nxd || jy < 0 || jy >= nyd || jz < 0 || jz >= nzd) continue;
+                            const int cidx = hkey(P, T, jx, jy, jz);
+                            const int start = cbeg[cidx];
+                            if (start == -1) continue;
+                            const int end = cend[cidx];
+                            for (int j = start; j < end; j++) {
+                                if (j == i) continue;
+                                const D4 pj = pos[j];
+                                const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                                const double ds = dx * dx + dy * dy + dz * dz;
+                                const double lim = 1.5 * (ri + pj.w);
+                                if (!(ds < lim * lim)) continue;
+                                if (T == my && i < j) {
+                                    if (ng < K) ef[(size_t)ng * P.N + k] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+                                    ng++;
+                                } else {
+                                    if (ni < K) eb[(size_t)ni * P.N + k] = j;
+                                    ni++;
+                                    if (T != my) {
+                                        if (nx < K) ex[(size_t)nx * P.N + k] = j;
+                                        nx++;
+                                    }
+                                }
+                            }
+                        }
+            }
+        }
+        fcnt[i] = ng;
+        bcnt[i] = ni;
+        xcnt[i] = nx;
+        if (ng > K || ni > K) atomicOr(overflow, 2u);
+    }
+    wave_max_to(ellmax, max(ng, ni));
+}
+
+// cross entries (j, i) at xoff[i], in discovery order (as phase 1 writes them)
+__global__ void k_cross_from_ell(int N, const int* sidx, const int* xcnt, const int* xoff, const int* ex,
+                                 uint32_t* xkey, int* xval) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = sidx ? sidx[k] : k;
+    const int n = xcnt[i], o = xoff[i];
+    for (int g = 0; g < n; ++g) {
+        xkey[o + g] = (uint32_t)ex[(size_t)g * N + k];
+        xval[o + g] = i;
+    }
+}
+
+// rows as phase 2 writes them: own fronts, then the cross partners
+// xsrt[xbeg[i] .. xend[i]); backs in reverse discovery order
+__global__ void k_hrows_from_ell(int N, const int* sidx, const int* cid, const int* foff, const int* bcnt,
+                                 const int* boff, const int* xbeg, const int* xend, const int* xsrt, const uint32_t* ef,
+                                 const int* eb, uint32_t* fword, int2* bent) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int i = sidx ? sidx[k] : k;
+    const int fo = foff[i], nf = foff[i + 1] - fo;
+    const int b = xbeg[i];
+    const int nxc = b >= 0 ? xend[i] - b : 0;
+    const int own = nf - nxc;
+    for (int g = 0; g < own; ++g) fword[fo + g] = ef[(size_t)g * N + k];
+    const int ci = cid[i];
+    for (int q = 0; q < nxc; ++q) {
+        const int j = xsrt[b + q];
+        fword[fo + own + q] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+    }
+    const int nb = bcnt[i], bo = boff[i];
+    for (int g = 0; g < nb; ++g) bent[bo + nb - 1 - g] = make_int2(eb[(size_t)g * N + k], -1);
+}
+
+// level of every sphere as a sort key (value: the sphere), for the
+// level-grouped thread order of the nested hierarchical search
+__global__ void k_level_keys(DevParams P, const D4* pos, const int* cid, uint32_t* key, int* val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    key[i] = (uint32_t)(P.clevel ? P.clevel[cid[i]] : level_of(P, pos[i].w));
+    val[i] = i;
+}
+
 // Owner slot of every back entry: src/dem_grid.cu:591-599 (Aref of the back
 // rows), -1 when the owner's own search did not list the pair. A wave takes
 // the back rows of 64 consecutive spheres (one contiguous segment), one entry
@@ -791,40 +985,6 @@ __global__ void k_hfront_total(int N, int half, int* fcnt, const int* bcnt, cons
 // so the first match is the only one).
 __global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, const uint32_t* fword, const int* boff,
                                                       int2* bent);
-
-// History inheritance, src/dem_grid.cu:603-647: each new front pair looks up
-// its pair in the old list of the lower old index and copies the history of
-// buffer p, negated when the orientation flipped. Unmatched pairs start from
-// zero. With nested grid levels a pair on two levels stays in the coarser
-// sphere's row whatever the indices, so its orientation never flips.
-template <class R>
-__global__ void k_hist_remap(DevParams P, const D4* pos, const int* cid, int p, const int* perm, const int* nfoff,
-                             uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword, const R* ohist) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.N) return;
-    const int io = perm[i];
-    const int li = P.nested ? (P.clevel ? P.clevel[cid[i]] : level_of(P, pos[i].w)) : 0;
-    for (int s = nfoff[i]; s < nfoff[i + 1]; ++s) {
-        uint32_t w = nfword[s];
-        const int j = (int)(w & JMASK);
-        const int jo = perm[j];
-        const int lj = P.nested ? (P.clevel ? P.clevel[cid[j]] : level_of(P, pos[j].w)) : 0;
-        const bool keep = li != lj || io < jo;  // the old row is i's own
-        const int a = keep ? io : jo, b = keep ? jo : io;
-        const bool neg = !keep;
-        int q = -1;
-        for (int t = ofoff[a]; t < ofoff[a + 1]; ++t)
-            if ((int)(ofword[t] & JMASK) == b) q = t;
-        w &= ~(VALID(0) | VALID(1));
-        if (q >= 0 && (ofword[q] & VALID(p)) && !(w & INTRA)) {
-            const R* h = ohist + (size_t)q * HSTRIDE;
-            R* d = nhist + (size_t)s * HSTRIDE;
-            for (int k = 0; k < HSTRIDE; ++k) d[k] = neg ? R(-1.0) * h[k] : h[k];
-            w |= VALID(p);
-        }
-        nfword[s] = w;
-    }
-}
 
 // History after a rebuild in DEMmain's cylinder and glue loops
 // (src/dem_simulation_cylind.cu:130-141): CreateContList inherits the
@@ -951,6 +1111,53 @@ __global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, co
         be.x = (slot >= 0 && (fword[slot] & INTRA)) ? (o | BINTRA) : o;
         be.y = slot;
         bent[e] = be;
+    }
+}
+
+// History inheritance, src/dem_grid.cu:603-647: each new front pair looks up
+// its pair in the old list of the lower old index and copies the history of
+// buffer p, negated when the orientation flipped. Unmatched pairs start from
+// zero. With nested grid levels a pair on two levels stays in the coarser
+// sphere's row whatever the indices, so its orientation never flips. A wave
+// takes the front rows of 64 consecutive spheres (one contiguous segment),
+// one pair per lane, so a sphere with a long row spreads its lookups over the
+// lanes instead of running them one after another.
+template <class R>
+__global__ __launch_bounds__(256) void k_hist_remap(DevParams P, const D4* pos, const int* cid, int p,
+                                                    const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
+                                                    const int* ofoff, const uint32_t* ofword, const R* ohist) {
+    __shared__ int s_fo[4][65];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= P.N) return;
+    load_rows(s_fo[wv], nfoff, i0, P.N);
+    const int end = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
+        const int s = k0 + lane;
+        if (s >= end) continue;
+        const int i = i0 + row_of(s_fo[wv], s);
+        uint32_t w = nfword[s];
+        const int j = (int)(w & JMASK);
+        const int io = perm[i], jo = perm[j];
+        bool keep = io < jo;  // the old row is i's own
+        if (!keep && P.nested) {
+            const int li = P.clevel ? P.clevel[cid[i]] : level_of(P, pos[i].w);
+            keep = li != (P.clevel ? P.clevel[cid[j]] : level_of(P, pos[j].w));
+        }
+        const int a = keep ? io : jo, b = keep ? jo : io;
+        const bool neg = !keep;
+        int q = -1;
+        const int te = ofoff[a + 1];
+        for (int t = ofoff[a]; t < te; ++t)
+            if ((int)(ofword[t] & JMASK) == b) { q = t; break; }
+        w &= ~(VALID(0) | VALID(1));
+        if (q >= 0 && (ofword[q] & VALID(p)) && !(w & INTRA)) {
+            const R* h = ohist + (size_t)q * HSTRIDE;
+            R* d = nhist + (size_t)s * HSTRIDE;
+            for (int k = 0; k < HSTRIDE; ++k) d[k] = neg ? R(-1.0) * h[k] : h[k];
+            w |= VALID(p);
+        }
+        nfword[s] = w;
     }
 }
 
@@ -2405,6 +2612,54 @@ int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* p
     return 0;
 }
 
+int launch_neighbors_ell(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* cell, const int* cbeg,
+                         const int* cend, const int* cid, int K, int* fcnt, int* bcnt, uint32_t* ef, int* eb,
+                         unsigned* overflow, unsigned* ellmax, int block) {
+    k_neighbors_ell<<<nblk(P.N, block), block, 0, st>>>(P, pos, cell, cbeg, cend, cid, K, fcnt, bcnt, ef, eb,
+                                                         overflow, ellmax);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_rows_from_ell(hipStream_t st, int N, const int* fcnt, const int* bcnt, const int* foff, const int* boff,
+                         const uint32_t* ef, const int* eb, uint32_t* fword, int2* bent, int block) {
+    k_rows_from_ell<<<nblk(N, block), block, 0, st>>>(N, fcnt, bcnt, foff, boff, ef, eb, fword, bent);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_hneighbors_ell(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* cell, const int* cbeg,
+                          const int* cend, const int* cid, const int* sidx, int K, int* fcnt, int* bcnt, int* xcnt,
+                          uint32_t* ef, int* eb, int* ex, unsigned* overflow, unsigned* ellmax, int block) {
+    k_hneighbors_ell<<<nblk(P.N, block), block, 0, st>>>(P, pos, cell, cbeg, cend, cid, sidx, K, fcnt, bcnt, xcnt, ef,
+                                                          eb, ex, overflow, ellmax);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_cross_from_ell(hipStream_t st, int N, const int* sidx, const int* xcnt, const int* xoff, const int* ex,
+                          uint32_t* xkey, int* xval, int block) {
+    k_cross_from_ell<<<nblk(N, block), block, 0, st>>>(N, sidx, xcnt, xoff, ex, xkey, xval);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_hrows_from_ell(hipStream_t st, int N, const int* sidx, const int* cid, const int* foff, const int* bcnt,
+                          const int* boff, const int* xbeg, const int* xend, const int* xsrt, const uint32_t* ef,
+                          const int* eb, uint32_t* fword, int2* bent, int block) {
+    k_hrows_from_ell<<<nblk(N, block), block, 0, st>>>(N, sidx, cid, foff, bcnt, boff, xbeg, xend, xsrt, ef, eb, fword,
+                                                        bent);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_level_keys(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, uint32_t* key, int* val,
+                      int block) {
+    k_level_keys<<<nblk(P.N, block), block, 0, st>>>(P, pos, cid, key, val);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* bcnt, const int* xbeg,
                         const int* xend, unsigned* overflow, int block) {
     k_hfront_total<<<nblk(N, block), block, 0, st>>>(N, half, fcnt, bcnt, xbeg, xend, overflow);
@@ -2496,8 +2751,8 @@ template <class R>
 int launch_hist_remap(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
                       const int* nfoff, uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword,
                       const R* ohist, int block) {
-    k_hist_remap<R><<<nblk(P.N, block), block, 0, st>>>(P, pos, cid, p, perm, nfoff, nfword, nhist, ofoff, ofword,
-                                                        ohist);
+    (void)block;
+    k_hist_remap<R><<<nblk(P.N, 256), 256, 0, st>>>(P, pos, cid, p, perm, nfoff, nfword, nhist, ofoff, ofword, ohist);
     LAUNCH_CHECK();
     return 0;
 }

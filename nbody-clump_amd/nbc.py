@@ -52,7 +52,7 @@ class Options(C.Structure):
         ("hgrid_levels", C.c_int32), ("graphs", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
         ("driver", C.c_int32), ("level_from_type", C.c_int32), ("hgrid_nested", C.c_int32),
-        ("reserved_i", C.c_int32), ("reserved", C.c_double),
+        ("single_pass", C.c_int32), ("reserved", C.c_double),
     ]
 
 
@@ -110,7 +110,7 @@ class Stats(C.Structure):
         ("back_entries", C.c_int64), ("touching_ss", C.c_int64), ("touching_wall", C.c_int64),
         ("kinetic_energy", C.c_double), ("pene_force", D3), ("pene_torque", D3),
         ("max_cum_disp", C.c_double), ("near_refreshes", C.c_int64), ("near_pairs", C.c_int64),
-        ("graph_replays", C.c_int64),
+        ("graph_replays", C.c_int64), ("single_pass_rebuilds", C.c_int64),
     ]
 
 
@@ -382,9 +382,10 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
                  cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0, level_from_type=0,
-                 hgrid_nested=0):
+                 hgrid_nested=0, single_pass=0):
     o = Options()
     o.hgrid_nested = hgrid_nested
+    o.single_pass = single_pass
     o.driver = driver
     o.level_from_type = level_from_type
     o.precision, o.wall_type, o.block_size, o.radix_bits = precision, wall_type, block_size, radix_bits

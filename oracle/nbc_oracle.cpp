@@ -149,6 +149,7 @@ struct orc_ctx {
     long long touching_ss = 0, touching_wall = 0;
     bool first = true;
     bool rebuild_next = false;
+    bool rebuilt_between = false;  // orc_rebuild ran since the last step
     // compact mode: per-step work and clears cover the used slot range
     // [0, S_used) and the used row entries only (a CSR pair list), instead of
     // all N*BranchLmtLen/2 slots and N*BranchLmtLen row entries
@@ -1521,6 +1522,20 @@ static void clump_kick(orc_ctx* c) {
 
 static void zero3(std::vector<d3>& v) { std::fill(v.begin(), v.end(), d3{0, 0, 0}); }
 
+// The rebuild branch of the loop body (src/dem_simulation_clump.cu:121-175).
+static int rebuild_in_step(orc_ctx* c) {
+    int e = rebuild(c);
+    if (e) return e;
+    if (c->driver == NBC_DRIVER_CYLINDER || c->driver == NBC_DRIVER_SPLIT) {
+        // src/dem_simulation_cylind.cu:141: ForceExchange follows the rebuild
+        // too, swapping the inherited histories out (and clearing them)
+        c->D.swap(c->Dpre);
+        if (c->compact) zero_range(c->D, c->S_high);
+        else c->D.zero();
+    }
+    return 0;
+}
+
 // One pass of the loop body of src/dem_simulation_clump.cu:110-226.
 static int step_once(orc_ctx* c) {
     integrate_before(c);
@@ -1529,21 +1544,18 @@ static int step_once(orc_ctx* c) {
     const int nf = c->wall_type == NBC_WALL_CUBE ? 6 : (c->wall_type == NBC_WALL_CYLINDER ? 3 : 0);
     if (c->Update != 0 || c->first || c->rebuild_next) {
         c->rebuild_next = false;
-        int e = rebuild(c);
+        int e = rebuild_in_step(c);
         if (e) return e;
-        if (c->driver == NBC_DRIVER_CYLINDER || c->driver == NBC_DRIVER_SPLIT) {
-            // src/dem_simulation_cylind.cu:141: ForceExchange follows the rebuild
-            // too, swapping the inherited histories out (and clearing them)
-            c->D.swap(c->Dpre);
-            if (c->compact) zero_range(c->D, c->S_high);
-            else c->D.zero();
-        }
+    } else if (c->rebuilt_between) {
+        // orc_rebuild stood in for this step's rebuild: the inherited
+        // histories are already the ones this step reads
     } else {
         c->D.swap(c->Dpre);  // DEMForce::ForceExchange
         if (c->compact) zero_range(c->D, c->S_used);
         else c->D.zero();
     }
     c->first = false;
+    c->rebuilt_between = false;
     for (int f = 0; f < nf; ++f) { c->WH[f].swap(c->WHpre[f]); c->WH[f].zero(); }
     if (c->P.isIntruder) { c->PH.swap(c->PHpre); c->PH.zero(); }
     // ForceZero (src/dem_force.cu:75-86, src/dem_cube_wall.cu:208-211)
@@ -1614,9 +1626,14 @@ extern "C" int orc_integrate_only(orc_ctx* c, int nsteps, const double* F, const
 // this is how a single-bed oracle follows it.
 extern "C" void orc_request_rebuild(orc_ctx* c) { c->rebuild_next = true; }
 
+// A rebuild between steps, on the positions the last step left: the next step
+// then skips its history exchange, as a step that rebuilds does, so the
+// inherited histories carry on (what a rebuild inside that step would give,
+// on these positions).
 extern "C" int orc_rebuild(orc_ctx* c) {
-    int e = rebuild(c);
+    int e = rebuild_in_step(c);
     c->first = false;
+    c->rebuilt_between = e == 0;
     return e;
 }
 
