@@ -594,7 +594,11 @@ extern "C" int nbc_save_bed_text(const char* dir, const nbc_bed_out* b, const nb
     {
         std::ofstream f(d + "/nbc.par");
         f << "WallType = " << (W.type == NBC_WALL_CYLINDER ? "cylinder" : "cube") << "\n";
-        if (b->params.GridLevels > 1) f << "HGridLevels = " << (int)b->params.GridLevels << "\n";
+        if (b->params.GridLevels > 1) {
+            f << "HGridLevels = " << (int)b->params.GridLevels << "\n";
+            // nested levels keep a coarse cell's spheres together (faster steps)
+            if (b->params.GridLevels <= 4) f << "HGridNested = 1\n";
+        }
     }
     (void)cmd;
     return 0;
