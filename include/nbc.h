@@ -471,7 +471,8 @@ int nbc_profile_get(nbc_ctx* ctx, int32_t kernel_class, double* total_ms, int64_
 /* Picks options for one bed by timing candidates on the device: coordinate
  * descent over near_skin, batch_steps, radix_bits, block_size and graphs (none
  * of them changes any result), then, when asked, the cell size (1, 1.25 or 1.5 times
- * 3*max radius) and the hierarchical grid (off, 2 or 3 levels), which change
+ * 3*max radius) and the hierarchical grid (off, 2 or 3 levels, level-major
+ * or nested), which change
  * the sphere order and so the FP64 sums by rounding. Each candidate gets its
  * own context: upload, warm-up steps (the first rebuild included), then
  * `steps` timed steps. params are world.par values NOT yet finalized (every

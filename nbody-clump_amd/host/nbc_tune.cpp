@@ -91,8 +91,13 @@ int tune(const Bed& b, const nbc_options* base, int32_t flags, nbc_options* best
         sweep([rmax](nbc_options& o, double v) { o.cell_size = v * 3.0 * rmax; o.hgrid_levels = 0; },
               {1.0, 1.25, 1.5});
     if ((flags & NBC_TUNE_HGRID) && b.nbeds == 1)
-        sweep([](nbc_options& o, double v) { o.hgrid_levels = (int32_t)v; if (v > 1) o.cell_size = 0.0; },
-              {0, 2, 3});
+        // negative: nested levels
+        sweep([](nbc_options& o, double v) {
+                  o.hgrid_levels = (int32_t)std::fabs(v);
+                  o.hgrid_nested = v < 0.0 ? 1 : 0;
+                  if (o.hgrid_levels > 1) o.cell_size = 0.0;
+              },
+              {0, 2, 3, -2, -3});
     if (n_entries) *n_entries = count;
     if (best_ms < 0.0) {
         nbc::set_global_error("nbc_autotune: no candidate ran");

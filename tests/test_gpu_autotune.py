@@ -41,9 +41,9 @@ def test_autotune_cell_and_hgrid_sweeps():
     bed = nbc.generate_bed(2, 800, seed=6, container=(0.02, 0, 0, 0))
     base = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CUBE)
     best, entries = nbc.autotune(bed, base, wall=bed.wall, steps=40, flags=nbc.NBC_TUNE_CELL | nbc.NBC_TUNE_HGRID)
-    assert len(entries) == 4 + 4 + 3 + 2 + 2 + 3 + 3
+    assert len(entries) == 4 + 4 + 3 + 2 + 2 + 3 + 5
     assert all(ms > 0 for _, ms in entries)
-    assert best.hgrid_levels in (0, 2, 3)
+    assert best.hgrid_levels in (0, 2, 3) and best.hgrid_nested in (0, 1)
     # the picked options run
     ctx = nbc.Context(bed.finalized_params(best), best)
     ctx.upload(bed, bed.wall)
