@@ -526,6 +526,17 @@ int nbc_write_contacts_bin(const char* dir, int32_t step, int32_t n, int32_t bra
                            const int32_t* nei, const double* fcn, const double* cpt);
 
 /* ---- the drop-in entry point ------------------------------------------ */
+/* Binary -> text state for resuming a clump run: OutPart/ph.<step>.bin and
+ * OutClump/ch.<step>.bin under rundir (layouts of src/dem_binary_io.h:19-181)
+ * become outdir/<IniConFile> and outdir/ch.00000000.bt in the text formats the
+ * loaders read (src/dem_subsphere_io.cu:174-175, src/dem_clumpsphere_io.cu:
+ * 46-48), rows in file order; world.par is copied with StartStep = step,
+ * nbc.par and Rocks/ are copied, and the penetrator's output record replaces
+ * its state in Rocks/Penetrator.bt. Pair, wall and penetrator histories are
+ * not in the outputs, so the resumed run starts them from zero. Refuses to
+ * overwrite ph/ch files in outdir. No counterpart in the reference, whose
+ * outputs cannot be fed back without such a conversion. */
+int nbc_resume_from_bin(const char* rundir, int32_t step, const char* outdir);
 /* Runs the whole simulation like DEMClumpMain::RunSimulationClump from the
  * files in workdir (world.par, IniConFile, ch.00000000.bt, Rocks/, optional
  * nbc.par), writing OutPart/OutClump/OutForce/OutCnt there. */

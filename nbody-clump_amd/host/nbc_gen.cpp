@@ -171,6 +171,36 @@ constexpr double kPi = 3.14159265358979323846;
 
 }  // namespace
 
+namespace nbc {
+// Rocks/Penetrator.bt in the format nbc_load_penetrator reads
+// (src/dem_penetrator_io.cu:21-116, "cone R H NSEG" for the triangulated
+// cone); without a penetrator, a far-away dummy sphere.
+int write_penetrator_text(const std::string& path, const nbc_penetrator* pene) {
+    std::ofstream f(path);
+    if (!f) return -1;
+    f << std::setprecision(17);
+    if (pene) {
+        if (pene->shape == NBC_PENE_SPHERE) f << "sphere " << pene->radius << "\n";
+        else if (pene->shape == NBC_PENE_CONE)  // the mesh holds two triangles per rim segment
+            f << "cone " << pene->radius << " " << pene->height << " " << (pene->ntri > 0 ? pene->ntri / 2 : 64)
+              << "\n";
+        else f << "cylinder " << pene->radius << " " << pene->height << "\n" << pene->dir[0] << " " << pene->dir[1]
+               << " " << pene->dir[2] << "\n";
+        f << pene->prop[0] << " " << pene->prop[1] << " " << pene->prop[2] << " " << pene->prop[3] << "\n"
+          << pene->pos[0] << " " << pene->pos[1] << " " << pene->pos[2] << "\n" << pene->vel[0] << " "
+          << pene->vel[1] << " " << pene->vel[2] << "\n" << pene->quat[0] << " " << pene->quat[1] << " "
+          << pene->quat[2] << " " << pene->quat[3] << "\n" << pene->angv[0] << " " << pene->angv[1] << " "
+          << pene->angv[2] << "\n" << "motion "
+          << (pene->motion == NBC_PENE_GRAVITY ? (pene->wait_for_contact ? "gravity_after_contact" : "gravity")
+                                               : "constant_velocity")
+          << "\n";
+    } else {
+        f << "sphere 0.001\n1 1 1 1\n0 0 100\n0 0 0\n0 0 0 1\n0 0 0\nmotion constant_velocity\n";
+    }
+    return f ? 0 : -1;
+}
+}  // namespace nbc
+
 extern "C" void nbc_free_bed(nbc_bed_out* o) {
     if (!o) return;
     void* p[] = {o->clump_id, o->pos, o->local, o->mass, o->radius, o->cpos, o->cvel, o->cquat, o->cangv, o->cprop};
@@ -569,27 +599,9 @@ extern "C" int nbc_save_bed_text(const char* dir, const nbc_bed_out* b, const nb
         else
             f << "1 1\n0 0 1\n1 1 1 1\n0 0 0\n0 0 0\n0 0 0 1\n0 0 0\n";
     }
-    {
-        std::ofstream f(rocks + "/Penetrator.bt");
-        f << std::setprecision(17);
-        if (pene) {
-            if (pene->shape == NBC_PENE_SPHERE) f << "sphere " << pene->radius << "\n";
-            else if (pene->shape == NBC_PENE_CONE)  // the mesh holds two triangles per rim segment
-                f << "cone " << pene->radius << " " << pene->height << " " << (pene->ntri > 0 ? pene->ntri / 2 : 64)
-                  << "\n";
-            else f << "cylinder " << pene->radius << " " << pene->height << "\n" << pene->dir[0] << " " << pene->dir[1]
-                   << " " << pene->dir[2] << "\n";
-            f << pene->prop[0] << " " << pene->prop[1] << " " << pene->prop[2] << " " << pene->prop[3] << "\n"
-              << pene->pos[0] << " " << pene->pos[1] << " " << pene->pos[2] << "\n" << pene->vel[0] << " "
-              << pene->vel[1] << " " << pene->vel[2] << "\n" << pene->quat[0] << " " << pene->quat[1] << " "
-              << pene->quat[2] << " " << pene->quat[3] << "\n" << pene->angv[0] << " " << pene->angv[1] << " "
-              << pene->angv[2] << "\n" << "motion "
-              << (pene->motion == NBC_PENE_GRAVITY ? (pene->wait_for_contact ? "gravity_after_contact" : "gravity")
-                                                   : "constant_velocity")
-              << "\n";
-        } else {
-            f << "sphere 0.001\n1 1 1 1\n0 0 100\n0 0 0\n0 0 0 1\n0 0 0\nmotion constant_velocity\n";
-        }
+    if (nbc::write_penetrator_text(rocks + "/Penetrator.bt", pene)) {
+        nbc::set_global_error("cannot write " + rocks + "/Penetrator.bt");
+        return NBC_E_IO;
     }
     {
         std::ofstream f(d + "/nbc.par");

@@ -146,6 +146,7 @@ _SIGS = {
     "nbc_load_cube_wall": ([C.c_char_p, C.POINTER(Wall)], C.c_int),
     "nbc_load_cylinder_wall": ([C.c_char_p, C.POINTER(Wall)], C.c_int),
     "nbc_load_penetrator": ([C.c_char_p, C.POINTER(Penetrator)], C.c_int),
+    "nbc_resume_from_bin": ([C.c_char_p, C.c_int32, C.c_char_p], C.c_int),
     "nbc_free_mesh": ([C.POINTER(Penetrator)], None),
     "nbc_make_cone_mesh": ([C.c_double, C.c_double, I32, P], C.c_int),
     "nbc_generate_bed": ([C.POINTER(BedSpec), C.POINTER(BedOut)], C.c_int),

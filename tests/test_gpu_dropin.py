@@ -38,22 +38,23 @@ def read_ph(path):
     return n, ints[1], ints[2], pos
 
 
-def oracle_for_fixture(steps):
+def oracle_for_fixture(steps, d=FIX):
+    d = str(d)
     p = nbc.Params()
     ini = C.create_string_buffer(256)
-    assert nbc.lib().nbc_load_world_par(os.path.join(FIX, "world.par").encode(), C.byref(p), ini, 256) == 0
+    assert nbc.lib().nbc_load_world_par(os.path.join(d, "world.par").encode(), C.byref(p), ini, 256) == 0
     n, nc = p.ParticleNum, p.ClumpNum
     cid = np.zeros(n, np.int32)
     pos, loc, m, r = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
-    assert nbc.lib().nbc_load_phase(os.path.join(FIX, "ph.00000000.bt").encode(), n,
+    assert nbc.lib().nbc_load_phase(os.path.join(d, ini.value.decode()).encode(), n,
                                     *[nbc.ptr(a) for a in (cid, pos, loc, m, r)]) == 0
     cp, cv, cq, cw, pr = np.zeros((nc, 3)), np.zeros((nc, 3)), np.zeros((nc, 4)), np.zeros((nc, 3)), np.zeros((nc, 4))
-    assert nbc.lib().nbc_load_clumps(os.path.join(FIX, "ch.00000000.bt").encode(), nc,
+    assert nbc.lib().nbc_load_clumps(os.path.join(d, "ch.00000000.bt").encode(), nc,
                                      *[nbc.ptr(a) for a in (cp, cv, cq, cw, pr)]) == 0
     wall = nbc.Wall()
-    assert nbc.lib().nbc_load_cube_wall(os.path.join(FIX, "Rocks", "CubeWall.bt").encode(), C.byref(wall)) == 0
+    assert nbc.lib().nbc_load_cube_wall(os.path.join(d, "Rocks", "CubeWall.bt").encode(), C.byref(wall)) == 0
     pene = nbc.Penetrator()
-    assert nbc.lib().nbc_load_penetrator(os.path.join(FIX, "Rocks", "Penetrator.bt").encode(), C.byref(pene)) == 0
+    assert nbc.lib().nbc_load_penetrator(os.path.join(d, "Rocks", "Penetrator.bt").encode(), C.byref(pene)) == 0
     bed = nbc.Bed(p, cid, pos, loc, m, r, cp, cv, cq, cw, pr, wall)
     o = orc.Oracle(orc.finalize_params(bed))
     o.upload(bed, wall, pene)
@@ -122,3 +123,33 @@ def test_mixed_precision_run_and_overflow_error(tmp_path):
     rc = nbc.lib().nbc_run_simulation_clump(str(d2).encode(), 0)
     assert rc == nbc.NBC_E_NEIGHBOR_OVERFLOW
     assert "Neighbor list overflow (BranchLmtLen=2)" in nbc.lib().nbc_last_error(None).decode()
+
+
+def test_resume_from_binary_outputs(tmp_path):
+    """A run continued from its own binary outputs: the shipped inputs run 150
+    steps, nbc_resume_from_bin turns the step-150 outputs into a run
+    directory (StartStep 150), which runs to step 300; its step-300 output
+    equals the FP64 oracle started from the same converted files (contact
+    histories start from zero there, as the outputs do not hold them)."""
+    print("[case] resume from the step-150 outputs of the shipped inputs", flush=True)
+    d = prepare(tmp_path, 150, 150)
+    assert nbc.lib().nbc_run_simulation_clump(str(d).encode(), 0) == 0, nbc.lib().nbc_last_error(None)
+    r = tmp_path / "resumed"
+    assert nbc.lib().nbc_resume_from_bin(str(d).encode(), 150, str(r).encode()) == 0, nbc.lib().nbc_last_error(None)
+    txt = (r / "world.par").read_text()
+    assert "StartStep = 150" in txt
+    (r / "world.par").write_text(txt.replace("EndStep\t\t= 150", "EndStep\t\t= 300"))
+    assert nbc.lib().nbc_run_simulation_clump(str(r).encode(), 0) == 0, nbc.lib().nbc_last_error(None)
+    assert not (r / "OutPart/ph.00000150.bin").exists() and (r / "OutPart/ph.00000300.bin").exists()
+    # the resumed run starts where the first one stopped
+    n0, fid0, _, pos0 = read_ph(d / "OutPart/ph.00000150.bin")
+    o0, bed0 = oracle_for_fixture(0, r)
+    x0 = np.zeros_like(pos0)
+    x0[fid0] = pos0
+    assert np.array_equal(bed0.pos, x0)
+    o0.close()
+    n, fid, cid, pos = read_ph(r / "OutPart/ph.00000300.bin")
+    o, bed = oracle_for_fixture(150, r)
+    so = o.spheres()
+    assert np.array_equal(fid, so["fid"]) and np.array_equal(cid, so["cid"])
+    assert np.abs(pos - so["pos"]).max() <= 1e-12

@@ -360,3 +360,73 @@ def test_generator_cli_writes_a_run_directory(tmp_path):
     nbc.lib().nbc_free_mesh(C.byref(pe))
     r2 = subprocess.run([exe, "--kind", "1", "--clumps", "150", "--out", str(d)], capture_output=True, text=True)
     assert r2.returncode == 1 and "refusing to overwrite" in r2.stderr
+
+
+def test_resume_from_binary_outputs(tmp_path):
+    """nbc_resume_from_bin turns the binary outputs of a step (sorted-order
+    ph with file indices, ch with the penetrator record first) back into a
+    run directory in the text formats: rows in file order, bit-exact values,
+    StartStep = step, the penetrator at its recorded state; it refuses to
+    overwrite and rejects a truncated output."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(nbc.LIB_PATH), "nbc_generate")
+    run = tmp_path / "run"
+    r = subprocess.run([exe, "--kind", "1", "--clumps", "120", "--seed", "4", "--cone", "0.004", "0.0069", "12",
+                        "0.1", "--out", str(run)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    bed = nbc.generate_bed(1, 120, seed=4)
+    n, nc = bed.n, bed.nc
+    rng = np.random.default_rng(1)
+    order = rng.permutation(n).astype(np.int32)  # sorted position k holds file sphere order[k]
+    pos = bed.pos + rng.normal(0, 1e-4, bed.pos.shape)
+    cvel = rng.normal(0, 0.1, (nc, 3))
+    cangv = rng.normal(0, 10.0, (nc, 3))
+    cpos = bed.cpos + 1e-4
+    ppos, pvel = np.array([0.001, -0.002, 0.03]), np.array([0.0, 0.0, -0.1])
+    pquat, pangv = np.array([0.0, 0.0, 0.0, 1.0]), np.zeros(3)
+    step = 40
+    sid = np.arange(n, dtype=np.int32)
+    cid = np.ascontiguousarray(bed.clump_id[order])
+    args = [np.ascontiguousarray(a) for a in (pos[order], bed.local[order], bed.mass[order], bed.radius[order])]
+    assert nbc.lib().nbc_write_particles_bin(str(run / "OutPart").encode(), step, n, nbc.ptr(sid), nbc.ptr(order),
+                                             nbc.ptr(cid), *[nbc.ptr(a) for a in args]) == 0
+    assert nbc.lib().nbc_write_clumps_bin(str(run / "OutClump").encode(), step, nc,
+                                          *[nbc.ptr(np.ascontiguousarray(a)) for a in
+                                            (cpos, cvel, bed.cquat, cangv, bed.cprop, ppos, pvel, pquat, pangv)]) == 0
+    out = tmp_path / "resume"
+    assert nbc.lib().nbc_resume_from_bin(str(run).encode(), step, str(out).encode()) == 0
+    p = nbc.Params()
+    ini = C.create_string_buffer(256)
+    assert nbc.lib().nbc_load_world_par(str(out / "world.par").encode(), C.byref(p), ini, 256) == 0
+    q = nbc.Params()
+    assert nbc.lib().nbc_load_world_par(str(run / "world.par").encode(), C.byref(q), ini, 256) == 0
+    assert p.StartStep == step and p.ParticleNum == n and p.StepSize == q.StepSize and p.Thresh == q.Thresh
+    c2 = np.zeros(n, np.int32)
+    x2, l2, m2, r2 = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+    assert nbc.lib().nbc_load_phase(str(out / ini.value.decode()).encode(), n,
+                                    *[nbc.ptr(a) for a in (c2, x2, l2, m2, r2)]) == 0
+    assert np.array_equal(c2, bed.clump_id) and np.array_equal(x2, pos) and np.array_equal(l2, bed.local)
+    assert np.array_equal(m2, bed.mass) and np.array_equal(r2, bed.radius)
+    cp, cv, cq, cw, cr = np.zeros((nc, 3)), np.zeros((nc, 3)), np.zeros((nc, 4)), np.zeros((nc, 3)), np.zeros((nc, 4))
+    assert nbc.lib().nbc_load_clumps(str(out / "ch.00000000.bt").encode(), nc,
+                                     *[nbc.ptr(a) for a in (cp, cv, cq, cw, cr)]) == 0
+    assert np.array_equal(cp, cpos) and np.array_equal(cv, cvel) and np.array_equal(cw, cangv)
+    assert np.array_equal(cq, bed.cquat) and np.array_equal(cr, bed.cprop)
+    pe, pe0 = nbc.Penetrator(), nbc.Penetrator()
+    assert nbc.lib().nbc_load_penetrator(str(out / "Rocks" / "Penetrator.bt").encode(), C.byref(pe)) == 0
+    assert nbc.lib().nbc_load_penetrator(str(run / "Rocks" / "Penetrator.bt").encode(), C.byref(pe0)) == 0
+    assert list(pe.pos) == ppos.tolist() and list(pe.vel) == pvel.tolist()
+    assert (pe.shape, pe.radius, pe.height, pe.ntri, pe.motion) == (pe0.shape, pe0.radius, pe0.height, pe0.ntri,
+                                                                     pe0.motion)
+    assert list(pe.prop) == list(pe0.prop)
+    nbc.lib().nbc_free_mesh(C.byref(pe))
+    nbc.lib().nbc_free_mesh(C.byref(pe0))
+    assert (out / "Rocks" / "CylindWall.bt").read_bytes() == (run / "Rocks" / "CylindWall.bt").read_bytes()
+    assert (out / "nbc.par").read_bytes() == (run / "nbc.par").read_bytes()
+    # refuses to overwrite; a truncated output is an I/O error
+    assert nbc.lib().nbc_resume_from_bin(str(run).encode(), step, str(out).encode()) == nbc.NBC_E_IO
+    f = run / "OutPart" / "ph.00000040.bin"
+    f.write_bytes(f.read_bytes()[:-8])
+    assert nbc.lib().nbc_resume_from_bin(str(run).encode(), step, str(tmp_path / "r2").encode()) == nbc.NBC_E_IO
+    assert "truncated" in nbc.lib().nbc_last_error(None).decode()
