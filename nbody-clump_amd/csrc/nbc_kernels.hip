@@ -2344,57 +2344,92 @@ __device__ __forceinline__ void finish_listed(const DevParams& P, const SphereBu
     }
 }
 
+// The member records of a wave's 64 clumps are one contiguous run of member
+// slots (members are sorted by clump). The wave reads the run in chunks of
+// K4Chunk records with coalesced loads (lane k takes record base + k, then
+// base + 64 + k, ...), finishes the wall-list records among them and leaves
+// the chunk in LDS; each lane then adds its clump's records of the chunk in
+// slot order, carrying the sums from chunk to chunk. The addition order is
+// the per-lane one (first member starts the sum, the rest in slot order).
+// Launched with one wave per block.
+template <class R>
+struct K4Chunk {
+    static constexpr int value = sizeof(R) == sizeof(double) ? 192 : 384;  // 12 KB of LDS either way
+};
+
 template <class R, bool NEXT>
-__global__ __launch_bounds__(256) void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, RecordBuf rec, int record,
+__global__ __launch_bounds__(64) void k_clump_after(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, WallBuf<R> WL, RecordBuf rec, int record,
                               const int* mbeg, const int* mend, const int* mem, int* halt, int n_arg, PeneState* pst,
                               unsigned long long* counters) {
+    typedef typename Vec4T<R>::type R4;
+    constexpr int CH = K4Chunk<R>::value;
+    __shared__ R4 s_rec[CH][2];
     const int n = step_of(P, n_arg);
-    if (*halt <= n) return;
+    if (*halt <= n) return;  // the same value for the whole wave
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = lane_id();
     // NEXT: K1 of step n + 1 follows in the same thread (it returns when K4 of
     // step n does, at a halt at or before n)
     if (NEXT && c == 0) k1_thread0(P, pst, counters, halt, n + 1);
-    if (c >= P.Nc) return;
-    const int b = mbeg[c], e = mend[c];
+    const bool valid = c < P.Nc;
+    const int b = valid ? mbeg[c] : -1, e = valid ? mend[c] : -1;
+    const bool has = b >= 0 && e > b;
     // clump state first: independent of the member sums
-    const D4 pr = C.prop[c];
-    const D4 vr = C.vw[2 * c], wr = C.vw[2 * c + 1];
-    D4 pc, qc;
-    if (NEXT) { pc = C.pq[2 * c]; qc = C.pq[2 * c + 1]; }
-    D4 Fr, Tr;
-    D4 F1r, T1r;  // what the next K1 reads (C.ft)
-    if (b >= 0 && e > b) {
-        // members in ascending sorted order; k_contacts wrote each sphere's
-        // totals at its member slot, so the clump's records are contiguous;
-        // the first member starts the sums, the rest are added in order
-        typedef typename Vec4T<R>::type R4;
-        constexpr int U = sizeof(R) == sizeof(double) ? 2 : 4;  // member records in flight
-        V3<double> F = mk<double>(0.0, 0.0, 0.0), T = F;
-        for (int k0 = b; k0 < e; k0 += U) {
-            R4 f[U], t[U];
+    D4 pr, vr, wr, pc, qc;
+    if (valid) {
+        pr = C.prop[c];
+        vr = C.vw[2 * c]; wr = C.vw[2 * c + 1];
+        if (NEXT) { pc = C.pq[2 * c]; qc = C.pq[2 * c + 1]; }
+    }
+    // the wave's member run [r0, r1)
+    int r0 = has ? b : 0x7fffffff, r1 = has ? e : -1;
+    for (int off = 32; off > 0; off >>= 1) {
+        r0 = min(r0, __shfl_xor(r0, off));
+        r1 = max(r1, __shfl_xor(r1, off));
+    }
+    V3<double> Fs = mk<double>(0.0, 0.0, 0.0), Ts = Fs;
+    for (int base = r0; base < r1; base += CH) {
+        const int cnt = min(CH, r1 - base);
+        R4 fr[CH / 64], tr[CH / 64];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (k0 + u < e) {
-                    f[u] = S.sft[2 * (size_t)(k0 + u)];
-                    t[u] = S.sft[2 * (size_t)(k0 + u) + 1];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (k0 + u >= e) continue;
-                if (t[u].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, k0 + u, mem, c, f[u], t[u]);
-                const V3<double> fu = mk<double>(f[u].x, f[u].y, f[u].z), tu = mk<double>(t[u].x, t[u].y, t[u].z);
-                if (k0 + u == b) { F = fu; T = tu; }
-                else { F = F + fu; T = T + tu; }
+        for (int r = 0; r < CH / 64; ++r) {
+            const int k = r * 64 + lane;
+            if (k < cnt) {
+                fr[r] = S.sft[2 * (size_t)(base + k)];
+                tr[r] = S.sft[2 * (size_t)(base + k) + 1];
             }
         }
-        Fr.x = F.x; Fr.y = F.y; Fr.z = F.z; Fr.w = 0.0;
-        Tr.x = T.x; Tr.y = T.y; Tr.z = T.z; Tr.w = 0.0;
+#pragma unroll
+        for (int r = 0; r < CH / 64; ++r) {
+            const int k = r * 64 + lane;
+            if (k < cnt) {
+                if (tr[r].w != R(0)) finish_listed<R>(P, S, C, WL, rec, record, base + k, mem, 0, fr[r], tr[r]);
+                s_rec[k][0] = fr[r];
+                s_rec[k][1] = tr[r];
+            }
+        }
+        __syncthreads();
+        if (has) {
+            const int lo = max(b, base), hi = min(e, base + cnt);
+            for (int k = lo; k < hi; ++k) {
+                const R4 f = s_rec[k - base][0], t = s_rec[k - base][1];
+                const V3<double> fu = mk<double>(f.x, f.y, f.z), tu = mk<double>(t.x, t.y, t.z);
+                if (k == b) { Fs = fu; Ts = tu; }
+                else { Fs = Fs + fu; Ts = Ts + tu; }
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    D4 Fr, Tr;
+    D4 F1r, T1r;  // what the next K1 reads (C.ft)
+    if (has) {
+        Fr.x = Fs.x; Fr.y = Fs.y; Fr.z = Fs.z; Fr.w = 0.0;
+        Tr.x = Ts.x; Tr.y = Ts.y; Tr.z = Ts.z; Tr.w = 0.0;
         if (P.driver && P.pene_on) {
             // DEMmain loops: the next drift's half-kick and the attitude leave
             // the penetrator term out (src/dem_integrate.cu:23-25, 74-75); the
             // clump is its one sphere, whose contact (+ wall) totals these are
-            typedef typename Vec4T<R>::type R4;
             const R4 f0 = S.sft[2 * (size_t)b], t0 = S.sft[2 * (size_t)b + 1];
             V3<R> a = Ld4<R>::xyz(f0), q = Ld4<R>::xyz(t0);
             if (t0.w != R(0)) {
