@@ -1616,6 +1616,21 @@ __device__ __forceinline__ bool pene_candidate(const DevParams& P, const PeneSta
     return norm < (P.prad + 0.80 * LatDx) + s2;
 }
 
+// Queue entry of the contact kernel: the other sphere's FP64 position as the
+// scan gathered it, overwritten by the lane that evaluates the pair with its
+// six result values (force, torque).
+template <class R> struct QEnt;
+template <> struct alignas(16) QEnt<float> { float v[8]; };
+template <> struct alignas(16) QEnt<double> { double v[6]; };
+template <class R>
+__device__ __forceinline__ void qent_put_pos(QEnt<R>& q, const D4& p) { __builtin_memcpy(q.v, &p, sizeof(D4)); }
+template <class R>
+__device__ __forceinline__ D4 qent_pos(const QEnt<R>& q) {
+    D4 p;
+    __builtin_memcpy(&p, q.v, sizeof(D4));
+    return p;
+}
+
 // K3: sphere-sphere contacts of 64 spheres per wave (src/dem_force_clump.cu:3-191),
 // then each sphere's totals with its wall and penetrator terms
 // (src/dem_clump.cu:89-101) and the clump torque term T + (x_s - x_c) x F
@@ -1643,7 +1658,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     __shared__ int q_e[4][Q];    // front near index of the pair; -1 once found not touching
     __shared__ int q_oth[4][Q];  // the other sphere
     __shared__ int q_key[4][Q];  // owner lane, +64 for back entries (non-decreasing)
-    __shared__ R q_res[4][Q][6];
+    __shared__ QEnt<R> q_ent[4][Q];  // other sphere's position, then the pair's results
     __shared__ int s_fo[4][65], s_bo[4][65];
     __shared__ F4 s_p32[4][64];
     __shared__ unsigned int s_cnt[4];
@@ -1684,7 +1699,9 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                 const int oth = q_oth[wv][k];
                 const int e = q_e[wv][k];
                 const int I = side ? oth : me, J = side ? me : oth;
-                const D4 pI = S.pos[I], pJ = S.pos[J];
+                // the other sphere's position comes from the scan's gather
+                const D4 pM = S.pos[me], pO = qent_pos<R>(q_ent[wv][k]);
+                const D4 pI = side ? pO : pM, pJ = side ? pM : pO;
                 const uint32_t w = L.nfw[e];
                 const V3<double> Dd = xyz(pJ) - xyz(pI);
                 const double DistS = Dd.x * Dd.x + Dd.y * Dd.y + Dd.z * Dd.z;
@@ -1732,7 +1749,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                         f = R(-1.0) * o.cfc;  // x - y == x + (-y) exactly: same sums as subtracting
                         t = o.ctc2;
                     }
-                    R* rr = q_res[wv][k];
+                    R* rr = q_ent[wv][k].v;
                     rr[0] = f.x; rr[1] = f.y; rr[2] = f.z; rr[3] = t.x; rr[4] = t.y; rr[5] = t.z;
                 }
             }
@@ -1749,13 +1766,13 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
         const int f0 = lbound(lane), f1 = lbound(lane + 1), b0 = lbound(lane + 64), b1 = lbound(lane + 65);
         for (int k = f0; k < f1; ++k) {
             if (q_e[wv][k] < 0) continue;
-            const R* rr = q_res[wv][k];
+            const R* rr = q_ent[wv][k].v;
             Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
             Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
         }
         for (int k = b0; k < b1; ++k) {
             if (q_e[wv][k] < 0) continue;
-            const R* rr = q_res[wv][k];
+            const R* rr = q_ent[wv][k].v;
             Fss = Fss + mk<R>(rr[0], rr[1], rr[2]);
             Tss = Tss + mk<R>(rr[3], rr[4], rr[5]);
         }
@@ -1763,13 +1780,14 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
         __builtin_amdgcn_wave_barrier();
         qn = 0;
     };
-    auto push = [&](bool cand, int key, int e, int oth) __attribute__((always_inline)) {
+    auto push = [&](bool cand, int key, int e, int oth, const D4& po) __attribute__((always_inline)) {
         const uint64_t m = __ballot(cand);
         if (cand) {
             const int pos = qn + __popcll(m & lanemask_lt());
             q_e[wv][pos] = e;
             q_oth[wv][pos] = oth;
             q_key[wv][pos] = key;
+            qent_put_pos<R>(q_ent[wv][pos], po);
         }
         qn += __popcll(m);
         if (qn > Q - 64) flush();
@@ -1792,6 +1810,8 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             if (e + 64 < F1) wnext = __builtin_nontemporal_load(L.nfw + e + 64);
             bool cand = false;
             int r = 0, j = 0;
+            D4 pj;
+            pj.x = pj.y = pj.z = pj.w = 0.0;
             if (e < F1) {
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
@@ -1799,11 +1819,11 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
                 j = (int)(w & JMASK);
                 // the partner's FP32 copy is made here from its FP64 position: a
                 // gather fetches the same DRAM sector for 16 or 32 bytes
-                const D4 pj = S.pos[j];
+                pj = S.pos[j];
                 cand = maybe_touch(P, s_p32[wv][r], pos32(P, xyz(pj), pj.w));
                 if (!cand && (w & VALID(p ^ 1))) L.nfw[e] = w & ~VALID(p ^ 1);
             }
-            push(cand, r, e, j);
+            push(cand, r, e, j, pj);
         }
     }
     // back entries: the owner holds the history; the same pair is evaluated
@@ -1818,14 +1838,16 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             }
             bool cand = false;
             int r = 0;
+            D4 po;
+            po.x = po.y = po.z = po.w = 0.0;
             if (e < B1) {
 #pragma unroll
                 for (int st = 32; st > 0; st >>= 1)
                     if (s_bo[wv][r + st] <= e) r += st;
-                const D4 po = S.pos[o];
+                po = S.pos[o];
                 cand = maybe_touch(P, s_p32[wv][r], pos32(P, xyz(po), po.w));
             }
-            push(cand, r + 64, ne, o);
+            push(cand, r + 64, ne, o, po);
         }
     }
     if (qn > 0) flush();
