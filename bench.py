@@ -145,7 +145,7 @@ def run_c2(a):
                         k_contacts_us=k3_ms / max(k3_n, 1) * 1e3, k_sphere_pose_us=k2_ms / max(k2_n, 1) * 1e3,
                         k_clump_after_us=k4_ms / max(k4_n, 1) * 1e3, rebuilds_in_steps=s1.rebuilds - s0.rebuilds,
                         rebuild_ms=(r_ms - r_ms0) / max(r_n - r_n0, 1), touching_ss_per_sphere=s1.touching_ss / bed.n,
-                        list_pairs_per_sphere=s1.front_pairs / bed.n)
+                        list_pairs_per_sphere=s1.front_pairs / bed.n, kinetic_energy_J=s1.kinetic_energy)
         print(f"[bench] C2 {key}: {out[key]}", file=sys.stderr, flush=True)
         ctx.close()
     h = out.get("hgrid3_nested") or next(iter(out.values()))
