@@ -626,6 +626,49 @@ __global__ void k_remap_cid(int N, int* cid, const int* newid) {
 // counts, FILL = true writes the CSR rows: front j > i in discovery order,
 // back j < i in reverse discovery order (the gather order of
 // src/dem_force_clump.cu:183-188).
+// Neighbour search over the 27 cells of (cix, ciy, ciz), z outer, x inner
+// (src/dem_grid.cu:466-552). The three cells of one (z, y) row have
+// consecutive keys, so their spheres are one contiguous run of the sorted
+// order: each row is walked as the run [s0, s1) in ascending index, which is
+// the reference's cell-by-cell order, with four candidate positions loaded
+// ahead. visit(j, pj) is called for every j != i within 1.5 (ri + rj).
+template <class Visit>
+__device__ __forceinline__ void search_27(const D4* pos, const int* cbeg, const int* cend, const uint32_t* G,
+                                          int coff, int i, const D4& pi, int cix, int ciy, int ciz, Visit visit) {
+    const double ri = pi.w;
+    const int jx0 = max(cix - 1, 0), jx1 = min(cix + 1, (int)G[0] - 1);
+    if (jx0 > jx1) return;
+    for (int cz = -1; cz <= 1; cz++)
+        for (int cy = -1; cy <= 1; cy++) {
+            const int jy = ciy + cy, jz = ciz + cz;
+            if (jy < 0 || (uint32_t)jy >= G[1] || jz < 0 || (uint32_t)jz >= G[2]) continue;
+            const int rowc = coff + (int)(((uint32_t)jz * G[1] + (uint32_t)jy) * G[0]);
+            int s0 = -1, s1 = -1;
+            for (int jx = jx0; jx <= jx1; ++jx) {
+                const int b = cbeg[rowc + jx];
+                if (b == -1) continue;
+                if (s0 < 0) s0 = b;
+                s1 = cend[rowc + jx];
+            }
+            if (s0 < 0) continue;
+            for (int j0 = s0; j0 < s1; j0 += 4) {
+                D4 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) q[u] = pos[min(j0 + u, s1 - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u;
+                    if (j >= s1 || j == i) continue;
+                    const D4 pj = q[u];
+                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const double ds = dx * dx + dy * dy + dz * dz;
+                    const double lim = 1.50 * (ri + pj.w);
+                    if (ds < lim * lim) visit(j);
+                }
+            }
+        }
+}
+
 template <bool FILL>
 __global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, const int* cbeg, const int* cend,
                             const int* cid, int* fcnt, int* bcnt, const int* foff, const int* boff, uint32_t* fword,
@@ -649,41 +692,21 @@ __global__ void k_neighbors(DevParams P, const D4* pos, const uint32_t* cell, co
         return;
     }
     const D4 pi = pos[i];
-    const double ri = pi.w;
     const int cix = (int)floor((pi.x - O[0]) * ic[0]);
     const int ciy = (int)floor((pi.y - O[1]) * ic[1]);
     const int ciz = (int)floor((pi.z - O[2]) * ic[2]);
     int ng = 0, ni = 0;
     int fb = 0, bb = 0, nbi = 0, ci = 0;
     if (FILL) { fb = foff[i]; bb = boff[i]; nbi = bcnt[i]; ci = cid[i]; }
-    for (int cz = -1; cz <= 1; cz++)
-        for (int cy = -1; cy <= 1; cy++)
-            for (int cx = -1; cx <= 1; cx++) {
-                const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
-                if (jx < 0 || (uint32_t)jx >= G[0] || jy < 0 || (uint32_t)jy >= G[1] || jz < 0 ||
-                    (uint32_t)jz >= G[2])
-                    continue;
-                const int cidx = coff + (int)(((uint32_t)jz * G[1] + (uint32_t)jy) * G[0] + (uint32_t)jx);
-                const int start = cbeg[cidx];
-                if (start == -1) continue;
-                const int end = cend[cidx];
-                for (int j = start; j < end; j++) {
-                    if (j == i) continue;
-                    const D4 pj = pos[j];
-                    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                    const double ds = dx * dx + dy * dy + dz * dz;
-                    const double lim = 1.50 * (ri + pj.w);
-                    if (ds < lim * lim) {
-                        if (i < j) {
-                            if (FILL && ng < half) fword[fb + ng] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
-                            ng++;
-                        } else {
-                            if (FILL && ni < half) bent[bb + (nbi - 1 - ni)] = make_int2(j, -1);
-                            ni++;
-                        }
-                    }
-                }
-            }
+    search_27(pos, cbeg, cend, G, coff, i, pi, cix, ciy, ciz, [&](int j) {
+        if (i < j) {
+            if (FILL && ng < half) fword[fb + ng] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+            ng++;
+        } else {
+            if (FILL && ni < half) bent[bb + (nbi - 1 - ni)] = make_int2(j, -1);
+            ni++;
+        }
+    });
     if (!FILL) {
         fcnt[i] = ng;
         bcnt[i] = ni;
@@ -821,39 +844,19 @@ __global__ void k_neighbors_ell(DevParams P, const D4* pos, const uint32_t* cell
         }
         if ((int)cell[i] != last) {
             const D4 pi = pos[i];
-            const double ri = pi.w;
             const int ci = cid[i];
             const int cix = (int)floor((pi.x - O[0]) * ic[0]);
             const int ciy = (int)floor((pi.y - O[1]) * ic[1]);
             const int ciz = (int)floor((pi.z - O[2]) * ic[2]);
-            for (int cz = -1; cz <= 1; cz++)
-                for (int cy = -1; cy <= 1; cy++)
-                    for (int cx = -1; cx <= 1; cx++) {
-                        const int jx = cix + cx, jy = ciy + cy, jz = ciz + cz;
-                        if (jx < 0 || (uint32_t)jx >= G[0] || jy < 0 || (uint32_t)jy >= G[1] || jz < 0 ||
-                            (uint32_t)jz >= G[2])
-                            continue;
-                        const int cidx = coff + (int)(((uint32_t)jz * G[1] + (uint32_t)jy) * G[0] + (uint32_t)jx);
-                        const int start = cbeg[cidx];
-                        if (start == -1) continue;
-                        const int end = cend[cidx];
-                        for (int j = start; j < end; j++) {
-                            if (j == i) continue;
-                            const D4 pj = pos[j];
-                            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                            const double ds = dx * dx + dy * dy + dz * dz;
-                            const double lim = 1.50 * (ri + pj.w);
-                            if (ds < lim * lim) {
-                                if (i < j) {
-                                    if (ng < K) ef[(size_t)ng * P.N + i] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
-                                    ng++;
-                                } else {
-                                    if (ni < K) eb[(size_t)ni * P.N + i] = j;
-                                    ni++;
-                                }
-                            }
-                        }
-                    }
+            search_27(pos, cbeg, cend, G, coff, i, pi, cix, ciy, ciz, [&](int j) {
+                if (i < j) {
+                    if (ng < K) ef[(size_t)ng * P.N + i] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+                    ng++;
+                } else {
+                    if (ni < K) eb[(size_t)ni * P.N + i] = j;
+                    ni++;
+                }
+            });
         }
         fcnt[i] = ng;
         bcnt[i] = ni;
