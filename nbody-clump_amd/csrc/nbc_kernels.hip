@@ -260,7 +260,27 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
     if (*halt < n) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false, near = false;
-    if (i < P.N) {
+    if (i < P.N && ref_disp<R>(P.driver)) {
+        // FP32-mixed clump loop: the loads that do not need the clump index
+        // are issued with it, so a wave waits for one gather level only
+        const int c = S.cid[i];
+        const typename Vec4T<R>::type lo = S.loc[i];
+        const D4 cu = S.cum[i];  // x_rebuild, radius
+        const float th = S.nthr[i];
+        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the wait for c
+        const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
+        double M[3][3];
+        QuatIvDCM<double>(q.x, q.y, q.z, q.w, M);
+        const V3<double> arm = Map<double>(M, mk<double>((double)lo.x, (double)lo.y, (double)lo.z));
+        const V3<double> up = xyz(pc) + arm;
+        // displacement = x - x_rebuild
+        const V3<double> cum = up - xyz(cu);
+        over = Length<double>(cum) > P.Thresh;
+        near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > (double)th;
+        D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = cu.w;
+        S.pos[i] = pn;
+        S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
+    } else if (i < P.N) {
         const int c = S.cid[i];
         const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
         const typename Vec4T<R>::type lo = S.loc[i];
@@ -269,33 +289,24 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         V3<double> arm = Map<double>(M, mk<double>((double)lo.x, (double)lo.y, (double)lo.z));
         V3<double> up = xyz(pc) + arm;
         const D4 cu = S.cum[i];
-        if (ref_disp<R>(P.driver)) {
-            // displacement = x - x_rebuild; cum holds x_rebuild and the radius
-            const V3<double> cum = up - xyz(cu);
-            over = Length<double>(cum) > P.Thresh;
-            near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > (double)S.nthr[i];
-            D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = cu.w;
-            S.pos[i] = pn;
+        const D4 po = S.pos[i];
+        V3<double> cum;
+        if (P.driver) {
+            const D4 dd = C.dist[c];
+            cum = xyz(cu) + xyz(dd);
         } else {
-            const D4 po = S.pos[i];
-            V3<double> cum;
-            if (P.driver) {
-                const D4 dd = C.dist[c];
-                cum = xyz(cu) + xyz(dd);
-            } else {
-                cum = xyz(cu) + up - xyz(po);
-            }
-            // glue loop: type 0 rebuilds past 0.1 r, the kinematic types never ask
-            // (src/dem_integrate_withglue.cu:40-43)
-            if (P.driver == NBC_DRIVER_SPLIT) over = C.type[c] == 0 && Length<double>(cum) > 0.10 * po.w;
-            else over = Length<double>(cum) > P.Thresh;
-            // the sphere left its near-list allowance (threshold kept in cum.w)
-            near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > cu.w;
-            D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
-            D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = cu.w;
-            S.pos[i] = pn;
-            S.cum[i] = cn;
+            cum = xyz(cu) + up - xyz(po);
         }
+        // glue loop: type 0 rebuilds past 0.1 r, the kinematic types never ask
+        // (src/dem_integrate_withglue.cu:40-43)
+        if (P.driver == NBC_DRIVER_SPLIT) over = C.type[c] == 0 && Length<double>(cum) > 0.10 * po.w;
+        else over = Length<double>(cum) > P.Thresh;
+        // the sphere left its near-list allowance (threshold kept in cum.w)
+        near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > cu.w;
+        D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = po.w;
+        D4 cn; cn.x = cum.x; cn.y = cum.y; cn.z = cum.z; cn.w = cu.w;
+        S.pos[i] = pn;
+        S.cum[i] = cn;
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
     }
     // the rebuild request (src/dem_clump.cu:132-134) records the step index:
