@@ -2407,33 +2407,43 @@ __global__ __launch_bounds__(64) void k_clump_after(DevParams P, SphereBuf<R> S,
     // NEXT: K1 of step n + 1 follows in the same thread (it returns when K4 of
     // step n does, at a halt at or before n)
     if (NEXT && c == 0) k1_thread0(P, pst, counters, halt, n + 1);
+    // the wave's member run [r0, r1) from its first and last clump, requested
+    // first, so that the member records can be requested while the clump
+    // state below is still on its way (over the lanes when one of the two
+    // clumps has no members)
+    const int c0 = blockIdx.x * blockDim.x, cl = min(c0 + (int)blockDim.x, P.Nc) - 1;
+    int r0 = mbeg[c0], r1 = mend[cl];
     const bool valid = c < P.Nc;
-    const int b = valid ? mbeg[c] : -1, e = valid ? mend[c] : -1;
+    // clump state: independent of the member sums (loaded without a branch,
+    // from a clamped index, so that no wait for it precedes the member loads)
+    const int cc = valid ? c : P.Nc - 1;
+    const int bb = mbeg[cc], ee = mend[cc];
+    const D4 pr = C.prop[cc];
+    const D4 vr = C.vw[2 * cc], wr = C.vw[2 * cc + 1];
+    D4 pc, qc;
+    if (NEXT) { pc = C.pq[2 * cc]; qc = C.pq[2 * cc + 1]; }
+    __builtin_amdgcn_sched_barrier(0);
+    const int b = valid ? bb : -1, e = valid ? ee : -1;
     const bool has = b >= 0 && e > b;
-    // clump state first: independent of the member sums
-    D4 pr, vr, wr, pc, qc;
-    if (valid) {
-        pr = C.prop[c];
-        vr = C.vw[2 * c]; wr = C.vw[2 * c + 1];
-        if (NEXT) { pc = C.pq[2 * c]; qc = C.pq[2 * c + 1]; }
-    }
-    // the wave's member run [r0, r1)
-    int r0 = has ? b : 0x7fffffff, r1 = has ? e : -1;
-    for (int off = 32; off > 0; off >>= 1) {
-        r0 = min(r0, __shfl_xor(r0, off));
-        r1 = max(r1, __shfl_xor(r1, off));
+    if (r0 < 0 || r1 < 0) {
+        r0 = has ? b : 0x7fffffff;
+        r1 = has ? e : -1;
+        for (int off = 32; off > 0; off >>= 1) {
+            r0 = min(r0, __shfl_xor(r0, off));
+            r1 = max(r1, __shfl_xor(r1, off));
+        }
     }
     V3<double> Fs = mk<double>(0.0, 0.0, 0.0), Ts = Fs;
-    for (int base = r0; base < r1; base += CH) {
+    auto chunk = [&](int base) __attribute__((always_inline)) {
         const int cnt = min(CH, r1 - base);
         R4 fr[CH / 64], tr[CH / 64];
+        // clamped indices: the loads need no branch, so nothing waits for the
+        // clump state before they issue
 #pragma unroll
         for (int r = 0; r < CH / 64; ++r) {
-            const int k = r * 64 + lane;
-            if (k < cnt) {
-                fr[r] = S.sft[2 * (size_t)(base + k)];
-                tr[r] = S.sft[2 * (size_t)(base + k) + 1];
-            }
+            const size_t k = (size_t)min(base + r * 64 + lane, r1 - 1);
+            fr[r] = S.sft[2 * k];
+            tr[r] = S.sft[2 * k + 1];
         }
 #pragma unroll
         for (int r = 0; r < CH / 64; ++r) {
@@ -2455,6 +2465,10 @@ __global__ __launch_bounds__(64) void k_clump_after(DevParams P, SphereBuf<R> S,
             }
         }
         __syncthreads();
+    };
+    if (r0 < r1) {
+        chunk(r0);  // the first chunk outside the loop: straight-line after the loads above
+        for (int base = r0 + CH; base < r1; base += CH) chunk(base);
     }
     if (!valid) return;
     D4 Fr, Tr;
