@@ -282,14 +282,15 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
     } else if (i < P.N) {
         const int c = S.cid[i];
-        const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
         const typename Vec4T<R>::type lo = S.loc[i];
+        const D4 cu = S.cum[i];
+        const D4 po = S.pos[i];
+        __builtin_amdgcn_sched_barrier(0);  // these loads ahead of the wait for c
+        const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
         double M[3][3];
         QuatIvDCM<double>(q.x, q.y, q.z, q.w, M);
         V3<double> arm = Map<double>(M, mk<double>((double)lo.x, (double)lo.y, (double)lo.z));
         V3<double> up = xyz(pc) + arm;
-        const D4 cu = S.cum[i];
-        const D4 po = S.pos[i];
         V3<double> cum;
         if (P.driver) {
             const D4 dd = C.dist[c];
