@@ -163,7 +163,8 @@ typedef struct nbc_options {
                                 spheres), 1 on, -1 off */
     double pene_frcoe;       /* penetrator friction; <= 0 means Frcoe          */
     double cell_size;        /* > 0 overrides 3*max radius (requires >= 3*rmax) */
-    double near_skin;        /* near-list skin as a fraction of Thresh (0: 0.25; >= 1: gap 2*Thresh,
+    double near_skin;        /* fixed near-list skin as a fraction of Thresh (0: the
+                                per-sphere skin of near_horizon; >= 1: gap 2*Thresh,
                                 refreshed at rebuilds only) */
     int32_t driver;          /* NBC_DRIVER_* (0: the clump loop)                */
     int32_t level_from_type; /* HGRID driver: a particle's grid level is its type
@@ -174,7 +175,11 @@ typedef struct nbc_options {
     int32_t single_pass;     /* neighbour search: 0 a single pass into fixed-capacity
                                 rows when they fit (else the count and fill
                                 passes), -1 always the count and fill passes */
-    double reserved[1];
+    double near_horizon;     /* per-sphere near-list skin: the sphere's speed bound
+                                (|v| + |w| |x - x_clump| of its clump) times dt times
+                                this many steps, clamped to [0.03, 1] Thresh. 0: 500
+                                steps when near_skin is 0, else the fixed near_skin;
+                                < 0: the fixed near_skin (0.25 when 0) */
 } nbc_options;
 
 /* ---- boundaries -------------------------------------------------------- */

@@ -127,6 +127,8 @@ struct DevParams {
     double half;          // largest box half-extent
     float f32_marg;       // absolute slack of the FP32 distance pre-filter
     double near_delta;    // near-list skin: a sphere's slack is min(2a + delta, Thresh + a), a = |cum| at the build
+    double near_horizon;  // > 0: per-sphere delta = speed bound * dt * near_horizon, clamped to [near_lo, near_hi]
+    double near_lo, near_hi;
     double wl_slack;      // wall list: distance a sphere may move after the list is built (2 * Thresh)
     double pene_move;     // wall list: penetrator travel allowed before the list is rebuilt
     int wall_type;        // NBC_WALL_*

@@ -181,9 +181,17 @@ class Engine : public EngineBase {
         const bool thresh_ok = std::isfinite(P.Thresh) && P.Thresh >= 0.0;
         const double inf = std::numeric_limits<double>::infinity();
         // near-list skin delta (a fraction of Thresh): smaller lists for the
-        // contact kernel, refreshed when a sphere leaves its allowance
+        // contact kernel, refreshed when a sphere leaves its allowance. By
+        // default each sphere's delta is the distance its clump's speed bound
+        // covers in near_horizon steps: slow spheres keep short lists, fast
+        // ones ask for fewer refreshes.
         const double skin = O.near_skin > 0.0 ? std::min(O.near_skin, 1.0) : 0.25;
         dp.near_delta = thresh_ok ? skin * P.Thresh : inf;
+        dp.near_horizon = O.near_horizon > 0.0 ? O.near_horizon
+                          : (O.near_horizon == 0.0 && !(O.near_skin > 0.0)) ? 500.0 : 0.0;
+        if (!thresh_ok || !(std::isfinite(dp.near_horizon) && std::isfinite(dp.dt))) dp.near_horizon = 0.0;
+        dp.near_lo = 0.03 * P.Thresh;
+        dp.near_hi = P.Thresh;
         // wall list: spheres move at most 2 * Thresh after a list build; the
         // penetrator may move pene_move before the list is rebuilt
         dp.wl_slack = thresh_ok ? 2.0 * P.Thresh * (1.0 + 1e-9) + 1e-12 : inf;
@@ -192,6 +200,7 @@ class Engine : public EngineBase {
             // glue loop: rebuilds at 0.1 r, and driven spheres move without
             // asking for one: every list pair is near, every sphere wall-listed
             dp.near_delta = inf;
+            dp.near_horizon = 0.0;
             dp.wl_slack = inf;
         }
         dp.wall_type = wall_type; dp.nfaces = nfaces;
@@ -911,7 +920,7 @@ class Engine : public EngineBase {
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));
-        KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, Ln.foff, Ln.fword,
+        KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, s.cid, C.pq, C.vw, Ln.foff, Ln.fword,
                              boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2,
                              block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1030,7 +1039,7 @@ class Engine : public EngineBase {
         FrontList& L = Lf[lcur];
         KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, L.fword, hn[q], L.hist[q], q, block));
         SphereBuf<R>& s = S[scur];
-        KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, L.foff, L.fword,
+        KC(launch_near_build(st, dp, s.pos, s.cum, ref_disp<R>(driver) ? s.nthr : nullptr, nslack, s.cid, C.pq, C.vw, L.foff, L.fword,
                              boff, bent, fcnt, nfoff, nfw, nfs, slot2near, bcnt, nboff, nbo, nbe, stmp, stotal + 2,
                              block));
         HIPC(hipMemcpyAsync(&near_total, nfoff + N, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -153,9 +153,11 @@ int launch_level_keys(hipStream_t st, const DevParams& P, const D4* pos, const i
 int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* bcnt, const int* xbeg,
                         const int* xend, unsigned* overflow, int block);
 // Near lists of the current list at the current positions; also sets each
-// sphere's slack and its near-list request threshold (cum.w).
+// sphere's slack and its near-list request threshold (cum.w). cid, pq and vw
+// (clump index, clump position, clump velocities) give the per-sphere skin
+// when P.near_horizon > 0.
 int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, float* nthr, double* slack,
-                      const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
+                      const int* cid, const D4* pq, const D4* vw, const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
                       uint32_t* nfw, int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp,
                       int* stotal, int block);
 template <class R>

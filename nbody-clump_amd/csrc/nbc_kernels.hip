@@ -1024,7 +1024,8 @@ __global__ void k_hist_carry(long long nnew, long long nold, int p, uint32_t* nf
 // ---------------------------------------------------------------------------
 // Near lists: the contact kernel's view of the Verlet list. Each sphere gets a
 // slack at the near-list build: with a = |cum| (its displacement since the
-// last rebuild) and delta the near skin, slack = 2a + delta, and k_sphere_pose
+// last rebuild) and delta its near skin (fixed, or the distance its clump's
+// speed bound covers in a set number of steps), slack = 2a + delta, and k_sphere_pose
 // asks for a new near list (ST_NEAR) once |cum| exceeds a + delta, so the
 // sphere has moved at most |cum| + a <= slack since the build. When
 // Thresh + a is smaller, the slack is Thresh + a and no request is needed
@@ -1048,16 +1049,30 @@ __device__ __forceinline__ bool near_pair(const D4& a, const D4& b, uint32_t w, 
 // it asks for a new near list), written into cum.w, or with nthr given
 // (cum = position at the last rebuild) into nthr rounded down, so that the
 // request comes no later than with the exact threshold.
-__global__ void k_near_slack(DevParams P, const D4* pos, D4* cum, float* nthr, double* slack) {
+__global__ void k_near_slack(DevParams P, const D4* pos, D4* cum, float* nthr, double* slack, const int* cid,
+                             const D4* pq, const D4* vw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.N) return;
     D4 c = cum[i];
+    const D4 p = pos[i];
     if (nthr) {
-        const D4 p = pos[i];
         c.x = p.x - c.x; c.y = p.y - c.y; c.z = p.z - c.z;
     }
     const double a = sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
-    const double d = P.near_delta;
+    double d = P.near_delta;
+    if (P.near_horizon > 0.0) {
+        // the distance the clump's speed bound |v| + |w| |x - x_clump| covers
+        // in near_horizon steps; any d >= 0 keeps the slack and the request
+        // threshold consistent, so this only trades list length for refreshes
+        const int k = cid[i];
+        const D4 v = vw[2 * k], w = vw[2 * k + 1], o = pq[2 * k];
+        const double rx = p.x - o.x, ry = p.y - o.y, rz = p.z - o.z;
+        const double sp = sqrt(v.x * v.x + v.y * v.y + v.z * v.z) +
+                          sqrt(w.x * w.x + w.y * w.y + w.z * w.z) * sqrt(rx * rx + ry * ry + rz * rz);
+        d = sp * P.dt * P.near_horizon;
+        d = d > P.near_lo ? (d < P.near_hi ? d : P.near_hi) : P.near_lo;
+        if (!(sp * 0.0 == 0.0)) d = P.near_hi;  // speed bound not finite
+    }
     double sl, trig;
     if (!(a == a) || !(d < 1e300)) {
         sl = __longlong_as_double(0x7ff0000000000000ll);  // no bound: every pair is near
@@ -2754,7 +2769,7 @@ int launch_hfront_total(hipStream_t st, int N, int half, int* fcnt, const int* b
 }
 
 int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum, float* nthr, double* slack,
-                      const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
+                      const int* cid, const D4* pq, const D4* vw, const int* foff, const uint32_t* fword, const int* boff, const int2* bent, int* ncnt, int* nfoff,
                       uint32_t* nfw, int* nfs, int* slot2near, int* nbcnt, int* nboff, int* nbo, int* nbe, int* stmp,
                       int* stotal, int block) {
     const int N = P.N;
@@ -2762,7 +2777,7 @@ int launch_near_build(hipStream_t st, const DevParams& P, const D4* pos, D4* cum
     int e;
     if ((e = launch_fill_i32(st, ncnt + N, 1, 0))) return e;
     if ((e = launch_fill_i32(st, nbcnt + N, 1, 0))) return e;
-    k_near_slack<<<g, block, 0, st>>>(P, pos, cum, nthr, slack);
+    k_near_slack<<<g, block, 0, st>>>(P, pos, cum, nthr, slack, cid, pq, vw);
     LAUNCH_CHECK();
     const unsigned gw = nblk(N, 256);  // 4 waves of 64 spheres per block
     k_near_scan<<<gw, 256, 0, st>>>(P, pos, slack, foff, fword, ncnt, slot2near);

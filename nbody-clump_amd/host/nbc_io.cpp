@@ -236,6 +236,7 @@ extern "C" int nbc_load_options(const char* path, nbc_options* out) {
         else if (k == "HGridNested") out->hgrid_nested = std::atoi(v.c_str());
         else if (k == "SinglePass") out->single_pass = std::atoi(v.c_str());
         else if (k == "NearSkin") out->near_skin = std::atof(v.c_str());
+        else if (k == "NearHorizon") out->near_horizon = std::atof(v.c_str());
         else if (k == "Graphs") out->graphs = std::atoi(v.c_str());
         else if (k == "BatchSteps") out->batch_steps = std::atoi(v.c_str());
         else if (k == "Driver") {

@@ -82,7 +82,8 @@ int tune(const Bed& b, const nbc_options* base, int32_t flags, nbc_options* best
         cur = pick;
         best_ms = pick_ms;
     };
-    sweep([](nbc_options& o, double v) { o.near_skin = v; }, {0.125, 0.25, 0.5, 1.0});
+    // near_skin 0: the per-sphere skin (near_horizon), else a fixed one
+    sweep([](nbc_options& o, double v) { o.near_skin = v; }, {0.0, 0.125, 0.25, 0.5, 1.0});
     sweep([](nbc_options& o, double v) { o.batch_steps = (int32_t)v; }, {2, 4, 8, 16});
     sweep([](nbc_options& o, double v) { o.radix_bits = (int32_t)v; }, {4, 6, 8});
     sweep([](nbc_options& o, double v) { o.block_size = (int32_t)v; }, {128, 256});

@@ -52,7 +52,7 @@ class Options(C.Structure):
         ("hgrid_levels", C.c_int32), ("graphs", C.c_int32),
         ("pene_frcoe", C.c_double), ("cell_size", C.c_double), ("near_skin", C.c_double),
         ("driver", C.c_int32), ("level_from_type", C.c_int32), ("hgrid_nested", C.c_int32),
-        ("single_pass", C.c_int32), ("reserved", C.c_double),
+        ("single_pass", C.c_int32), ("near_horizon", C.c_double),
     ]
 
 
@@ -383,7 +383,7 @@ def generate_bed(kind, nclumps=0, seed=1, container=(0, 0, 0, 0)):
 
 def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pair_capacity=0, pene_frcoe=0.0,
                  cell_size=0.0, batch_steps=0, hgrid_levels=0, near_skin=0.0, graphs=0, driver=0, level_from_type=0,
-                 hgrid_nested=0, single_pass=0):
+                 hgrid_nested=0, single_pass=0, near_horizon=0.0):
     o = Options()
     o.hgrid_nested = hgrid_nested
     o.single_pass = single_pass
@@ -393,6 +393,7 @@ def make_options(precision=NBC_FP64, wall_type=0, block_size=0, radix_bits=0, pa
     o.pair_capacity, o.pene_frcoe, o.cell_size, o.batch_steps = pair_capacity, pene_frcoe, cell_size, batch_steps
     o.hgrid_levels = hgrid_levels
     o.near_skin = near_skin
+    o.near_horizon = near_horizon
     o.graphs = graphs
     return o
 

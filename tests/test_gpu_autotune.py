@@ -18,10 +18,10 @@ def test_autotune_picks_result_preserving_options():
     bed.params.StepSize = 5e-6
     base = nbc.make_options(precision=nbc.NBC_FP64, wall_type=nbc.NBC_WALL_CYLINDER)
     best, entries = nbc.autotune(bed, base, wall=bed.wall, steps=60)
-    assert len(entries) == 4 + 4 + 3 + 2 + 2
+    assert len(entries) == 5 + 4 + 3 + 2 + 2
     assert all(ms > 0 for _, ms in entries)
     # a sweep keeps the incoming value (0: default) unless a candidate beats it
-    assert best.near_skin in (0.125, 0.25, 0.5, 1.0) and best.batch_steps in (0, 2, 4, 8, 16)
+    assert best.near_skin in (0.0, 0.125, 0.25, 0.5, 1.0) and best.batch_steps in (0, 2, 4, 8, 16)
     assert best.cell_size == 0.0 and best.hgrid_levels == 0
     out = []
     for opt in (base, best):
@@ -41,7 +41,7 @@ def test_autotune_cell_and_hgrid_sweeps():
     bed = nbc.generate_bed(2, 800, seed=6, container=(0.02, 0, 0, 0))
     base = nbc.make_options(precision=nbc.NBC_MIXED, wall_type=nbc.NBC_WALL_CUBE)
     best, entries = nbc.autotune(bed, base, wall=bed.wall, steps=40, flags=nbc.NBC_TUNE_CELL | nbc.NBC_TUNE_HGRID)
-    assert len(entries) == 4 + 4 + 3 + 2 + 2 + 3 + 5
+    assert len(entries) == 5 + 4 + 3 + 2 + 2 + 3 + 5
     assert all(ms > 0 for _, ms in entries)
     assert best.hgrid_levels in (0, 2, 3) and best.hgrid_nested in (0, 1)
     # the picked options run
@@ -62,7 +62,7 @@ def test_autotune_batch_picks_result_preserving_options():
         b.params.StepSize = 5e-6
     base = nbc.make_options(precision=nbc.NBC_FP64, wall_type=nbc.NBC_WALL_CYLINDER)
     best, entries = nbc.autotune_batch(beds, base, steps=40)
-    assert len(entries) == 4 + 4 + 3 + 2 + 2
+    assert len(entries) == 5 + 4 + 3 + 2 + 2
     assert all(ms > 0 for _, ms in entries)
     assert best.cell_size == 0.0 and best.hgrid_levels == 0
     out = []

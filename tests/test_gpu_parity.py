@@ -205,19 +205,22 @@ def test_sqrt_div_rounding_on_device_matches_host():
     assert ctx.stats().touching_ss == ref.stats().touching_ss
 
 
+@pytest.mark.parametrize("horizon", [-1.0, 0.0, 20.0])
 @pytest.mark.parametrize("prec", [nbc.NBC_FP64, nbc.NBC_MIXED])
-def test_fast_collisions_between_rebuilds(prec):
+def test_fast_collisions_between_rebuilds(prec, horizon):
     """Clumps thrown at each other at up to 0.5 m/s: pairs close from outside
     the contact kernel's near range (gap 2 * Thresh at the last rebuild) and
     collide between rebuilds. Touching sets, histories and states must follow
-    the oracle, which scans every list pair every step."""
-    banner(f"fast collisions between rebuilds, precision {prec}")
+    the oracle, which scans every list pair every step, with the fixed near
+    skin (horizon -1), the per-sphere skin of the default 500-step horizon
+    (0) and a 20-step horizon that asks for many near-list refreshes."""
+    banner(f"fast collisions between rebuilds, precision {prec}, near horizon {horizon}")
     bed = nbc.generate_bed(1, 300, seed=8)
     rng = np.random.default_rng(8)
     bed.cvel[:] = rng.uniform(-0.5, 0.5, bed.cvel.shape)
     bed.cangv[:] = rng.uniform(-50.0, 50.0, bed.cangv.shape)
     bed.params.StepSize = 1e-5  # up to 5 um per step: a rebuild every ~50 steps
-    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER)
+    opt = nbc.make_options(precision=prec, wall_type=nbc.NBC_WALL_CYLINDER, near_horizon=horizon)
     ctx, ref = pair_setup(bed, opt)
     seen = 0
     for k in range(20):
@@ -230,7 +233,8 @@ def test_fast_collisions_between_rebuilds(prec):
     sb = ref.stats()
     assert sb.rebuilds >= 4 and seen > 0
     # the contact kernel's near lists were refreshed between rebuilds too
-    assert ctx.stats().near_refreshes > 0
+    if horizon != 0.0:
+        assert ctx.stats().near_refreshes > 0
     if prec == nbc.NBC_FP64:
         assert_lists_equal(ctx, ref)
         assert np.abs(ctx.spheres()["pos"] - ref.spheres()["pos"]).max() <= 1e-12

@@ -248,7 +248,7 @@ def test_nbc_par_keys(tmp_path):
     f = tmp_path / "nbc.par"
     f.write_text("Precision = mixed\nWallType = cylinder\nBlockSize = 128\nRadixBits = 6\nPairCapacity = 20\n"
                  "PeneFrcoe = 0.4\nCellSize = 0.006\nHGridLevels = 3\nNearSkin = 0.5\nGraphs = -1\n"
-                 "BatchSteps = 8\nDriver = split\nHGridNested = 1\nSinglePass = -1\n")
+                 "BatchSteps = 8\nDriver = split\nHGridNested = 1\nSinglePass = -1\nNearHorizon = 250\n")
     o = nbc.Options()
     assert nbc.lib().nbc_load_options(str(f).encode(), C.byref(o)) == 0
     assert (o.precision, o.wall_type, o.block_size, o.radix_bits, o.pair_capacity) == \
@@ -256,6 +256,7 @@ def test_nbc_par_keys(tmp_path):
     assert (o.pene_frcoe, o.cell_size, o.hgrid_levels, o.near_skin, o.graphs, o.batch_steps) == \
         (0.4, 0.006, 3, 0.5, -1, 8)
     assert o.driver == nbc.NBC_DRIVER_SPLIT and o.hgrid_nested == 1 and o.single_pass == -1
+    assert o.near_horizon == 250.0
     for name, want in (("clump", 0), ("cube", 1), ("cylinder", 2), ("split", 3), ("hgrid", 4)):
         f.write_text(f"Driver = {name}\n")
         assert nbc.lib().nbc_load_options(str(f).encode(), C.byref(o)) == 0 and o.driver == want
