@@ -248,7 +248,7 @@ class Engine : public EngineBase {
         HIPC(dalloc(&hist, radix_tmp_ints(nk, 8))); HIPC(dalloc(&hscan, radix_tmp_ints(nk, 8)));
         HIPC(dalloc(&stmp, scan_tmp_ints(std::max(radix_tmp_ints(nk, 8), N + 1))));
         HIPC(dalloc(&stotal, 4));
-        HIPC(dalloc(&cellkey, N)); HIPC(dalloc(&perm, N));
+        HIPC(dalloc(&cellkey, N)); HIPC(dalloc(&perm, N)); HIPC(dalloc(&iperm, N));
         HIPC(dalloc(&cbeg, P.CellNum)); HIPC(dalloc(&cend, P.CellNum));
         HIPC(dalloc(&fcnt, (size_t)N + 1)); HIPC(dalloc(&bcnt, (size_t)N + 1));
         if (dp.L > 1) {
@@ -380,7 +380,7 @@ class Engine : public EngineBase {
         F(cfile); F(cfile2); F(cfirst); F(cnew);
         F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near); F(hn[0]); F(hn[1]);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
-        F(cellkey); F(perm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
+        F(cellkey); F(perm); F(iperm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
         F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
@@ -910,13 +910,15 @@ class Engine : public EngineBase {
                                 bent, dstatus + 2, block));
         ell_fills += from_ell ? 1 : 0;
         KC(launch_back_resolve(st, N, Ln.foff, Ln.fword, boff, bent, block));
-        // history-valid bits of the old near list back into the old pair words
-        KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, Lo.fword, hn[p], Lo.hist[p], p, block));
-        if (driver == NBC_DRIVER_CYLINDER || driver == NBC_DRIVER_SPLIT)
+        if (driver == NBC_DRIVER_CYLINDER || driver == NBC_DRIVER_SPLIT) {
+            // history-valid bits of the old near list back into the old pair words
+            KC(launch_near_sync<R>(st, N, nfoff, nfw, nfs, Lo.fword, hn[p], Lo.hist[p], p, block));
             KC(launch_hist_carry<R>(st, tot[0], nfront, p, Ln.fword, Ln.hist[p], Lo.fword, Lo.hist[p], block));
-        else
-            KC(launch_hist_remap<R>(st, dp, s.pos, s.cid, p, perm, Ln.foff, Ln.fword, Ln.hist[p], Lo.foff, Lo.fword,
-                                    Lo.hist[p], block));
+        } else {
+            // valid pairs of the old near list hand their histories to the new list
+            KC(launch_hist_push<R>(st, dp, s.pos, s.cid, p, perm, iperm, nfoff, nfw, hn[p], Ln.foff, Ln.fword,
+                                   Ln.hist[p], block));
+        }
         lcur ^= 1;
         // near lists of the new list (the contact kernel's candidates)
         HIPC(grow_near(Ln.cap));
@@ -1663,7 +1665,7 @@ class Engine : public EngineBase {
     BedGeo* dbeds = nullptr;
     int* dbid = nullptr;
     int *val0 = nullptr, *val1 = nullptr, *hist = nullptr, *hscan = nullptr, *stmp = nullptr, *stotal = nullptr;
-    int *perm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
+    int *perm = nullptr, *iperm = nullptr, *cbeg = nullptr, *cend = nullptr, *fcnt = nullptr, *bcnt = nullptr;
     int *mbeg = nullptr, *mend = nullptr, *mem = nullptr;
     uint32_t* dstatus = nullptr;  // [0] halt step, [1] smallest bad sphere, [2] overflow, [3] clump state not finite
     uint32_t* hstatus = nullptr;

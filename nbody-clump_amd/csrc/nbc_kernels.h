@@ -113,10 +113,13 @@ int launch_sphere_pose(hipStream_t st, const DevParams& P, SphereBuf<R> S, Clump
                        int block);
 template <class R>
 int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm, SphereBuf<R> src, SphereBuf<R> dst);
+// History inheritance at a rebuild from the old near list (onfoff/onfw,
+// histories ohn in near order) into the new front list, whose pair words
+// carry no valid bits yet; inv receives the inverse of perm.
 template <class R>
-int launch_hist_remap(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
-                      const int* nfoff, uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword,
-                      const R* ohist, int block);
+int launch_hist_push(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
+                     int* inv, const int* onfoff, const uint32_t* onfw, const R* ohn, const int* nfoff,
+                     uint32_t* nfword, R* nhist, int block);
 // Clump renumbering at a rebuild: first[c] = first sphere of clump c (first
 // filled with N before), then the clump arrays move to the order of first
 // (order[new] = old) and every sphere's clump index follows.

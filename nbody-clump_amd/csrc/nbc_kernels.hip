@@ -644,6 +644,49 @@ __global__ void k_remap_cid(int N, int* cid, const int* newid) {
 // order: each row is walked as the run [s0, s1) in ascending index, which is
 // the reference's cell-by-cell order, with four candidate positions loaded
 // ahead. visit(j, pj) is called for every j != i within 1.5 (ri + rj).
+// The list passes work like the contact kernel: a wave takes 64 consecutive
+// spheres, whose rows form one contiguous segment of the CSR list, and reads
+// it 64 entries at a time (coalesced); an entry finds its row by binary search
+// over the 65 row offsets kept in LDS.
+__device__ __forceinline__ int row_of(const int* fo, int e) {
+    int r = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1)
+        if (fo[r + st] <= e) r += st;
+    return r;
+}
+
+// Loads the row offsets of the wave's spheres i0 .. i0+63 into fo[0..64].
+__device__ __forceinline__ void load_rows(int* fo, const int* off, int i0, int N) {
+    const int lane = lane_id();
+    fo[lane] = off[min(i0 + lane, N)];
+    if (lane == 0) fo[64] = off[min(i0 + 64, N)];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Index of partner j in the front row [t0, t1), or -1: eight words per round
+// in flight instead of one dependent load per entry (pairs are listed once,
+// so any match is the only one).
+__device__ __forceinline__ int find_in_row(const uint32_t* fword, int t0, int t1, int j) {
+    for (int t = t0; t < t1; t += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = fword[min(t + u, t1 - 1)];
+        int hit = -1;
+#pragma unroll
+        for (int u = 7; u >= 0; --u)
+            if ((int)(w[u] & JMASK) == j) hit = t + u;
+        if (hit >= 0) return hit < t1 ? hit : -1;
+    }
+    return -1;
+}
+
 template <class Visit>
 __device__ __forceinline__ void search_27(const D4* pos, const int* cbeg, const int* cend, const uint32_t* G,
                                           int coff, int i, const D4& pi, int cix, int ciy, int ciz, Visit visit) {
@@ -878,14 +921,35 @@ __global__ void k_neighbors_ell(DevParams P, const D4* pos, const uint32_t* cell
     wave_max_to(ellmax, max(ng, ni));
 }
 
-__global__ void k_rows_from_ell(int N, const int* fcnt, const int* bcnt, const int* foff, const int* boff,
-                                const uint32_t* ef, const int* eb, uint32_t* fword, int2* bent) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int nf = fcnt[i], fo = foff[i];
-    for (int g = 0; g < nf; ++g) fword[fo + g] = ef[(size_t)g * N + i];
-    const int nb = bcnt[i], bo = boff[i];
-    for (int g = 0; g < nb; ++g) bent[bo + nb - 1 - g] = make_int2(eb[(size_t)g * N + i], -1);
+// A wave takes 64 consecutive spheres, whose front rows (and back rows) form
+// one contiguous CSR segment, and writes it 64 entries at a time (coalesced);
+// an entry finds its row by binary search over the 65 row offsets in LDS.
+// Back rows hold the owners in descending order (the search found them
+// ascending).
+__global__ __launch_bounds__(256) void k_rows_from_ell(int N, const int* foff, const int* boff, const uint32_t* ef,
+                                                       const int* eb, uint32_t* fword, int2* bent) {
+    __shared__ int s_fo[4][65];
+    __shared__ int s_bo[4][65];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int i0 = (blockIdx.x * 4 + wv) * 64;
+    if (i0 >= N) return;
+    load_rows(s_fo[wv], foff, i0, N);
+    load_rows(s_bo[wv], boff, i0, N);
+    const int fend = s_fo[wv][64];
+    for (int k0 = s_fo[wv][0]; k0 < fend; k0 += 64) {
+        const int e = k0 + lane;
+        if (e >= fend) continue;
+        const int r = row_of(s_fo[wv], e);
+        fword[e] = ef[(size_t)(e - s_fo[wv][r]) * N + i0 + r];
+    }
+    const int bend = s_bo[wv][64];
+    for (int k0 = s_bo[wv][0]; k0 < bend; k0 += 64) {
+        const int e = k0 + lane;
+        if (e >= bend) continue;
+        const int r = row_of(s_bo[wv], e);
+        const int g = s_bo[wv][r + 1] - 1 - e;
+        bent[e] = make_int2(eb[(size_t)g * N + i0 + r], -1);
+    }
 }
 
 // Hierarchical grid: the loop of k_hneighbors (phase 0) storing own fronts,
@@ -1094,32 +1158,6 @@ __global__ void k_near_slack(DevParams P, const D4* pos, D4* cum, float* nthr, d
     }
 }
 
-// The near-list passes work like the contact kernel: a wave takes 64
-// consecutive spheres, whose rows form one contiguous segment of the CSR
-// list, and reads it 64 entries at a time (coalesced); an entry finds its row
-// by binary search over the 65 row offsets kept in LDS.
-__device__ __forceinline__ int row_of(const int* fo, int e) {
-    int r = 0;
-#pragma unroll
-    for (int st = 32; st > 0; st >>= 1)
-        if (fo[r + st] <= e) r += st;
-    return r;
-}
-
-// Loads the row offsets of the wave's spheres i0 .. i0+63 into fo[0..64].
-__device__ __forceinline__ void load_rows(int* fo, const int* off, int i0, int N) {
-    const int lane = lane_id();
-    fo[lane] = off[min(i0 + lane, N)];
-    if (lane == 0) fo[64] = off[min(i0 + 64, N)];
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 __global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, const uint32_t* fword, const int* boff,
                                                       int2* bent) {
     __shared__ int s_fo[4][65];
@@ -1133,10 +1171,7 @@ __global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, co
         if (e >= end) continue;
         const int i = i0 + row_of(s_fo[wv], e);
         const int o = bent[e].x;
-        int slot = -1;
-        const int se = foff[o + 1];
-        for (int t = foff[o]; t < se; ++t)
-            if ((int)(fword[t] & JMASK) == i) { slot = t; break; }
+        const int slot = find_in_row(fword, foff[o], foff[o + 1], i);
         int2 be;
         be.x = (slot >= 0 && (fword[slot] & INTRA)) ? (o | BINTRA) : o;
         be.y = slot;
@@ -1144,51 +1179,50 @@ __global__ __launch_bounds__(256) void k_back_resolve(int N, const int* foff, co
     }
 }
 
-// History inheritance, src/dem_grid.cu:603-647: each new front pair looks up
-// its pair in the old list of the lower old index and copies the history of
-// buffer p, negated when the orientation flipped. Unmatched pairs start from
-// zero. With nested grid levels a pair on two levels stays in the coarser
-// sphere's row whatever the indices, so its orientation never flips. A wave
-// takes the front rows of 64 consecutive spheres (one contiguous segment),
-// one pair per lane, so a sphere with a long row spreads its lookups over the
-// lanes instead of running them one after another.
+// History inheritance, src/dem_grid.cu:603-647: a new pair takes the history
+// (buffer p) of the same pair in the old list, negated when its orientation
+// flipped; unmatched pairs start from zero. The reference pulls: every new
+// pair searches the old row of its lower old index. Here the old pairs push:
+// the old near list holds every pair with a valid history (a valid pair is
+// always near) and, between builds, its latest history in near order, so each
+// old near pair (a, b) valid in buffer p finds its pair in the new list (new
+// indices inv[a], inv[b]; owner the lower one, or with nested grid levels the
+// coarser sphere, which owned it before too and keeps it whatever the
+// indices), copies the history there and sets the valid bit. The same pairs
+// inherit the same values, but only the ~0.3 touching pairs per sphere search
+// a row instead of the ~12 new pairs per sphere. The new pair words must come
+// without valid bits (as the neighbour search writes them); inv is the
+// inverse of perm (new index -> old index).
 template <class R>
-__global__ __launch_bounds__(256) void k_hist_remap(DevParams P, const D4* pos, const int* cid, int p,
-                                                    const int* perm, const int* nfoff, uint32_t* nfword, R* nhist,
-                                                    const int* ofoff, const uint32_t* ofword, const R* ohist) {
-    __shared__ int s_fo[4][65];
-    const int lane = lane_id(), wv = threadIdx.x >> 6;
-    const int i0 = (blockIdx.x * 4 + wv) * 64;
-    if (i0 >= P.N) return;
-    load_rows(s_fo[wv], nfoff, i0, P.N);
-    const int end = s_fo[wv][64];
-    for (int k0 = s_fo[wv][0]; k0 < end; k0 += 64) {
-        const int s = k0 + lane;
-        if (s >= end) continue;
-        const int i = i0 + row_of(s_fo[wv], s);
-        uint32_t w = nfword[s];
-        const int j = (int)(w & JMASK);
-        const int io = perm[i], jo = perm[j];
-        bool keep = io < jo;  // the old row is i's own
+__global__ void k_hist_push(DevParams P, const D4* pos, const int* cid, int p, const int* inv, const int* onfoff,
+                            const uint32_t* onfw, const R* ohn, const int* nfoff, uint32_t* nfword, R* nhist) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.N) return;
+    const int oe = onfoff[a + 1];
+    for (int o = onfoff[a]; o < oe; ++o) {
+        const uint32_t w = onfw[o];
+        if (!(w & VALID(p))) continue;
+        const int na = inv[a], nb = inv[(int)(w & JMASK)];
+        bool keep = na < nb;
         if (!keep && P.nested) {
-            const int li = P.clevel ? P.clevel[cid[i]] : level_of(P, pos[i].w);
-            keep = li != (P.clevel ? P.clevel[cid[j]] : level_of(P, pos[j].w));
+            const int la = P.clevel ? P.clevel[cid[na]] : level_of(P, pos[na].w);
+            keep = la != (P.clevel ? P.clevel[cid[nb]] : level_of(P, pos[nb].w));
         }
-        const int a = keep ? io : jo, b = keep ? jo : io;
-        const bool neg = !keep;
-        int q = -1;
-        const int te = ofoff[a + 1];
-        for (int t = ofoff[a]; t < te; ++t)
-            if ((int)(ofword[t] & JMASK) == b) { q = t; break; }
-        w &= ~(VALID(0) | VALID(1));
-        if (q >= 0 && (ofword[q] & VALID(p)) && !(w & INTRA)) {
-            const R* h = ohist + (size_t)q * HSTRIDE;
-            R* d = nhist + (size_t)s * HSTRIDE;
-            for (int k = 0; k < HSTRIDE; ++k) d[k] = neg ? R(-1.0) * h[k] : h[k];
-            w |= VALID(p);
-        }
-        nfword[s] = w;
+        const int i = keep ? na : nb, j = keep ? nb : na;
+        const int s = find_in_row(nfword, nfoff[i], nfoff[i + 1], j);
+        if (s < 0) continue;
+        const uint32_t nw = nfword[s];
+        if (nw & INTRA) continue;
+        const R* h = ohn + (size_t)o * HSTRIDE;
+        R* d = nhist + (size_t)s * HSTRIDE;
+        for (int k = 0; k < HSTRIDE; ++k) d[k] = keep ? h[k] : R(-1.0) * h[k];
+        nfword[s] = nw | VALID(p);
     }
+}
+
+__global__ void k_inverse_perm(int N, const int* perm, int* inv) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) inv[perm[n]] = n;
 }
 
 // Pass 1: near flag of every front entry (into flag[]) and near count per sphere.
@@ -2724,7 +2758,8 @@ int launch_neighbors_ell(hipStream_t st, const DevParams& P, const D4* pos, cons
 
 int launch_rows_from_ell(hipStream_t st, int N, const int* fcnt, const int* bcnt, const int* foff, const int* boff,
                          const uint32_t* ef, const int* eb, uint32_t* fword, int2* bent, int block) {
-    k_rows_from_ell<<<nblk(N, block), block, 0, st>>>(N, fcnt, bcnt, foff, boff, ef, eb, fword, bent);
+    (void)fcnt; (void)bcnt; (void)block;
+    k_rows_from_ell<<<nblk(N, 256), 256, 0, st>>>(N, foff, boff, ef, eb, fword, bent);
     LAUNCH_CHECK();
     return 0;
 }
@@ -2849,11 +2884,13 @@ int launch_gather(hipStream_t st, const DevParams& P, int nface, const int* perm
     return 0;
 }
 template <class R>
-int launch_hist_remap(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
-                      const int* nfoff, uint32_t* nfword, R* nhist, const int* ofoff, const uint32_t* ofword,
-                      const R* ohist, int block) {
-    (void)block;
-    k_hist_remap<R><<<nblk(P.N, 256), 256, 0, st>>>(P, pos, cid, p, perm, nfoff, nfword, nhist, ofoff, ofword, ohist);
+int launch_hist_push(hipStream_t st, const DevParams& P, const D4* pos, const int* cid, int p, const int* perm,
+                     int* inv, const int* onfoff, const uint32_t* onfw, const R* ohn, const int* nfoff,
+                     uint32_t* nfword, R* nhist, int block) {
+    if (P.N <= 0) return 0;
+    k_inverse_perm<<<nblk(P.N, block), block, 0, st>>>(P.N, perm, inv);
+    LAUNCH_CHECK();
+    k_hist_push<R><<<nblk(P.N, block), block, 0, st>>>(P, pos, cid, p, inv, onfoff, onfw, ohn, nfoff, nfword, nhist);
     LAUNCH_CHECK();
     return 0;
 }
@@ -2946,8 +2983,8 @@ int launch_member_slot(hipStream_t st, int N, const int* mem, int* slot) {
     template int launch_near_sync<R>(hipStream_t, int, const int*, const uint32_t*, const int*, uint32_t*,          \
                                      const R*, R*, int, int);                                                       \
     template int launch_near_hist_gather<R>(hipStream_t, int, const uint32_t*, const int*, const R*, R*, int);      \
-    template int launch_hist_remap<R>(hipStream_t, const DevParams&, const D4*, const int*, int, const int*,         \
-                                      const int*, uint32_t*, R*, const int*, const uint32_t*, const R*, int);        \
+    template int launch_hist_push<R>(hipStream_t, const DevParams&, const D4*, const int*, int, const int*, int*,     \
+                                     const int*, const uint32_t*, const R*, const int*, uint32_t*, R*, int);         \
     template int launch_clump_gather<R>(hipStream_t, int, const int*, ClumpBuf<R>, ClumpBuf<R>, const int*, int*,   \
                                         int*, int*, int, int);                                                         \
     template int launch_hist_carry<R>(hipStream_t, long long, long long, int, uint32_t*, R*, const uint32_t*,       \
