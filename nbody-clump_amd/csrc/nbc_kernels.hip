@@ -878,47 +878,156 @@ __device__ __forceinline__ void wave_max_to(unsigned* dst, int v) {
     if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, (unsigned)v);
 }
 
-// Uniform grid (src/dem_grid.cu:466-552): the loop of k_neighbors; fronts as
-// pair words, backs in discovery order (reversed by the compaction).
-__global__ void k_neighbors_ell(DevParams P, const D4* pos, const uint32_t* cell, const int* cbeg, const int* cend,
-                                const int* cid, int K, int* fcnt, int* bcnt, uint32_t* ef, int* eb, unsigned* overflow,
-                                unsigned* ellmax) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int ng = 0, ni = 0;
-    if (i < P.N) {
-        const int half = P.B / 2;
-        const double* O = P.origin;
-        const double* ic = P.icell;
-        const uint32_t* G = P.G;
-        int coff = 0, last = P.CellNum - 1;
-        if (P.nbeds > 1) {
-            const BedGeo& g = P.beds[P.bid[i]];
-            O = g.origin; ic = g.icell; G = g.G;
-            coff = g.cell_off;
-            last = g.cell_off + g.cell_num - 1;
+// Single-pass uniform-grid search (src/dem_grid.cu:466-552, the loop of
+// k_neighbors) into entry-major tables of capacity K: fronts as pair words,
+// backs in discovery order (reversed by the compaction). One wave per cell:
+// the spheres of a cell share its 27-cell stencil, whose candidates (nine
+// x-runs of three cells, each run contiguous in sorted order, the runs
+// ascending) the wave loads 64 at a time, one per lane. Each home sphere then
+// tests the 64 loaded candidates at once, its position broadcast from its own
+// lane, and keeps the hits as a 64-bit mask; its lane writes them in
+// candidate order, the order in which the per-sphere loop visits them. A
+// per-sphere loop reloads every candidate once per home sphere (about 450
+// loads of 32 bytes per sphere on a settled bed); here a candidate is loaded
+// once per cell. A sphere whose cell coordinates, computed from its position
+// as the per-sphere loop does, differ from those of its hashed cell runs the
+// per-sphere loop (search_27) instead.
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__global__ __launch_bounds__(256) void k_neighbors_cell(DevParams P, const D4* pos, const int* cbeg, const int* cend,
+                                                        const int* cid, int K, int* fcnt, int* bcnt, uint32_t* ef,
+                                                        int* eb, unsigned* overflow, unsigned* ellmax) {
+    __shared__ int s_j[4][64];
+    __shared__ int s_c[4][64];
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const int h = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + wv));
+    if (h >= P.CellNum) return;
+    const int hb = cbeg[h];
+    if (hb < 0) return;
+    const int he = cend[h];
+    const int half = P.B / 2;
+    const double* O = P.origin;
+    const double* ic = P.icell;
+    const uint32_t* G = P.G;
+    int coff = 0, last = P.CellNum - 1;
+    if (P.nbeds > 1) {
+        const BedGeo& g = P.beds[P.bid[hb]];
+        O = g.origin; ic = g.icell; G = g.G;
+        coff = g.cell_off;
+        last = g.cell_off + g.cell_num - 1;
+    }
+    if (h == last) {
+        // out-of-box bucket: never searched
+        for (int i = hb + lane; i < he; i += 64) { fcnt[i] = 0; bcnt[i] = 0; }
+        return;
+    }
+    const int G0 = (int)G[0], G1 = (int)G[1], G2 = (int)G[2];
+    const int l = h - coff;
+    const int hx = l % G0, hy = (l / G0) % G1, hz = l / (G0 * G1);
+    // stencil row r = 3 (dz + 1) + (dy + 1): its run [s0, s1) over the cells
+    // hx-1 .. hx+1 that exist
+    int rs = 0, rl = 0;
+    if (lane < 9) {
+        const int jy = hy + lane % 3 - 1, jz = hz + lane / 3 - 1;
+        const int jx0 = max(hx - 1, 0), jx1 = min(hx + 1, G0 - 1);
+        if (jy >= 0 && jy < G1 && jz >= 0 && jz < G2) {
+            const int rowc = coff + (jz * G1 + jy) * G0;
+            int s0 = -1, s1 = -1;
+            for (int jx = jx0; jx <= jx1; ++jx) {
+                const int b = cbeg[rowc + jx];
+                if (b == -1) continue;
+                if (s0 < 0) s0 = b;
+                s1 = cend[rowc + jx];
+            }
+            if (s0 >= 0) { rs = s0; rl = s1 - s0; }
         }
-        if ((int)cell[i] != last) {
-            const D4 pi = pos[i];
-            const int ci = cid[i];
-            const int cix = (int)floor((pi.x - O[0]) * ic[0]);
-            const int ciy = (int)floor((pi.y - O[1]) * ic[1]);
-            const int ciz = (int)floor((pi.z - O[2]) * ic[2]);
-            search_27(pos, cbeg, cend, G, coff, i, pi, cix, ciy, ciz, [&](int j) {
-                if (i < j) {
-                    if (ng < K) ef[(size_t)ng * P.N + i] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+    }
+    int rst[9], roff[10];
+    roff[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        rst[r] = __builtin_amdgcn_readlane(rs, r);
+        roff[r + 1] = roff[r] + __builtin_amdgcn_readlane(rl, r);
+    }
+    const int M = roff[9];
+    int gmax = 0;
+    for (int g0 = hb; g0 < he; g0 += 64) {
+        const int i = g0 + lane;
+        const bool act = i < he;
+        const D4 pi = pos[act ? i : hb];
+        const int ci = cid[act ? i : hb];
+        const int cix = (int)floor((pi.x - O[0]) * ic[0]);
+        const int ciy = (int)floor((pi.y - O[1]) * ic[1]);
+        const int ciz = (int)floor((pi.z - O[2]) * ic[2]);
+        const bool reg = act && cix == hx && ciy == hy && ciz == hz;
+        const uint64_t regm = __ballot(reg);
+        int ng = 0, ni = 0;
+        for (int c0 = 0; c0 < M && regm; c0 += 64) {
+            const int t = c0 + lane;
+            const bool cv = t < M;
+            int j = hb;
+#pragma unroll
+            for (int r = 0; r < 9; ++r)
+                if (t >= roff[r] && t < roff[r + 1]) j = rst[r] + (t - roff[r]);
+            const D4 pj = pos[j];
+            const int cj = cid[j];
+            uint64_t mym = 0;
+            // home spheres in turn, each broadcast from its own lane
+            for (uint64_t rm = regm; rm; rm &= rm - 1ull) {
+                const int k = (int)__builtin_ctzll(rm);
+                const double kx = readlane_f64(pi.x, k), ky = readlane_f64(pi.y, k);
+                const double kz = readlane_f64(pi.z, k), kr = readlane_f64(pi.w, k);
+                const double dx = kx - pj.x, dy = ky - pj.y, dz = kz - pj.z;
+                const double ds = dx * dx + dy * dy + dz * dz;
+                const double lim = 1.50 * (kr + pj.w);
+                const uint64_t m = __ballot(cv && j != g0 + k && ds < lim * lim);
+                if (lane == k) mym = m;
+            }
+            s_j[wv][lane] = j;
+            s_c[wv][lane] = cj;
+            wave_sync();
+            while (mym) {
+                const int b = __builtin_ctzll(mym);
+                mym &= mym - 1ull;
+                const int jj = s_j[wv][b];
+                if (jj > i) {
+                    if (ng < K) ef[(size_t)ng * P.N + i] = (uint32_t)jj | (s_c[wv][b] == ci ? INTRA : 0u);
                     ng++;
                 } else {
-                    if (ni < K) eb[(size_t)ni * P.N + i] = j;
+                    if (ni < K) eb[(size_t)ni * P.N + i] = jj;
                     ni++;
                 }
-            });
+            }
+            wave_sync();
         }
-        fcnt[i] = ng;
-        bcnt[i] = ni;
-        if (ng > half || ni > half) atomicOr(overflow, 1u);
-        if (ng > K || ni > K) atomicOr(overflow, 2u);
+        if (act && !reg) {
+            int ng2 = 0, ni2 = 0;
+            search_27(pos, cbeg, cend, G, coff, i, pi, cix, ciy, ciz, [&](int j) {
+                if (i < j) {
+                    if (ng2 < K) ef[(size_t)ng2 * P.N + i] = (uint32_t)j | (cid[j] == ci ? INTRA : 0u);
+                    ng2++;
+                } else {
+                    if (ni2 < K) eb[(size_t)ni2 * P.N + i] = j;
+                    ni2++;
+                }
+            });
+            ng = ng2;
+            ni = ni2;
+        }
+        if (act) {
+            fcnt[i] = ng;
+            bcnt[i] = ni;
+            if (ng > half || ni > half) atomicOr(overflow, 1u);
+            if (ng > K || ni > K) atomicOr(overflow, 2u);
+            gmax = max(gmax, max(ng, ni));
+        }
     }
-    wave_max_to(ellmax, max(ng, ni));
+    wave_max_to(ellmax, gmax);
 }
 
 // A wave takes 64 consecutive spheres, whose front rows (and back rows) form
@@ -2750,8 +2859,9 @@ int launch_hneighbors(hipStream_t st, const DevParams& P, int phase, const D4* p
 int launch_neighbors_ell(hipStream_t st, const DevParams& P, const D4* pos, const uint32_t* cell, const int* cbeg,
                          const int* cend, const int* cid, int K, int* fcnt, int* bcnt, uint32_t* ef, int* eb,
                          unsigned* overflow, unsigned* ellmax, int block) {
-    k_neighbors_ell<<<nblk(P.N, block), block, 0, st>>>(P, pos, cell, cbeg, cend, cid, K, fcnt, bcnt, ef, eb,
-                                                         overflow, ellmax);
+    (void)cell; (void)block;
+    k_neighbors_cell<<<nblk(P.CellNum, 4), 256, 0, st>>>(P, pos, cbeg, cend, cid, K, fcnt, bcnt, ef, eb, overflow,
+                                                         ellmax);
     LAUNCH_CHECK();
     return 0;
 }
