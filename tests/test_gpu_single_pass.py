@@ -77,3 +77,59 @@ def test_single_pass_rows_bit_exact_through_rebuilds(kind, single_pass):
     assert np.abs(ctx.spheres()["pos"] - ref.spheres()["pos"]).max() <= 1e-12
     ctx.close()
     ref.close()
+
+
+def test_cell_search_spheres_on_cell_faces():
+    """The uniform-grid search runs one wave per cell and hands a sphere to the
+    per-sphere loop when its cell coordinates, computed from its position with
+    the inverse cell size, differ from its hashed cell (computed with a
+    division). Spheres are placed on cell-face values where the two roundings
+    disagree (a 4.3 mm cell makes such values common); the rows stay
+    bit-exact to the oracle, which makes the same two computations."""
+    print("[case] uniform-grid search with spheres on cell faces", flush=True)
+    bed = nbc.generate_bed(0, 600, seed=21)
+    cs_opt = 0.0043
+    opt = nbc.make_options(precision=nbc.NBC_FP64, wall_type=nbc.NBC_WALL_CUBE, cell_size=cs_opt)
+    P = bed.finalized_params(opt)
+    O, cs, ic = (np.array(v[:3]) for v in (P.WorldOrigin, P.CellSize, P.i_CellSize))
+
+    def split(x, a):
+        return np.floor((x - O[a]) / cs[a]) != np.floor((x - O[a]) * ic[a])
+
+    moved = 0
+    for k in range(bed.nc):
+        # the middle sphere of a C0 clump sits at the clump centre (offset 0)
+        m = np.nonzero((bed.clump_id == k) & np.all(bed.local == 0.0, axis=1))[0]
+        if len(m) != 1:
+            continue
+        for a in (0, 1):
+            base = O[a] + np.round((bed.cpos[k, a] - O[a]) / cs[a]) * cs[a]
+            xs = [base]
+            up = dn = base
+            for _ in range(50):
+                up, dn = np.nextafter(up, np.inf), np.nextafter(dn, -np.inf)
+                xs += [up, dn]
+            hit = [x for x in xs if split(x, a)]
+            if hit:
+                bed.cpos[k, a] = hit[0]
+                bed.pos[m[0], a] = hit[0]
+                moved += 1
+                break
+    assert moved >= 50, moved
+    ctx = nbc.Context(P, opt)
+    ctx.upload(bed, bed.wall)
+    ref = orc.Oracle(orc.finalize_params(bed, cs_opt), opt)
+    ref.upload(bed, bed.wall)
+    for k in range(2):
+        ctx.rebuild()
+        ref.rebuild()
+        pos = ctx.spheres()["pos"]
+        nsplit = int(np.count_nonzero(split(pos[:, 0], 0) | split(pos[:, 1], 1)))
+        print(f"  rebuild {k}: {nsplit} spheres on split cell faces", flush=True)
+        assert nsplit >= 50
+        diff = first_row_difference(ctx, ref)
+        assert diff is None, diff
+        assert_lists_equal(ctx, ref)
+    assert ctx.stats().single_pass_rebuilds >= 1
+    ctx.close()
+    ref.close()
