@@ -256,18 +256,23 @@ __global__ void k_clump_before(DevParams P, ClumpBuf<R> C, PeneState* pst, unsig
 // K2: sphere pose from the clump pose, cumulative displacement, rebuild flag.
 template <class R>
 __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* halt, int n_arg) {
-    const int n = step_of(P, n_arg);
-    if (*halt < n) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false, near = false;
-    if (i < P.N && ref_disp<R>(P.driver)) {
+    int n;
+    if (ref_disp<R>(P.driver)) {
         // FP32-mixed clump loop: the loads that do not need the clump index
-        // are issued with it, so a wave waits for one gather level only
-        const int c = S.cid[i];
-        const typename Vec4T<R>::type lo = S.loc[i];
-        const D4 cu = S.cum[i];  // x_rebuild, radius
-        const float th = S.nthr[i];
-        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the wait for c
+        // are issued with it, and ahead of the halt check (from a clamped
+        // index; the arrays are valid whatever the halt), so a wave waits for
+        // one gather level only
+        const int ii = i < P.N ? i : P.N - 1;
+        const int c = S.cid[ii];
+        const typename Vec4T<R>::type lo = S.loc[ii];
+        const D4 cu = S.cum[ii];  // x_rebuild, radius
+        const float th = S.nthr[ii];
+        n = step_of(P, n_arg);
+        const int hv = *halt;
+        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the waits
+        if (hv < n) return;
         const D4 pc = C.pq[2 * c], q = C.pq[2 * c + 1];
         double M[3][3];
         QuatIvDCM<double>(q.x, q.y, q.z, q.w, M);
@@ -278,9 +283,17 @@ __global__ void k_sphere_pose(DevParams P, SphereBuf<R> S, ClumpBuf<R> C, int* h
         over = Length<double>(cum) > P.Thresh;
         near = cum.x * cum.x + cum.y * cum.y + cum.z * cum.z > (double)th;
         D4 pn; pn.x = up.x; pn.y = up.y; pn.z = up.z; pn.w = cu.w;
-        S.pos[i] = pn;
-        S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
-    } else if (i < P.N) {
+        if (i < P.N) {
+            S.pos[i] = pn;
+            S.kin[i] = mk4<R>((R)arm.x, (R)arm.y, (R)arm.z, lo.w);
+        } else {
+            over = near = false;
+        }
+    } else {
+        n = step_of(P, n_arg);
+        if (*halt < n) return;
+    }
+    if (!ref_disp<R>(P.driver) && i < P.N) {
         const int c = S.cid[i];
         const typename Vec4T<R>::type lo = S.loc[i];
         const D4 cu = S.cum[i];
@@ -2586,13 +2599,8 @@ __global__ __launch_bounds__(64) void k_clump_after(DevParams P, SphereBuf<R> S,
     typedef typename Vec4T<R>::type R4;
     constexpr int CH = K4Chunk<R>::value;
     __shared__ R4 s_rec[CH][2];
-    const int n = step_of(P, n_arg);
-    if (*halt <= n) return;  // the same value for the whole wave
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = lane_id();
-    // NEXT: K1 of step n + 1 follows in the same thread (it returns when K4 of
-    // step n does, at a halt at or before n)
-    if (NEXT && c == 0) k1_thread0(P, pst, counters, halt, n + 1);
     // the wave's member run [r0, r1) from its first and last clump, requested
     // first, so that the member records can be requested while the clump
     // state below is still on its way (over the lanes when one of the two
@@ -2608,7 +2616,14 @@ __global__ __launch_bounds__(64) void k_clump_after(DevParams P, SphereBuf<R> S,
     const D4 vr = C.vw[2 * cc], wr = C.vw[2 * cc + 1];
     D4 pc, qc;
     if (NEXT) { pc = C.pq[2 * cc]; qc = C.pq[2 * cc + 1]; }
+    // the halt check overlaps these loads (the arrays are valid whatever it says)
+    const int n = step_of(P, n_arg);
+    const int hv = *halt;
     __builtin_amdgcn_sched_barrier(0);
+    if (hv <= n) return;  // the same value for the whole wave
+    // NEXT: K1 of step n + 1 follows in the same thread (it returns when K4 of
+    // step n does, at a halt at or before n)
+    if (NEXT && c == 0) k1_thread0(P, pst, counters, halt, n + 1);
     const int b = valid ? bb : -1, e = valid ? ee : -1;
     const bool has = b >= 0 && e > b;
     if (r0 < 0 || r1 < 0) {
