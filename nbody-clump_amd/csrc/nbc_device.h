@@ -145,6 +145,9 @@ struct DevParams {
     // graph replays: the step kernels' step argument is an offset from *sbase
     // (read when the kernel runs); nullptr: the argument is the step itself
     const int* sbase;
+    // touching sphere-sphere pairs of the last contact launch, one count per
+    // wave of its sphere blocks (the host adds them up; no same-address atomics)
+    unsigned* wss;
     // penetrator (static part)
     int pshape, pmotion, pwait, ntri;
     double prad, pheight, pdir[3], pmass;

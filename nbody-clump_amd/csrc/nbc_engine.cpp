@@ -259,7 +259,10 @@ class Engine : public EngineBase {
         HIPC(dalloc(&dstatus, 4));
         HIPC(dalloc(&ellmax, 1));
         ell_k = O.single_pass < 0 ? 0 : std::min(P.BranchLmtLen / 2, 32);
-        HIPC(dalloc(&counters, 2));  // touching pairs, touching wall contacts
+        HIPC(dalloc(&counters, 2));  // [1]: touching wall contacts of the last step ([0] unused)
+        nwss = 4 * (((size_t)N + 255) / 256);  // one per wave of the contact launch's sphere blocks
+        HIPC(dalloc(&wss, nwss));
+        dp.wss = wss;
         HIPC(dalloc(&pst, 1));
         nbk = contacts_blocks(N, block);
         HIPC(dalloc(&pene_partial, 6 * (((size_t)N + 63) / 64)));  // one per wave of k_walls
@@ -381,7 +384,7 @@ class Engine : public EngineBase {
         F(boff); F(bent); F(nfoff); F(nboff); F(nfw); F(nfs); F(nbo); F(nbe); F(slot2near); F(hn[0]); F(hn[1]);
         F(key0); F(key1); F(val0); F(val1); F(hist); F(hscan); F(stmp); F(stotal);
         F(cellkey); F(perm); F(iperm); F(cbeg); F(cend); F(fcnt); F(bcnt); F(mbeg); F(mend); F(mem);
-        F(dstatus); F(counters); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
+        F(dstatus); F(counters); F(wss); F(pst); F(pene_partial); F(wb.list); F(wb.pos_of); F(wb.res); F(wl_off); F(cache_f); F(cache_t); F(ke_partial); F(dtri);
         F(rec.tot); F(rec.parts); F(rec.pene); F(rec.pair_fcn); F(rec.pair_cpt);
         F(xcnt); F(xoff); F(xbeg); F(xend); F(xk0); F(xk1); F(xv0); F(xv1); F(xhist); F(xhscan); F(xstmp);
         F(ell_f); F(ell_b); F(ell_x); F(ellmax);
@@ -722,6 +725,7 @@ class Engine : public EngineBase {
             HIPC(hipMemcpy(dstatus, init, sizeof(init), hipMemcpyHostToDevice));
         }
         HIPC(hipMemset(counters, 0, 2 * sizeof(unsigned long long)));
+        KC(launch_fill_u32(st, wss, (long long)nwss, 0u));
         // penetrator
         PeneState ps;
         std::memset(&ps, 0, sizeof(ps));
@@ -1583,9 +1587,11 @@ class Engine : public EngineBase {
         KC(launch_kinetic(st, Nc, C.vw, C.prop, ke_partial));
         std::vector<double> kp((Nc + 255) / 256);
         unsigned long long ctr[2];
+        std::vector<unsigned> wc(nwss);
         PeneState ps;
         HIPC(hipMemcpyAsync(kp.data(), ke_partial, sizeof(double) * kp.size(), hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(ctr, counters, sizeof(ctr), hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(wc.data(), wss, sizeof(unsigned) * nwss, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(&ps, pst, sizeof(ps), hipMemcpyDeviceToHost, st));
         HIPC(hipStreamSynchronize(st));
         double ke = 0.0;
@@ -1594,7 +1600,9 @@ class Engine : public EngineBase {
         s->rebuilds = nrebuild;
         s->front_pairs = nfront;
         s->back_entries = nback;
-        s->touching_ss = (int64_t)ctr[0];
+        unsigned long long tss = 0;
+        for (unsigned v : wc) tss += v;
+        s->touching_ss = (int64_t)tss;
         s->touching_wall = (int64_t)ctr[1];
         s->near_refreshes = nnear;
         s->near_pairs = near_total;
@@ -1690,6 +1698,8 @@ class Engine : public EngineBase {
     uint32_t* hring = nullptr;
     hipEvent_t ev_stat[kRing] = {};
     unsigned long long* counters = nullptr;
+    unsigned* wss = nullptr;
+    size_t nwss = 0;
     PeneState* pst = nullptr;
     double *pene_partial = nullptr, *cache_f = nullptr, *cache_t = nullptr, *ke_partial = nullptr, *dtri = nullptr;
     WallBuf<R> wb = {nullptr, nullptr, nullptr};

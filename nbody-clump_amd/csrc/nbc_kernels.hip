@@ -52,6 +52,17 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int step_of(const DevParams& P, int n) { return P.sbase ? *P.sbase + n : n; }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+// Waves per block of the contact kernels. A block keeps its LDS until its
+// slowest wave ends, and the waves of one block differ in how many flush
+// rounds they run: one-wave blocks let each wave's slot refill as soon as it
+// is done (FP32 K3 72.4 -> 69.9 us on the settled 1.12 M-sphere bed).
+#ifndef K3_NW_F32
+#define K3_NW_F32 1
+#endif
+#ifndef K3_NW_F64
+#define K3_NW_F64 1
+#endif
+
 // Logical block index giving each XCD runs of G consecutive blocks, the runs
 // of the 8 XCDs interleaved: neighbouring blocks (which share spheres) meet in
 // one L2, and every XCD still gets work from the whole bed.
@@ -112,8 +123,7 @@ __device__ __forceinline__ bool maybe_touch(const DevParams& P, const F4& a, con
 // (src/dem_integrate_penetrator.cu:3-41) for step n.
 __device__ __forceinline__ void k1_thread0(const DevParams& P, PeneState* pst, unsigned long long* counters,
                                            int* halt, int n) {
-    counters[0] = 0ull;
-    counters[1] = 0ull;
+    counters[1] = 0ull;  // touching wall contacts (the sphere-sphere count is per wave, DevParams::wss)
     if (!P.pene_on) return;
     const double Dt = P.dt;
     if (P.pmotion == NBC_PENE_GRAVITY && (!P.pwait || pst->contact_state != 0)) {
@@ -886,9 +896,15 @@ __global__ void k_hfront_total(int N, int half, int* fcnt, const int* bcnt, cons
 // the host then fills the rows with the multi-pass kernels. ellmax receives
 // the longest row (front or back), from which the host sizes K for the next
 // rebuild.
+// Same-address device atomics are serialised at the memory side (~13 ns each
+// on this part), so a wave only issues its atomicMax when its value exceeds
+// the maximum recorded so far (a device-coherent load; a stale read only
+// costs an extra atomic).
 __device__ __forceinline__ void wave_max_to(unsigned* dst, int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0 && v > 0) atomicMax(dst, (unsigned)v);
+    if ((threadIdx.x & 63) == 0 && v > 0 &&
+        (unsigned)v > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(dst, (unsigned)v);
 }
 
 // Single-pass uniform-grid search (src/dem_grid.cu:466-552, the loop of
@@ -1833,7 +1849,7 @@ __device__ __forceinline__ D4 qent_pos(const QEnt<R>& q) {
 // history-valid bits. Each lane adds its sphere's results in queue order,
 // which is its slot order: front entries ascending, then back entries in
 // back-row order.
-template <class R, bool RECORD, bool DM>
+template <class R, bool RECORD, bool DM, int NW>
 __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
                                               const ListBuf<R>& L, int p, const WallBuf<R>& WL,
                                               unsigned long long* counters, int* bad, const RecordBuf& rec, int remap,
@@ -1841,25 +1857,24 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     typedef typename Vec4T<R>::type R4;
     const int* status = halt + 3;  // ST_* bits
     constexpr int Q = 128;
-    __shared__ int q_e[4][Q];    // front near index of the pair; -1 once found not touching
-    __shared__ int q_oth[4][Q];  // the other sphere
-    __shared__ int q_key[4][Q];  // owner lane, +64 for back entries (non-decreasing)
+    __shared__ int q_e[NW][Q];    // front near index of the pair; -1 once found not touching
+    __shared__ int q_oth[NW][Q];  // the other sphere
+    __shared__ int q_key[NW][Q];  // owner lane, +64 for back entries (non-decreasing)
     // FP64 form: the scan also gathers the other sphere's clump and the pair's
     // history-valid bits, so no load of the flush waits on a clump index or a
     // pair word (121 -> 117.6 us on the settled 1.12 M-sphere bed); the FP32
     // form, slower with more loads in flight during the scan (+0.5 us), loads
     // them in the flush
     constexpr bool CARRY = sizeof(R) == sizeof(double);
-    __shared__ uint32_t q_cx[4][CARRY ? Q : 1];  // other sphere's clump | the pair word's history-valid bits
-    __shared__ QEnt<R> q_ent[4][Q];  // other sphere's position, then the pair's results
-    __shared__ int s_fo[4][65], s_bo[4][65];
-    __shared__ F4 s_p32[4][64];
-    __shared__ int s_cid[4][CARRY ? 64 : 1];
-    __shared__ unsigned int s_cnt[4];
-    const int lb = remap ? xcd_group_block(vblock, nvblocks, 16) : vblock;
-    const int i = lb * blockDim.x + threadIdx.x;
+    __shared__ uint32_t q_cx[NW][CARRY ? Q : 1];  // other sphere's clump | the pair word's history-valid bits
+    __shared__ QEnt<R> q_ent[NW][Q];  // other sphere's position, then the pair's results
+    __shared__ int s_fo[NW][65], s_bo[NW][65];
+    __shared__ F4 s_p32[NW][64];
+    __shared__ int s_cid[NW][CARRY ? 64 : 1];
+    const int lb = remap ? xcd_group_block(vblock, nvblocks, 64 / NW) : vblock;
+    const int i = lb * (64 * NW) + threadIdx.x;
     const bool active = i < P.N;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = NW > 1 ? threadIdx.x >> 6 : 0;
     const int wbase = i - lane;
     R* hcur = L.hist[p];
     R* hnew = L.hist[p ^ 1];
@@ -2122,16 +2137,14 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             }
         }
     }
-    // touching sphere-sphere count
+    // touching sphere-sphere count of this wave's spheres, stored in the
+    // wave's own slot: one device-scope atomicAdd per block on a single
+    // counter had cost the FP32 kernel 17 us per launch (89.5 -> 72.4 us on
+    // the settled 1.12 M-sphere bed), same-address atomics being serialised
+    // at the memory side
     unsigned a0 = (unsigned)n_ss;
     for (int o = 32; o > 0; o >>= 1) a0 += __shfl_down(a0, o, 64);
-    if (lane == 0) s_cnt[wv] = a0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int k = 0; k < ((int)blockDim.x + 63) >> 6; ++k) t += s_cnt[k];
-        if (t) atomicAdd(&counters[0], t);
-    }
+    if (lane == 0) P.wss[wbase >> 6] = a0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2148,7 +2161,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
                                            unsigned long long* counters, int* bad, const RecordBuf& rec, int vblock) {
     typedef typename Vec4T<R>::type R4;
     const int lane = threadIdx.x & 63;
-    const int k = vblock * 256 + threadIdx.x;
+    const int k = vblock * (int)blockDim.x + threadIdx.x;
     const int gw = k >> 6;  // wave index: penetrator partial slot
     const bool active = k < len;
     int n_w = 0;
@@ -2442,7 +2455,7 @@ __device__ __forceinline__ void walls_body(const DevParams& P, const SphereBuf<R
 // the other writes after k_clump_after). The FP32 form asks for five waves per
 // SIMD, which the contact blocks need and the rarer wall blocks pay for with
 // a few spills.
-template <class R, bool RECORD, bool DM>
+template <class R, bool RECORD, bool DM, int NW>
 __device__ __forceinline__ void step3_body(const DevParams& P, const SphereBuf<R>& S, const ClumpBuf<R>& C,
                                            const ListBuf<R>& L, int p, const WallBuf<R>& WL, int len, int nwb,
                                            PeneState* pst, double* pene_partial, unsigned long long* counters,
@@ -2451,26 +2464,28 @@ __device__ __forceinline__ void step3_body(const DevParams& P, const SphereBuf<R
     if ((int)blockIdx.x < nwb)
         walls_body<R, RECORD, DM>(P, S, C, WL, len, pst, pene_partial, counters, bad, rec, blockIdx.x);
     else
-        contacts_body<R, RECORD, DM>(P, S, C, L, p, WL, counters, bad, rec, remap, halt, (int)blockIdx.x - nwb,
+        contacts_body<R, RECORD, DM, NW>(P, S, C, L, p, WL, counters, bad, rec, remap, halt, (int)blockIdx.x - nwb,
                                  (int)gridDim.x - nwb);
 }
 
 template <bool RECORD, bool DM>
-__global__ __launch_bounds__(256, 5) void k_contacts_f32(DevParams P, SphereBuf<float> S, ClumpBuf<float> C,
+__global__ __launch_bounds__(64 * K3_NW_F32, 5) void k_contacts_f32(DevParams P, SphereBuf<float> S, ClumpBuf<float> C,
                                                          ListBuf<float> L, int p, WallBuf<float> WL, int len, int nwb,
                                                          PeneState* pst, double* pene_partial,
                                                          unsigned long long* counters, int* bad, RecordBuf rec,
                                                          int remap, const int* halt, int n) {
-    step3_body<float, RECORD, DM>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
+    step3_body<float, RECORD, DM, K3_NW_F32>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap,
+                                            halt, n);
 }
 
 template <bool RECORD, bool DM>
-__global__ __launch_bounds__(256, 3) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
+__global__ __launch_bounds__(64 * K3_NW_F64, 3) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
                                                       ListBuf<double> L, int p, WallBuf<double> WL, int len, int nwb,
                                                       PeneState* pst, double* pene_partial,
                                                       unsigned long long* counters, int* bad, RecordBuf rec,
                                                       int remap, const int* halt, int n) {
-    step3_body<double, RECORD, DM>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec, remap, halt, n);
+    step3_body<double, RECORD, DM, K3_NW_F64>(P, S, C, L, p, WL, len, nwb, pst, pene_partial, counters, bad, rec,
+                                              remap, halt, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -3073,9 +3088,10 @@ template <class R>
 int launch_contacts(hipStream_t st, const DevParams& P, SphereBuf<R> S, ClumpBuf<R> C, ListBuf<R> L, int p,
                     WallBuf<R> WL, int wlen, PeneState* pst, double* partial, unsigned long long* ctr, int* bad,
                     RecordBuf rec, bool record, int remap, const int* halt, int step) {
-    const int nwb = wlen > 0 ? (int)nblk(wlen, 256) : 0;
-    const unsigned nb = nblk(P.N, 256) + (unsigned)nwb;
-#define NBC_K3(K, REC, DM) K<REC, DM><<<nb, 256, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, \
+    const int bs = 64 * (sizeof(R) == sizeof(float) ? K3_NW_F32 : K3_NW_F64);
+    const int nwb = wlen > 0 ? (int)nblk(wlen, bs) : 0;
+    const unsigned nb = nblk(P.N, bs) + (unsigned)nwb;
+#define NBC_K3(K, REC, DM) K<REC, DM><<<nb, bs, 0, st>>>(P, S, C, L, p, WL, wlen, nwb, pst, partial, ctr, bad, rec, \
                                                           remap, halt, step)
     const bool dm = P.driver != NBC_DRIVER_CLUMP;
     if constexpr (sizeof(R) == sizeof(float)) {
