@@ -62,6 +62,16 @@ __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) -
 #ifndef K3_NW_F64
 #define K3_NW_F64 1
 #endif
+// Waves per SIMD the contact kernels are compiled for: at five the FP32 form
+// needs 96 VGPRs without spilling on its contact path; four (72.5 us) leaves
+// latency unhidden, six (78.4 us) spills in the flush. The FP64 form spills
+// above three.
+#ifndef K3_WAVES_F32
+#define K3_WAVES_F32 5
+#endif
+#ifndef K3_WAVES_F64
+#define K3_WAVES_F64 3
+#endif
 
 // Logical block index giving each XCD runs of G consecutive blocks, the runs
 // of the 8 XCDs interleaved: neighbouring blocks (which share spheres) meet in
@@ -1868,7 +1878,10 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     constexpr bool CARRY = sizeof(R) == sizeof(double);
     __shared__ uint32_t q_cx[NW][CARRY ? Q : 1];  // other sphere's clump | the pair word's history-valid bits
     __shared__ QEnt<R> q_ent[NW][Q];  // other sphere's position, then the pair's results
-    __shared__ int s_fo[NW][65], s_bo[NW][65];
+    // row offsets stay in each lane's registers; the binary search reads them
+    // with lane shuffles, which need no LDS allocation (6.7 instead of 7.2 KB
+    // per block; 69.8 -> 68.5 us on the settled 1.12 M-sphere bed)
+    int fo_l = 0, bo_l = 0, fo_e = 0, bo_e = 0;
     __shared__ F4 s_p32[NW][64];
     __shared__ int s_cid[NW][CARRY ? 64 : 1];
     const int lb = remap ? xcd_group_block(vblock, nvblocks, 64 / NW) : vblock;
@@ -1882,12 +1895,12 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     V3<R> Fss = mk<R>(R(0), R(0), R(0)), Tss = Fss;
     {
         const int ir = i < P.N ? i : P.N;
-        s_fo[wv][lane] = L.nfoff[ir];
-        s_bo[wv][lane] = L.nboff[ir];
+        fo_l = L.nfoff[ir];
+        bo_l = L.nboff[ir];
         if (lane == 63) {
             const int ie = i + 1 < P.N ? i + 1 : P.N;
-            s_fo[wv][64] = L.nfoff[ie];
-            s_bo[wv][64] = L.nboff[ie];
+            fo_e = L.nfoff[ie];
+            bo_e = L.nboff[ie];
         }
         const D4 pa = S.pos[i < P.N ? i : P.N - 1];  // clamped load, then a value select
         if (CARRY) s_cid[wv][lane] = S.cid[i < P.N ? i : P.N - 1];
@@ -2016,7 +2029,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     };
 
     // the back list's first chunk is fetched before the front scan
-    const int B0 = s_bo[wv][0], B1 = s_bo[wv][64];
+    const int B0 = __builtin_amdgcn_readfirstlane(bo_l), B1 = __builtin_amdgcn_readlane(bo_e, 63);
     int onext = 0, enext = 0;
     if (B0 + lane < B1) {
         onext = __builtin_nontemporal_load(L.nbo + B0 + lane);
@@ -2024,7 +2037,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     }
     // front entries: pairs the row's sphere owns (it writes their history)
     {
-        const int F0 = s_fo[wv][0], F1 = s_fo[wv][64];
+        const int F0 = __builtin_amdgcn_readfirstlane(fo_l), F1 = __builtin_amdgcn_readlane(fo_e, 63);
         uint32_t wnext = F0 + lane < F1 ? __builtin_nontemporal_load(L.nfw + F0 + lane) : 0u;
         for (int c = F0; c < F1; c += 64) {
             const int e = c + lane;
@@ -2034,10 +2047,11 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             int r = 0, j = 0, cj = 0;
             D4 pj;
             pj.x = pj.y = pj.z = pj.w = 0.0;
-            if (e < F1) {
+            // the entry's row; every lane of the wave takes part in the shuffles
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1)
-                    if (s_fo[wv][r + st] <= e) r += st;
+            for (int st = 32; st > 0; st >>= 1)
+                if (__shfl(fo_l, r + st, 64) <= e) r += st;
+            if (e < F1) {
                 j = (int)(w & JMASK);
                 // the partner's FP32 copy is made here from its FP64 position: a
                 // gather fetches the same DRAM sector for 16 or 32 bytes
@@ -2064,10 +2078,10 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             uint32_t wo = 0;
             D4 po;
             po.x = po.y = po.z = po.w = 0.0;
-            if (e < B1) {
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1)
-                    if (s_bo[wv][r + st] <= e) r += st;
+            for (int st = 32; st > 0; st >>= 1)
+                if (__shfl(bo_l, r + st, 64) <= e) r += st;
+            if (e < B1) {
                 po = S.pos[o];
                 if (CARRY) {
                     co = S.cid[o];
@@ -2469,7 +2483,7 @@ __device__ __forceinline__ void step3_body(const DevParams& P, const SphereBuf<R
 }
 
 template <bool RECORD, bool DM>
-__global__ __launch_bounds__(64 * K3_NW_F32, 5) void k_contacts_f32(DevParams P, SphereBuf<float> S, ClumpBuf<float> C,
+__global__ __launch_bounds__(64 * K3_NW_F32, K3_WAVES_F32) void k_contacts_f32(DevParams P, SphereBuf<float> S, ClumpBuf<float> C,
                                                          ListBuf<float> L, int p, WallBuf<float> WL, int len, int nwb,
                                                          PeneState* pst, double* pene_partial,
                                                          unsigned long long* counters, int* bad, RecordBuf rec,
@@ -2479,7 +2493,7 @@ __global__ __launch_bounds__(64 * K3_NW_F32, 5) void k_contacts_f32(DevParams P,
 }
 
 template <bool RECORD, bool DM>
-__global__ __launch_bounds__(64 * K3_NW_F64, 3) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
+__global__ __launch_bounds__(64 * K3_NW_F64, K3_WAVES_F64) void k_contacts_f64(DevParams P, SphereBuf<double> S, ClumpBuf<double> C,
                                                       ListBuf<double> L, int p, WallBuf<double> WL, int len, int nwb,
                                                       PeneState* pst, double* pene_partial,
                                                       unsigned long long* counters, int* bad, RecordBuf rec,
