@@ -154,7 +154,8 @@ typedef struct nbc_params {
 typedef struct nbc_options {
     int32_t precision;       /* NBC_FP64 (default) or NBC_MIXED                */
     int32_t wall_type;       /* 0 = NBC_WALL_CUBE default when isWallForce     */
-    int32_t block_size;      /* threads per block for per-sphere kernels (64..1024) */
+    int32_t block_size;      /* threads per block for per-sphere kernels (64..1024, used up to 256;
+                                the contact kernels run fixed one-wave blocks) */
     int32_t radix_bits;      /* LSD radix digit width for the cell sort (1..8) */
     int32_t pair_capacity;   /* front-pair slots reserved per sphere (average) */
     int32_t batch_steps;     /* steps per chunk between status checks (default 16 with graphs, else 4) */

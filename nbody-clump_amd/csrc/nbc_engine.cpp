@@ -119,7 +119,7 @@ class Engine : public EngineBase {
         block = O.block_size > 0 ? O.block_size : 256;
         if (block < 64 || block > 1024 || (block % 64) != 0)
             return fail(NBC_E_BADINPUT, "nbc_create: block_size must be a multiple of 64 in [64, 1024]");
-        if (block > 256) block = 256;  // k_contacts is launched with __launch_bounds__(256)
+        if (block > 256) block = 256;  // the per-sphere kernels are tuned and tested up to 256 threads
         rbits = O.radix_bits > 0 ? O.radix_bits : 8;
         if (rbits > 8) rbits = 8;
         wall_type = P.isWallForce ? (O.wall_type ? O.wall_type : NBC_WALL_CUBE) : NBC_WALL_NONE;
