@@ -1882,6 +1882,7 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     // with lane shuffles, which need no LDS allocation (6.7 instead of 7.2 KB
     // per block; 69.8 -> 68.5 us on the settled 1.12 M-sphere bed)
     int fo_l = 0, bo_l = 0, fo_e = 0, bo_e = 0;
+    int t_slot = 0, t_wk = -1;  // the tail's member slot and wall-list position, loaded with the prologue
     __shared__ F4 s_p32[NW][64];
     __shared__ int s_cid[NW][CARRY ? 64 : 1];
     // XCD runs of 16,384 spheres (runs of 1,024 / 4,096 / 65,536 / 131,072
@@ -1906,6 +1907,8 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
             bo_e = L.nboff[ie];
         }
         const D4 pa = S.pos[i < P.N ? i : P.N - 1];  // clamped load, then a value select
+        t_slot = S.slot[i < P.N ? i : P.N - 1];
+        if (P.nfaces > 0 || P.pene_on) t_wk = WL.pos_of[i < P.N ? i : P.N - 1];
         if (CARRY) s_cid[wv][lane] = S.cid[i < P.N ? i : P.N - 1];
         F4 a = pos32(P, xyz(pa), pa.w);
         if (!active) a.w = -1.0f;
@@ -2098,9 +2101,8 @@ __device__ __forceinline__ void contacts_body(const DevParams& P, const SphereBu
     if (qn > 0) flush();
     if (active) {
         const R4 ki = S.kin[i];
-        const size_t so = (size_t)S.slot[i];  // k_clump_after reads the totals in member order
-        int wk = -1;
-        if (P.nfaces > 0 || P.pene_on) wk = WL.pos_of[i];
+        const size_t so = (size_t)t_slot;  // k_clump_after reads the totals in member order
+        const int wk = t_wk;
         if (P.nfaces > 0 && wk < 0) {
             // input check of the wall kernel (src/dem_force_cube_wall_clump.cu:196-197)
             // for spheres off the wall list: exact only when a cheap sign is off
